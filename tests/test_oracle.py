@@ -1,0 +1,370 @@
+"""Pins the CPU oracle against closed forms, brute force and invariants.
+
+The reference ships no tests, golden vectors or fixtures for this path, and
+it cannot be built (CUDA texture references, missing headers), so every
+oracle function is pinned here by mathematics that a plausible mistake (a
+dropped term, a wrong sign or axis, a transposed operand) would violate.
+"""
+import numpy as np
+import pytest
+
+from tval3_amd import workload as wl
+
+
+def rnd(shape, seed):
+    return np.random.default_rng(seed).normal(size=shape)
+
+
+# ---------------------------------------------------------------- ray list
+def test_build_paths_matches_enumeration(orc):
+    E, R, mps = 5, 7, 3
+    rng = np.random.default_rng(0)
+    mask = (rng.uniform(size=E * R) < 0.6).astype(np.uint8)
+    pe, pr = orc.build_paths(E, R, mps, mask)
+    exp = [(mp * E + e, mp * R + r) for mp in range(mps) for e in range(E) for r in range(R)
+           if mask[e * R + r]]
+    assert list(zip(pe.tolist(), pr.tolist())) == exp
+    pe, pr = orc.build_paths(E, R, 1, None)
+    assert len(pe) == E * R and pe[R] == 1 and pr[R] == 0
+
+
+# ---------------------------------------------------------------- Bresenham
+def _coords(vox, rv):
+    rvx, rvy, rvz = rv
+    z = vox // (rvx * rvy)
+    x = (vox % (rvx * rvy)) // rvy
+    y = vox % rvy
+    return np.stack([x, y, z], 1)
+
+
+def test_bresenham_axis_and_diagonal(orc):
+    rv = (16, 16, 16)
+    v = orc.bresenham_ray((2, 3, 4), (12, 3, 4), rv)
+    c = _coords(v, rv)
+    assert len(v) == 11 and (c[:, 0] == np.arange(2, 13)).all()
+    assert (c[:, 1] == 3).all() and (c[:, 2] == 4).all()
+    v = orc.bresenham_ray((1, 1, 1), (9, 9, 9), rv)
+    c = _coords(v, rv)
+    assert len(v) == 9 and (c == np.arange(1, 10)[:, None]).all()
+    # backwards along z (z is the fast axis: dz > dx, dz > dy)
+    c = _coords(orc.bresenham_ray((5, 5, 10), (5, 7, 2), rv), rv)
+    assert len(c) == 9 and (c[:, 2] == np.arange(10, 1, -1)).all()
+
+
+def test_bresenham_closed_form(orc):
+    """Every step of the walk of 3d/tval3_gpu_3d_kernels.cu:1672-1759 puts the
+    slow coordinates at ceil(j*m_k/m1 - 1/2) (round half down), and the fast
+    axis follows the >= cascade (x, then y, then z) of :1679,:1687."""
+    rv = (40, 40, 40)
+    rng = np.random.default_rng(1)
+    for _ in range(400):
+        a = rng.integers(0, 40, 3)
+        b = rng.integers(0, 40, 3)
+        if rng.uniform() < 0.3:                # force ties in |d|
+            b[1] = np.clip(a[1] + (b[0] - a[0]), 0, 39)
+        c = _coords(orc.bresenham_ray(a, b, rv), rv)
+        d = b - a
+        m = np.abs(d)
+        if m[0] >= m[1] and m[0] >= m[2]:
+            order = (0, 1, 2)
+        elif m[1] >= m[0] and m[1] >= m[2]:
+            order = (1, 0, 2)
+        else:
+            order = (2, 0, 1)
+        f, s2, s3 = order
+        m1 = m[f]
+        assert len(c) == m1 + 1
+        j = np.arange(m1 + 1)
+        assert (c[:, f] == a[f] + np.sign(d[f] if d[f] else 1) * j).all()
+        for s in (s2, s3):
+            if m1 == 0:
+                continue
+            inc = -1 if d[s] < 0 else 1
+            # ceil((2 j m - m1) / (2 m1)) in integers, clipped at 0 for j = 0
+            k = -((-(2 * j * m[s] - m1)) // (2 * m1))
+            k[0] = 0
+            assert (c[:, s] == a[s] + inc * k).all(), (a, b)
+
+
+# ---------------------------------------------------------------- Siddon
+def _chord(p0, p1, lo, hi):
+    d = p1 - p0
+    amin, amax = 0.0, 1.0
+    for a in range(3):
+        if d[a] == 0:
+            if not (lo[a] < p0[a] < hi[a]):
+                return 0.0
+            continue
+        t0, t1 = sorted(((lo[a] - p0[a]) / d[a], (hi[a] - p0[a]) / d[a]))
+        amin, amax = max(amin, t0), min(amax, t1)
+    return max(0.0, amax - amin) * np.linalg.norm(d)
+
+
+def test_siddon_chord_length(orc):
+    rv = (7, 9, 5)
+    origin = np.array([-1.0, -0.5, -0.8])
+    h = np.array([0.3, 0.2, 0.35])
+    hi = origin + np.array(rv) * h
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        p0 = rng.uniform(-2.5, 2.5, 3)
+        p1 = rng.uniform(-2.5, 2.5, 3)
+        vox, ln = orc.siddon_ray(p0, p1, origin, h, rv)
+        assert np.isclose(ln.sum(), _chord(p0, p1, origin, hi), rtol=1e-12, atol=1e-12)
+        assert (ln > 0).all()
+        assert (vox >= 0).all() and (vox < np.prod(rv)).all()
+
+
+def test_siddon_axis_ray_through_centres(orc):
+    rv = (6, 5, 4)
+    origin = np.zeros(3)
+    h = np.array([0.5, 0.25, 1.0])
+    p0 = np.array([-1.0, 2.5 * 0.25, 1.5])       # y centre of voxel 2, z centre of voxel 1
+    p1 = np.array([10.0, 2.5 * 0.25, 1.5])
+    vox, ln = orc.siddon_ray(p0, p1, origin, h, rv)
+    assert len(vox) == 6 and np.allclose(ln, 0.5, rtol=0, atol=1e-14)
+    c = _coords(vox, rv)
+    assert (c[:, 0] == np.arange(6)).all() and (c[:, 1] == 2).all() and (c[:, 2] == 1).all()
+
+
+def test_siddon_supersampled_line_integral(orc):
+    """Per-voxel lengths against brute-force sampling of the segment."""
+    rng = np.random.default_rng(3)
+    for N in (4, 6, 8):
+        rv = (N, N, N)
+        origin = -np.ones(3)
+        h = np.full(3, 2.0 / N)
+        for _ in range(10):
+            p0, p1 = wl.sphere_points(2, 1.1, seed=int(rng.integers(1 << 30)))
+            vox, ln = orc.siddon_ray(p0, p1, origin, h, rv)
+            K = 400000
+            t = (np.arange(K) + 0.5) / K
+            P = p0[None] + t[:, None] * (p1 - p0)[None]
+            idx = np.floor((P - origin) / h).astype(int)
+            ok = ((idx >= 0) & (idx < N)).all(1)
+            lin = idx[ok, 2] * N * N + idx[ok, 0] * N + idx[ok, 1]
+            L = np.linalg.norm(p1 - p0)
+            bf = np.bincount(lin, minlength=N ** 3) * (L / K)
+            got = np.zeros(N ** 3)
+            np.add.at(got, vox, ln)
+            assert np.abs(got - bf).max() < 3 * L / K + 1e-12
+
+
+def test_siddon_csr_matches_rays(orc):
+    g = wl.transducer_geometry(6, 10, N=8)
+    op = orc.siddon_op(g)
+    rowptr, col, val = orc.siddon_csr(op)
+    assert rowptr[-1] == len(col)
+    for k in (0, 13, 59):
+        e, r = k // 10, k % 10
+        vox, ln = orc.siddon_ray(g["e_pos"][e], g["r_pos"][r], g["origin"], g["voxel"], g["rv"])
+        assert (col[rowptr[k]:rowptr[k + 1]] == vox).all()
+        assert np.allclose(val[rowptr[k]:rowptr[k + 1]], ln)
+
+
+# ---------------------------------------------------------------- products
+def test_dense_and_csr_matvec_brute_force(orc):
+    rng = np.random.default_rng(4)
+    m, n = 23, 31
+    A = rng.normal(size=(m, n)) * (rng.uniform(size=(m, n)) < 0.3)
+    x, y = rng.normal(size=n), rng.normal(size=m)
+    for op in (orc.dense_op(A), None):
+        if op is None:
+            rows, cols = np.nonzero(A)
+            rowptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=m))])
+            op = orc.csr_op(m, n, rowptr, cols, A[rows, cols])
+        assert np.allclose(orc.matvec(op, 0, x), A @ x, rtol=1e-13, atol=1e-13)
+        assert np.allclose(orc.matvec(op, 1, y), A.T @ y, rtol=1e-13, atol=1e-13)
+        # integer-valued inputs: FP32 result is exact
+        Ai = np.round(A * 4)
+    rows, cols = np.nonzero(Ai)
+    rowptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=m))])
+    op = orc.csr_op(m, n, rowptr, cols, Ai[rows, cols])
+    xi = rng.integers(-5, 6, n).astype(np.float32)
+    assert (orc.matvec(op, 0, xi, np.float32) == (Ai @ xi).astype(np.float32)).all()
+
+
+def test_csr_transpose_round_trip(orc):
+    rng = np.random.default_rng(5)
+    m, n = 40, 17
+    A = rng.normal(size=(m, n)) * (rng.uniform(size=(m, n)) < 0.25)
+    rows, cols = np.nonzero(A)
+    rowptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=m))])
+    cp, rw, cv = orc.csr_transpose(m, n, rowptr, cols, A[rows, cols])
+    B = np.zeros((n, m))
+    for j in range(n):
+        seg = slice(cp[j], cp[j + 1])
+        assert (np.diff(rw[seg]) > 0).all()            # rows ascending within a column
+        B[j, rw[seg]] = cv[seg]
+    assert (B == A.T).all()
+    rp2, c2, v2 = orc.csr_transpose(n, m, cp, rw, cv)
+    assert (rp2 == rowptr).all() and (c2 == cols).all() and (v2 == A[rows, cols]).all()
+
+
+def test_bresenham_and_siddon_adjoint(orc):
+    g = wl.transducer_geometry(5, 9, N=6)
+    for op in (orc.bresenham_op(g), orc.siddon_op(g)):
+        x = rnd(op.s.n, 6)
+        y = rnd(op.s.m, 7)
+        lhs = orc.matvec(op, 0, x) @ y
+        rhs = x @ orc.matvec(op, 1, y)
+        assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1)
+
+
+# ---------------------------------------------------------------- stencils
+def _vol(U, dims):
+    p, q, r = dims
+    return U.reshape(r, q, p)          # [z, x, y]
+
+
+def test_D_closed_form_and_adjoint(orc):
+    dims = (5, 4, 3)
+    U = rnd(60, 8)
+    Ux, Uy, Uz = orc.D(U, dims)
+    V = _vol(U, dims)
+    assert np.allclose(_vol(Uy, dims), np.roll(V, -1, 2) - V)
+    assert np.allclose(_vol(Ux, dims), np.roll(V, -1, 1) - V)
+    assert np.allclose(_vol(Uz, dims), np.roll(V, -1, 0) - V)
+    X, Y, Z = rnd(60, 9), rnd(60, 10), rnd(60, 11)
+    lhs = Ux @ X + Uy @ Y + Uz @ Z
+    rhs = U @ orc.Dt(X, Y, Z, dims)
+    assert abs(lhs - rhs) < 1e-12 * (abs(lhs) + 1)
+    c = np.full(60, 3.25)
+    assert all((d == 0).all() for d in orc.D(c, dims))
+    # each difference sums to zero along its own axis
+    assert np.allclose(_vol(Ux, dims).sum(1), 0)
+
+
+def test_DtD_is_periodic_laplacian(orc):
+    dims = (6, 5, 4)
+    p, q, r = dims
+    U = rnd(p * q * r, 12)
+    L = orc.Dt(*orc.D(U, dims), dims)
+    lam = sum(np.meshgrid(2 - 2 * np.cos(2 * np.pi * np.arange(r) / r),
+                          2 - 2 * np.cos(2 * np.pi * np.arange(q) / q),
+                          2 - 2 * np.cos(2 * np.pi * np.arange(p) / p), indexing="ij"))
+    ref = np.real(np.fft.ifftn(lam * np.fft.fftn(_vol(U, dims))))
+    assert np.allclose(_vol(L, dims), ref, atol=1e-12)
+
+
+# ---------------------------------------------------------------- shrinkage
+def test_shrink_aniso_closed_form(orc):
+    n = 500
+    U = [rnd(n, 20 + k) for k in range(3)]
+    S = [rnd(n, 30 + k) for k in range(3)]
+    beta = 4.0
+    Wx, Wy, Wz, lam1 = orc.shrink(*U, *S, beta, iso=False)
+    exp = []
+    for u, s in zip(U, S):
+        ub = u - s / beta
+        exp.append(np.sign(ub) * np.maximum(np.abs(ub) - 1 / beta, 0))
+    for w, e in zip((Wx, Wy, Wz), exp):
+        assert np.allclose(w, e, atol=1e-15)
+    assert np.isclose(lam1, sum(np.abs(e).sum() for e in exp))
+    # dead zone
+    z = np.full(n, 0.2)
+    W = orc.shrink(z, -z, z, 0 * z, 0 * z, 0 * z, 4.0)
+    assert all((w == 0).all() for w in W[:3]) and W[3] == 0
+
+
+def test_shrink_iso_closed_form(orc):
+    n = 500
+    U = [rnd(n, 40 + k) for k in range(3)]
+    S = [rnd(n, 50 + k) for k in range(3)]
+    beta = 2.0
+    Wx, Wy, Wz, lam1 = orc.shrink(*U, *S, beta, iso=True)
+    Ub = np.stack([u - s / beta for u, s in zip(U, S)])
+    nrm = np.linalg.norm(Ub, axis=0)
+    fac = np.maximum(nrm - 1 / beta, 0) / np.where(nrm > 0, nrm, 1)
+    for k, w in enumerate((Wx, Wy, Wz)):
+        assert np.allclose(w, Ub[k] * fac, atol=1e-15)
+    assert np.isclose(lam1, np.maximum(nrm - 1 / beta, 0).sum())
+    # one non-zero component: iso == aniso
+    z = np.zeros(n)
+    a = orc.shrink(U[0], z, z, S[0], z, z, beta, iso=True)
+    b = orc.shrink(U[0], z, z, S[0], z, z, beta, iso=False)
+    assert np.allclose(a[0], b[0]) and np.isclose(a[3], b[3])
+
+
+def test_scaleb_closed_form(orc):
+    b = np.array([0.1, -0.05, 0.2, 0.15])
+    s, scl = orc.scaleb(b)
+    # smallest |b| is -0.05, largest 0.2: range 0.25 -> scaled to 0.5
+    assert np.isclose(scl, 2.0) and np.allclose(s, 2 * b)
+    b = np.array([3.0, -1.0, 0.5])
+    s, scl = orc.scaleb(b)                      # range 3.0 - 0.5 = 2.5 -> 1.5
+    assert np.isclose(scl, 1.5 / 2.5)
+    s, scl = orc.scaleb(np.array([1.0, 2.0]))   # range 1.0: unchanged
+    assert scl == 1.0
+
+
+# ---------------------------------------------------------------- whole loop
+def test_solve_noise_free_well_conditioned(orc):
+    """m = n Gaussian A, no noise: TVAL3 recovers a piecewise-constant image."""
+    N = 16
+    u = wl.phantom_2d(N, 4, seed=7)
+    A = wl.dense_gaussian(N * N, N * N, seed=8)
+    b = A @ u
+    op = orc.dense_op(A)
+    o = orc.default_opts(mu=2.0 ** 12, mu0=2.0 ** 12, beta=2.0 ** 7, beta0=2.0 ** 7,
+                         tol=1e-6, tol_inn=1e-4, maxit=600)
+    U, info, hist, st = orc.solve(op, (N, N, 1), b, o, Ut=u)
+    assert st == 0
+    assert info.rel_error < 2e-3, info.rel_error
+    assert info.total_iters <= 600 and info.outer_iters <= o.maxcnt
+
+
+def test_solve_nonmonotone_invariants(orc):
+    N = 16
+    u = wl.phantom_2d(N, 5, seed=9)
+    A = wl.dense_gaussian(100, N * N, seed=10)
+    b = wl.add_noise(A @ u, 1e-3, seed=11)
+    o = orc.default_opts(maxit=150, tol_inn=1e-3, tol=1e-5)
+    U, info, hist, st = orc.solve(orc.dense_op(A), (N, N, 1), b, o, Ut=u)
+    assert st == 0 and info.total_iters == len(hist["f"]) <= 150
+    assert (U >= 0).all()                                 # nonneg projection
+    assert info.outer_iters <= o.maxcnt
+    # mu, beta follow the continuation scheme and never exceed their caps
+    assert (hist["mu"] <= o.mu).all() and (hist["beta"] <= o.beta).all()
+    assert np.all(np.diff(hist["outer"]) >= 0)
+
+
+def test_solve_maxit_cap_and_fallback_branch(orc):
+    """With gamma = 1 the backtracking never shrinks alpha, so an iteration
+    either passes the test |f| > |C - alpha c| at once or takes the cnt==5
+    fallback (3d/tval3_gpu_3d.cpp:848,875-883).  The test uses absolute
+    values, so a huge c makes it accept rather than fail."""
+    N = 8
+    u = wl.phantom_2d(N, 3, seed=12)
+    A = wl.dense_gaussian(40, N * N, seed=13)
+    b = wl.add_noise(A @ u, 0.05, seed=14)
+    o = orc.default_opts(maxit=40, gamma=1.0, c=0.0, gam=0.0, tol_inn=1e-12)
+    U, info, hist, st = orc.solve(orc.dense_op(A), (N, N, 1), b, o)
+    assert info.total_iters == 40 == len(hist["cnt"])
+    assert set(hist["cnt"].tolist()) <= {0, 6}
+    assert info.fallbacks == int((hist["cnt"] == 6).sum()) > 0
+    assert info.backtracks == 5 * info.fallbacks
+
+
+def test_solve_rejects_nonfinite(orc):
+    A = wl.dense_gaussian(10, 16, seed=1)
+    b = np.ones(10)
+    b[3] = np.nan
+    U, info, hist, st = orc.solve(orc.dense_op(A), (4, 4, 1), b)
+    assert st == 6
+
+
+def test_solve_fp32_tracks_fp64_config0(orc):
+    """Config 0 (2-D 64x64, dense A, m = 0.4 N, 0.1% noise, aniso TV,
+    mu=2^8, beta=2^5, tol 1e-4, tol_inn 1e-3, maxit 300): FP32 vs FP64."""
+    N = 64
+    u = wl.phantom_2d(N, 8)
+    A = wl.dense_gaussian(1638, N * N)
+    b = wl.add_noise(A @ u, 1e-3)
+    op = orc.dense_op(A)
+    o = orc.default_opts(tol=1e-4, tol_inn=1e-3, maxit=300)
+    U64, i64, h64, _ = orc.solve(op, (N, N, 1), b, o, Ut=u)
+    U32, i32, h32, _ = orc.solve(op, (N, N, 1), b, o, Ut=u, dtype=np.float32)
+    rel = np.linalg.norm(U32 - U64) / np.linalg.norm(U64)
+    assert i64.rel_error < 0.2
+    assert rel < 5e-3, rel
