@@ -1,0 +1,227 @@
+/*
+ * tval3_mi355x.h -- C ABI of the MI355X-native TVAL3 solver.
+ *
+ * Plain C, no exceptions, no HIP or torch types in the signatures: callable
+ * from C, from C++ (include/tval3_gpu_3d.h restores the reference's C++
+ * overload set on top of it) and from Python via ctypes.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference root).
+ *
+ * Conventions
+ *   - Volumes are column-major with dim_y fastest: U[y + x*dim_y + z*dim_y*dim_x]
+ *     (3d/container_device.h:21-23).  dims[3] = {dim_y, dim_x, dim_z}.
+ *   - Ray order of dynamic operators is the reference's compacted order:
+ *     measurement position, then emitter, then receiver, masked pairs skipped
+ *     (3d/mat_vec_mul_dynamic.h:501-545).
+ *   - "_dev" pointers are device pointers, "_host" pointers host memory.
+ *   - Every function returns a tv3_status; tv3_last_error() gives the text.
+ *   - A context is confined to one host thread (the reference keeps its
+ *     handles and streams as process globals, 3d/tval3_gpu_3d.cpp:71-79).
+ */
+#ifndef TVAL3_MI355X_H_
+#define TVAL3_MI355X_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TV3_ABI_VERSION 1
+
+typedef enum {
+    TV3_OK = 0,
+    TV3_E_ARG = 1,        /* invalid argument (NULL, negative size, bad enum)          */
+    TV3_E_SHAPE = 2,      /* sizes inconsistent (3d/tval3_gpu_3d.cpp:1369-1380)        */
+    TV3_E_FORMAT = 3,     /* unsupported storage (3d/tval3_gpu_3d.cpp:1404-1406,1463)  */
+    TV3_E_NOMEM = 4,
+    TV3_E_HIP = 5,        /* HIP runtime error (reference: HANDLE_ERROR / cuda_exception) */
+    TV3_E_NONFINITE = 6,  /* NaN/Inf in b, A or the iterates                            */
+    TV3_E_INTERNAL = 7,
+    TV3_E_UNSUPPORTED = 8 /* option the reference has but this build does not support    */
+} tv3_status;
+
+typedef enum { TV3_N = 0, TV3_T = 1 } tv3_trans;   /* CUBLAS_OP_N / CUBLAS_OP_T */
+
+typedef enum { TV3_F32 = 0, TV3_F16 = 1, TV3_BF16 = 2, TV3_F64 = 3 } tv3_dtype;
+
+typedef enum {
+    TV3_WEIGHT_BRESENHAM = 0, /* integer 3-D walk, constant weight (3d/tval3_gpu_3d_kernels.cu:1612-1764) */
+    TV3_WEIGHT_SIDDON = 1     /* exact intersection lengths (Siddon 1985)                                  */
+} tv3_weight_mode;
+
+typedef enum { TV3_TV_ANISO = 0, TV3_TV_ISO = 1 } tv3_tv_mode;
+
+typedef enum {
+    TV3_SOLVER_PLAIN = 0,  /* reference-shaped host loop: one kernel per vector op, host scalars */
+    TV3_SOLVER_FUSED = 1   /* fused stencil kernels, device FP64 scalars, hipGraph iterations     */
+} tv3_solver_kind;
+
+typedef struct tv3_ctx tv3_ctx; /* owns the stream, workspace, graphs and tuning cache */
+typedef struct tv3_op tv3_op;   /* owns device copies of a system matrix / geometry   */
+
+/* Options: names and defaults of tval3_options (2d/tval3_types.h:7-34) plus
+ * the 3-D skip parameters (3d/tval3_gpu_3d.cpp:151-157) and build-specific
+ * fields.  Always initialise with tv3_opts_default(). */
+typedef struct {
+    double mu, mu0, beta, beta0, rate_cnt, tol, tol_inn;
+    int32_t maxcnt, maxit;
+    double c, gamma, gam, rate_gam;
+    int32_t isreal, nonneg;
+    uint32_t numInitIterations;   /* accepted; only skipMulRatio == 0 is supported */
+    double skipMulRatio, maxRelativeChange;
+    int32_t tv_mode;              /* tv3_tv_mode                                    */
+    int32_t solver;               /* tv3_solver_kind                                */
+    int32_t warm_start;           /* 1: U on entry is the initial guess, else A^T b */
+    int32_t graph_iters;          /* fused: inner iterations per graph launch, 0 = auto */
+    int32_t verbose;
+} tv3_opts;
+
+/* tval3_info (2d/tval3_types.h:37-43; 3-D adds outer_secs, 3d/tval3_gpu_3d.cpp:1507) */
+typedef struct {
+    int32_t total_iters, outer_iters;
+    double rel_chg, rel_error;
+    double secs;          /* GPU time of the whole solve (events), as info.secs        */
+    double outer_secs;    /* host wall time of the entry point including uploads        */
+    double loop_secs;     /* GPU time of the outer loop only                            */
+    int32_t backtracks, fallbacks;
+    int64_t n_matvec_N, n_matvec_T;
+    int32_t status;
+} tv3_info;
+
+/* Optional per-inner-iteration history; arrays are caller owned, length cap. */
+typedef struct {
+    int32_t cap, count;
+    double *f, *lam1, *lam2, *lam3, *lam4, *lam5, *tau, *alpha, *relchg, *mu, *beta, *C;
+    int32_t *cnt;     /* backtracking steps; 6 = the cnt==5 fallback */
+    int32_t *outer;
+} tv3_history;
+
+/* Transducer geometry of a dynamic operator (geometry_host / geometry_device,
+ * 3d/container_device.h:103-172).  Volume: rv_x * rv_y * rv_z voxels, rv_y
+ * fastest, so dims = {rv_y, rv_x, rv_z}. */
+typedef struct {
+    int32_t num_emitters, num_receivers, num_mps;
+    int32_t rv_x, rv_y, rv_z;
+    float scale_factor;                      /* Bresenham weight per voxel           */
+    const int32_t *x_emitters, *y_emitters, *z_emitters;    /* num_mps*E, Bresenham */
+    const int32_t *x_receivers, *y_receivers, *z_receivers; /* num_mps*R, Bresenham */
+    const float *pos_emitters;               /* num_mps*E*3 world (x,y,z), Siddon     */
+    const float *pos_receivers;              /* num_mps*R*3 world (x,y,z), Siddon     */
+    float origin[3];                         /* world (x,y,z) of the volume corner    */
+    float voxel[3];                          /* voxel size (x,y,z)                    */
+    const uint8_t *use_path;                 /* E*R mask, NULL = all pairs            */
+} tv3_geometry;
+
+/* ---- version / defaults ------------------------------------------------- */
+int32_t tv3_abi_version(void);
+void tv3_opts_default(tv3_opts *opts);             /* 2d/tval3_types.h:29-34 */
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the process globals of 3d/tval3_gpu_3d.cpp:71-79 (streams, handles,
+ * majorRevision) and the cublasCreate/cudaStreamCreate calls at :1110-1113,1473. */
+tv3_status tv3_ctx_create(int32_t device, tv3_ctx **out);
+void tv3_ctx_destroy(tv3_ctx *ctx);
+const char *tv3_last_error(const tv3_ctx *ctx);    /* NULL ctx: last create error */
+void *tv3_ctx_stream(tv3_ctx *ctx);                /* the context's hipStream_t    */
+
+/* ---- operators ---------------------------------------------------------- */
+/* Sparse operator: replaces sparse_mm_multGPU(A, 1, {dev}, dev)
+ * (3d/mat_vec_mul_sparse.h:79-694, single device) and sparse_mat_device
+ * (3d/container_device.h:48-98).  Host arrays; values in val_type (F32, F16,
+ * BF16 or F64).  csc_* may be NULL: the CSC (used for A^T) is then built on
+ * the device by a stable transpose.  The reference requires sparse_mat_both
+ * (3d/tval3_gpu_3d.cpp:1463-1465). */
+tv3_status tv3_op_csr_create(tv3_ctx *ctx, int64_t m, int64_t n, int64_t nnz,
+                             const int32_t *rowptr, const int32_t *col, const void *val,
+                             tv3_dtype val_type, const int32_t *csc_colptr,
+                             const int32_t *csc_row, const void *csc_val, tv3_op **out);
+
+/* Dense operator: replaces dense_mm (3d/mat_vec_mul_dense.h:13-20).
+ * Column-major m x n with leading dimension lda >= m, host memory. */
+tv3_status tv3_op_dense_create(tv3_ctx *ctx, int64_t m, int64_t n, const void *A, int64_t lda,
+                               tv3_dtype val_type, tv3_op **out);
+
+/* Matrix-free operator: replaces geom_mm_multGPU(A, 1, {dev}, dev, 16)
+ * (3d/mat_vec_mul_dynamic.h:330-637, entry at 3d/tval3_gpu_3d.cpp:1570).
+ * Ray weights are generated inside every product. */
+tv3_status tv3_op_dynamic_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weight_mode mode,
+                                 tv3_op **out);
+
+/* Hybrid operator: replaces sparse_geom_mm (3d/mat_vec_mul.h:74-756): A x by
+ * the dynamic traversal, A^T x by the CSC of a precomputed matrix of the same
+ * geometry (built on the device from the traversal). */
+tv3_status tv3_op_hybrid_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weight_mode mode,
+                                tv3_dtype val_type, tv3_op **out);
+
+/* Precompute the sparse matrix of a dynamic operator on the device (CSR by a
+ * count pass, a scan and a fill pass; CSC by a stable transpose).  Values are
+ * stored as val_type (F32, F16, BF16).  No reference counterpart (the
+ * reference expects host-built matrices). */
+tv3_status tv3_op_materialize(tv3_op *dynamic_op, tv3_dtype val_type, tv3_op **out_csr);
+
+void tv3_op_destroy(tv3_op *op);
+tv3_status tv3_op_info(const tv3_op *op, int64_t *m, int64_t *n, int64_t *nnz);
+
+/* Copy a sparse operator's CSR to host memory (values converted to double).
+ * Pass NULL arrays to query nnz only. */
+tv3_status tv3_op_csr_download(tv3_op *op, int32_t *rowptr, int32_t *col, double *val);
+
+/* ---- products ----------------------------------------------------------- */
+/* mat_vec_mul(trans, A, x, y, stream, mulChain) overload set
+ * (3d/mat_vec_mul_sparse.h:1703-1780, 3d/mat_vec_mul_dense.h:25-73,
+ * 3d/mat_vec_mul_dynamic.h:682-967, 3d/mat_vec_mul.h:732-756).
+ * x, y device pointers; y is fully overwritten; asynchronous on stream
+ * (NULL = the context stream).  mulChain is a multi-GPU hint and is ignored. */
+tv3_status tv3_matvec(tv3_op *op, tv3_trans trans, const float *x_dev, float *y_dev,
+                      void *stream);
+tv3_status tv3_matvec_f64(tv3_op *op, tv3_trans trans, const double *x_dev, double *y_dev,
+                          void *stream);
+
+/* ---- solver ------------------------------------------------------------- */
+/* tval3_gpu_3d(U, A, b, opts, Ut, pagelocked, mainGPU, numGPUs, gpuArray)
+ * (3d/tval3_gpu_3d.cpp:1673-1745, core loop :1088-1345) for float data.
+ * b_host: m floats; U_host: n floats (output; input too if opts->warm_start);
+ * Ut_host: n floats or NULL (info->rel_error); hist may be NULL. */
+tv3_status tv3_solve(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const int32_t dims[3],
+                     const float *b_host, float *U_host, const float *Ut_host, tv3_info *info,
+                     tv3_history *hist);
+
+/* Same with device-resident b, U, Ut (no host transfers inside). */
+tv3_status tv3_solve_device(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts,
+                            const int32_t dims[3], const float *b_dev, float *U_dev,
+                            const float *Ut_dev, tv3_info *info, tv3_history *hist);
+
+/* double-precision overloads (tval3_info<double> tval3_gpu_3d(...), :1710-1745):
+ * the plain solver in FP64 on the device. */
+tv3_status tv3_solve_f64(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const int32_t dims[3],
+                         const double *b_host, double *U_host, const double *Ut_host,
+                         tv3_info *info, tv3_history *hist);
+
+/* ---- device building blocks (exposed for tests and autograd bindings) ---- */
+/* D (3d/tval3_gpu_3d.cpp:271-314): periodic forward differences.
+ * Ux along dim_x (stride dim_y), Uy along dim_y (stride 1), Uz along dim_z. */
+tv3_status tv3_D(tv3_ctx *ctx, const int32_t dims[3], const float *U_dev, float *Ux_dev,
+                 float *Uy_dev, float *Uz_dev, void *stream);
+/* D^T (3d/tval3_gpu_3d.cpp:319-361). */
+tv3_status tv3_Dt(tv3_ctx *ctx, const int32_t dims[3], const float *X_dev, const float *Y_dev,
+                  const float *Z_dev, float *res_dev, void *stream);
+/* shrinkage_gpu (3d/tval3_gpu_3d.cpp:924-960): W = shrink(DU - sigma/beta),
+ * returns lam1 = sum |W| (aniso) or sum |W|_2 (iso). Synchronous. */
+tv3_status tv3_shrink(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, const float *Ux,
+                      const float *Uy, const float *Uz, const float *sx, const float *sy,
+                      const float *sz, double beta, float *Wx, float *Wy, float *Wz,
+                      double *lam1);
+
+/* ---- tuning ------------------------------------------------------------- */
+/* Time the sparse kernels of op over their launch shapes (lanes per row) and
+ * keep the fastest; returns the chosen lanes for N and T. */
+tv3_status tv3_op_autotune(tv3_op *op, int32_t *lanes_N, int32_t *lanes_T);
+tv3_status tv3_op_set_lanes(tv3_op *op, int32_t lanes_N, int32_t lanes_T);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,640 @@
+// tv3_abi.hip -- the C ABI (include/tval3_mi355x.h): argument validation,
+// object lifetimes, dispatch to the kernels, exception -> status mapping.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "vec.hpp"
+
+using namespace tv3;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+template <typename F>
+tv3_status guard(tv3_ctx *ctx, F &&f) {
+    try {
+        f();
+        return TV3_OK;
+    } catch (const tv3_error &e) {
+        (ctx ? ctx->err : g_create_err) = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        (ctx ? ctx->err : g_create_err) = "host allocation failed";
+        return TV3_E_NOMEM;
+    } catch (const std::exception &e) {
+        (ctx ? ctx->err : g_create_err) = std::string("internal error: ") + e.what();
+        return TV3_E_INTERNAL;
+    }
+}
+
+void set_device(tv3_ctx *ctx) { TV3_HIP(hipSetDevice(ctx->device)); }
+
+template <typename T>
+void upload(DevBuf<T> &d, const T *h, size_t count, hipStream_t s) {
+    d.alloc(count > 0 ? count : 1);
+    if (count) TV3_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+}
+
+void upload_bytes(DevBuf<uint8_t> &d, const void *h, size_t bytes, hipStream_t s) {
+    d.alloc(bytes > 0 ? bytes : 1);
+    if (bytes) TV3_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+}
+
+void check_csr_host(int64_t rows, int64_t cols, int64_t nnz, const int32_t *ptr,
+                    const int32_t *ind, const char *what) {
+    TV3_REQUIRE(ptr && (nnz == 0 || ind), TV3_E_ARG, std::string(what) + ": NULL array");
+    TV3_REQUIRE(ptr[0] == 0 && ptr[rows] == nnz, TV3_E_FORMAT,
+                std::string(what) + ": pointer array must start at 0 and end at nnz");
+    for (int64_t i = 0; i < rows; ++i)
+        TV3_REQUIRE(ptr[i + 1] >= ptr[i], TV3_E_FORMAT,
+                    std::string(what) + ": pointer array not monotone");
+    for (int64_t k = 0; k < nnz; ++k)
+        TV3_REQUIRE(ind[k] >= 0 && ind[k] < cols, TV3_E_FORMAT,
+                    std::string(what) + ": index out of range");
+}
+
+bool finite_host(const void *v, tv3_dtype t, int64_t count) {
+    if (t == TV3_F32) {
+        const float *p = static_cast<const float *>(v);
+        for (int64_t k = 0; k < count; ++k) if (!std::isfinite(p[k])) return false;
+    } else if (t == TV3_F64) {
+        const double *p = static_cast<const double *>(v);
+        for (int64_t k = 0; k < count; ++k) if (!std::isfinite(p[k])) return false;
+    } else {
+        // F16 / BF16: exponent all ones means Inf or NaN
+        const uint16_t *p = static_cast<const uint16_t *>(v);
+        const uint16_t mask = t == TV3_F16 ? 0x7C00 : 0x7F80;
+        for (int64_t k = 0; k < count; ++k) if ((p[k] & mask) == mask) return false;
+    }
+    return true;
+}
+
+double to_double(const uint8_t *p, tv3_dtype t, int64_t k) {
+    switch (t) {
+    case TV3_F32: { float v; std::memcpy(&v, p + 4 * k, 4); return v; }
+    case TV3_F64: { double v; std::memcpy(&v, p + 8 * k, 8); return v; }
+    case TV3_F16: {
+        uint16_t h; std::memcpy(&h, p + 2 * k, 2);
+        int sgn = h >> 15, e = (h >> 10) & 31, mant = h & 1023;
+        double v = e == 0 ? std::ldexp((double)mant, -24)
+                          : (e == 31 ? INFINITY : std::ldexp((double)(mant | 1024), e - 25));
+        return sgn ? -v : v;
+    }
+    case TV3_BF16: {
+        uint16_t h; std::memcpy(&h, p + 2 * k, 2);
+        uint32_t u = (uint32_t)h << 16; float v; std::memcpy(&v, &u, 4); return v;
+    }
+    }
+    return 0.0;
+}
+
+void validate_opts(const tv3_opts *o) {
+    TV3_REQUIRE(o, TV3_E_ARG, "opts is NULL");
+    TV3_REQUIRE(o->skipMulRatio == 0.0, TV3_E_UNSUPPORTED,
+                "skipMulRatio != 0 (A^T skipping) is not supported");
+    TV3_REQUIRE(o->maxit >= 0 && o->maxcnt >= 0, TV3_E_ARG, "maxit/maxcnt must be >= 0");
+    TV3_REQUIRE(o->mu > 0 && o->mu0 > 0 && o->beta > 0 && o->beta0 > 0, TV3_E_ARG,
+                "mu, mu0, beta, beta0 must be positive");
+    TV3_REQUIRE(o->tv_mode == TV3_TV_ANISO || o->tv_mode == TV3_TV_ISO, TV3_E_ARG, "bad tv_mode");
+    TV3_REQUIRE(o->solver == TV3_SOLVER_PLAIN || o->solver == TV3_SOLVER_FUSED, TV3_E_ARG,
+                "bad solver");
+}
+
+int64_t dims_n(const int32_t dims[3]) {
+    TV3_REQUIRE(dims && dims[0] > 0 && dims[1] > 0 && dims[2] > 0, TV3_E_ARG,
+                "dims must be three positive sizes");
+    return (int64_t)dims[0] * dims[1] * dims[2];
+}
+
+}  // namespace
+
+namespace tv3 {
+
+void op_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s) {
+    switch (op->kind) {
+    case OpKind::CSR: spmv_csr(trans ? op->AT : op->A, x, y, s); break;
+    case OpKind::DENSE: dense_gemv(op, trans, x, y, s); break;
+    case OpKind::DYNAMIC: dynamic_apply(op, trans, x, y, s); break;
+    case OpKind::HYBRID:
+        if (trans) spmv_csr(op->AT, x, y, s);
+        else dynamic_apply(op, false, x, y, s);
+        break;
+    }
+}
+
+void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s) {
+    switch (op->kind) {
+    case OpKind::CSR: spmv_csr(trans ? op->AT : op->A, x, y, s); break;
+    case OpKind::DENSE: dense_gemv(op, trans, x, y, s); break;
+    case OpKind::DYNAMIC: dynamic_apply(op, trans, x, y, s); break;
+    case OpKind::HYBRID:
+        TV3_REQUIRE(op->AT.vtype == TV3_F32 || op->AT.vtype == TV3_F64, TV3_E_FORMAT,
+                    "FP64 products of a hybrid operator need F32 weights");
+        if (trans) spmv_csr(op->AT, x, y, s);
+        else dynamic_apply(op, false, x, y, s);
+        break;
+    }
+}
+
+}  // namespace tv3
+
+template <typename T>
+static tv3_status solve_host(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts,
+                             const int32_t dims[3], const T *b, T *U, const T *Ut,
+                             tv3_info *info, tv3_history *hist) {
+    if (!ctx) return TV3_E_ARG;
+    auto t0 = std::chrono::steady_clock::now();
+    tv3_status st = guard(ctx, [&] {
+        validate_opts(opts);
+        TV3_REQUIRE(op && b && U && info, TV3_E_ARG, "NULL argument");
+        int64_t n = dims_n(dims);
+        TV3_REQUIRE(n == op->n, TV3_E_SHAPE, "length of U must equal the columns of A");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        DevBuf<T> bd(op->m), Ud(n), Utd;
+        TV3_HIP(hipMemcpyAsync(bd.p, b, sizeof(T) * op->m, hipMemcpyHostToDevice, s));
+        if (opts->warm_start)
+            TV3_HIP(hipMemcpyAsync(Ud.p, U, sizeof(T) * n, hipMemcpyHostToDevice, s));
+        if (Ut) {
+            Utd.alloc(n);
+            TV3_HIP(hipMemcpyAsync(Utd.p, Ut, sizeof(T) * n, hipMemcpyHostToDevice, s));
+        }
+        std::memset(info, 0, sizeof(*info));
+        if (hist) hist->count = 0;
+        if constexpr (std::is_same<T, float>::value) {
+            if (opts->solver == TV3_SOLVER_FUSED)
+                solve_fused(ctx, op, *opts, dims, bd.p, Ud.p, Ut ? Utd.p : nullptr, info, hist);
+            else
+                solve_plain<float>(ctx, op, *opts, dims, bd.p, Ud.p, Ut ? Utd.p : nullptr, info,
+                                   hist);
+        } else {
+            solve_plain<double>(ctx, op, *opts, dims, bd.p, Ud.p, Ut ? Utd.p : nullptr, info,
+                                hist);
+        }
+        TV3_HIP(hipMemcpyAsync(U, Ud.p, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+        TV3_HIP(hipStreamSynchronize(s));
+    });
+    if (info) {
+        info->outer_secs =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (st == TV3_OK && info->status != TV3_OK) {
+            ctx->err = "iterates became non-finite";
+            return (tv3_status)info->status;
+        }
+    }
+    return st;
+}
+
+extern "C" {
+
+int32_t tv3_abi_version(void) { return TV3_ABI_VERSION; }
+
+void tv3_opts_default(tv3_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    // 2d/tval3_types.h:29-34
+    o->mu = 256.0; o->mu0 = 256.0; o->beta = 32.0; o->beta0 = 32.0;
+    o->rate_cnt = 2.0; o->tol = 1e-6; o->tol_inn = 1e-3;
+    o->maxcnt = 10; o->maxit = 1025;
+    o->c = 1e-5; o->gamma = 0.6; o->gam = 0.9995; o->rate_gam = 0.9;
+    o->isreal = 1; o->nonneg = 1;
+    o->numInitIterations = 0; o->skipMulRatio = 0.0; o->maxRelativeChange = 0.0;
+    o->tv_mode = TV3_TV_ANISO;
+    o->solver = TV3_SOLVER_PLAIN;
+    o->warm_start = 0;
+    o->graph_iters = 0;
+    o->verbose = 0;
+}
+
+tv3_status tv3_ctx_create(int32_t device, tv3_ctx **out) {
+    return guard(nullptr, [&] {
+        TV3_REQUIRE(out, TV3_E_ARG, "out is NULL");
+        *out = nullptr;
+        int count = 0;
+        TV3_HIP(hipGetDeviceCount(&count));
+        TV3_REQUIRE(count > 0, TV3_E_HIP, "no HIP device");
+        TV3_REQUIRE(device >= 0 && device < count, TV3_E_ARG, "device index out of range");
+        TV3_HIP(hipSetDevice(device));
+        std::unique_ptr<tv3_ctx> c(new tv3_ctx());
+        c->device = device;
+        TV3_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        TV3_HIP(hipGetDeviceProperties(&prop, device));
+        c->num_cus = prop.multiProcessorCount;
+        *out = c.release();
+    });
+}
+
+void tv3_ctx_destroy(tv3_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->red_partials.release();
+    ctx->scratch.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *tv3_last_error(const tv3_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+void *tv3_ctx_stream(tv3_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+tv3_status tv3_op_csr_create(tv3_ctx *ctx, int64_t m, int64_t n, int64_t nnz,
+                             const int32_t *rowptr, const int32_t *col, const void *val,
+                             tv3_dtype vt, const int32_t *cptr, const int32_t *crow,
+                             const void *cval, tv3_op **out) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(out, TV3_E_ARG, "out is NULL");
+        *out = nullptr;
+        TV3_REQUIRE(m > 0 && n > 0 && nnz >= 0 && nnz <= INT32_MAX, TV3_E_SHAPE,
+                    "bad sparse sizes");
+        TV3_REQUIRE(dtype_size(vt) > 0, TV3_E_ARG, "bad value type");
+        TV3_REQUIRE(nnz == 0 || val, TV3_E_ARG, "val is NULL");
+        check_csr_host(m, n, nnz, rowptr, col, "CSR");
+        TV3_REQUIRE(finite_host(val, vt, nnz), TV3_E_NONFINITE, "A holds NaN or Inf");
+        const bool has_csc = cptr != nullptr;
+        if (has_csc) {
+            TV3_REQUIRE(nnz == 0 || (crow && cval), TV3_E_ARG, "partial CSC arrays");
+            check_csr_host(n, m, nnz, cptr, crow, "CSC");
+        }
+        set_device(ctx);
+        std::unique_ptr<tv3_op> op(new tv3_op());
+        op->ctx = ctx;
+        op->kind = OpKind::CSR;
+        op->m = m; op->n = n;
+        hipStream_t s = ctx->stream;
+        SparseRows &A = op->A;
+        A.rows = m; A.cols = n; A.nnz = nnz; A.vtype = vt;
+        upload(A.ptr, rowptr, m + 1, s);
+        upload(A.ind, col, nnz, s);
+        upload_bytes(A.val, val, nnz * dtype_size(vt), s);
+        A.lanes = default_lanes(m, nnz);
+        if (has_csc) {
+            SparseRows &T = op->AT;
+            T.rows = n; T.cols = m; T.nnz = nnz; T.vtype = vt;
+            upload(T.ptr, cptr, n + 1, s);
+            upload(T.ind, crow, nnz, s);
+            upload_bytes(T.val, cval, nnz * dtype_size(vt), s);
+            T.lanes = default_lanes(n, nnz);
+        } else {
+            build_transpose(ctx, A, op->AT);
+        }
+        TV3_HIP(hipStreamSynchronize(s));
+        *out = op.release();
+    });
+}
+
+tv3_status tv3_op_dense_create(tv3_ctx *ctx, int64_t m, int64_t n, const void *A, int64_t lda,
+                               tv3_dtype vt, tv3_op **out) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(out, TV3_E_ARG, "out is NULL");
+        *out = nullptr;
+        TV3_REQUIRE(m > 0 && n > 0 && lda >= m, TV3_E_SHAPE, "bad dense sizes (lda >= m)");
+        TV3_REQUIRE(A, TV3_E_ARG, "A is NULL");
+        TV3_REQUIRE(dtype_size(vt) > 0, TV3_E_ARG, "bad value type");
+        TV3_REQUIRE(finite_host(A, vt, lda * n), TV3_E_NONFINITE, "A holds NaN or Inf");
+        set_device(ctx);
+        std::unique_ptr<tv3_op> op(new tv3_op());
+        op->ctx = ctx;
+        op->kind = OpKind::DENSE;
+        op->m = m; op->n = n; op->lda = lda; op->dtype = vt;
+        upload_bytes(op->dense, A, (size_t)lda * n * dtype_size(vt), ctx->stream);
+        TV3_HIP(hipStreamSynchronize(ctx->stream));
+        *out = op.release();
+    });
+}
+
+static void fill_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_weight_mode mode,
+                          tv3_op *op) {
+    TV3_REQUIRE(g, TV3_E_ARG, "geometry is NULL");
+    TV3_REQUIRE(g->num_emitters > 0 && g->num_receivers > 0 && g->num_mps > 0, TV3_E_SHAPE,
+                "geometry needs emitters, receivers and measurement positions");
+    TV3_REQUIRE(g->rv_x > 0 && g->rv_y > 0 && g->rv_z > 0, TV3_E_SHAPE, "bad volume size");
+    TV3_REQUIRE(mode == TV3_WEIGHT_BRESENHAM || mode == TV3_WEIGHT_SIDDON, TV3_E_ARG,
+                "bad weight mode");
+    const int E = g->num_emitters, R = g->num_receivers, mps = g->num_mps;
+    const int64_t ne = (int64_t)E * mps, nr = (int64_t)R * mps;
+    DynGeom &G = op->geo;
+    G.E = E; G.R = R; G.mps = mps;
+    G.rv_x = g->rv_x; G.rv_y = g->rv_y; G.rv_z = g->rv_z;
+    G.scale = g->scale_factor;
+    G.mode = mode;
+    hipStream_t s = ctx->stream;
+    if (mode == TV3_WEIGHT_BRESENHAM) {
+        TV3_REQUIRE(g->x_emitters && g->y_emitters && g->z_emitters && g->x_receivers &&
+                        g->y_receivers && g->z_receivers,
+                    TV3_E_ARG, "Bresenham mode needs integer transducer coordinates");
+        std::vector<int32_t> ec(3 * ne), rc(3 * nr);
+        for (int64_t k = 0; k < ne; ++k) {
+            ec[3 * k] = g->x_emitters[k]; ec[3 * k + 1] = g->y_emitters[k];
+            ec[3 * k + 2] = g->z_emitters[k];
+        }
+        for (int64_t k = 0; k < nr; ++k) {
+            rc[3 * k] = g->x_receivers[k]; rc[3 * k + 1] = g->y_receivers[k];
+            rc[3 * k + 2] = g->z_receivers[k];
+        }
+        upload(G.ecoord, ec.data(), ec.size(), s);
+        upload(G.rcoord, rc.data(), rc.size(), s);
+        TV3_HIP(hipStreamSynchronize(s));
+    } else {
+        TV3_REQUIRE(g->pos_emitters && g->pos_receivers, TV3_E_ARG,
+                    "Siddon mode needs transducer positions");
+        TV3_REQUIRE(g->voxel[0] > 0 && g->voxel[1] > 0 && g->voxel[2] > 0, TV3_E_ARG,
+                    "voxel sizes must be positive");
+        for (int64_t k = 0; k < 3 * ne; ++k)
+            TV3_REQUIRE(std::isfinite(g->pos_emitters[k]), TV3_E_NONFINITE, "emitter position");
+        for (int64_t k = 0; k < 3 * nr; ++k)
+            TV3_REQUIRE(std::isfinite(g->pos_receivers[k]), TV3_E_NONFINITE, "receiver position");
+        upload(G.epos, g->pos_emitters, 3 * ne, s);
+        upload(G.rpos, g->pos_receivers, 3 * nr, s);
+        for (int a = 0; a < 3; ++a) { G.origin[a] = g->origin[a]; G.voxel[a] = g->voxel[a]; }
+        TV3_HIP(hipStreamSynchronize(s));
+    }
+    // compacted ray list, 3d/mat_vec_mul_dynamic.h:501-545
+    std::vector<int32_t> path;
+    const int64_t per = (int64_t)E * R, tot = per * mps;
+    path.reserve(2 * tot);
+    for (int64_t i = 0; i < tot; ++i) {
+        if (g->use_path && !g->use_path[i % per]) continue;
+        path.push_back((int32_t)(i / R));
+        path.push_back((int32_t)((i / per) * R + i % R));
+    }
+    G.num_paths = (int64_t)path.size() / 2;
+    if (mode == TV3_WEIGHT_BRESENHAM) {
+        int32_t mw = 1;
+        for (int64_t k = 0; k < G.num_paths; ++k) {
+            int64_t e = path[2 * k], r = path[2 * k + 1];
+            int32_t dx = std::abs(g->x_receivers[r] - g->x_emitters[e]);
+            int32_t dy = std::abs(g->y_receivers[r] - g->y_emitters[e]);
+            int32_t dz = std::abs(g->z_receivers[r] - g->z_emitters[e]);
+            mw = std::max(mw, std::max(dx, std::max(dy, dz)) + 1);
+        }
+        G.max_walk = mw;
+    }
+    TV3_REQUIRE(G.num_paths > 0, TV3_E_SHAPE, "use_path masks every ray");
+    upload(G.path, path.data(), path.size(), s);
+    TV3_HIP(hipStreamSynchronize(s));
+    op->m = G.num_paths;
+    op->n = (int64_t)g->rv_x * g->rv_y * g->rv_z;
+}
+
+tv3_status tv3_op_dynamic_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weight_mode mode,
+                                 tv3_op **out) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(out, TV3_E_ARG, "out is NULL");
+        *out = nullptr;
+        set_device(ctx);
+        std::unique_ptr<tv3_op> op(new tv3_op());
+        op->ctx = ctx;
+        op->kind = OpKind::DYNAMIC;
+        fill_geometry(ctx, geom, mode, op.get());
+        *out = op.release();
+    });
+}
+
+tv3_status tv3_op_hybrid_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weight_mode mode,
+                                tv3_dtype vt, tv3_op **out) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(out, TV3_E_ARG, "out is NULL");
+        *out = nullptr;
+        set_device(ctx);
+        std::unique_ptr<tv3_op> op(new tv3_op());
+        op->ctx = ctx;
+        op->kind = OpKind::HYBRID;
+        fill_geometry(ctx, geom, mode, op.get());
+        SparseRows A;
+        dynamic_materialize(op.get(), vt, A);
+        build_transpose(ctx, A, op->AT);
+        *out = op.release();
+    });
+}
+
+tv3_status tv3_op_materialize(tv3_op *dyn, tv3_dtype vt, tv3_op **out) {
+    if (!dyn) return TV3_E_ARG;
+    tv3_ctx *ctx = dyn->ctx;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(out, TV3_E_ARG, "out is NULL");
+        *out = nullptr;
+        TV3_REQUIRE(dyn->kind == OpKind::DYNAMIC || dyn->kind == OpKind::HYBRID, TV3_E_ARG,
+                    "materialize needs a dynamic operator");
+        set_device(ctx);
+        std::unique_ptr<tv3_op> op(new tv3_op());
+        op->ctx = ctx;
+        op->kind = OpKind::CSR;
+        op->m = dyn->m; op->n = dyn->n;
+        dynamic_materialize(dyn, vt, op->A);
+        build_transpose(ctx, op->A, op->AT);
+        *out = op.release();
+    });
+}
+
+void tv3_op_destroy(tv3_op *op) {
+    if (!op) return;
+    (void)hipSetDevice(op->ctx->device);
+    (void)hipStreamSynchronize(op->ctx->stream);
+    delete op;
+}
+
+tv3_status tv3_op_info(const tv3_op *op, int64_t *m, int64_t *n, int64_t *nnz) {
+    if (!op) return TV3_E_ARG;
+    if (m) *m = op->m;
+    if (n) *n = op->n;
+    if (nnz) {
+        switch (op->kind) {
+        case OpKind::CSR: *nnz = op->A.nnz; break;
+        case OpKind::HYBRID: *nnz = op->AT.nnz; break;
+        case OpKind::DENSE: *nnz = op->m * op->n; break;
+        case OpKind::DYNAMIC: *nnz = -1; break;
+        }
+    }
+    return TV3_OK;
+}
+
+tv3_status tv3_op_csr_download(tv3_op *op, int32_t *rowptr, int32_t *col, double *val) {
+    if (!op) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        TV3_REQUIRE(op->kind == OpKind::CSR, TV3_E_ARG, "not a CSR operator");
+        set_device(op->ctx);
+        const SparseRows &A = op->A;
+        hipStream_t s = op->ctx->stream;
+        if (rowptr)
+            TV3_HIP(hipMemcpyAsync(rowptr, A.ptr.p, sizeof(int32_t) * (A.rows + 1),
+                                   hipMemcpyDeviceToHost, s));
+        if (col && A.nnz)
+            TV3_HIP(hipMemcpyAsync(col, A.ind.p, sizeof(int32_t) * A.nnz, hipMemcpyDeviceToHost, s));
+        std::vector<uint8_t> raw;
+        if (val && A.nnz) {
+            raw.resize(A.nnz * dtype_size(A.vtype));
+            TV3_HIP(hipMemcpyAsync(raw.data(), A.val.p, raw.size(), hipMemcpyDeviceToHost, s));
+        }
+        TV3_HIP(hipStreamSynchronize(s));
+        if (val)
+            for (int64_t k = 0; k < A.nnz; ++k) val[k] = to_double(raw.data(), A.vtype, k);
+    });
+}
+
+tv3_status tv3_matvec(tv3_op *op, tv3_trans trans, const float *x, float *y, void *stream) {
+    if (!op) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        TV3_REQUIRE(x && y, TV3_E_ARG, "x or y is NULL");
+        TV3_REQUIRE(trans == TV3_N || trans == TV3_T, TV3_E_ARG, "bad trans");
+        set_device(op->ctx);
+        hipStream_t s = stream ? (hipStream_t)stream : op->ctx->stream;
+        op_apply(op, trans == TV3_T, x, y, s);
+        TV3_HIP(hipGetLastError());
+    });
+}
+
+tv3_status tv3_matvec_f64(tv3_op *op, tv3_trans trans, const double *x, double *y, void *stream) {
+    if (!op) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        TV3_REQUIRE(x && y, TV3_E_ARG, "x or y is NULL");
+        TV3_REQUIRE(trans == TV3_N || trans == TV3_T, TV3_E_ARG, "bad trans");
+        set_device(op->ctx);
+        hipStream_t s = stream ? (hipStream_t)stream : op->ctx->stream;
+        op_apply(op, trans == TV3_T, x, y, s);
+        TV3_HIP(hipGetLastError());
+    });
+}
+
+static void solve_device_impl(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts,
+                              const int32_t dims[3], const float *b, float *U, const float *Ut,
+                              tv3_info *info, tv3_history *hist) {
+    validate_opts(opts);
+    TV3_REQUIRE(op && b && U && info, TV3_E_ARG, "NULL argument");
+    TV3_REQUIRE(dims_n(dims) == op->n, TV3_E_SHAPE, "length of U must equal the columns of A");
+    std::memset(info, 0, sizeof(*info));
+    if (hist) hist->count = 0;
+    set_device(ctx);
+    if (opts->solver == TV3_SOLVER_FUSED)
+        solve_fused(ctx, op, *opts, dims, b, U, Ut, info, hist);
+    else
+        solve_plain<float>(ctx, op, *opts, dims, b, U, Ut, info, hist);
+}
+
+tv3_status tv3_solve_device(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const int32_t dims[3],
+                            const float *b, float *U, const float *Ut, tv3_info *info,
+                            tv3_history *hist) {
+    if (!ctx) return TV3_E_ARG;
+    tv3_status st = guard(ctx, [&] { solve_device_impl(ctx, op, opts, dims, b, U, Ut, info, hist); });
+    if (st == TV3_OK && info && info->status != TV3_OK) {
+        ctx->err = "iterates became non-finite";
+        return (tv3_status)info->status;
+    }
+    return st;
+}
+
+tv3_status tv3_solve(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const int32_t dims[3],
+                     const float *b, float *U, const float *Ut, tv3_info *info,
+                     tv3_history *hist) {
+    return solve_host<float>(ctx, op, opts, dims, b, U, Ut, info, hist);
+}
+
+tv3_status tv3_solve_f64(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const int32_t dims[3],
+                         const double *b, double *U, const double *Ut, tv3_info *info,
+                         tv3_history *hist) {
+    return solve_host<double>(ctx, op, opts, dims, b, U, Ut, info, hist);
+}
+
+tv3_status tv3_D(tv3_ctx *ctx, const int32_t dims[3], const float *U, float *Ux, float *Uy,
+                 float *Uz, void *stream) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        dims_n(dims);
+        TV3_REQUIRE(U && Ux && Uy && Uz, TV3_E_ARG, "NULL argument");
+        set_device(ctx);
+        launch_D<float>(dims, U, Ux, Uy, Uz, stream ? (hipStream_t)stream : ctx->stream);
+        TV3_HIP(hipGetLastError());
+    });
+}
+
+tv3_status tv3_Dt(tv3_ctx *ctx, const int32_t dims[3], const float *X, const float *Y,
+                  const float *Z, float *res, void *stream) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        dims_n(dims);
+        TV3_REQUIRE(X && Y && Z && res, TV3_E_ARG, "NULL argument");
+        set_device(ctx);
+        launch_Dt<float>(dims, X, Y, Z, res, stream ? (hipStream_t)stream : ctx->stream);
+        TV3_HIP(hipGetLastError());
+    });
+}
+
+tv3_status tv3_shrink(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, const float *Ux, const float *Uy,
+                      const float *Uz, const float *sx, const float *sy, const float *sz,
+                      double beta, float *Wx, float *Wy, float *Wz, double *lam1) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(n > 0 && Ux && Uy && Uz && sx && sy && sz && Wx && Wy && Wz, TV3_E_ARG,
+                    "NULL argument");
+        TV3_REQUIRE(beta > 0, TV3_E_ARG, "beta must be positive");
+        set_device(ctx);
+        double l = shrink_host<float>(ctx, n, mode == TV3_TV_ISO, Ux, Uy, Uz, sx, sy, sz, beta, Wx,
+                                      Wy, Wz, ctx->stream);
+        if (lam1) *lam1 = l;
+    });
+}
+
+static float time_spmv(tv3_ctx *ctx, const SparseRows &S, int L, const float *x, float *y) {
+    hipStream_t s = ctx->stream;
+    hipEvent_t a, b;
+    TV3_HIP(hipEventCreate(&a)); TV3_HIP(hipEventCreate(&b));
+    spmv_csr(S, x, y, s, L);               // warm up
+    TV3_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < 5; ++r) spmv_csr(S, x, y, s, L);
+    TV3_HIP(hipEventRecord(b, s));
+    TV3_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    TV3_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+    return ms / 5;
+}
+
+tv3_status tv3_op_autotune(tv3_op *op, int32_t *lanes_N, int32_t *lanes_T) {
+    if (!op) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        TV3_REQUIRE(op->kind == OpKind::CSR || op->kind == OpKind::HYBRID, TV3_E_ARG,
+                    "autotune needs a sparse operator");
+        set_device(op->ctx);
+        DevBuf<float> x(op->n), y(op->m > op->n ? op->m : op->n);
+        TV3_HIP(hipMemsetAsync(x.p, 0, x.bytes(), op->ctx->stream));
+        DevBuf<float> xm(op->m);
+        TV3_HIP(hipMemsetAsync(xm.p, 0, xm.bytes(), op->ctx->stream));
+        const int cand[] = {2, 4, 8, 16, 32, 64};
+        auto best = [&](SparseRows &S, const float *xx) {
+            int bl = S.lanes;
+            float bt = 1e30f;
+            for (int L : cand) {
+                float t = time_spmv(op->ctx, S, L, xx, y.p);
+                if (t < bt) { bt = t; bl = L; }
+            }
+            S.lanes = bl;
+            return bl;
+        };
+        int ln = op->kind == OpKind::CSR ? best(op->A, x.p) : 0;
+        int lt = best(op->AT, xm.p);
+        if (lanes_N) *lanes_N = ln;
+        if (lanes_T) *lanes_T = lt;
+    });
+}
+
+tv3_status tv3_op_set_lanes(tv3_op *op, int32_t lanes_N, int32_t lanes_T) {
+    if (!op) return TV3_E_ARG;
+    auto ok = [](int L) { return L == 0 || (L >= 2 && L <= 64 && (L & (L - 1)) == 0); };
+    if (!ok(lanes_N) || !ok(lanes_T)) return TV3_E_ARG;
+    if (lanes_N) op->A.lanes = lanes_N;
+    if (lanes_T) op->AT.lanes = lanes_T;
+    return TV3_OK;
+}
+
+}  // extern "C"

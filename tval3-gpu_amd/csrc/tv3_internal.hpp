@@ -1,0 +1,197 @@
+// tv3_internal.hpp -- internal types of the MI355X TVAL3 library.
+//
+// One process, one GPU, one context per host thread.  All device memory is
+// owned by RAII buffers; errors travel as tv3_error exceptions inside the
+// library and become tv3_status codes at the C ABI (tv3_abi.cpp).
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tval3_mi355x.h"
+
+namespace tv3 {
+
+struct tv3_error : std::runtime_error {
+    tv3_status status;
+    tv3_error(tv3_status s, const std::string &msg) : std::runtime_error(msg), status(s) {}
+};
+
+#define TV3_HIP(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::tv3::tv3_error(TV3_E_HIP, std::string(#expr) + ": " +                 \
+                                                  hipGetErrorString(e_));                 \
+    } while (0)
+
+#define TV3_REQUIRE(cond, st, msg)                                                        \
+    do {                                                                                  \
+        if (!(cond)) throw ::tv3::tv3_error(st, msg);                                     \
+    } while (0)
+
+// Device buffer of T elements (move-only).
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw tv3_error(e == hipErrorOutOfMemory ? TV3_E_NOMEM : TV3_E_HIP,
+                            std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                                " bytes): " + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Pinned host buffer (hipHostMalloc).
+template <typename T>
+struct PinnedBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    void alloc(size_t count) {
+        if (count <= n) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        TV3_HIP(hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+};
+
+inline size_t dtype_size(tv3_dtype t) {
+    switch (t) {
+    case TV3_F32: return 4;
+    case TV3_F16: return 2;
+    case TV3_BF16: return 2;
+    case TV3_F64: return 8;
+    }
+    return 0;
+}
+
+enum class OpKind { CSR, DENSE, DYNAMIC, HYBRID };
+
+// Compressed sparse rows of a matrix (also used for the CSC = CSR of A^T).
+struct SparseRows {
+    int64_t rows = 0, cols = 0, nnz = 0;
+    tv3_dtype vtype = TV3_F32;
+    DevBuf<int32_t> ptr, ind;
+    DevBuf<uint8_t> val;   // nnz * dtype_size(vtype)
+    int lanes = 0;         // lanes per row of the SpMV kernel (power of two <= 64)
+};
+
+struct DynGeom {
+    int32_t E = 0, R = 0, mps = 0, rv_x = 0, rv_y = 0, rv_z = 0;
+    float scale = 1.f;
+    tv3_weight_mode mode = TV3_WEIGHT_SIDDON;
+    int64_t num_paths = 0;
+    int32_t max_walk = 0;      // longest Bresenham walk (voxels) over all rays
+    DevBuf<int32_t> path;      // 2 * num_paths: (emitter, receiver) global indices
+    DevBuf<int32_t> ecoord;    // 3 * mps*E  (x, y, z) Bresenham
+    DevBuf<int32_t> rcoord;    // 3 * mps*R
+    DevBuf<float> epos, rpos;  // 3 * mps*E / 3 * mps*R world coordinates, Siddon
+    float origin[3] = {0, 0, 0}, voxel[3] = {1, 1, 1};
+};
+
+}  // namespace tv3
+
+struct tv3_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 256;
+    // reduction scratch
+    tv3::DevBuf<double> red_partials;
+    tv3::PinnedBuf<double> red_host;
+    tv3::DevBuf<uint8_t> scratch;   // general workspace (grown on demand)
+};
+
+struct tv3_op {
+    tv3_ctx *ctx = nullptr;
+    tv3::OpKind kind = tv3::OpKind::CSR;
+    int64_t m = 0, n = 0;
+    // CSR / HYBRID: A rows and A^T rows (the CSC)
+    tv3::SparseRows A, AT;
+    // DENSE
+    tv3_dtype dtype = TV3_F32;
+    tv3::DevBuf<uint8_t> dense;
+    int64_t lda = 0;
+    // DYNAMIC / HYBRID
+    tv3::DynGeom geo;
+};
+
+namespace tv3 {
+
+// ---------------------------------------------------------------- launchers
+// sparse (spmv.hip)
+void spmv_csr(const SparseRows &S, const float *x, float *y, hipStream_t s, int lanes = 0);
+void spmv_csr(const SparseRows &S, const double *x, double *y, hipStream_t s, int lanes = 0);
+int default_lanes(int64_t rows, int64_t nnz);
+// Stable device transpose of a CSR (rows ordered ascending within each column).
+void build_transpose(tv3_ctx *ctx, const SparseRows &A, SparseRows &AT);
+// exclusive scan of cnt[0..n) into out[0..n] (out[n] = total); returns total
+int64_t exclusive_scan_i32(tv3_ctx *ctx, const int32_t *cnt, int32_t *out, int64_t n);
+
+// dense (dense.hip)
+void dense_gemv(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s);
+void dense_gemv(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s);
+
+// dynamic (dynamic.hip)
+void dynamic_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s);
+void dynamic_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s);
+void dynamic_materialize(tv3_op *op, tv3_dtype vt, SparseRows &A);
+
+// generic apply: y = op x (trans = 0) or op^T x
+void op_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s);
+void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s);
+
+// solvers
+template <typename T>
+void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims[3],
+                 const T *b_dev, T *U_dev, const T *Ut_dev, tv3_info *info, tv3_history *hist);
+void solve_fused(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims[3],
+                 const float *b_dev, float *U_dev, const float *Ut_dev, tv3_info *info,
+                 tv3_history *hist);
+
+// stencil building blocks (vec.hip), float and double
+template <typename T>
+void launch_D(const int32_t dims[3], const T *U, T *Ux, T *Uy, T *Uz, hipStream_t s);
+template <typename T>
+void launch_Dt(const int32_t dims[3], const T *X, const T *Y, const T *Z, T *res, hipStream_t s);
+template <typename T>
+double shrink_host(tv3_ctx *ctx, int64_t n, int iso, const T *Ux, const T *Uy, const T *Uz,
+                   const T *sx, const T *sy, const T *sz, double beta, T *Wx, T *Wy, T *Wz,
+                   hipStream_t s);
+
+}  // namespace tv3
