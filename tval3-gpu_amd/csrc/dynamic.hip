@@ -158,25 +158,27 @@ __device__ __forceinline__ void ray_ends(const GeoView &g, int64_t k, R p0[3], R
     for (int a = 0; a < 3; ++a) { p0[a] = (R)g.ep[3 * e + a]; p1[a] = (R)g.rp[3 * r + a]; }
 }
 
-template <typename T>
+template <typename T, typename AccT>
 __global__ __launch_bounds__(256) void k_dyn_N(GeoView g, int siddon, const T *__restrict__ x,
                                                T *__restrict__ y) {
     int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= g.P) return;
-    T acc = 0;
+    AccT acc = 0;
     if (siddon) {
         T p0[3], p1[3], o[3] = {(T)g.org[0], (T)g.org[1], (T)g.org[2]},
                         h[3] = {(T)g.h[0], (T)g.h[1], (T)g.h[2]};
         int N[3] = {g.rvx, g.rvy, g.rvz};
         ray_ends<T>(g, k, p0, p1);
-        siddon_walk<T>(p0, p1, o, h, N, g.rvy, [&](int64_t v, T w) { acc += w * x[v]; });
+        siddon_walk<T>(p0, p1, o, h, N, g.rvy,
+                       [&](int64_t v, T w) { acc += (AccT)w * (AccT)x[v]; });
     } else {
         int e = g.path[2 * k], r = g.path[2 * k + 1];
         T w = (T)g.scale;
         bresenham_walk(g.ec[3 * e], g.ec[3 * e + 1], g.ec[3 * e + 2], g.rc[3 * r], g.rc[3 * r + 1],
-                       g.rc[3 * r + 2], g.rvx, g.rvy, g.rvz, [&](int64_t v) { acc += x[v] * w; });
+                       g.rc[3 * r + 2], g.rvx, g.rvy, g.rvz,
+                       [&](int64_t v) { acc += (AccT)x[v] * (AccT)w; });
     }
-    y[k] = acc;
+    y[k] = (T)acc;
 }
 
 template <typename T>
@@ -200,13 +202,13 @@ __global__ __launch_bounds__(256) void k_dyn_T(GeoView g, int siddon, const T *_
     }
 }
 
-template <typename T>
+template <typename T, typename AccT>
 static void dyn_apply(tv3_op *op, bool trans, const T *x, T *y, hipStream_t s) {
     GeoView g = view(op->geo);
     int sid = op->geo.mode == TV3_WEIGHT_SIDDON;
     unsigned blocks = (unsigned)((g.P + 255) / 256);
     if (!trans) {
-        if (blocks) hipLaunchKernelGGL((k_dyn_N<T>), dim3(blocks), dim3(256), 0, s, g, sid, x, y);
+        if (blocks) hipLaunchKernelGGL((k_dyn_N<T, AccT>), dim3(blocks), dim3(256), 0, s, g, sid, x, y);
     } else {
         TV3_HIP(hipMemsetAsync(y, 0, sizeof(T) * op->n, s));
         if (blocks) hipLaunchKernelGGL((k_dyn_T<T>), dim3(blocks), dim3(256), 0, s, g, sid, x, y);
@@ -214,11 +216,11 @@ static void dyn_apply(tv3_op *op, bool trans, const T *x, T *y, hipStream_t s) {
     TV3_HIP(hipGetLastError());
 }
 
-void dynamic_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s) {
-    dyn_apply<float>(op, trans, x, y, s);
+void dynamic_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s, bool acc64) {
+    acc64 ? dyn_apply<float, double>(op, trans, x, y, s) : dyn_apply<float, float>(op, trans, x, y, s);
 }
-void dynamic_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s) {
-    dyn_apply<double>(op, trans, x, y, s);
+void dynamic_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s, bool) {
+    dyn_apply<double, double>(op, trans, x, y, s);
 }
 
 // ------------------------------------------------------------- materialize

@@ -9,7 +9,11 @@
 //    update_W read V on a stream that did not wait for its producer);
 //  * lam2/lam4 use all three components with sigma in X,Y,Z order;
 //  * inner products accumulate in FP64 (block partials summed in a fixed
-//    order on the host), scalars are FP64;
+//    order on the host), scalars are FP64, and every entry of a product
+//    A x is summed in FP64 and rounded once, so the FP32 iterates restate
+//    the oracle's rounding instead of the accumulation order of a library;
+//  * vector kernels are compiled without FP contraction (a*x+y rounds
+//    twice, as written);
 //  * every work vector is allocated once per solve (the reference allocates
 //    and frees 3 n-vectors inside get_g/update_g/update_W/update_mlp);
 //  * only skipMulRatio == 0 (the skip heuristic re-subtracts Atb from a
@@ -58,7 +62,7 @@ struct PlainState {
         TV3_HIP(hipMemcpyAsync(dst, src, sizeof(T) * k, hipMemcpyDeviceToDevice, s));
     }
     void A(bool trans, const T *x, T *y) {
-        op_apply(op, trans, x, y, s);
+        op_apply(op, trans, x, y, s, /*acc64=*/true);
         (trans ? nT : nN)++;
     }
     double fval() const { return lam1 + beta / 2 * lam2 + mu / 2 * lam3 - lam4 - lam5; }
@@ -262,9 +266,7 @@ void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
         gam = o.gam; Q = 1; C = S.f;
     }
     TV3_HIP(hipEventRecord(evl1, s));
-    scale<T>(n, U, 1.0 / scl, U, s);
-    // :1275-1277 divides each entry by scl in Type; multiply by the FP64
-    // reciprocal rounded once is used here
+    divide<T>(n, U, scl, U, s);    // :1275-1277
     TV3_HIP(hipEventRecord(ev1, s));
     TV3_HIP(hipStreamSynchronize(s));
     float ms = 0, msl = 0;

@@ -26,7 +26,10 @@ template <> __device__ __forceinline__ float ldw<__hip_bfloat16>(const __hip_bfl
 __device__ __forceinline__ double ldw64(const double *p, int64_t k) { return p[k]; }
 __device__ __forceinline__ double ldw64(const float *p, int64_t k) { return (double)p[k]; }
 
-template <int L, typename V>
+// Acc = float: FP32 accumulation (the fast path).  Acc = double: each row
+// sum is formed in FP64 and rounded once, which restates the oracle's
+// rounding (used by the reference-shaped solver).
+template <int L, typename V, typename Acc>
 __global__ __launch_bounds__(256) void k_csr_f32(int64_t rows, const int32_t *__restrict__ ptr,
                                                  const int32_t *__restrict__ ind,
                                                  const V *__restrict__ val,
@@ -36,21 +39,21 @@ __global__ __launch_bounds__(256) void k_csr_f32(int64_t rows, const int32_t *__
     const int lane = threadIdx.x & (L - 1);
     if (row >= rows) return;          // whole groups leave together (L divides 256)
     const int beg = ptr[row], end = ptr[row + 1];
-    float s = 0.f;
+    Acc s = 0;
     int k = beg + lane;
     // two independent chains keep two gathers in flight per lane
-    float s2 = 0.f;
+    Acc s2 = 0;
     for (; k + L < end; k += 2 * L) {
         const int c0 = ind[k], c1 = ind[k + L];
-        const float w0 = ldw<V>(val, k), w1 = ldw<V>(val, k + L);
-        s = fmaf(w0, x[c0], s);
-        s2 = fmaf(w1, x[c1], s2);
+        const Acc w0 = (Acc)ldw<V>(val, k), w1 = (Acc)ldw<V>(val, k + L);
+        s = fma(w0, (Acc)x[c0], s);
+        s2 = fma(w1, (Acc)x[c1], s2);
     }
-    if (k < end) s = fmaf(ldw<V>(val, k), x[ind[k]], s);
+    if (k < end) s = fma((Acc)ldw<V>(val, k), (Acc)x[ind[k]], s);
     s += s2;
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
-    if (lane == 0) y[row] = s;
+    if (lane == 0) y[row] = (float)s;
 }
 
 template <int L, typename V>
@@ -77,7 +80,7 @@ int default_lanes(int64_t rows, int64_t nnz) {
     return L;
 }
 
-template <typename V>
+template <typename V, typename Acc>
 static void launch_f32(const SparseRows &S, const float *x, float *y, hipStream_t s, int L) {
     const V *val = reinterpret_cast<const V *>(S.val.p);
     const int64_t threads = S.rows * L;
@@ -85,7 +88,7 @@ static void launch_f32(const SparseRows &S, const float *x, float *y, hipStream_
     if (blocks == 0) return;
 #define TV3_CASE(LL)                                                                      \
     case LL:                                                                              \
-        hipLaunchKernelGGL((k_csr_f32<LL, V>), dim3(blocks), dim3(256), 0, s, S.rows,     \
+        hipLaunchKernelGGL((k_csr_f32<LL, V, Acc>), dim3(blocks), dim3(256), 0, s, S.rows,\
                            S.ptr.p, S.ind.p, val, x, y);                                  \
         break;
     switch (L) {
@@ -115,17 +118,26 @@ static void launch_f64(const SparseRows &S, const double *x, double *y, hipStrea
     TV3_HIP(hipGetLastError());
 }
 
-void spmv_csr(const SparseRows &S, const float *x, float *y, hipStream_t s, int lanes) {
+void spmv_csr(const SparseRows &S, const float *x, float *y, hipStream_t s, int lanes,
+              bool acc64) {
     int L = lanes ? lanes : (S.lanes ? S.lanes : default_lanes(S.rows, S.nnz));
     switch (S.vtype) {
-    case TV3_F32: launch_f32<float>(S, x, y, s, L); break;
-    case TV3_F16: launch_f32<__half>(S, x, y, s, L); break;
-    case TV3_BF16: launch_f32<__hip_bfloat16>(S, x, y, s, L); break;
+    case TV3_F32:
+        acc64 ? launch_f32<float, double>(S, x, y, s, L) : launch_f32<float, float>(S, x, y, s, L);
+        break;
+    case TV3_F16:
+        acc64 ? launch_f32<__half, double>(S, x, y, s, L) : launch_f32<__half, float>(S, x, y, s, L);
+        break;
+    case TV3_BF16:
+        acc64 ? launch_f32<__hip_bfloat16, double>(S, x, y, s, L)
+              : launch_f32<__hip_bfloat16, float>(S, x, y, s, L);
+        break;
     default: throw tv3_error(TV3_E_FORMAT, "FP32 products need F32, F16 or BF16 values");
     }
 }
 
-void spmv_csr(const SparseRows &S, const double *x, double *y, hipStream_t s, int lanes) {
+void spmv_csr(const SparseRows &S, const double *x, double *y, hipStream_t s, int lanes,
+              bool) {
     int L = lanes ? lanes : (S.lanes ? S.lanes : default_lanes(S.rows, S.nnz));
     switch (S.vtype) {
     case TV3_F64: launch_f64<double>(S, x, y, s, L); break;

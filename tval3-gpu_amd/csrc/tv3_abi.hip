@@ -114,19 +114,19 @@ int64_t dims_n(const int32_t dims[3]) {
 
 namespace tv3 {
 
-void op_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s) {
+void op_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s, bool acc64) {
     switch (op->kind) {
-    case OpKind::CSR: spmv_csr(trans ? op->AT : op->A, x, y, s); break;
-    case OpKind::DENSE: dense_gemv(op, trans, x, y, s); break;
-    case OpKind::DYNAMIC: dynamic_apply(op, trans, x, y, s); break;
+    case OpKind::CSR: spmv_csr(trans ? op->AT : op->A, x, y, s, 0, acc64); break;
+    case OpKind::DENSE: dense_gemv(op, trans, x, y, s, acc64); break;
+    case OpKind::DYNAMIC: dynamic_apply(op, trans, x, y, s, acc64); break;
     case OpKind::HYBRID:
-        if (trans) spmv_csr(op->AT, x, y, s);
-        else dynamic_apply(op, false, x, y, s);
+        if (trans) spmv_csr(op->AT, x, y, s, 0, acc64);
+        else dynamic_apply(op, false, x, y, s, acc64);
         break;
     }
 }
 
-void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s) {
+void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s, bool) {
     switch (op->kind) {
     case OpKind::CSR: spmv_csr(trans ? op->AT : op->A, x, y, s); break;
     case OpKind::DENSE: dense_gemv(op, trans, x, y, s); break;

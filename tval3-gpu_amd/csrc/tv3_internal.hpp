@@ -155,8 +155,11 @@ namespace tv3 {
 
 // ---------------------------------------------------------------- launchers
 // sparse (spmv.hip)
-void spmv_csr(const SparseRows &S, const float *x, float *y, hipStream_t s, int lanes = 0);
-void spmv_csr(const SparseRows &S, const double *x, double *y, hipStream_t s, int lanes = 0);
+// acc64: FP64 accumulation of each output entry, rounded once (FP32 vectors)
+void spmv_csr(const SparseRows &S, const float *x, float *y, hipStream_t s, int lanes = 0,
+              bool acc64 = false);
+void spmv_csr(const SparseRows &S, const double *x, double *y, hipStream_t s, int lanes = 0,
+              bool acc64 = true);
 int default_lanes(int64_t rows, int64_t nnz);
 // Stable device transpose of a CSR (rows ordered ascending within each column).
 void build_transpose(tv3_ctx *ctx, const SparseRows &A, SparseRows &AT);
@@ -164,17 +167,23 @@ void build_transpose(tv3_ctx *ctx, const SparseRows &A, SparseRows &AT);
 int64_t exclusive_scan_i32(tv3_ctx *ctx, const int32_t *cnt, int32_t *out, int64_t n);
 
 // dense (dense.hip)
-void dense_gemv(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s);
-void dense_gemv(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s);
+void dense_gemv(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,
+                bool acc64 = false);
+void dense_gemv(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
+                bool acc64 = true);
 
 // dynamic (dynamic.hip)
-void dynamic_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s);
-void dynamic_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s);
+void dynamic_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,
+                   bool acc64 = false);
+void dynamic_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
+                   bool acc64 = true);
 void dynamic_materialize(tv3_op *op, tv3_dtype vt, SparseRows &A);
 
 // generic apply: y = op x (trans = 0) or op^T x
-void op_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s);
-void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s);
+void op_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,
+              bool acc64 = false);
+void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
+              bool acc64 = true);
 
 // solvers
 template <typename T>

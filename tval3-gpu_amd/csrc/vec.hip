@@ -43,6 +43,13 @@ __global__ void k_scale(int64_t n, T *y, T a, const T *x) {
         y[i] = a * x[i];
 }
 
+template <typename T>
+__global__ void k_div(int64_t n, T *y, T a, const T *x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = x[i] / a;
+}
+
 static inline unsigned ew_blocks(int64_t n) {
     int64_t b = (n + 255) / 256;
     if (b > 8192) b = 8192;
@@ -61,6 +68,11 @@ void grad(int64_t n, T *d, double a, const T *g, const T *g2, const T *dts, hipS
 template <typename T>
 void nonneg(int64_t n, T *u, hipStream_t s) {
     hipLaunchKernelGGL(k_nonneg<T>, dim3(ew_blocks(n)), dim3(256), 0, s, n, u);
+}
+template <typename T>
+void divide(int64_t n, T *y, double a, const T *x, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_div<T>, dim3(ew_blocks(n)), dim3(256), 0, s, n, y, (T)a, x);
 }
 template <typename T>
 void scale(int64_t n, T *y, double a, const T *x, hipStream_t s) {
@@ -294,6 +306,7 @@ std::array<double, 2> diff_norm(tv3_ctx *ctx, int64_t n, const T *a, const T *b,
     template void grad<T>(int64_t, T *, double, const T *, const T *, const T *, hipStream_t);  \
     template void nonneg<T>(int64_t, T *, hipStream_t);                                         \
     template void scale<T>(int64_t, T *, double, const T *, hipStream_t);                       \
+    template void divide<T>(int64_t, T *, double, const T *, hipStream_t);                      \
     template void launch_D<T>(const int32_t *, const T *, T *, T *, T *, hipStream_t);          \
     template void launch_Dt<T>(const int32_t *, const T *, const T *, const T *, T *,           \
                                hipStream_t);                                                    \

@@ -35,6 +35,7 @@ template <typename T>
 void grad(int64_t n, T *d, double a, const T *g, const T *g2, const T *dts, hipStream_t s);
 template <typename T> void nonneg(int64_t n, T *u, hipStream_t s);
 template <typename T> void scale(int64_t n, T *y, double a, const T *x, hipStream_t s);
+template <typename T> void divide(int64_t n, T *y, double a, const T *x, hipStream_t s);
 template <typename T> double dot(tv3_ctx *ctx, int64_t n, const T *a, const T *b, hipStream_t s);
 template <typename T>
 std::array<double, 2> lam24(tv3_ctx *ctx, int64_t n, const T *Ux, const T *Uy, const T *Uz,
