@@ -174,7 +174,9 @@ tv3_status tv3_op_csr_download(tv3_op *op, int32_t *rowptr, int32_t *col, double
  * (3d/mat_vec_mul_sparse.h:1703-1780, 3d/mat_vec_mul_dense.h:25-73,
  * 3d/mat_vec_mul_dynamic.h:682-967, 3d/mat_vec_mul.h:732-756).
  * x, y device pointers; y is fully overwritten; asynchronous on stream
- * (NULL = the context stream).  mulChain is a multi-GPU hint and is ignored. */
+ * (a hipStream_t; NULL is the null stream, as for the reference's
+ * stream = 0 default; pass tv3_ctx_stream(ctx) for the context stream).
+ * mulChain is a multi-GPU hint and has no counterpart here. */
 tv3_status tv3_matvec(tv3_op *op, tv3_trans trans, const float *x_dev, float *y_dev,
                       void *stream);
 tv3_status tv3_matvec_f64(tv3_op *op, tv3_trans trans, const double *x_dev, double *y_dev,
@@ -216,8 +218,10 @@ tv3_status tv3_shrink(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, const float *Ux
                       double *lam1);
 
 /* ---- tuning ------------------------------------------------------------- */
-/* Time the sparse kernels of op over their launch shapes (lanes per row) and
- * keep the fastest; returns the chosen lanes for N and T. */
+/* Time the sparse kernels of op over their launch shapes and keep the
+ * fastest: the row-blocked kernel with 1024/2048/4096-entry blocks (returned
+ * as -1024/-2048/-4096) or the row-group kernel with L lanes per row
+ * (returned as L).  tv3_op_set_lanes accepts the same codes (0 = keep). */
 tv3_status tv3_op_autotune(tv3_op *op, int32_t *lanes_N, int32_t *lanes_T);
 tv3_status tv3_op_set_lanes(tv3_op *op, int32_t lanes_N, int32_t lanes_T);
 

@@ -50,12 +50,33 @@ def test_csr_integer_bit_exact_all_lanes(ctx):
     op = api.SparseOperator(ctx, m, n, rp, c, v)
     x = np.random.default_rng(2).integers(-4, 5, n).astype(np.float32)
     r = np.random.default_rng(3).integers(-4, 5, m).astype(np.float32)
-    for L in (2, 4, 8, 16, 32, 64):
+    # row-group kernel (L lanes per row) and row-blocked kernel (-NB entries)
+    for L in (2, 4, 8, 16, 32, 64, -1024, -2048, -4096):
         op.set_lanes(L, L)
         y = H(op.mat_vec_mul(T(x)))
         z = H(op.mat_vec_mul(T(r), trans=True))
         assert (y == (A @ x).astype(np.float32)).all(), L
         assert (z == (A.T @ r).astype(np.float32)).all(), L
+
+
+def test_csr_stream_long_rows_and_empty(ctx):
+    # rows longer than a block, empty rows and a block of many short rows
+    rng = np.random.default_rng(21)
+    m, n = 40, 9000
+    lens = np.array([0, 5000, 3, 0, 1200, 4100] + [7] * 34)
+    rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = np.concatenate([rng.choice(n, l, replace=False) for l in lens]).astype(np.int32)
+    val = rng.integers(-3, 4, len(col)).astype(np.float64)
+    A = np.zeros((m, n))
+    for i in range(m):
+        A[i, col[rowptr[i]:rowptr[i + 1]]] = val[rowptr[i]:rowptr[i + 1]]
+    op = api.SparseOperator(ctx, m, n, rowptr, col, val)
+    x = rng.integers(-4, 5, n).astype(np.float32)
+    r = rng.integers(-4, 5, m).astype(np.float32)
+    for L in (-1024, -2048, -4096, 64):
+        op.set_lanes(L, L)
+        assert (H(op.mat_vec_mul(T(x))) == (A @ x).astype(np.float32)).all(), L
+        assert (H(op.mat_vec_mul(T(r), trans=True)) == (A.T @ r).astype(np.float32)).all(), L
 
 
 def test_csr_float_matches_oracle_and_given_csc(ctx):

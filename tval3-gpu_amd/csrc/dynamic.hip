@@ -335,6 +335,7 @@ void dynamic_materialize(tv3_op *op, tv3_dtype vt, SparseRows &A) {
     TV3_HIP(hipGetLastError());
     TV3_HIP(hipStreamSynchronize(s));
     A.lanes = default_lanes(A.rows, A.nnz);
+    build_row_blocks(ctx, A, 2048);
 }
 
 }  // namespace tv3

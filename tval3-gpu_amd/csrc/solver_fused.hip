@@ -1,9 +1,913 @@
-// solver_fused.hip -- placeholder until the fused solver lands.
-#include "tv3_internal.hpp"
+// solver_fused.hip -- TVAL3 with fused kernels, device-side FP64 scalars and
+// hipGraph-captured inner iterations.
+//
+// Algorithm: 3d/tval3_gpu_3d.cpp:1088-1345 (same formulas, same order of
+// decisions).  What changes is how the work is laid out:
+//
+//  * one steady inner iteration is seven kernels:
+//      descend   U_t = max(U - tau d, 0) and, from U and U_t on the fly,
+//                DU, W = shrink(DU - sigma/beta), V0 = DU - W, V1 = DU_t - W
+//                with |V1|^2, <V1,sigma>, <V0,V1>                  (:671-701)
+//      A x       Au_t = A U_t, reductions |r1|^2, <delta,r1>, <r0,r1>
+//                of r = Au - b                                    (:382, :500)
+//      A^T x     g_t = A^T Au_t - Atb                            (:423, :565)
+//      search    the nonmonotone Armijo test (:839-890) on f(alpha): U, Au,
+//                g, DU are affine in alpha (:717-737), so lam2/lam3 are
+//                quadratic and lam4/lam5 linear in alpha and the up to five
+//                backtracking sweeps of update_g become scalar work
+//      apply     U, g, Au = prev + alpha (trial - prev), skipped if alpha = 1
+//      update_W  W = shrink(DU - sigma/beta), V = DU - W, g2 = D^T V,
+//                d = g2 + mu/beta g + DtsAtd, and the Barzilai-Borwein sums
+//                ss = |U - Up|^2, sy = <U - Up, d - d_prev> (equal to the
+//                reference's <uup, dg2 + mu/beta dg> inside an inner loop,
+//                where mu/beta and DtsAtd are fixed)          (:965-1029, :896)
+//      end       lam1, lam2, lam4, f, Q, C, RelChg, tau, stop flags (:1176-1202)
+//  * W, g2, Ux..Uz and their previous copies are never stored: W and DU are
+//    recomputed from U and sigma where needed (the D^T of get_g, :536, is
+//    dead in the reference: g2 is overwritten in update_W before any read);
+//  * every kernel starts by reading a device control block and returns at
+//    once when the inner loop has ended, so K iterations are captured in one
+//    graph and the host reads the control block once per graph;
+//  * the first iteration of an inner loop (tau from |d|^2, |Dd|^2, |Ad|^2,
+//    :583-625), the cnt == 5 fallback (:876-883) and the outer update
+//    (:1206-1249) run as ordinary launches.
+//  * vectors are FP32 in the layout of layout.hip, products accumulate in
+//    FP32, all inner products / norms accumulate in FP64 (block partials
+//    reduced in a fixed order), all scalars are FP64.
+#include <cmath>
+#include <cstring>
+
+#include "spmv_stream.hpp"
+#include "vec.hpp"
 
 namespace tv3 {
-void solve_fused(tv3_ctx *, tv3_op *, const tv3_opts &, const int32_t *, const float *, float *,
-                 const float *, tv3_info *, tv3_history *) {
-    throw tv3_error(TV3_E_UNSUPPORTED, "fused solver not built yet");
+
+namespace {
+
+// ------------------------------------------------------------------ control
+struct FCtl {
+    // parameters
+    double mu, beta, muDbeta, c, gamma, rate_gam, tol_inn, gam_opt, rate_cnt, mu_max, beta_max;
+    int32_t maxit, nonneg, iso, hist_cap;
+    // state
+    double f, lam1, lam2, lam3, lam4, lam5, C, Q, gam, tau, alpha, dd, nrmup2, relchg;
+    double ratio, mdb_old, relout;
+    int32_t total, fst, active, inner_done, need_fallback, nonfinite, sigw_old, cnt;
+    int32_t backtracks, fallbacks, inner_loops;
+};
+
+constexpr int HF = 15;   // history fields per iteration
+
+struct Lay {
+    int p, q, r, nby, nbx;
+    int64_t pq;
+};
+
+template <int B>
+__device__ __forceinline__ int64_t vidx(const Lay &L, int y, int x, int z) {
+    if (B == 1) return (int64_t)z * L.pq + (int64_t)x * L.p + y;
+    return ((((int64_t)(z >> 2) * L.nbx + (x >> 2)) * L.nby + (y >> 2)) << 6) + ((z & 3) << 4) +
+           ((x & 3) << 2) + (y & 3);
 }
+
+// Voxel groups of 256 threads: B = 4 -> four bricks consecutive in y (one
+// wavefront per brick); B = 1 -> 64 (y) x 4 (x) voxels of one z-plane.
+// Workgroups of the 1-D grid walk contiguous chunks of groups, consecutive
+// chunks on one XCD.
+struct Groups {
+    int gy, gx, gz;      // groups along y, x, z
+    int64_t count;
+};
+
+template <int B>
+static Groups groups_of(const Lay &L) {
+    Groups g;
+    if (B == 4) { g.gy = (L.nby + 3) / 4; g.gx = L.nbx; g.gz = L.r / 4; }
+    else { g.gy = (L.p + 63) / 64; g.gx = (L.q + 3) / 4; g.gz = L.r; }
+    g.count = (int64_t)g.gy * g.gx * g.gz;
+    return g;
+}
+
+template <int B>
+__device__ __forceinline__ bool voxel_of(const Lay &L, const Groups &G, int64_t grp, int &y, int &x,
+                                         int &z) {
+    const int t = threadIdx.x;
+    const int gy = (int)(grp % G.gy);
+    const int64_t rest = grp / G.gy;
+    const int gx = (int)(rest % G.gx), gz = (int)(rest / G.gx);
+    if (B == 4) {
+        const int by = gy * 4 + (t >> 6), lane = t & 63;
+        y = by * 4 + (lane & 3);
+        x = gx * 4 + ((lane >> 2) & 3);
+        z = gz * 4 + (lane >> 4);
+        return by < L.nby;
+    }
+    y = gy * 64 + (t & 63);
+    x = gx * 4 + (t >> 6);
+    z = gz;
+    return y < L.p && x < L.q;
+}
+
+constexpr int kMaxStencilGrid = 1 << 30;
+
+static unsigned stencil_grid(const Groups &G) {
+    return (unsigned)(G.count < kMaxStencilGrid ? G.count : kMaxStencilGrid);
+}
+
+// range of groups of this workgroup
+__device__ __forceinline__ void group_range(int64_t count, int64_t &g0, int64_t &g1) {
+    const int64_t Gd = gridDim.x, chunk = (count + Gd - 1) / Gd;
+    const int64_t w = xcd_remap(blockIdx.x, (int)Gd);
+    g0 = w * chunk;
+    g1 = min(count, g0 + chunk);
+}
+
+__device__ __forceinline__ int nxt(int v, int n) { return v + 1 == n ? 0 : v + 1; }
+__device__ __forceinline__ int prv(int v, int n) { return v == 0 ? n - 1 : v - 1; }
+
+// W = shrink(DU - sigma/beta); returns the lam1 contribution
+__device__ __forceinline__ float shrink3(float ux, float uy, float uz, float sx, float sy, float sz,
+                                        float beta, float ib, bool iso, float &wx, float &wy,
+                                        float &wz) {
+    const float bx = ux - sx / beta, by = uy - sy / beta, bz = uz - sz / beta;
+    if (!iso) {
+        const float tx = fmaxf(fabsf(bx) - ib, 0.f), ty = fmaxf(fabsf(by) - ib, 0.f),
+                    tz = fmaxf(fabsf(bz) - ib, 0.f);
+        wx = bx >= 0 ? tx : -tx;
+        wy = by >= 0 ? ty : -ty;
+        wz = bz >= 0 ? tz : -tz;
+        return tx + ty + tz;
+    }
+    const float nrm = sqrtf(bx * bx + by * by + bz * bz);
+    const float sh = nrm - ib;
+    if (sh > 0.f) {
+        const float f = sh / nrm;
+        wx = bx * f; wy = by * f; wz = bz * f;
+        return sh;
+    }
+    wx = wy = wz = 0.f;
+    return 0.f;
+}
+
+// ------------------------------------------------------------------ stencils
+// descend: U_t = max(U_c - tau d, 0) (:690-696); V0/V1 statistics.
+template <int B>
+__global__ __launch_bounds__(256) void k_descend(Lay L, Groups G, const float *__restrict__ Uc,
+                                                 const float *__restrict__ d,
+                                                 const float *__restrict__ sx,
+                                                 const float *__restrict__ sy,
+                                                 const float *__restrict__ sz,
+                                                 const float *__restrict__ wsx,
+                                                 const float *__restrict__ wsy,
+                                                 const float *__restrict__ wsz,
+                                                 float *__restrict__ Un, const FCtl *C,
+                                                 double *part) {
+    if (!C->active) return;
+    double acc[3] = {0, 0, 0};
+    const float tau = (float)C->tau, beta = (float)C->beta, ib = 1.f / beta;
+    const bool nn = C->nonneg, iso = C->iso;
+    const bool old = C->sigw_old;
+    int64_t g0, g1;
+    group_range(G.count, g0, g1);
+    for (int64_t grp = g0; grp < g1; ++grp) {
+        int y, x, z;
+        if (!voxel_of<B>(L, G, grp, y, x, z)) continue;
+        const int64_t i = vidx<B>(L, y, x, z);
+        const int64_t iy = vidx<B>(L, nxt(y, L.p), x, z), ix = vidx<B>(L, y, nxt(x, L.q), z),
+                      iz = vidx<B>(L, y, x, nxt(z, L.r));
+        const float u0 = Uc[i], uy = Uc[iy], ux = Uc[ix], uz = Uc[iz];
+        auto step = [&](float u, float dv) {
+            float t = u - tau * dv;
+            return (nn && t < 0.f) ? 0.f : t;
+        };
+        const float t0 = step(u0, d[i]), ty = step(uy, d[iy]), tx = step(ux, d[ix]),
+                    tz = step(uz, d[iz]);
+        const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
+        float wx, wy, wz;
+        shrink3(ux - u0, uy - u0, uz - u0, old ? wsx[i] : s_x, old ? wsy[i] : s_y,
+                old ? wsz[i] : s_z, beta, ib, iso, wx, wy, wz);
+        const float v0x = (ux - u0) - wx, v0y = (uy - u0) - wy, v0z = (uz - u0) - wz;
+        const float v1x = (tx - t0) - wx, v1y = (ty - t0) - wy, v1z = (tz - t0) - wz;
+        acc[0] += (double)v1x * v1x + (double)v1y * v1y + (double)v1z * v1z;
+        acc[1] += (double)v1x * s_x + (double)v1y * s_y + (double)v1z * s_z;
+        acc[2] += (double)v0x * v1x + (double)v0y * v1y + (double)v0z * v1z;
+        Un[i] = t0;
+    }
+    block_sum_store<3>(acc, part, gridDim.x);
+}
+
+// update_W + D^T + gradient + BB sums (:965-1029, :896-919, :1017-1067)
+template <int B>
+__global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *__restrict__ U,
+                                                  const float *__restrict__ Uc,
+                                                  const float *__restrict__ sx,
+                                                  const float *__restrict__ sy,
+                                                  const float *__restrict__ sz,
+                                                  const float *__restrict__ g,
+                                                  const float *__restrict__ dts,
+                                                  float *__restrict__ d, const FCtl *C,
+                                                  double *part) {
+    if (!C->active) return;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
+    const bool iso = C->iso;
+    int64_t g0, g1;
+    group_range(G.count, g0, g1);
+    for (int64_t grp = g0; grp < g1; ++grp) {
+        int y, x, z;
+        if (!voxel_of<B>(L, G, grp, y, x, z)) continue;
+        const int yp = nxt(y, L.p), ym = prv(y, L.p), xp = nxt(x, L.q), xm = prv(x, L.q),
+                  zp = nxt(z, L.r), zm = prv(z, L.r);
+        const int64_t i = vidx<B>(L, y, x, z);
+        const float u0 = U[i];
+        const float uyp = U[vidx<B>(L, yp, x, z)], uxp = U[vidx<B>(L, y, xp, z)],
+                    uzp = U[vidx<B>(L, y, x, zp)];
+        // V at i
+        float wx, wy, wz;
+        const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
+        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, beta, ib, iso, wx,
+                                 wy, wz);
+        const float vx = (uxp - u0) - wx, vy = (uyp - u0) - wy, vz = (uzp - u0) - wz;
+        // V_a at i - e_a (needs the full DU there)
+        float vyb, vxb, vzb;
+        {
+            const int64_t j = vidx<B>(L, ym, x, z);
+            const float uj = U[j];
+            const float dyj = u0 - uj, dxj = U[vidx<B>(L, ym, xp, z)] - uj,
+                        dzj = U[vidx<B>(L, ym, x, zp)] - uj;
+            float a, b, c;
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, c);
+            vyb = dyj - b;
+        }
+        {
+            const int64_t j = vidx<B>(L, y, xm, z);
+            const float uj = U[j];
+            const float dxj = u0 - uj, dyj = U[vidx<B>(L, yp, xm, z)] - uj,
+                        dzj = U[vidx<B>(L, y, xm, zp)] - uj;
+            float a, b, c;
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, c);
+            vxb = dxj - a;
+        }
+        {
+            const int64_t j = vidx<B>(L, y, x, zm);
+            const float uj = U[j];
+            const float dzj = u0 - uj, dxj = U[vidx<B>(L, y, xp, zm)] - uj,
+                        dyj = U[vidx<B>(L, yp, x, zm)] - uj;
+            float a, b, c;
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, c);
+            vzb = dzj - c;
+        }
+        const float g2 = vyb - vy + vxb - vx + vzb - vz;            // D^T V, :501
+        const float dn = mdb * g[i] + g2 + dts[i];                   // :915-918
+        const float uc = Uc[i], du = u0 - uc;
+        acc[0] += l1;
+        acc[1] += (double)vx * vx + (double)vy * vy + (double)vz * vz;
+        acc[2] += (double)vx * s_x + (double)vy * s_y + (double)vz * s_z;
+        acc[3] += (double)du * du;
+        acc[4] += (double)du * (double)(dn - d[i]);
+        acc[5] += (double)uc * uc;
+        acc[6] += (double)dn * dn;
+        d[i] = dn;
+    }
+    block_sum_store<7>(acc, part, gridDim.x);
+}
+
+// |D d|^2 for the first-iteration step length (:587-596)
+template <int B>
+__global__ __launch_bounds__(256) void k_dd_norm(Lay L, Groups G, const float *__restrict__ d,
+                                                 double *part) {
+    double acc[1] = {0};
+    int64_t g0, g1;
+    group_range(G.count, g0, g1);
+    for (int64_t grp = g0; grp < g1; ++grp) {
+        int y, x, z;
+        if (!voxel_of<B>(L, G, grp, y, x, z)) continue;
+        const int64_t i = vidx<B>(L, y, x, z);
+        const float d0 = d[i];
+        const float a = d[vidx<B>(L, nxt(y, L.p), x, z)] - d0,
+                    b = d[vidx<B>(L, y, nxt(x, L.q), z)] - d0,
+                    c = d[vidx<B>(L, y, x, nxt(z, L.r))] - d0;
+        acc[0] += (double)a * a + (double)b * b + (double)c * c;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// update_mlp, sigma part (:1036-1069): keeps the old sigma for the W of the
+// next descend and returns lam4 = <sigma_new, V>
+template <int B>
+__global__ __launch_bounds__(256) void k_mlp_sigma(Lay L, Groups G, const float *__restrict__ U,
+                                                   float *__restrict__ sx, float *__restrict__ sy,
+                                                   float *__restrict__ sz, float *__restrict__ ox,
+                                                   float *__restrict__ oy, float *__restrict__ oz,
+                                                   const FCtl *C, double *part) {
+    double acc[1] = {0};
+    const float beta = (float)C->beta, ib = 1.f / beta;
+    int64_t g0, g1;
+    group_range(G.count, g0, g1);
+    for (int64_t grp = g0; grp < g1; ++grp) {
+        int y, x, z;
+        if (!voxel_of<B>(L, G, grp, y, x, z)) continue;
+        const int64_t i = vidx<B>(L, y, x, z);
+        const float u0 = U[i];
+        const float dy = U[vidx<B>(L, nxt(y, L.p), x, z)] - u0,
+                    dx = U[vidx<B>(L, y, nxt(x, L.q), z)] - u0,
+                    dz = U[vidx<B>(L, y, x, nxt(z, L.r))] - u0;
+        const float a = sx[i], b = sy[i], c = sz[i];
+        float wx, wy, wz;
+        shrink3(dx, dy, dz, a, b, c, beta, ib, C->iso, wx, wy, wz);
+        const float vx = dx - wx, vy = dy - wy, vz = dz - wz;
+        const float na = a - beta * vx, nb = b - beta * vy, nc = c - beta * vz;
+        ox[i] = a; oy[i] = b; oz[i] = c;
+        sx[i] = na; sy[i] = nb; sz[i] = nc;
+        acc[0] += (double)na * vx + (double)nb * vy + (double)nc * vz;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// ------------------------------------------------------------------ m-vectors
+__global__ __launch_bounds__(256) void k_mlp_delta(int64_t m, const float *__restrict__ Au,
+                                                   const float *__restrict__ b,
+                                                   float *__restrict__ delta, const FCtl *C,
+                                                   double *part) {
+    double acc[1] = {0};
+    const float mu = (float)C->mu;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
+        const float r = Au[i] - b[i];
+        const float dn = delta[i] - mu * r;
+        delta[i] = dn;
+        acc[0] += (double)dn * r;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void k_apply_alpha(int64_t n, int64_t m,
+                                                     const float *__restrict__ Uc,
+                                                     float *__restrict__ Un,
+                                                     const float *__restrict__ gc,
+                                                     float *__restrict__ gn,
+                                                     const float *__restrict__ Auc,
+                                                     float *__restrict__ Aun, const FCtl *C) {
+    if (!C->active || C->alpha == 1.0) return;
+    const float a = (float)C->alpha;
+    const int64_t mx = n > m ? n : m;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mx; i += (int64_t)gridDim.x * 256) {
+        if (i < n) {
+            const float u = Uc[i], g = gc[i];
+            Un[i] = u + a * (Un[i] - u);
+            gn[i] = g + a * (gn[i] - g);
+        }
+        if (i < m) {
+            const float v = Auc[i];
+            Aun[i] = v + a * (Aun[i] - v);
+        }
+    }
+}
+
+// outer update of d and DtsAtd (:1239-1246) without a stored g2
+__global__ __launch_bounds__(256) void k_outer_d(int64_t n, float *__restrict__ d,
+                                                 const float *__restrict__ g,
+                                                 float *__restrict__ dts, const FCtl *C,
+                                                 double *part) {
+    double acc[1] = {0};
+    const float ratio = (float)C->ratio, mo = (float)C->mdb_old, mn = (float)C->muDbeta;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float dv = d[i], gv = g[i];
+        const float g2 = dv - mo * gv - dts[i];
+        const float nd = ratio * dv;
+        const float dn = mn * gv + g2 + nd;
+        dts[i] = nd;
+        d[i] = dn;
+        acc[0] += (double)dn * dn;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// |U - rcdU|^2 then rcdU = U (:1208-1217)
+__global__ __launch_bounds__(256) void k_relout(int64_t n, const float *__restrict__ U,
+                                                float *__restrict__ rcd, double *part) {
+    double acc[1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float u = U[i], t = u - rcd[i];
+        acc[0] += (double)t * t;
+        rcd[i] = u;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// ------------------------------------------------------------------ product epilogues
+struct EpiRes {          // Au_n = s; r1 = s - b; sums |r1|^2, <delta,r1>, <r0,r1>
+    float *Aun;
+    const float *b, *delta, *Auc;
+    const FCtl *C;
+    double *part;
+    mutable double acc[3];
+    __device__ bool skip() const { return !C->active; }
+    __device__ __forceinline__ void row(int r, float s, bool active) const {
+        if (active) {
+            Aun[r] = s;
+            const float bv = b[r], r1 = s - bv, r0 = Auc[r] - bv;
+            acc[0] += (double)r1 * r1;
+            acc[1] += (double)delta[r] * r1;
+            acc[2] += (double)r0 * r1;
+        }
+    }
+    __device__ __forceinline__ void finish() const { block_sum_store<3>(acc, part, gridDim.x); }
+};
+
+struct EpiNorm {         // |s|^2 only (A d for the first-iteration step)
+    const FCtl *C;
+    double *part;
+    mutable double acc[1];
+    __device__ bool skip() const { return false; }
+    __device__ __forceinline__ void row(int, float s, bool active) const {
+        if (active) acc[0] += (double)s * s;
+    }
+    __device__ __forceinline__ void finish() const { block_sum_store<1>(acc, part, gridDim.x); }
+};
+
+struct EpiGrad {         // g_n = s - Atb
+    float *g;
+    const float *Atb;
+    const FCtl *C;
+    __device__ bool skip() const { return !C->active; }
+    __device__ __forceinline__ void row(int r, float s, bool active) const {
+        if (active) g[r] = s - Atb[r];
+    }
+    __device__ __forceinline__ void finish() const {}
+};
+
+// ------------------------------------------------------------------ scalar kernels
+// Sum K partial arrays of nb entries each in a fixed order (1024 threads, four
+// independent load chains per thread); result in out[K] on thread 0.
+template <int K>
+__device__ void reduce_parts(const double *part, int nb, double *out) {
+    __shared__ double sh[K][1024];
+    const int t = threadIdx.x;
+    for (int k = 0; k < K; ++k) {
+        const double *p = part + (int64_t)k * nb;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        int j = t;
+        for (; j + 3 * 1024 < nb; j += 4 * 1024) {
+            a0 += p[j]; a1 += p[j + 1024]; a2 += p[j + 2048]; a3 += p[j + 3072];
+        }
+        for (; j < nb; j += 1024) a0 += p[j];
+        sh[k][t] = (a0 + a1) + (a2 + a3);
+    }
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off)
+            for (int k = 0; k < K; ++k) sh[k][t] += sh[k][t + off];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < K; ++k) out[k] = sh[k][0];
+}
+
+// Second reduction stage: K = KA + KB partial arrays (nbA resp. nbB entries)
+// -> K arrays of S2 entries, workgroup j summing a contiguous slice in a fixed
+// order.  Keeps the single-workgroup scalar kernels off long load chains.
+constexpr int S2 = 64;
+
+template <int KA, int KB>
+__global__ __launch_bounds__(256) void k_stage2(const FCtl *C, int check, const double *pa, int nbA,
+                                                const double *pb, int nbB, double *out) {
+    if (check && !C->active) return;
+    __shared__ double sh[4];
+    const int j = blockIdx.x, t = threadIdx.x;
+    for (int k = 0; k < KA + KB; ++k) {
+        const double *p = k < KA ? pa + (int64_t)k * nbA : pb + (int64_t)(k - KA) * nbB;
+        const int nb = k < KA ? nbA : nbB;
+        const int chunk = (nb + S2 - 1) / S2, lo = j * chunk, hi = min(nb, lo + chunk);
+        double a0 = 0, a1 = 0;
+        int i = lo + t;
+        for (; i + 256 < hi; i += 512) { a0 += p[i]; a1 += p[i + 256]; }
+        if (i < hi) a0 += p[i];
+        double v = a0 + a1;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((t & 63) == 0) sh[t >> 6] = v;
+        __syncthreads();
+        if (t == 0) out[(int64_t)k * S2 + j] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ double fval(const FCtl *C, double l1, double l2, double l3, double l4,
+                                      double l5) {
+    return l1 + C->beta / 2 * l2 + C->mu / 2 * l3 - l4 - l5;
+}
+
+// nonmonotone Armijo line search on f(alpha) (:839-890); mode 1 = accept
+__global__ void k_line_search(FCtl *C, const double *p2, int mode) {
+    if (!C->active) return;
+    double a[3], b[3];
+    reduce_parts<3>(p2, S2, a);
+    reduce_parts<3>(p2 + 3 * S2, S2, b);
+    if (threadIdx.x != 0) return;
+    const double l2t = a[0], l4t = a[1], x01 = a[2], l3t = b[0], l5t = b[1], r01 = b[2];
+    const double l2c = C->lam2, l4c = C->lam4, l3c = C->lam3, l5c = C->lam5;
+    auto at = [&](double al, double *o) {
+        const double bl = 1.0 - al;
+        o[0] = bl * bl * l2c + 2 * al * bl * x01 + al * al * l2t;
+        o[1] = bl * bl * l3c + 2 * al * bl * r01 + al * al * l3t;
+        o[2] = bl * l4c + al * l4t;
+        o[3] = bl * l5c + al * l5t;
+    };
+    double L[4] = {l2t, l3t, l4t, l5t};
+    double f = fval(C, C->lam1, L[0], L[1], L[2], L[3]);
+    double alpha = 1.0;
+    int cnt = 0;
+    const double c_arm = C->dd * C->tau * C->c * C->beta;
+    if (mode == 1) {
+        cnt = 6;
+    } else if (fabs(f) > fabs(C->C - alpha * c_arm)) {
+        while (fabs(f) > fabs(C->C - alpha * c_arm)) {
+            if (cnt == 5) {
+                C->need_fallback = 1;
+                C->active = 0;
+                return;
+            }
+            alpha *= C->gamma;
+            at(alpha, L);
+            f = fval(C, C->lam1, L[0], L[1], L[2], L[3]);
+            ++cnt;
+        }
+    }
+    C->alpha = alpha;
+    C->cnt = cnt;
+    C->backtracks += cnt > 5 ? 5 : cnt;
+    C->lam2 = L[0]; C->lam3 = L[1]; C->lam4 = L[2]; C->lam5 = L[3];
+    C->f = f;
+}
+
+__global__ void k_iter_end(FCtl *C, const double *p2, double *hist) {
+    if (!C->active) return;
+    double a[7];
+    reduce_parts<7>(p2, S2, a);
+    if (threadIdx.x != 0) return;
+    C->lam1 = a[0]; C->lam2 = a[1]; C->lam4 = a[2];
+    const double ss = a[3], sy = a[4];
+    C->nrmup2 = a[5];
+    C->dd = a[6];
+    C->f = fval(C, C->lam1, C->lam2, C->lam3, C->lam4, C->lam5);   // :973, :1028
+    const double Qp = C->Q;
+    C->Q = C->gam * Qp + 1;
+    C->C = (C->gam * Qp * C->C + C->f) / C->Q;                      // :1176-1177
+    C->relchg = sqrt(ss) / sqrt(C->nrmup2);                          // :1189-1197
+    const int k = C->total;
+    if (k < C->hist_cap) {
+        double *h = hist + (int64_t)k * HF;
+        h[0] = C->f; h[1] = C->lam1; h[2] = C->lam2; h[3] = C->lam3; h[4] = C->lam4;
+        h[5] = C->lam5; h[6] = C->tau; h[7] = C->alpha; h[8] = C->relchg; h[9] = C->mu;
+        h[10] = C->beta; h[11] = C->C; h[12] = C->cnt; h[13] = C->inner_loops - 1; h[14] = 0;
+    }
+    C->total = k + 1;
+    C->tau = fabs(ss / sy);                                          // BB step, :663
+    C->sigw_old = 0;
+    if (!isfinite(C->f)) { C->nonfinite = 1; C->active = 0; }
+    if (C->relchg < C->tol_inn && !C->fst) { C->inner_done = 1; C->active = 0; }
+    C->fst = 0;
+    if (C->total >= C->maxit) C->active = 0;
+}
+
+// first-iteration step length tau = |dd / (|Dd|^2 + mu/beta |Ad|^2)| (:583-625);
+// fallback = 1 also shrinks gam (:877)
+__global__ void k_tau_first(FCtl *C, const double *pD, int nbD, const double *pA, int nbA,
+                            int fallback) {
+    double a[1], b[1];
+    reduce_parts<1>(pD, nbD, a);
+    reduce_parts<1>(pA, nbA, b);
+    if (threadIdx.x != 0) return;
+    if (fallback) {
+        C->gam *= C->rate_gam;
+        C->fallbacks += 1;
+        C->need_fallback = 0;
+        C->active = 1;
+    }
+    C->tau = fabs(C->dd / (a[0] + C->muDbeta * b[0]));
+}
+
+// after the initial get_g / update (:1146-1153)
+__global__ void k_init_scalars(FCtl *C, const double *pN, int nbN, const double *pW, int nbW) {
+    double a[3], b[7];
+    reduce_parts<3>(pN, nbN, a);
+    reduce_parts<7>(pW, nbW, b);
+    if (threadIdx.x != 0) return;
+    C->lam3 = a[0]; C->lam5 = a[1];
+    C->lam1 = b[0]; C->lam2 = b[1]; C->lam4 = b[2]; C->dd = b[6];
+    C->f = fval(C, C->lam1, C->lam2, C->lam3, C->lam4, C->lam5);
+    C->Q = 1; C->C = C->f; C->gam = C->gam_opt;
+    C->total = 0; C->fst = 1; C->active = 1; C->inner_done = 0; C->need_fallback = 0;
+    C->nonfinite = 0; C->sigw_old = 0; C->cnt = 0; C->alpha = 1; C->backtracks = 0;
+    C->fallbacks = 0; C->inner_loops = 1;
+}
+
+__global__ void k_relout_scalar(FCtl *C, const double *p, int nb) {
+    double a[1];
+    reduce_parts<1>(p, nb, a);
+    if (threadIdx.x == 0) C->relout = sqrt(a[0]) / sqrt(C->nrmup2);
+}
+
+// update_mlp bookkeeping and continuation (:1034-1083, :1222-1249)
+__global__ void k_outer_scalars(FCtl *C, const double *p4, int nb4, const double *p5, int nb5) {
+    double a[1], b[1];
+    reduce_parts<1>(p4, nb4, a);
+    reduce_parts<1>(p5, nb5, b);
+    if (threadIdx.x != 0) return;
+    C->lam4 = a[0]; C->lam5 = b[0];
+    const double beta0 = C->beta;
+    C->beta *= C->rate_cnt;
+    C->mu *= C->rate_cnt;
+    if (C->beta > C->beta_max) C->beta = C->beta_max;
+    if (C->mu > C->mu_max) C->mu = C->mu_max;
+    C->mdb_old = C->muDbeta;
+    C->muDbeta = C->mu / C->beta;
+    C->ratio = beta0 / C->beta;
+    C->f = fval(C, C->lam1, C->lam2, C->lam3, C->lam4, C->lam5);
+    C->gam = C->gam_opt; C->Q = 1; C->C = C->f;
+    C->sigw_old = 1; C->fst = 1; C->inner_done = 0; C->active = 1; C->need_fallback = 0;
+    C->inner_loops += 1;
+}
+
+__global__ void k_set_dd(FCtl *C, const double *p, int nb) {
+    double a[1];
+    reduce_parts<1>(p, nb, a);
+    if (threadIdx.x == 0) C->dd = a[0];
+}
+
+// ------------------------------------------------------------------ host side
+int product_blocks(const SparseRows &S) {
+    if (S.blk_nnz) return (int)S.nblk;
+    const int L = S.lanes ? S.lanes : 32;
+    return (int)((S.rows * L + 255) / 256);
+}
+
+template <int B>
+struct Fused {
+    tv3_ctx *ctx;
+    FusedLayout *F;
+    const tv3_opts *o;
+    hipStream_t s;
+    Lay L;
+    int64_t n, m;
+    int nbS, nbN, nbNd, nbM;
+    Groups G;
+    DevBuf<float> pool;
+    float *U[2], *g[2], *Au[2], *d, *sx, *sy, *sz, *ox, *oy, *oz, *dts, *Atb, *rcd, *b, *delta,
+        *tmp;
+    DevBuf<float> tmpbuf;
+    DevBuf<double> parts, hist, st2;
+    double *pD, *pN, *pW, *pDd, *pAd, *pS4, *pM5, *pOD, *pRO;
+    DevBuf<FCtl> ctl;
+    PinnedBuf<FCtl> hctl;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_iters = 8;
+
+    void alloc() {
+        auto pad = [](int64_t k) { return (k + 63) / 64 * 64; };
+        const int nv = 2 + 2 + 1 + 3 + 3 + 1 + 1 + 1 + 1, mv = 2 + 2 + 1;
+        pool.alloc((size_t)nv * pad(n) + (size_t)mv * pad(m));
+        TV3_HIP(hipMemsetAsync(pool.p, 0, pool.bytes(), s));
+        float *p = pool.p;
+        float **nvp[] = {&U[0], &U[1], &g[0], &g[1], &d, &sx, &sy, &sz, &ox, &oy, &oz, &dts, &Atb,
+                         &rcd};
+        for (float **q : nvp) { *q = p; p += pad(n); }
+        float **mvp[] = {&Au[0], &Au[1], &b, &delta};
+        for (float **q : mvp) { *q = p; p += pad(m); }
+        tmpbuf.alloc(n > m ? n : m);
+        tmp = tmpbuf.p;
+        G = groups_of<B>(L);
+        nbS = (int)stencil_grid(G);
+        nbN = product_blocks(F->A);
+        nbM = 2048;
+        const size_t tot = 3 * (size_t)nbS + 3 * (size_t)nbN + 7 * (size_t)nbS + (size_t)nbS +
+                           (size_t)nbN + (size_t)nbS + nbM + nbM + nbM;
+        parts.alloc(tot);
+        double *q = parts.p;
+        pD = q; q += 3 * (size_t)nbS;
+        pN = q; q += 3 * (size_t)nbN;
+        pW = q; q += 7 * (size_t)nbS;
+        pDd = q; q += nbS;
+        pAd = q; q += nbN;
+        pS4 = q; q += nbS;
+        pM5 = q; q += nbM;
+        pOD = q; q += nbM;
+        pRO = q; q += nbM;
+        st2.alloc(16 * S2);
+        ctl.alloc(1);
+        hctl.alloc(1);
+        hist.alloc((size_t)(o->maxit > 0 ? o->maxit : 1) * HF);
+    }
+
+    void read_ctl() {
+        TV3_HIP(hipMemcpyAsync(hctl.p, ctl.p, sizeof(FCtl), hipMemcpyDeviceToHost, s));
+        TV3_HIP(hipStreamSynchronize(s));
+    }
+    void write_ctl() {
+        TV3_HIP(hipMemcpyAsync(ctl.p, hctl.p, sizeof(FCtl), hipMemcpyHostToDevice, s));
+    }
+
+    // one inner iteration from U[c] into U[1-c]; mode 1 = accept (fallback)
+    void iteration(int c, int mode) {
+        const int n_ = 1 - c;
+        const dim3 gS(nbS);
+        hipLaunchKernelGGL(k_descend<B>, gS, dim3(256), 0, s, L, G, U[c], d, sx, sy, sz, ox, oy,
+                           oz, U[n_], ctl.p, pD);
+        launch_product<float>(F->A, U[n_], EpiRes{Au[n_], b, delta, Au[c], ctl.p, pN, {0, 0, 0}}, s);
+        launch_product<float>(F->AT, Au[n_], EpiGrad{g[n_], Atb, ctl.p}, s);
+        hipLaunchKernelGGL((k_stage2<3, 3>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pD, nbS, pN, nbN,
+                           st2.p);
+        hipLaunchKernelGGL(k_line_search, dim3(1), dim3(1024), 0, s, ctl.p, st2.p, mode);
+        hipLaunchKernelGGL(k_apply_alpha, dim3(2048), dim3(256), 0, s, n, m, U[c], U[n_], g[c],
+                           g[n_], Au[c], Au[n_], ctl.p);
+        hipLaunchKernelGGL(k_update_w<B>, gS, dim3(256), 0, s, L, G, U[n_], U[c], sx, sy, sz,
+                           g[n_], dts, d, ctl.p, pW);
+        hipLaunchKernelGGL((k_stage2<7, 0>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pW, nbS, pW, 0,
+                           st2.p);
+        hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1024), 0, s, ctl.p, st2.p, hist.p);
+    }
+
+    void tau_first(int fallback) {
+        hipLaunchKernelGGL(k_dd_norm<B>, dim3(nbS), dim3(256), 0, s, L, G, d, pDd);
+        launch_product<float>(F->A, d, EpiNorm{ctl.p, pAd, {0}}, s);
+        hipLaunchKernelGGL(k_tau_first, dim3(1), dim3(1024), 0, s, ctl.p, pDd, nbS, pAd, nbN,
+                           fallback);
+    }
+
+    void capture(int c) {
+        hipGraph_t gr;
+        TV3_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < graph_iters; ++k) iteration((c + k) & 1, 0);
+        TV3_HIP(hipStreamEndCapture(s, &gr));
+        TV3_HIP(hipGraphInstantiate(&graph[c], gr, nullptr, nullptr, 0));
+        TV3_HIP(hipGraphDestroy(gr));
+    }
+
+    ~Fused() {
+        for (auto &gx : graph)
+            if (gx) (void)hipGraphExecDestroy(gx);
+    }
+};
+
+template <int B>
+void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_dev, float *U_out,
+               const float *Ut_dev, tv3_info *info, tv3_history *hist) {
+    Fused<B> S;
+    S.ctx = ctx; S.F = &FL; S.o = &o; S.s = ctx->stream;
+    hipStream_t s = S.s;
+    S.L.p = FL.dims[0]; S.L.q = FL.dims[1]; S.L.r = FL.dims[2];
+    S.L.nby = FL.nby; S.L.nbx = FL.nbx;
+    S.L.pq = (int64_t)S.L.p * S.L.q;
+    S.n = (int64_t)S.L.p * S.L.q * S.L.r;
+    S.m = FL.A.rows;
+    const int64_t n = S.n, m = S.m;
+    S.graph_iters = o.graph_iters > 0 ? o.graph_iters : 8;
+    S.alloc();
+
+    hipEvent_t ev0, ev1, evl0, evl1;
+    TV3_HIP(hipEventCreate(&ev0)); TV3_HIP(hipEventCreate(&ev1));
+    TV3_HIP(hipEventCreate(&evl0)); TV3_HIP(hipEventCreate(&evl1));
+    TV3_HIP(hipEventRecord(ev0, s));
+
+    // scaleb (:225-266) on a host copy; b into the internal ray order
+    std::vector<float> bh(m);
+    TV3_HIP(hipMemcpyAsync(bh.data(), b_dev, sizeof(float) * m, hipMemcpyDeviceToHost, s));
+    TV3_HIP(hipStreamSynchronize(s));
+    float bmin = bh[0], bmax = bmin, amin = std::fabs(bmin), amax = amin;
+    for (float v : bh) {
+        TV3_REQUIRE(std::isfinite(v), TV3_E_NONFINITE, "b holds NaN or Inf");
+        const float a = std::fabs(v);
+        if (a < amin) { bmin = v; amin = a; }
+        if (a > amax) { bmax = v; amax = a; }
+    }
+    const float dif = std::fabs(bmax - bmin);
+    float scl = 1.f;
+    if (dif < 0.5f) scl = 0.5f / dif;
+    else if (dif > 1.5f) scl = 1.5f / dif;
+    permute_gather(b_dev, FL.r_int2nat.p, S.tmp, m, s);
+    scale<float>(m, S.b, scl, S.tmp, s);
+
+    FCtl &H = *S.hctl.p;
+    std::memset(&H, 0, sizeof(FCtl));
+    H.mu = std::min(o.mu0, o.mu); H.beta = std::min(o.beta0, o.beta); H.muDbeta = H.mu / H.beta;
+    H.c = o.c; H.gamma = o.gamma; H.rate_gam = o.rate_gam; H.tol_inn = o.tol_inn;
+    H.gam_opt = o.gam; H.gam = o.gam; H.rate_cnt = o.rate_cnt; H.mu_max = o.mu; H.beta_max = o.beta;
+    H.maxit = o.maxit; H.nonneg = o.nonneg; H.iso = o.tv_mode == TV3_TV_ISO;
+    H.hist_cap = o.maxit; H.active = 1; H.alpha = 1;
+    S.write_ctl();
+
+    // initialisation (:1128-1153)
+    launch_product<float>(FL.AT, S.b, EpiStore<float>{S.Atb}, s);
+    if (o.warm_start) {
+        permute_gather(U_out, FL.v_int2nat.p, S.tmp, n, s);
+        scale<float>(n, S.U[0], scl, S.tmp, s);
+    } else {
+        TV3_HIP(hipMemcpyAsync(S.U[0], S.Atb, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    }
+    TV3_HIP(hipMemcpyAsync(S.rcd, S.U[0], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    launch_product<float>(FL.A, S.U[0], EpiRes{S.Au[0], S.b, S.delta, S.Au[0], S.ctl.p, S.pN, {0, 0, 0}}, s);
+    launch_product<float>(FL.AT, S.Au[0], EpiGrad{S.g[0], S.Atb, S.ctl.p}, s);
+    hipLaunchKernelGGL(k_update_w<B>, dim3(S.nbS), dim3(256), 0, s, S.L, S.G, S.U[0], S.U[0], S.sx,
+                       S.sy, S.sz, S.g[0], S.dts, S.d, S.ctl.p, S.pW);
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pN, S.nbN, S.pW, S.nbS);
+    TV3_HIP(hipGetLastError());
+
+    S.capture(0);
+    S.capture(1);
+    TV3_HIP(hipEventRecord(evl0, s));
+
+    // U[c] holds the accepted iterate; every completed iteration flips c
+    int c = 0, outer = 0;
+    S.read_ctl();
+    while (outer < o.maxcnt && H.total < o.maxit) {
+        int prev = H.total;
+        S.tau_first(0);              // first iteration of the inner loop
+        S.iteration(c, 0);
+        S.read_ctl();
+        for (;;) {
+            const int done = H.total - prev;
+            if (done & 1) c ^= 1;
+            prev = H.total;
+            if (H.need_fallback) {   // cnt == 5: restart from U[c] (:876-883)
+                S.tau_first(1);
+                S.iteration(c, 1);
+                S.read_ctl();
+                continue;
+            }
+            if (H.inner_done || H.nonfinite || H.total >= o.maxit) break;
+            TV3_HIP(hipGraphLaunch(S.graph[c], s));
+            S.read_ctl();
+        }
+        if (H.nonfinite) break;
+        ++outer;
+        hipLaunchKernelGGL(k_relout, dim3(S.nbM), dim3(256), 0, s, n, S.U[c], S.rcd, S.pRO);
+        hipLaunchKernelGGL(k_relout_scalar, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pRO, S.nbM);
+        S.read_ctl();
+        if (H.relout < o.tol) break;
+        hipLaunchKernelGGL(k_mlp_sigma<B>, dim3(S.nbS), dim3(256), 0, s, S.L, S.G, S.U[c], S.sx,
+                           S.sy, S.sz, S.ox, S.oy, S.oz, S.ctl.p, S.pS4);
+        hipLaunchKernelGGL(k_mlp_delta, dim3(S.nbM), dim3(256), 0, s, m, S.Au[c], S.b, S.delta,
+                           S.ctl.p, S.pM5);
+        hipLaunchKernelGGL(k_outer_scalars, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pS4, S.nbS, S.pM5,
+                           S.nbM);
+        hipLaunchKernelGGL(k_outer_d, dim3(S.nbM), dim3(256), 0, s, n, S.d, S.g[c], S.dts, S.ctl.p,
+                           S.pOD);
+        hipLaunchKernelGGL(k_set_dd, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pOD, S.nbM);
+        S.read_ctl();
+    }
+    TV3_HIP(hipEventRecord(evl1, s));
+
+    // U (natural order) = U_int / scl (:1275-1277)
+    divide<float>(n, S.tmp, scl, S.U[c], s);
+    permute_scatter(S.tmp, FL.v_int2nat.p, U_out, n, s);
+    TV3_HIP(hipEventRecord(ev1, s));
+    S.read_ctl();
+    float ms = 0, msl = 0;
+    TV3_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    TV3_HIP(hipEventElapsedTime(&msl, evl0, evl1));
+    info->total_iters = H.total;
+    info->outer_iters = outer;
+    info->rel_chg = H.relout;
+    info->secs = ms / 1000.0;
+    info->loop_secs = msl / 1000.0;
+    info->backtracks = H.backtracks;
+    info->fallbacks = H.fallbacks;
+    // per completed iteration 1 A + 1 A^T; per inner loop one A d; per
+    // fallback the rejected trial (A, A^T) and one A d; setup A^T b, A U, A^T
+    info->n_matvec_N = 1 + (int64_t)H.total + H.inner_loops + 2 * (int64_t)H.fallbacks;
+    info->n_matvec_T = 2 + (int64_t)H.total + H.fallbacks;
+    info->status = H.nonfinite ? TV3_E_NONFINITE : TV3_OK;
+    info->rel_error = 0;
+    if (Ut_dev) {
+        auto e = diff_norm<float>(ctx, n, Ut_dev, U_out, s);
+        info->rel_error = std::sqrt(e[0]) / std::sqrt(e[1]);
+    }
+    if (hist && hist->cap > 0) {
+        const int cnt = std::min(H.total, hist->cap);
+        std::vector<double> hh((size_t)std::max(cnt, 1) * HF);
+        if (cnt > 0)
+            TV3_HIP(hipMemcpy(hh.data(), S.hist.p, sizeof(double) * HF * cnt, hipMemcpyDeviceToHost));
+        for (int k = 0; k < cnt; ++k) {
+            const double *h = &hh[(size_t)k * HF];
+            hist->f[k] = h[0]; hist->lam1[k] = h[1]; hist->lam2[k] = h[2]; hist->lam3[k] = h[3];
+            hist->lam4[k] = h[4]; hist->lam5[k] = h[5]; hist->tau[k] = h[6]; hist->alpha[k] = h[7];
+            hist->relchg[k] = h[8]; hist->mu[k] = h[9]; hist->beta[k] = h[10]; hist->C[k] = h[11];
+            hist->cnt[k] = (int32_t)h[12]; hist->outer[k] = (int32_t)h[13];
+        }
+        hist->count = cnt;
+    }
+    (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1);
+    (void)hipEventDestroy(evl0); (void)hipEventDestroy(evl1);
+}
+
+}  // namespace
+
+void solve_fused(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims[3],
+                 const float *b_dev, float *U_dev, const float *Ut_dev, tv3_info *info,
+                 tv3_history *hist) {
+    FusedLayout &FL = fused_layout(op, dims);
+    if (FL.B == 4) run_fused<4>(ctx, FL, o, b_dev, U_dev, Ut_dev, info, hist);
+    else run_fused<1>(ctx, FL, o, b_dev, U_dev, Ut_dev, info, hist);
+}
+
 }  // namespace tv3

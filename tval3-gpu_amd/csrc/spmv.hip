@@ -11,7 +11,7 @@
 // consecutive memory: the col/val streams are fully coalesced whatever L is.
 // x is gathered (it stays in L2 / MALL); the group sum is a butterfly over L
 // lanes.  L is chosen from the mean row length or by the autotuner.
-#include "tv3_internal.hpp"
+#include "spmv_stream.hpp"
 
 namespace tv3 {
 
@@ -35,7 +35,8 @@ __global__ __launch_bounds__(256) void k_csr_f32(int64_t rows, const int32_t *__
                                                  const V *__restrict__ val,
                                                  const float *__restrict__ x,
                                                  float *__restrict__ y) {
-    const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) / L;
+    const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t row = (bid * 256 + threadIdx.x) / L;
     const int lane = threadIdx.x & (L - 1);
     if (row >= rows) return;          // whole groups leave together (L divides 256)
     const int beg = ptr[row], end = ptr[row + 1];
@@ -118,8 +119,39 @@ static void launch_f64(const SparseRows &S, const double *x, double *y, hipStrea
     TV3_HIP(hipGetLastError());
 }
 
+void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz) {
+    std::vector<int32_t> ptr(S.rows + 1);
+    TV3_HIP(hipMemcpyAsync(ptr.data(), S.ptr.p, sizeof(int32_t) * (S.rows + 1),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    TV3_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> blk;
+    blk.reserve(S.nnz / (blk_nnz / 2) + S.rows / 256 + 2);
+    int64_t start = 0;
+    blk.push_back(0);
+    for (int64_t r = 0; r < S.rows; ++r) {
+        // close the current block before row r if adding it would overflow
+        if (r > start && ((int64_t)ptr[r + 1] - ptr[start] > blk_nnz || r - start >= 256)) {
+            blk.push_back((int32_t)r);
+            start = r;
+        }
+    }
+    blk.push_back((int32_t)S.rows);
+    S.nblk = (int64_t)blk.size() - 1;
+    S.blk.alloc(blk.size());
+    TV3_HIP(hipMemcpyAsync(S.blk.p, blk.data(), sizeof(int32_t) * blk.size(),
+                           hipMemcpyHostToDevice, ctx->stream));
+    TV3_HIP(hipStreamSynchronize(ctx->stream));
+    S.blk_nnz = blk_nnz;
+}
+
 void spmv_csr(const SparseRows &S, const float *x, float *y, hipStream_t s, int lanes,
               bool acc64) {
+    if (!lanes && S.blk_nnz) {
+        if (acc64) launch_stream_any<double>(S, x, EpiStore<double>{y}, s);
+        else launch_stream_any<float>(S, x, EpiStore<float>{y}, s);
+        TV3_HIP(hipGetLastError());
+        return;
+    }
     int L = lanes ? lanes : (S.lanes ? S.lanes : default_lanes(S.rows, S.nnz));
     switch (S.vtype) {
     case TV3_F32:
@@ -259,29 +291,91 @@ __global__ void k_scatter(int64_t rows, const int32_t *ptr, const int32_t *ind,
     }
 }
 
-// Sort each column's entries by row (insertion sort per thread): makes the
-// transpose independent of the atomic arrival order.
+// Sort each column's entries by row: makes the transpose independent of the
+// atomic arrival order.  Entries are packed as (row << 32 | value bits)
+// keys.  Columns of up to 64 entries: one thread, insertion sort in private
+// memory; longer columns (rays start inside the volume, so a transducer's
+// voxel can hold thousands of entries): one workgroup each, bitonic sort in
+// LDS.
 template <int VB>
-__global__ void k_sort_cols(int64_t cols, const int32_t *cptr, int32_t *row, uint8_t *val) {
+__device__ __forceinline__ uint64_t pack_key(const int32_t *row, const uint8_t *val, int64_t k) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int t = 0; t < VB; ++t) v |= (uint32_t)val[k * VB + t] << (8 * t);
+    return ((uint64_t)(uint32_t)row[k] << 32) | v;
+}
+template <int VB>
+__device__ __forceinline__ void unpack_key(uint64_t key, int32_t *row, uint8_t *val, int64_t k) {
+    row[k] = (int32_t)(key >> 32);
+    const uint32_t v = (uint32_t)key;
+#pragma unroll
+    for (int t = 0; t < VB; ++t) val[k * VB + t] = (uint8_t)(v >> (8 * t));
+}
+
+template <int VB>
+__global__ void k_sort_cols(int64_t cols, const int32_t *cptr, int32_t *row, uint8_t *val,
+                            int32_t *long_list, int32_t *long_count) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cols) return;
-    int b = cptr[j], e = cptr[j + 1];
-    for (int a = b + 1; a < e; ++a) {
-        int32_t r = row[a];
-        uint8_t v[VB];
-#pragma unroll
-        for (int t = 0; t < VB; ++t) v[t] = val[(int64_t)a * VB + t];
-        int c = a - 1;
-        while (c >= b && row[c] > r) {
-            row[c + 1] = row[c];
-#pragma unroll
-            for (int t = 0; t < VB; ++t) val[(int64_t)(c + 1) * VB + t] = val[(int64_t)c * VB + t];
-            --c;
-        }
-        row[c + 1] = r;
-#pragma unroll
-        for (int t = 0; t < VB; ++t) val[(int64_t)(c + 1) * VB + t] = v[t];
+    const int b = cptr[j], e = cptr[j + 1], len = e - b;
+    if (len < 2) return;
+    if (len > 64) {
+        long_list[atomicAdd(long_count, 1)] = (int32_t)j;
+        return;
     }
+    uint64_t key[64];
+    for (int a = 0; a < len; ++a) key[a] = pack_key<VB>(row, val, b + a);
+    for (int a = 1; a < len; ++a) {
+        const uint64_t k = key[a];
+        int c = a - 1;
+        while (c >= 0 && key[c] > k) { key[c + 1] = key[c]; --c; }
+        key[c + 1] = k;
+    }
+    for (int a = 0; a < len; ++a) unpack_key<VB>(key[a], row, val, b + a);
+}
+
+// 8-byte values (FP64 operators): in-place insertion sort by row.
+__global__ void k_sort_cols_f64(int64_t cols, const int32_t *cptr, int32_t *row, double *val) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cols) return;
+    const int b = cptr[j], e = cptr[j + 1];
+    for (int a = b + 1; a < e; ++a) {
+        const int32_t r = row[a];
+        const double v = val[a];
+        int c = a - 1;
+        while (c >= b && row[c] > r) { row[c + 1] = row[c]; val[c + 1] = val[c]; --c; }
+        row[c + 1] = r;
+        val[c + 1] = v;
+    }
+}
+
+constexpr int kLongSortCap = 8192;   // LDS keys per workgroup (64 KiB)
+
+template <int VB>
+__global__ __launch_bounds__(256) void k_sort_long(const int32_t *list, const int32_t *cptr,
+                                                   int32_t *row, uint8_t *val) {
+    __shared__ uint64_t key[kLongSortCap];
+    const int j = list[blockIdx.x];
+    const int b = cptr[j], len = cptr[j + 1] - b;
+    int P = 1;
+    while (P < len) P <<= 1;
+    for (int a = threadIdx.x; a < P; a += 256)
+        key[a] = a < len ? pack_key<VB>(row, val, b + a) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int i = threadIdx.x; i < P; i += 256) {
+                const int l = i ^ jj;
+                if (l > i) {
+                    const uint64_t x = key[i], y = key[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) { key[i] = y; key[l] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int a = threadIdx.x; a < len; a += 256) unpack_key<VB>(key[a], row, val, b + a);
 }
 
 void build_transpose(tv3_ctx *ctx, const SparseRows &A, SparseRows &AT) {
@@ -302,27 +396,49 @@ void build_transpose(tv3_ctx *ctx, const SparseRows &A, SparseRows &AT) {
     exclusive_scan_i32(ctx, cnt.p, AT.ptr.p, AT.rows);
     TV3_HIP(hipMemcpyAsync(cnt.p, AT.ptr.p, sizeof(int32_t) * AT.rows, hipMemcpyDeviceToDevice, s));
     unsigned gb = (unsigned)((A.rows + 255) / 256), gc = (unsigned)((AT.rows + 255) / 256);
-    if (A.nnz > 0) {
+    if (A.nnz > 0 && vb == 8) {
+        hipLaunchKernelGGL(k_scatter<8>, dim3(gb), dim3(256), 0, s, A.rows, A.ptr.p, A.ind.p,
+                           A.val.p, cnt.p, AT.ind.p, AT.val.p);
+        hipLaunchKernelGGL(k_sort_cols_f64, dim3(gc), dim3(256), 0, s, AT.rows, AT.ptr.p, AT.ind.p,
+                           reinterpret_cast<double *>(AT.val.p));
+    } else if (A.nnz > 0) {
+        DevBuf<int32_t> list(AT.rows > 0 ? AT.rows : 1), lcount(1);
+        TV3_HIP(hipMemsetAsync(lcount.p, 0, sizeof(int32_t), s));
         if (vb == 2) {
             hipLaunchKernelGGL(k_scatter<2>, dim3(gb), dim3(256), 0, s, A.rows, A.ptr.p, A.ind.p,
                                A.val.p, cnt.p, AT.ind.p, AT.val.p);
             hipLaunchKernelGGL(k_sort_cols<2>, dim3(gc), dim3(256), 0, s, AT.rows, AT.ptr.p,
-                               AT.ind.p, AT.val.p);
-        } else if (vb == 4) {
+                               AT.ind.p, AT.val.p, list.p, lcount.p);
+        } else {
             hipLaunchKernelGGL(k_scatter<4>, dim3(gb), dim3(256), 0, s, A.rows, A.ptr.p, A.ind.p,
                                A.val.p, cnt.p, AT.ind.p, AT.val.p);
             hipLaunchKernelGGL(k_sort_cols<4>, dim3(gc), dim3(256), 0, s, AT.rows, AT.ptr.p,
-                               AT.ind.p, AT.val.p);
-        } else {
-            hipLaunchKernelGGL(k_scatter<8>, dim3(gb), dim3(256), 0, s, A.rows, A.ptr.p, A.ind.p,
-                               A.val.p, cnt.p, AT.ind.p, AT.val.p);
-            hipLaunchKernelGGL(k_sort_cols<8>, dim3(gc), dim3(256), 0, s, AT.rows, AT.ptr.p,
-                               AT.ind.p, AT.val.p);
+                               AT.ind.p, AT.val.p, list.p, lcount.p);
+        }
+        int32_t nlong = 0;
+        TV3_HIP(hipMemcpyAsync(&nlong, lcount.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        TV3_HIP(hipStreamSynchronize(s));
+        if (nlong > 0) {
+            // every long column must fit the LDS sort
+            std::vector<int32_t> lst(nlong), cp(AT.rows + 1);
+            TV3_HIP(hipMemcpy(lst.data(), list.p, sizeof(int32_t) * nlong, hipMemcpyDeviceToHost));
+            TV3_HIP(hipMemcpy(cp.data(), AT.ptr.p, sizeof(int32_t) * (AT.rows + 1),
+                              hipMemcpyDeviceToHost));
+            for (int32_t j : lst)
+                TV3_REQUIRE(cp[j + 1] - cp[j] <= kLongSortCap, TV3_E_SHAPE,
+                            "a column holds more than 8192 entries (device transpose limit)");
+            if (vb == 2)
+                hipLaunchKernelGGL(k_sort_long<2>, dim3(nlong), dim3(256), 0, s, list.p, AT.ptr.p,
+                                   AT.ind.p, AT.val.p);
+            else
+                hipLaunchKernelGGL(k_sort_long<4>, dim3(nlong), dim3(256), 0, s, list.p, AT.ptr.p,
+                                   AT.ind.p, AT.val.p);
         }
     }
     TV3_HIP(hipGetLastError());
     TV3_HIP(hipStreamSynchronize(s));
     AT.lanes = default_lanes(AT.rows, AT.nnz);
+    build_row_blocks(ctx, AT, 2048);
 }
 
 }  // namespace tv3

@@ -276,6 +276,7 @@ tv3_status tv3_op_csr_create(tv3_ctx *ctx, int64_t m, int64_t n, int64_t nnz,
         upload(A.ind, col, nnz, s);
         upload_bytes(A.val, val, nnz * dtype_size(vt), s);
         A.lanes = default_lanes(m, nnz);
+        build_row_blocks(ctx, A, 2048);
         if (has_csc) {
             SparseRows &T = op->AT;
             T.rows = n; T.cols = m; T.nnz = nnz; T.vtype = vt;
@@ -283,6 +284,7 @@ tv3_status tv3_op_csr_create(tv3_ctx *ctx, int64_t m, int64_t n, int64_t nnz,
             upload(T.ind, crow, nnz, s);
             upload_bytes(T.val, cval, nnz * dtype_size(vt), s);
             T.lanes = default_lanes(n, nnz);
+            build_row_blocks(ctx, T, 2048);
         } else {
             build_transpose(ctx, A, op->AT);
         }
@@ -344,6 +346,8 @@ static void fill_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_weight_mode m
         upload(G.ecoord, ec.data(), ec.size(), s);
         upload(G.rcoord, rc.data(), rc.size(), s);
         TV3_HIP(hipStreamSynchronize(s));
+        G.epos_h.assign(ec.begin(), ec.end());
+        G.rpos_h.assign(rc.begin(), rc.end());
     } else {
         TV3_REQUIRE(g->pos_emitters && g->pos_receivers, TV3_E_ARG,
                     "Siddon mode needs transducer positions");
@@ -355,6 +359,8 @@ static void fill_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_weight_mode m
             TV3_REQUIRE(std::isfinite(g->pos_receivers[k]), TV3_E_NONFINITE, "receiver position");
         upload(G.epos, g->pos_emitters, 3 * ne, s);
         upload(G.rpos, g->pos_receivers, 3 * nr, s);
+        G.epos_h.assign(g->pos_emitters, g->pos_emitters + 3 * ne);
+        G.rpos_h.assign(g->pos_receivers, g->pos_receivers + 3 * nr);
         for (int a = 0; a < 3; ++a) { G.origin[a] = g->origin[a]; G.voxel[a] = g->voxel[a]; }
         TV3_HIP(hipStreamSynchronize(s));
     }
@@ -382,6 +388,7 @@ static void fill_geometry(tv3_ctx *ctx, const tv3_geometry *g, tv3_weight_mode m
     TV3_REQUIRE(G.num_paths > 0, TV3_E_SHAPE, "use_path masks every ray");
     upload(G.path, path.data(), path.size(), s);
     TV3_HIP(hipStreamSynchronize(s));
+    G.path_h = std::move(path);
     op->m = G.num_paths;
     op->n = (int64_t)g->rv_x * g->rv_y * g->rv_z;
 }
@@ -434,6 +441,7 @@ tv3_status tv3_op_materialize(tv3_op *dyn, tv3_dtype vt, tv3_op **out) {
         op->m = dyn->m; op->n = dyn->n;
         dynamic_materialize(dyn, vt, op->A);
         build_transpose(ctx, op->A, op->AT);
+        op->ray_order = geometry_ray_order(dyn->geo);
         *out = op.release();
     });
 }
@@ -489,7 +497,7 @@ tv3_status tv3_matvec(tv3_op *op, tv3_trans trans, const float *x, float *y, voi
         TV3_REQUIRE(x && y, TV3_E_ARG, "x or y is NULL");
         TV3_REQUIRE(trans == TV3_N || trans == TV3_T, TV3_E_ARG, "bad trans");
         set_device(op->ctx);
-        hipStream_t s = stream ? (hipStream_t)stream : op->ctx->stream;
+        hipStream_t s = (hipStream_t)stream;   // NULL: the null stream
         op_apply(op, trans == TV3_T, x, y, s);
         TV3_HIP(hipGetLastError());
     });
@@ -501,7 +509,7 @@ tv3_status tv3_matvec_f64(tv3_op *op, tv3_trans trans, const double *x, double *
         TV3_REQUIRE(x && y, TV3_E_ARG, "x or y is NULL");
         TV3_REQUIRE(trans == TV3_N || trans == TV3_T, TV3_E_ARG, "bad trans");
         set_device(op->ctx);
-        hipStream_t s = stream ? (hipStream_t)stream : op->ctx->stream;
+        hipStream_t s = (hipStream_t)stream;   // NULL: the null stream
         op_apply(op, trans == TV3_T, x, y, s);
         TV3_HIP(hipGetLastError());
     });
@@ -553,7 +561,7 @@ tv3_status tv3_D(tv3_ctx *ctx, const int32_t dims[3], const float *U, float *Ux,
         dims_n(dims);
         TV3_REQUIRE(U && Ux && Uy && Uz, TV3_E_ARG, "NULL argument");
         set_device(ctx);
-        launch_D<float>(dims, U, Ux, Uy, Uz, stream ? (hipStream_t)stream : ctx->stream);
+        launch_D<float>(dims, U, Ux, Uy, Uz, (hipStream_t)stream);
         TV3_HIP(hipGetLastError());
     });
 }
@@ -565,7 +573,7 @@ tv3_status tv3_Dt(tv3_ctx *ctx, const int32_t dims[3], const float *X, const flo
         dims_n(dims);
         TV3_REQUIRE(X && Y && Z && res, TV3_E_ARG, "NULL argument");
         set_device(ctx);
-        launch_Dt<float>(dims, X, Y, Z, res, stream ? (hipStream_t)stream : ctx->stream);
+        launch_Dt<float>(dims, X, Y, Z, res, (hipStream_t)stream);
         TV3_HIP(hipGetLastError());
     });
 }
@@ -610,16 +618,27 @@ tv3_status tv3_op_autotune(tv3_op *op, int32_t *lanes_N, int32_t *lanes_T) {
         TV3_HIP(hipMemsetAsync(x.p, 0, x.bytes(), op->ctx->stream));
         DevBuf<float> xm(op->m);
         TV3_HIP(hipMemsetAsync(xm.p, 0, xm.bytes(), op->ctx->stream));
-        const int cand[] = {2, 4, 8, 16, 32, 64};
+        // candidates: row-blocked stream kernel with 1024/2048/4096-entry
+        // blocks (reported as -NB) and the row-group kernel with L lanes
         auto best = [&](SparseRows &S, const float *xx) {
-            int bl = S.lanes;
+            int choice = 0;
             float bt = 1e30f;
-            for (int L : cand) {
-                float t = time_spmv(op->ctx, S, L, xx, y.p);
-                if (t < bt) { bt = t; bl = L; }
+            for (int nb : {1024, 2048, 4096}) {
+                build_row_blocks(op->ctx, S, nb);
+                float t = time_spmv(op->ctx, S, 0, xx, y.p);
+                if (t < bt) { bt = t; choice = -nb; }
             }
-            S.lanes = bl;
-            return bl;
+            for (int L : {4, 8, 16, 32, 64}) {
+                float t = time_spmv(op->ctx, S, L, xx, y.p);
+                if (t < bt) { bt = t; choice = L; }
+            }
+            if (choice < 0) {
+                build_row_blocks(op->ctx, S, -choice);
+            } else {
+                S.lanes = choice;
+                S.blk_nnz = 0;
+            }
+            return choice;
         };
         int ln = op->kind == OpKind::CSR ? best(op->A, x.p) : 0;
         int lt = best(op->AT, xm.p);
@@ -630,11 +649,19 @@ tv3_status tv3_op_autotune(tv3_op *op, int32_t *lanes_N, int32_t *lanes_T) {
 
 tv3_status tv3_op_set_lanes(tv3_op *op, int32_t lanes_N, int32_t lanes_T) {
     if (!op) return TV3_E_ARG;
-    auto ok = [](int L) { return L == 0 || (L >= 2 && L <= 64 && (L & (L - 1)) == 0); };
+    auto ok = [](int L) {
+        return L == 0 || (L >= 2 && L <= 64 && (L & (L - 1)) == 0) || L == -1024 || L == -2048 ||
+               L == -4096;
+    };
     if (!ok(lanes_N) || !ok(lanes_T)) return TV3_E_ARG;
-    if (lanes_N) op->A.lanes = lanes_N;
-    if (lanes_T) op->AT.lanes = lanes_T;
-    return TV3_OK;
+    return guard(op->ctx, [&] {
+        auto apply = [&](SparseRows &S, int L) {
+            if (L > 0) { S.lanes = L; S.blk_nnz = 0; }
+            else if (L < 0) build_row_blocks(op->ctx, S, -L);
+        };
+        if (op->A.rows) apply(op->A, lanes_N);
+        if (op->AT.rows) apply(op->AT, lanes_T);
+    });
 }
 
 }  // extern "C"

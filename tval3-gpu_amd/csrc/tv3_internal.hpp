@@ -108,8 +108,16 @@ struct SparseRows {
     tv3_dtype vtype = TV3_F32;
     DevBuf<int32_t> ptr, ind;
     DevBuf<uint8_t> val;   // nnz * dtype_size(vtype)
-    int lanes = 0;         // lanes per row of the SpMV kernel (power of two <= 64)
+    int lanes = 0;         // row-group kernel: lanes per row (power of two <= 64)
+    // row-blocked ("stream") kernel: blocks of consecutive rows holding at most
+    // blk_nnz entries (or one longer row); blk[b] = first row of block b
+    int blk_nnz = 0;       // 0: use the row-group kernel
+    int64_t nblk = 0;
+    DevBuf<int32_t> blk;
 };
+
+// Partition rows into blocks for the stream kernel (host greedy over rowptr).
+void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz);
 
 struct DynGeom {
     int32_t E = 0, R = 0, mps = 0, rv_x = 0, rv_y = 0, rv_z = 0;
@@ -122,6 +130,25 @@ struct DynGeom {
     DevBuf<int32_t> rcoord;    // 3 * mps*R
     DevBuf<float> epos, rpos;  // 3 * mps*E / 3 * mps*R world coordinates, Siddon
     float origin[3] = {0, 0, 0}, voxel[3] = {1, 1, 1};
+    // host copies used to order rays for cache locality
+    std::vector<float> epos_h, rpos_h;   // 3 per transducer (voxel coords for Bresenham)
+    std::vector<int32_t> path_h;         // 2 per ray
+};
+
+// Spatial order of rays (internal -> natural index) for cache locality:
+// transducers sorted by polar band and serpentine azimuth, rays sorted by
+// (measurement position, emitter rank, receiver rank).
+std::vector<int32_t> geometry_ray_order(const DynGeom &g);
+
+// The fused solver's internal layout of a sparse operator: voxels in 4x4x4
+// bricks (when every dimension is a multiple of 4, else natural order), rays
+// in a locality order; A and its CSC in that numbering.
+struct FusedLayout {
+    int32_t dims[3] = {0, 0, 0};
+    int B = 1, nby = 0, nbx = 0;
+    SparseRows A, AT;
+    DevBuf<int32_t> v_nat2int, v_int2nat;   // voxel maps (n)
+    DevBuf<int32_t> r_int2nat;              // ray map (m): internal ray -> natural ray
 };
 
 }  // namespace tv3
@@ -149,6 +176,10 @@ struct tv3_op {
     int64_t lda = 0;
     // DYNAMIC / HYBRID
     tv3::DynGeom geo;
+    // CSR built from a geometry: the locality order of its rays (may be empty)
+    std::vector<int32_t> ray_order;
+    // fused solver layout, built on first use for a given dims
+    std::unique_ptr<tv3::FusedLayout> fused;
 };
 
 namespace tv3 {
@@ -184,6 +215,11 @@ void op_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,
               bool acc64 = false);
 void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
               bool acc64 = true);
+
+// internal layout (layout.hip)
+FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]);
+void permute_gather(const float *in, const int32_t *map, float *out, int64_t n, hipStream_t s);
+void permute_scatter(const float *in, const int32_t *map, float *out, int64_t n, hipStream_t s);
 
 // solvers
 template <typename T>
