@@ -10,7 +10,7 @@
 #include <cmath>
 #include <numeric>
 
-#include "tv3_internal.hpp"
+#include "spmv_stream.hpp"
 
 namespace tv3 {
 
@@ -118,6 +118,45 @@ void permute_scatter(const float *in, const int32_t *map, float *out, int64_t n,
                            map, out, n);
 }
 
+// Autotuner: time the product of S over the candidate launch shapes (row
+// blocks of 1024/2048/4096 entries, row groups of 8..64 lanes) and keep the
+// fastest.  Runs once per operator and volume shape.
+static void tune_shape(tv3_ctx *ctx, SparseRows &S, int64_t xlen) {
+    hipStream_t s = ctx->stream;
+    DevBuf<float> x(xlen > 0 ? xlen : 1), y(S.rows > 0 ? S.rows : 1);
+    TV3_HIP(hipMemsetAsync(x.p, 0, x.bytes(), s));
+    hipEvent_t e0, e1;
+    TV3_HIP(hipEventCreate(&e0));
+    TV3_HIP(hipEventCreate(&e1));
+    auto timed = [&]() {
+        launch_product<float>(S, x.p, EpiStore<float>{y.p}, s);
+        TV3_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 4; ++r) launch_product<float>(S, x.p, EpiStore<float>{y.p}, s);
+        TV3_HIP(hipEventRecord(e1, s));
+        TV3_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        TV3_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+    };
+    float best = 1e30f;
+    int choice = 0;
+    for (int nb : {1024, 2048, 4096}) {
+        build_row_blocks(ctx, S, nb);
+        const float t = timed();
+        if (t < best) { best = t; choice = -nb; }
+    }
+    for (int L : {8, 16, 32, 64}) {
+        S.blk_nnz = 0;
+        S.lanes = L;
+        const float t = timed();
+        if (t < best) { best = t; choice = L; }
+    }
+    if (choice < 0) build_row_blocks(ctx, S, -choice);
+    else { S.blk_nnz = 0; S.lanes = choice; }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
 FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
     TV3_REQUIRE(op->kind == OpKind::CSR, TV3_E_UNSUPPORTED,
                 "the fused solver needs a sparse operator (materialize a dynamic operator first)");
@@ -166,11 +205,8 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
         throw tv3_error(TV3_E_FORMAT, "the fused solver needs F32, F16 or BF16 weights");
     TV3_HIP(hipGetLastError());
     build_transpose(ctx, N, L->AT);
-    // launch shapes measured on 128^3 ray matrices: rows of ~140 entries on
-    // 32-lane groups, voxel rows (~17 entries) on 1024-entry blocks
-    N.lanes = 32;
-    N.blk_nnz = 0;
-    build_row_blocks(ctx, L->AT, 1024);
+    tune_shape(ctx, N, n);
+    tune_shape(ctx, L->AT, m);
     TV3_HIP(hipStreamSynchronize(s));
     op->fused = std::move(L);
     return *op->fused;

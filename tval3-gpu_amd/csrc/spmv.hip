@@ -137,8 +137,13 @@ void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz) {
     }
     blk.push_back((int32_t)S.rows);
     S.nblk = (int64_t)blk.size() - 1;
+    std::vector<int32_t> bp(blk.size());
+    for (size_t b = 0; b < blk.size(); ++b) bp[b] = ptr[blk[b]];
     S.blk.alloc(blk.size());
+    S.bptr.alloc(blk.size());
     TV3_HIP(hipMemcpyAsync(S.blk.p, blk.data(), sizeof(int32_t) * blk.size(),
+                           hipMemcpyHostToDevice, ctx->stream));
+    TV3_HIP(hipMemcpyAsync(S.bptr.p, bp.data(), sizeof(int32_t) * bp.size(),
                            hipMemcpyHostToDevice, ctx->stream));
     TV3_HIP(hipStreamSynchronize(ctx->stream));
     S.blk_nnz = blk_nnz;

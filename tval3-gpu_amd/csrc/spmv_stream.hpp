@@ -47,6 +47,7 @@ struct EpiStore {
 
 template <int NB, typename V, typename Acc, typename Epi>
 __global__ __launch_bounds__(256) void k_csr_stream(const int32_t *__restrict__ blk,
+                                                    const int32_t *__restrict__ bptr,
                                                     const int32_t *__restrict__ ptr,
                                                     const int32_t *__restrict__ ind,
                                                     const V *__restrict__ val,
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(256) void k_csr_stream(const int32_t *__restrict__ 
     const int t = threadIdx.x;
     for (int b = b0; b < b1; ++b) {
         const int r0 = blk[b], r1 = blk[b + 1];
-        const int p0 = ptr[r0], p1 = ptr[r1];
+        const int p0 = bptr[b], p1 = bptr[b + 1];
         const int len = p1 - p0;
         if (len <= NB) {
 #pragma unroll
@@ -176,15 +177,15 @@ void launch_stream(const SparseRows &S, const float *x, const Epi &epi, hipStrea
     switch (S.blk_nnz) {
     case 1024:
         hipLaunchKernelGGL((k_csr_stream<1024, V, Acc, Epi>), dim3(g), dim3(256), 0, s, S.blk.p,
-                           S.ptr.p, S.ind.p, val, x, epi, nb);
+                           S.bptr.p, S.ptr.p, S.ind.p, val, x, epi, nb);
         break;
     case 2048:
         hipLaunchKernelGGL((k_csr_stream<2048, V, Acc, Epi>), dim3(g), dim3(256), 0, s, S.blk.p,
-                           S.ptr.p, S.ind.p, val, x, epi, nb);
+                           S.bptr.p, S.ptr.p, S.ind.p, val, x, epi, nb);
         break;
     case 4096:
         hipLaunchKernelGGL((k_csr_stream<4096, V, Acc, Epi>), dim3(g), dim3(256), 0, s, S.blk.p,
-                           S.ptr.p, S.ind.p, val, x, epi, nb);
+                           S.bptr.p, S.ptr.p, S.ind.p, val, x, epi, nb);
         break;
     default: throw tv3_error(TV3_E_ARG, "stream block size must be 1024, 2048 or 4096");
     }

@@ -113,7 +113,8 @@ struct SparseRows {
     // blk_nnz entries (or one longer row); blk[b] = first row of block b
     int blk_nnz = 0;       // 0: use the row-group kernel
     int64_t nblk = 0;
-    DevBuf<int32_t> blk;
+    DevBuf<int32_t> blk;   // nblk + 1 first rows
+    DevBuf<int32_t> bptr;  // nblk + 1 first entries (ptr[blk[b]])
 };
 
 // Partition rows into blocks for the stream kernel (host greedy over rowptr).
