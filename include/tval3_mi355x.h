@@ -217,6 +217,42 @@ tv3_status tv3_shrink(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, const float *Ux
                       const float *sz, double beta, float *Wx, float *Wy, float *Wz,
                       double *lam1);
 
+/* ---- the fused solver's operator layout (no reference counterpart) ------ */
+typedef struct tv3_layout_info {
+    int32_t brick;          /* voxel numbering: 8 (8^3 tiles of 4^3 bricks), 4 (bricks), 1 */
+    int32_t panels;         /* 1: A^T products stage each panel's rays in LDS */
+    int32_t code_N, code_T; /* launch shapes as tv3_op_autotune codes (T: panel block size
+                               as -NB when panels) */
+    int64_t npan, nseg;     /* panels, staged ray values over all panels */
+    int32_t max_seg;        /* longest panel ray list */
+    int32_t reserved;
+} tv3_layout_info;
+
+/* Build the fused solver's layout of a sparse operator for a volume shape:
+ * voxels and rays renumbered for locality, A and its CSC in that numbering,
+ * the CSC's panels, launch shapes timed on this GPU.  A FUSED solve builds it
+ * on first use; this call takes that setup out of the first solve.  Kept
+ * until the operator is destroyed or prepared for another shape. */
+tv3_status tv3_op_prepare(tv3_op *op, const int32_t dims[3], tv3_layout_info *info);
+
+/* Override the layout's tuned launch shapes (tests, experiments): code_N,
+ * code_T as tv3_op_set_lanes codes (0 = keep); pan_nnz > 0 with code_T one of
+ * -1024/-2048/-4096 rebuilds the A^T panels (at most pan_nnz entries each,
+ * row blocks of -code_T entries), pan_nnz = 0 turns panels off. */
+tv3_status tv3_op_layout_force(tv3_op *op, const int32_t dims[3], int32_t code_N, int32_t code_T,
+                               int32_t pan_nnz, tv3_layout_info *info);
+
+/* y = A x (TV3_N) or A^T x (TV3_T) by the fused solver's product kernels in
+ * its layout; x and y in natural order (permuted in and out), on stream. */
+tv3_status tv3_op_layout_matvec(tv3_op *op, const int32_t dims[3], tv3_trans trans,
+                                const float *x_dev, float *y_dev, void *stream);
+
+/* Average duration (microseconds) of one launch of the layout's A (TV3_N) or
+ * A^T (TV3_T) product kernel over `reps` back-to-back launches on the
+ * context stream (x = 1), timed with HIP events on that stream. */
+tv3_status tv3_op_layout_time(tv3_op *op, const int32_t dims[3], tv3_trans trans, int32_t reps,
+                              double *us);
+
 /* ---- tuning ------------------------------------------------------------- */
 /* Time the sparse kernels of op over their launch shapes and keep the
  * fastest: the row-blocked kernel with 1024/2048/4096-entry blocks (returned

@@ -54,8 +54,10 @@ def test_struct_layouts_match_header():
 #include <stddef.h>
 #include "tval3_mi355x.h"
 int main(void){
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(tv3_opts), sizeof(tv3_info), sizeof(tv3_history),
-         sizeof(tv3_geometry), offsetof(tv3_opts, solver), offsetof(tv3_geometry, origin));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tv3_opts), sizeof(tv3_info),
+         sizeof(tv3_history), sizeof(tv3_geometry), offsetof(tv3_opts, solver),
+         offsetof(tv3_geometry, origin), sizeof(tv3_layout_info),
+         offsetof(tv3_layout_info, max_seg));
   return 0;
 }'''
     tmp = os.path.join(ROOT, "tests", "_layout_probe")
@@ -69,7 +71,8 @@ int main(void){
             if os.path.exists(p):
                 os.remove(p)
     exp = [C.sizeof(L.Opts), C.sizeof(L.Info), C.sizeof(L.History), C.sizeof(L.Geometry),
-           L.Opts.solver.offset, L.Geometry.origin.offset]
+           L.Opts.solver.offset, L.Geometry.origin.offset, C.sizeof(L.LayoutInfo),
+           L.LayoutInfo.max_seg.offset]
     assert got == exp
 
 

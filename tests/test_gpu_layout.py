@@ -1,0 +1,123 @@
+"""The fused solver's product kernels in its internal layout (8x8x8 tiles of
+4x4x4 bricks, bricks, or natural voxel order; rays renumbered; A^T in panels
+whose rays are staged in LDS) against the oracle's products.
+
+Integer-valued weights and vectors make every product exact in FP32 whatever
+the summation order, so those cases are compared bit-exactly and catch any
+dropped, doubled or misrouted entry.  Real Siddon weights are compared within
+the FP32 rounding bound gamma_k * (|A| |x|)_i, k = row length + 1."""
+import numpy as np
+import pytest
+
+import tv3_oracle as orc
+from tval3_amd import api, workload as wl
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available()
+    return api.Context(0)
+
+
+def T(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def H(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def siddon_structure(ctx, dims, E, R, seed=1):
+    g = wl.transducer_geometry(E, R, dims=dims, seed=seed)
+    dyn = api.DynamicOperator(ctx, g, "siddon")
+    A = dyn.materialize("f32")
+    rp, c, v = A.download_csr()
+    return A.m, A.n, rp, c, v
+
+
+def integer_case(ctx, dims, E, R, dtype):
+    """Siddon sparsity pattern with weights 1..7 (exact in FP16 and BF16)."""
+    m, n, rp, c, _ = siddon_structure(ctx, dims, E, R)
+    w = (1 + (np.arange(len(c)) * 2654435761 % 7)).astype(np.float64)
+    op = api.SparseOperator(ctx, m, n, rp, c, w, dtype=dtype)
+    oop = orc.csr_op(m, n, rp, c, w)
+    rng = np.random.default_rng(7)
+    x = rng.integers(0, 1024, n).astype(np.float32)
+    r = rng.integers(0, 1024, m).astype(np.float32)
+    return op, oop, x, r
+
+
+SHAPES = [
+    # (code_N, code_T, pan_nnz): panels with three block sizes, panels split
+    # into brick runs (small pan_nnz), and the plain kernels
+    (32, -1024, 16384), (-1024, -2048, 16384), (64, -4096, 16384),
+    (16, -2048, 600), (-2048, -1024, 100),
+    (8, -2048, 0), (-4096, 16, 0),
+]
+
+
+@pytest.mark.parametrize("dims,brick", [((16, 16, 16), 8), ((24, 16, 8), 8), ((12, 12, 12), 4),
+                                        ((12, 10, 6), 1)])
+def test_layout_products_exact(ctx, dims, brick):
+    op, oop, x, r = integer_case(ctx, dims, 24, 64, "f16")
+    y_ref = orc.matvec(oop, 0, x.astype(np.float64)).astype(np.float32)
+    z_ref = orc.matvec(oop, 1, r.astype(np.float64)).astype(np.float32)
+    assert np.abs(z_ref).max() < 2 ** 24 and np.abs(y_ref).max() < 2 ** 24
+    info = op.prepare(dims)
+    assert info["brick"] == brick
+    for code_N, code_T, pan in SHAPES:
+        info = op.layout_force(dims, code_N, code_T, pan)
+        assert info["panels"] == (pan > 0)
+        y = H(op.layout_mat_vec_mul(dims, T(x)))
+        z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+        assert (y == y_ref).all(), (dims, code_N)
+        assert (z == z_ref).all(), (dims, code_T, pan, info)
+
+
+def test_layout_panels_with_empty_tiles(ctx):
+    """Two emitters and four receivers leave most tiles without rays: empty
+    panels still produce zeros for their voxels."""
+    dims = (32, 16, 16)
+    op, oop, x, r = integer_case(ctx, dims, 2, 4, "f32")
+    z_ref = orc.matvec(oop, 1, r.astype(np.float64)).astype(np.float32)
+    assert (z_ref == 0).sum() > z_ref.size // 2
+    for code_T, pan in ((-1024, 16384), (-2048, 64)):
+        info = op.layout_force(dims, 0, code_T, pan)
+        assert info["panels"] == 1 and info["npan"] >= 16
+        z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+        assert (z == z_ref).all()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16", "bf16"])
+def test_layout_siddon_rounding_bound(ctx, dtype):
+    dims = (32, 32, 32)
+    g = wl.transducer_geometry(48, 96, dims=dims)
+    dyn = api.DynamicOperator(ctx, g, "siddon")
+    op = dyn.materialize(dtype)
+    rp, c, v = op.download_csr()          # the stored (rounded) weights
+    oop = orc.csr_op(op.m, op.n, rp, c, v)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, op.n).astype(np.float32)
+    r = rng.uniform(-1, 1, op.m).astype(np.float32)
+    info = op.prepare(dims)
+    u = 2.0 ** -24
+    aop = orc.csr_op(op.m, op.n, rp, c, np.abs(v))
+    lens_N = np.diff(rp)
+    lens_T = np.bincount(c, minlength=op.n)
+    for code_N, code_T, pan in [(0, 0, 0)] + SHAPES[:3]:
+        if pan:
+            op.layout_force(dims, code_N, code_T, pan)
+        y = H(op.layout_mat_vec_mul(dims, T(x))).astype(np.float64)
+        z = H(op.layout_mat_vec_mul(dims, T(r), trans=True)).astype(np.float64)
+        for got, trans, vec, lens in ((y, 0, x, lens_N), (z, 1, r, lens_T)):
+            ref = orc.matvec(oop, trans, vec.astype(np.float64))
+            mag = orc.matvec(aop, trans, np.abs(vec).astype(np.float64))
+            k = lens + 1
+            bound = k * u / (1 - k * u) * mag + 2 * u * np.abs(ref)   # + final FP32 store
+            assert (np.abs(got - ref) <= bound + 1e-30).all(), (dtype, trans, code_T, info)

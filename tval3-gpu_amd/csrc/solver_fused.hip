@@ -65,13 +65,13 @@ struct Lay {
 
 template <int B>
 __device__ __forceinline__ int64_t vidx(const Lay &L, int y, int x, int z) {
-    if (B == 1) return (int64_t)z * L.pq + (int64_t)x * L.p + y;
-    return ((((int64_t)(z >> 2) * L.nbx + (x >> 2)) * L.nby + (y >> 2)) << 6) + ((z & 3) << 4) +
-           ((x & 3) << 2) + (y & 3);
+    return brick_index(B, L.pq, L.p, L.nby, L.nbx, y, x, z);
 }
 
-// Voxel groups of 256 threads: B = 4 -> four bricks consecutive in y (one
-// wavefront per brick); B = 1 -> 64 (y) x 4 (x) voxels of one z-plane.
+// Voxel groups of 256 threads (one wavefront per brick where there are
+// bricks): B = 8 -> the four bricks of one z-half of a tile (256 consecutive
+// internal voxels); B = 4 -> four bricks consecutive in y; B = 1 -> 64 (y) x 4
+// (x) voxels of one z-plane.
 // Workgroups of the 1-D grid walk contiguous chunks of groups, consecutive
 // chunks on one XCD.
 struct Groups {
@@ -82,7 +82,8 @@ struct Groups {
 template <int B>
 static Groups groups_of(const Lay &L) {
     Groups g;
-    if (B == 4) { g.gy = (L.nby + 3) / 4; g.gx = L.nbx; g.gz = L.r / 4; }
+    if (B == 8) { g.gy = L.nby; g.gx = L.nbx; g.gz = L.r / 4; }
+    else if (B == 4) { g.gy = (L.nby + 3) / 4; g.gx = L.nbx; g.gz = L.r / 4; }
     else { g.gy = (L.p + 63) / 64; g.gx = (L.q + 3) / 4; g.gz = L.r; }
     g.count = (int64_t)g.gy * g.gx * g.gz;
     return g;
@@ -95,6 +96,13 @@ __device__ __forceinline__ bool voxel_of(const Lay &L, const Groups &G, int64_t 
     const int gy = (int)(grp % G.gy);
     const int64_t rest = grp / G.gy;
     const int gx = (int)(rest % G.gx), gz = (int)(rest / G.gx);
+    if (B == 8) {
+        const int lane = t & 63;
+        y = gy * 8 + ((t >> 4) & 4) + (lane & 3);
+        x = gx * 8 + ((t >> 5) & 4) + ((lane >> 2) & 3);
+        z = gz * 4 + (lane >> 4);
+        return true;
+    }
     if (B == 4) {
         const int by = gy * 4 + (t >> 6), lane = t & 63;
         y = by * 4 + (lane & 3);
@@ -401,13 +409,15 @@ struct EpiRes {          // Au_n = s; r1 = s - b; sums |r1|^2, <delta,r1>, <r0,r
     const FCtl *C;
     double *part;
     mutable double acc[3];
+    struct Pre { float b, d, a; };   // b, delta, Auc of the row
     __device__ bool skip() const { return !C->active; }
-    __device__ __forceinline__ void row(int r, float s, bool active) const {
+    __device__ __forceinline__ Pre load(int r) const { return {b[r], delta[r], Auc[r]}; }
+    __device__ __forceinline__ void row(int r, float s, bool active, const Pre &p) const {
         if (active) {
             Aun[r] = s;
-            const float bv = b[r], r1 = s - bv, r0 = Auc[r] - bv;
+            const float r1 = s - p.b, r0 = p.a - p.b;
             acc[0] += (double)r1 * r1;
-            acc[1] += (double)delta[r] * r1;
+            acc[1] += (double)p.d * r1;
             acc[2] += (double)r0 * r1;
         }
     }
@@ -418,8 +428,10 @@ struct EpiNorm {         // |s|^2 only (A d for the first-iteration step)
     const FCtl *C;
     double *part;
     mutable double acc[1];
+    struct Pre {};
     __device__ bool skip() const { return false; }
-    __device__ __forceinline__ void row(int, float s, bool active) const {
+    __device__ __forceinline__ Pre load(int) const { return {}; }
+    __device__ __forceinline__ void row(int, float s, bool active, const Pre &) const {
         if (active) acc[0] += (double)s * s;
     }
     __device__ __forceinline__ void finish() const { block_sum_store<1>(acc, part, gridDim.x); }
@@ -429,9 +441,11 @@ struct EpiGrad {         // g_n = s - Atb
     float *g;
     const float *Atb;
     const FCtl *C;
+    struct Pre { float a; };         // Atb of the row
     __device__ bool skip() const { return !C->active; }
-    __device__ __forceinline__ void row(int r, float s, bool active) const {
-        if (active) g[r] = s - Atb[r];
+    __device__ __forceinline__ Pre load(int r) const { return {Atb[r]}; }
+    __device__ __forceinline__ void row(int r, float s, bool active, const Pre &p) const {
+        if (active) g[r] = s - p.a;
     }
     __device__ __forceinline__ void finish() const {}
 };
@@ -714,7 +728,7 @@ struct Fused {
         hipLaunchKernelGGL(k_descend<B>, gS, dim3(256), 0, s, L, G, U[c], d, sx, sy, sz, ox, oy,
                            oz, U[n_], ctl.p, pD);
         launch_product<float>(F->A, U[n_], EpiRes{Au[n_], b, delta, Au[c], ctl.p, pN, {0, 0, 0}}, s);
-        launch_product<float>(F->AT, Au[n_], EpiGrad{g[n_], Atb, ctl.p}, s);
+        launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p}, s);
         hipLaunchKernelGGL((k_stage2<3, 3>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pD, nbS, pN, nbN,
                            st2.p);
         hipLaunchKernelGGL(k_line_search, dim3(1), dim3(1024), 0, s, ctl.p, st2.p, mode);
@@ -797,7 +811,7 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     S.write_ctl();
 
     // initialisation (:1128-1153)
-    launch_product<float>(FL.AT, S.b, EpiStore<float>{S.Atb}, s);
+    launch_transposed<float>(FL, S.b, EpiStore<float>{S.Atb}, s);
     if (o.warm_start) {
         permute_gather(U_out, FL.v_int2nat.p, S.tmp, n, s);
         scale<float>(n, S.U[0], scl, S.tmp, s);
@@ -806,7 +820,7 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     }
     TV3_HIP(hipMemcpyAsync(S.rcd, S.U[0], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     launch_product<float>(FL.A, S.U[0], EpiRes{S.Au[0], S.b, S.delta, S.Au[0], S.ctl.p, S.pN, {0, 0, 0}}, s);
-    launch_product<float>(FL.AT, S.Au[0], EpiGrad{S.g[0], S.Atb, S.ctl.p}, s);
+    launch_transposed<float>(FL, S.Au[0], EpiGrad{S.g[0], S.Atb, S.ctl.p}, s);
     hipLaunchKernelGGL(k_update_w<B>, dim3(S.nbS), dim3(256), 0, s, S.L, S.G, S.U[0], S.U[0], S.sx,
                        S.sy, S.sz, S.g[0], S.dts, S.d, S.ctl.p, S.pW);
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pN, S.nbN, S.pW, S.nbS);
@@ -906,7 +920,8 @@ void solve_fused(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
                  const float *b_dev, float *U_dev, const float *Ut_dev, tv3_info *info,
                  tv3_history *hist) {
     FusedLayout &FL = fused_layout(op, dims);
-    if (FL.B == 4) run_fused<4>(ctx, FL, o, b_dev, U_dev, Ut_dev, info, hist);
+    if (FL.B == 8) run_fused<8>(ctx, FL, o, b_dev, U_dev, Ut_dev, info, hist);
+    else if (FL.B == 4) run_fused<4>(ctx, FL, o, b_dev, U_dev, Ut_dev, info, hist);
     else run_fused<1>(ctx, FL, o, b_dev, U_dev, Ut_dev, info, hist);
 }
 

@@ -647,6 +647,73 @@ tv3_status tv3_op_autotune(tv3_op *op, int32_t *lanes_N, int32_t *lanes_T) {
     });
 }
 
+static void fill_layout_info(const FusedLayout &F, tv3_layout_info *info) {
+    if (!info) return;
+    std::memset(info, 0, sizeof(*info));
+    auto code = [](const SparseRows &S) { return S.blk_nnz ? -S.blk_nnz : S.lanes; };
+    info->brick = F.B;
+    info->panels = F.use_panels ? 1 : 0;
+    info->code_N = code(F.A);
+    info->code_T = F.use_panels ? -F.PT.blk_nnz : code(F.AT);
+    info->npan = F.use_panels ? F.PT.npan : 0;
+    info->nseg = F.use_panels ? F.PT.nseg : 0;
+    info->max_seg = F.use_panels ? F.PT.max_seg : 0;
+}
+
+static void check_dims(const tv3_op *op, const int32_t dims[3]) {
+    TV3_REQUIRE(dims && dims[0] > 0 && dims[1] > 0 && dims[2] > 0, TV3_E_ARG, "bad dims");
+    TV3_REQUIRE((int64_t)dims[0] * dims[1] * dims[2] == op->n, TV3_E_ARG,
+                "dims do not match the operator's column count");
+}
+
+tv3_status tv3_op_prepare(tv3_op *op, const int32_t dims[3], tv3_layout_info *info) {
+    if (!op) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        check_dims(op, dims);
+        set_device(op->ctx);
+        fill_layout_info(fused_layout(op, dims), info);
+    });
+}
+
+tv3_status tv3_op_layout_force(tv3_op *op, const int32_t dims[3], int32_t code_N, int32_t code_T,
+                               int32_t pan_nnz, tv3_layout_info *info) {
+    if (!op) return TV3_E_ARG;
+    auto ok = [](int L) {
+        return L == 0 || (L >= 2 && L <= 64 && (L & (L - 1)) == 0) || L == -1024 || L == -2048 ||
+               L == -4096;
+    };
+    if (!ok(code_N) || !ok(code_T) || pan_nnz < 0) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        check_dims(op, dims);
+        set_device(op->ctx);
+        layout_force(op, dims, code_N, code_T, pan_nnz);
+        fill_layout_info(fused_layout(op, dims), info);
+    });
+}
+
+tv3_status tv3_op_layout_matvec(tv3_op *op, const int32_t dims[3], tv3_trans trans,
+                                const float *x, float *y, void *stream) {
+    if (!op) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        TV3_REQUIRE(x && y, TV3_E_ARG, "x or y is NULL");
+        TV3_REQUIRE(trans == TV3_N || trans == TV3_T, TV3_E_ARG, "bad trans");
+        check_dims(op, dims);
+        set_device(op->ctx);
+        layout_matvec(op, dims, trans == TV3_T, x, y, (hipStream_t)stream);
+    });
+}
+
+tv3_status tv3_op_layout_time(tv3_op *op, const int32_t dims[3], tv3_trans trans, int32_t reps,
+                              double *us) {
+    if (!op || !us) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        TV3_REQUIRE(trans == TV3_N || trans == TV3_T, TV3_E_ARG, "bad trans");
+        check_dims(op, dims);
+        set_device(op->ctx);
+        *us = layout_time(op, dims, trans == TV3_T, reps);
+    });
+}
+
 tv3_status tv3_op_set_lanes(tv3_op *op, int32_t lanes_N, int32_t lanes_T) {
     if (!op) return TV3_E_ARG;
     auto ok = [](int L) {

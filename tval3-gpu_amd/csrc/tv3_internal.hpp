@@ -141,13 +141,67 @@ struct DynGeom {
 // (measurement position, emitter rank, receiver rank).
 std::vector<int32_t> geometry_ray_order(const DynGeom &g);
 
-// The fused solver's internal layout of a sparse operator: voxels in 4x4x4
-// bricks (when every dimension is a multiple of 4, else natural order), rays
-// in a locality order; A and its CSC in that numbering.
+// Internal index of voxel (y, x, z) in the fused solver's numbering.
+//   B = 1: natural order (y fastest, then x, then z).
+//   B = 4: 4x4x4 bricks of 64 voxels (y fastest inside), bricks ordered y, x, z.
+//   B = 8: 8x8x8 tiles of 512 voxels ordered y, x, z; inside a tile its eight
+//          4x4x4 bricks in (y, x, z) bit order, inside a brick y fastest.
+// nby, nbx = bricks along y, x (B = 4); tiles along y, x (B = 8).
+__host__ __device__ __forceinline__ int64_t brick_index(int B, int64_t pq, int p, int nby, int nbx,
+                                                        int y, int x, int z) {
+    if (B == 1) return (int64_t)z * pq + (int64_t)x * p + y;
+    const int64_t in_brick = ((z & 3) << 4) + ((x & 3) << 2) + (y & 3);
+    if (B == 4)
+        return ((((int64_t)(z >> 2) * nbx + (x >> 2)) * nby + (y >> 2)) << 6) + in_brick;
+    const int64_t tile = ((int64_t)(z >> 3) * nbx + (x >> 3)) * nby + (y >> 3);
+    const int sub = (((z >> 2) & 1) << 2) | (((x >> 2) & 1) << 1) | ((y >> 2) & 1);
+    return (tile << 9) + (sub << 6) + in_brick;
+}
+
+// A^T split into panels for the LDS-staged product.  A panel is a run of
+// consecutive internal voxel rows (an 8x8x8 tile or a brick-aligned part of
+// one) together with the sorted list of the distinct rays its entries touch;
+// the product stages those rays' values in LDS once and every entry of the
+// panel addresses them with a 16-bit local index.  Row blocks (at most
+// blk_nnz entries and 256 rows, or one longer row) never cross a panel.
+// Entries are stored per block, padded to a multiple of 4 so that a thread
+// loads 4 of them with one 16-byte access: 16-bit weights packed with their
+// index into one 32-bit word (index low, weight bits high), F32 weights in a
+// separate array.
+struct PanelT {
+    int64_t npan = 0, nseg = 0;
+    int max_seg = 0;       // largest ray list of a panel (LDS floats)
+    int blk_nnz = 0;
+    tv3_dtype vtype = TV3_F16;
+    DevBuf<int32_t> order;     // npan panels, decreasing entry count
+    DevBuf<int32_t> pan_seg;   // npan + 1 offsets into seg_ray
+    DevBuf<int32_t> pan_blk;   // npan + 1 offsets into blk
+    DevBuf<int32_t> seg_ray;   // nseg internal ray indices
+    DevBuf<int32_t> blk, bptr; // row blocks: first row, first CSC entry (+1 sentinel)
+    DevBuf<int32_t> bpad;      // row blocks: first padded entry (+1 sentinel)
+    DevBuf<uint32_t> pk;       // padded packed entries (16-bit weights)
+    DevBuf<uint16_t> sidx;     // padded local ray indices (F32 weights)
+    DevBuf<float> pval;        // padded weights (F32 weights)
+};
+
+// Build P from S (the CSC of A, rows = internal voxels, sorted columns): rows
+// grouped in units of `unit` rows (one tile), split so that a panel holds at
+// most pan_nnz entries where a unit has more.  Returns false (P untouched)
+// when a panel would need more than 65536 rays.
+bool build_panels(tv3_ctx *ctx, const SparseRows &S, int unit, int pan_nnz, int blk_nnz,
+                  PanelT &P);
+
+// The fused solver's internal layout of a sparse operator: voxels numbered by
+// brick_index (B = 8 when every dimension is a multiple of 8, else 4 when a
+// multiple of 4, else 1), rays in a locality order; A and its CSC in that
+// numbering, and the CSC's panels when use_panels.
 struct FusedLayout {
     int32_t dims[3] = {0, 0, 0};
+
     int B = 1, nby = 0, nbx = 0;
     SparseRows A, AT;
+    PanelT PT;
+    bool use_panels = false;
     DevBuf<int32_t> v_nat2int, v_int2nat;   // voxel maps (n)
     DevBuf<int32_t> r_int2nat;              // ray map (m): internal ray -> natural ray
 };
@@ -219,6 +273,13 @@ void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
 
 // internal layout (layout.hip)
 FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]);
+// y = A x or A^T x with the layout's kernels, natural order in and out
+void layout_matvec(tv3_op *op, const int32_t dims[3], bool trans, const float *x, float *y,
+                   hipStream_t s);
+// override the layout's launch shapes (tv3_op_layout_force)
+void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz);
+// average duration (us) of one layout product kernel launch
+double layout_time(tv3_op *op, const int32_t dims[3], bool trans, int reps);
 void permute_gather(const float *in, const int32_t *map, float *out, int64_t n, hipStream_t s);
 void permute_scatter(const float *in, const int32_t *map, float *out, int64_t n, hipStream_t s);
 

@@ -72,6 +72,12 @@ class Geometry(C.Structure):
                 ("origin", C.c_float * 3), ("voxel", C.c_float * 3), ("use_path", _P)]
 
 
+class LayoutInfo(C.Structure):
+    _fields_ = [("brick", C.c_int32), ("panels", C.c_int32), ("code_N", C.c_int32),
+                ("code_T", C.c_int32), ("npan", C.c_int64), ("nseg", C.c_int64),
+                ("max_seg", C.c_int32), ("reserved", C.c_int32)]
+
+
 # (name, restype, argtypes) of every exported entry point of the header
 SIGNATURES = [
     ("tv3_abi_version", C.c_int32, []),
@@ -104,6 +110,11 @@ SIGNATURES = [
     ("tv3_Dt", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     ("tv3_shrink", C.c_int, [_P, C.c_int64, C.c_int, _P, _P, _P, _P, _P, _P, C.c_double, _P,
                              _P, _P, C.POINTER(C.c_double)]),
+    ("tv3_op_prepare", C.c_int, [_P, _P, C.POINTER(LayoutInfo)]),
+    ("tv3_op_layout_force", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32,
+                                      C.POINTER(LayoutInfo)]),
+    ("tv3_op_layout_matvec", C.c_int, [_P, _P, C.c_int, _P, _P, _P]),
+    ("tv3_op_layout_time", C.c_int, [_P, _P, C.c_int, C.c_int32, C.POINTER(C.c_double)]),
     ("tv3_op_autotune", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("tv3_op_set_lanes", C.c_int, [_P, C.c_int32, C.c_int32]),
 ]

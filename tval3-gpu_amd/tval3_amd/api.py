@@ -142,6 +142,46 @@ class Operator:
         op.dtype = dtype
         return op
 
+    def prepare(self, dims):
+        """Build the fused solver's layout for a volume shape (renumbering,
+        CSC panels, tuned launch shapes); returns its description."""
+        d = np.ascontiguousarray(dims, np.int32)
+        info = L.LayoutInfo()
+        self.ctx.check(L.load().tv3_op_prepare(self.h, _vp(d), C.byref(info)))
+        return {k: getattr(info, k) for k, _ in L.LayoutInfo._fields_ if k != "reserved"}
+
+    def layout_force(self, dims, code_N=0, code_T=0, pan_nnz=0):
+        """Override the layout's launch shapes (codes as set_lanes; pan_nnz > 0
+        with code_T in (-1024, -2048, -4096) rebuilds A^T panels)."""
+        d = np.ascontiguousarray(dims, np.int32)
+        info = L.LayoutInfo()
+        self.ctx.check(L.load().tv3_op_layout_force(self.h, _vp(d), code_N, code_T, pan_nnz,
+                                                    C.byref(info)))
+        return {k: getattr(info, k) for k, _ in L.LayoutInfo._fields_ if k != "reserved"}
+
+    def layout_mat_vec_mul(self, dims, x, y=None, trans=False):
+        """A x or A^T x by the fused solver's kernels in its layout (natural
+        order in and out) on torch's current stream."""
+        import torch
+        out_len = self.n if trans else self.m
+        if y is None:
+            y = torch.empty(out_len, dtype=torch.float32, device=x.device)
+        assert x.dtype == torch.float32 and x.is_contiguous() and y.is_contiguous()
+        assert x.numel() == (self.m if trans else self.n) and y.numel() == out_len
+        d = np.ascontiguousarray(dims, np.int32)
+        self.ctx.check(L.load().tv3_op_layout_matvec(self.h, _vp(d), 1 if trans else 0, _vp(x),
+                                                     _vp(y), _torch_stream()))
+        return y
+
+    def layout_time(self, dims, trans=False, reps=20):
+        """Average duration (s) of one launch of the layout's product kernel,
+        HIP events on the context stream."""
+        d = np.ascontiguousarray(dims, np.int32)
+        us = C.c_double()
+        self.ctx.check(L.load().tv3_op_layout_time(self.h, _vp(d), 1 if trans else 0, reps,
+                                                   C.byref(us)))
+        return us.value * 1e-6
+
     def autotune(self):
         a, b = C.c_int32(), C.c_int32()
         self.ctx.check(L.load().tv3_op_autotune(self.h, C.byref(a), C.byref(b)))
