@@ -49,10 +49,14 @@ def build_workload(ctx, N, E, R):
 
 
 def bytes_A(nnz, m, n, sv):
+    """CSR product y = A x: entries (weight + int32 column), row pointers,
+    x read once, y written."""
     return nnz * (sv + 4) + 4 * (m + 1) + 4 * n + 4 * m
 
 
 def bytes_AT(nnz, m, n, sv):
+    """CSC product y = A^T r: entries (weight + int32 row index), column
+    pointers, r read once, y written."""
     return nnz * (sv + 4) + 4 * (n + 1) + 4 * m + 4 * n
 
 
@@ -90,7 +94,7 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--emitters", type=int, default=256)
     ap.add_argument("--receivers", type=int, default=1024)
-    ap.add_argument("--path", default="plain-f32", choices=["plain-f32", "fused-f32", "fused-f16"])
+    ap.add_argument("--path", default="fused-f16", choices=["plain-f32", "fused-f32", "fused-f16"])
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
@@ -121,8 +125,15 @@ def main():
 
     wdt = "f16" if args.path.endswith("f16") else "f32"
     op = A32 if wdt == "f32" else dyn.materialize(wdt)
-    lanes = op.autotune()
     solver = "plain" if args.path.startswith("plain") else "fused"
+    layout = None
+    if solver == "fused":
+        t0 = time.time()
+        layout = op.prepare(dims)      # renumbering, panels, tuned shapes: setup, not timed
+        lanes = (layout["code_N"], layout["code_T"])
+        log(f"layout: {layout} ({time.time() - t0:.1f}s)")
+    else:
+        lanes = op.autotune()
     base = dict(mu=2.0 ** 9, mu0=2.0 ** 9, tv_mode="iso", solver=solver)
     # fixed iteration count: no outer cap, no stopping test
     fixed = dict(base, tol=0.0, tol_inn=0.0, maxcnt=1 << 30)
@@ -136,15 +147,27 @@ def main():
     log(f"timed: {args.steps} iters in {t:.3f}s -> {value:.1f} iter/s "
         f"(loop-only {info['total_iters'] / info['loop_secs']:.1f}/s) rel_err={info['rel_error']:.4e}")
 
-    # dominant kernels: A and A^T products of the headline operator
+    # dominant kernels: the A and A^T products the solver runs, timed alone
+    # with HIP events on the launch stream (layout_time) for the fused path
     sv = {"f32": 4, "f16": 2, "bf16": 2}[wdt]
-    tA = time_spmv(op, False)
-    tAT = time_spmv(op, True)
-    gA = bytes_A(nnz, m, n, sv) / tA
-    gAT = bytes_AT(nnz, m, n, sv) / tAT
+    panels = layout["panels"] if layout else 0
+    if layout:
+        tA = op.layout_time(dims, False, 50)
+        tAT = op.layout_time(dims, True, 50)
+    else:
+        tA = time_spmv(op, False)
+        tAT = time_spmv(op, True)
+    bA, bAT = bytes_A(nnz, m, n, sv), bytes_AT(nnz, m, n, sv)
+    gA, gAT = bA / tA, bAT / tAT
     log(f"SpMV A: {tA * 1e6:.1f} us {gA / 1e9:.0f} GB/s; A^T: {tAT * 1e6:.1f} us {gAT / 1e9:.0f} GB/s")
-    dom_t, dom_b, dom_name = (tAT, bytes_AT(nnz, m, n, sv), "A^T (CSC) SpMV") if tAT >= tA else \
-        (tA, bytes_A(nnz, m, n, sv), "A (CSR) SpMV")
+    if tAT >= tA:
+        dom_t, dom_b = tAT, bAT
+        dom_name = ("A^T product, panel form (k_panel, rays staged in LDS)" if panels & 2
+                    else "A^T product (CSC row blocks)")
+    else:
+        dom_t, dom_b = tA, bA
+        dom_name = ("A product, panel form (k_panel segment sums + k_seg_sum)" if panels & 1
+                    else "A product (CSR row blocks)")
 
     extra = {}
     if not args.no_extra:
@@ -212,6 +235,7 @@ def main():
         "rel_err_vs_utrue": info["rel_error"],
         "loop_iter_per_s": info["total_iters"] / info["loop_secs"],
         "lanes": list(lanes),
+        "layout": layout,
     }
     out.update(extra)
     print(json.dumps(out))

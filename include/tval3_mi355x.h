@@ -220,11 +220,13 @@ tv3_status tv3_shrink(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, const float *Ux
 /* ---- the fused solver's operator layout (no reference counterpart) ------ */
 typedef struct tv3_layout_info {
     int32_t brick;          /* voxel numbering: 8 (8^3 tiles of 4^3 bricks), 4 (bricks), 1 */
-    int32_t panels;         /* 1: A^T products stage each panel's rays in LDS */
-    int32_t code_N, code_T; /* launch shapes as tv3_op_autotune codes (T: panel block size
-                               as -NB when panels) */
-    int64_t npan, nseg;     /* panels, staged ray values over all panels */
-    int32_t max_seg;        /* longest panel ray list */
+    int32_t panels;         /* bit 0: A x in panel form (per-tile ray segments, then a
+                               per-ray sum); bit 1: A^T r in panel form (per-tile rays
+                               staged in LDS) */
+    int32_t code_N, code_T; /* launch shapes as tv3_op_autotune codes; for a panel form its
+                               row block size as -NB */
+    int64_t npan, nseg;     /* tiles (panels) and ray segments (tile, ray) of the plan */
+    int32_t max_seg;        /* most rays crossing one panel */
     int32_t reserved;
 } tv3_layout_info;
 
@@ -235,12 +237,13 @@ typedef struct tv3_layout_info {
  * until the operator is destroyed or prepared for another shape. */
 tv3_status tv3_op_prepare(tv3_op *op, const int32_t dims[3], tv3_layout_info *info);
 
-/* Override the layout's tuned launch shapes (tests, experiments): code_N,
- * code_T as tv3_op_set_lanes codes (0 = keep); pan_nnz > 0 with code_T one of
- * -1024/-2048/-4096 rebuilds the A^T panels (at most pan_nnz entries each,
- * row blocks of -code_T entries), pan_nnz = 0 turns panels off. */
+/* Override the layout's tuned launch shapes (tests, experiments).  panels:
+ * bit 0 puts A x, bit 1 puts A^T r in panel form, rebuilt with panels of at
+ * most pan_nnz entries and row blocks of -code_N / -code_T entries (-1024,
+ * -2048 or -4096); a direction not in panel form takes code_N / code_T as a
+ * tv3_op_set_lanes code (0 = keep). */
 tv3_status tv3_op_layout_force(tv3_op *op, const int32_t dims[3], int32_t code_N, int32_t code_T,
-                               int32_t pan_nnz, tv3_layout_info *info);
+                               int32_t pan_nnz, int32_t panels, tv3_layout_info *info);
 
 /* y = A x (TV3_N) or A^T x (TV3_T) by the fused solver's product kernels in
  * its layout; x and y in natural order (permuted in and out), on stream. */

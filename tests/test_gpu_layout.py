@@ -1,6 +1,8 @@
 """The fused solver's product kernels in its internal layout (8x8x8 tiles of
-4x4x4 bricks, bricks, or natural voxel order; rays renumbered; A^T in panels
-whose rays are staged in LDS) against the oracle's products.
+4x4x4 bricks, bricks, or natural voxel order; rays renumbered; panel forms:
+A^T r with each tile's rays staged in LDS, A u as per-tile ray-segment sums
+over the tile's voxels staged in LDS, then per-ray sums) against the oracle's
+products.
 
 Integer-valued weights and vectors make every product exact in FP32 whatever
 the summation order, so those cases are compared bit-exactly and catch any
@@ -54,11 +56,13 @@ def integer_case(ctx, dims, E, R, dtype):
 
 
 SHAPES = [
-    # (code_N, code_T, pan_nnz): panels with three block sizes, panels split
-    # into brick runs (small pan_nnz), and the plain kernels
-    (32, -1024, 16384), (-1024, -2048, 16384), (64, -4096, 16384),
-    (16, -2048, 600), (-2048, -1024, 100),
-    (8, -2048, 0), (-4096, 16, 0),
+    # (code_N, code_T, pan_nnz, panels): both products in panel form with
+    # three block sizes, panels split into brick runs (small pan_nnz), one
+    # direction in panel form, and the plain kernels
+    (-1024, -1024, 16384, 3), (-2048, -2048, 16384, 3), (-4096, -4096, 16384, 3),
+    (-1024, -2048, 600, 3), (-2048, -1024, 100, 3),
+    (32, -1024, 16384, 2), (-1024, 64, 16384, 1),
+    (8, -2048, 0, 0), (-4096, 16, 0, 0),
 ]
 
 
@@ -71,12 +75,12 @@ def test_layout_products_exact(ctx, dims, brick):
     assert np.abs(z_ref).max() < 2 ** 24 and np.abs(y_ref).max() < 2 ** 24
     info = op.prepare(dims)
     assert info["brick"] == brick
-    for code_N, code_T, pan in SHAPES:
-        info = op.layout_force(dims, code_N, code_T, pan)
-        assert info["panels"] == (pan > 0)
+    for code_N, code_T, pan, panels in SHAPES:
+        info = op.layout_force(dims, code_N, code_T, pan, panels)
+        assert info["panels"] == panels
         y = H(op.layout_mat_vec_mul(dims, T(x)))
         z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
-        assert (y == y_ref).all(), (dims, code_N)
+        assert (y == y_ref).all(), (dims, code_N, pan, info)
         assert (z == z_ref).all(), (dims, code_T, pan, info)
 
 
@@ -85,13 +89,33 @@ def test_layout_panels_with_empty_tiles(ctx):
     panels still produce zeros for their voxels."""
     dims = (32, 16, 16)
     op, oop, x, r = integer_case(ctx, dims, 2, 4, "f32")
+    y_ref = orc.matvec(oop, 0, x.astype(np.float64)).astype(np.float32)
     z_ref = orc.matvec(oop, 1, r.astype(np.float64)).astype(np.float32)
     assert (z_ref == 0).sum() > z_ref.size // 2
-    for code_T, pan in ((-1024, 16384), (-2048, 64)):
-        info = op.layout_force(dims, 0, code_T, pan)
-        assert info["panels"] == 1 and info["npan"] >= 16
+    for code, pan in ((-1024, 16384), (-2048, 64)):
+        info = op.layout_force(dims, code, code, pan, 3)
+        assert info["panels"] == 3 and info["npan"] >= 16
+        y = H(op.layout_mat_vec_mul(dims, T(x)))
         z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+        assert (y == y_ref).all()
         assert (z == z_ref).all()
+
+
+def test_layout_panels_first_tile_empty(ctx):
+    """At 64^3 the corner tiles lie outside the transducer sphere, so no ray
+    crosses them: the segment form of A has panels without rows (the first
+    one at row 0), which must get no row blocks."""
+    dims = (64, 64, 64)
+    op, oop, x, r = integer_case(ctx, dims, 16, 32, "f16")
+    y_ref = orc.matvec(oop, 0, x.astype(np.float64)).astype(np.float32)
+    z_ref = orc.matvec(oop, 1, r.astype(np.float64)).astype(np.float32)
+    info = op.layout_force(dims, -1024, -1024, 16384, 3)
+    assert info["panels"] == 3 and info["brick"] == 8
+    y = H(op.layout_mat_vec_mul(dims, T(x)))
+    z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+    assert (y == y_ref).all()
+    assert (z == z_ref).all()
+    assert (z_ref[:8] == 0).all()          # voxels (0..7, 0, 0) of the corner tile
 
 
 @pytest.mark.parametrize("dtype", ["f32", "f16", "bf16"])
@@ -110,9 +134,9 @@ def test_layout_siddon_rounding_bound(ctx, dtype):
     aop = orc.csr_op(op.m, op.n, rp, c, np.abs(v))
     lens_N = np.diff(rp)
     lens_T = np.bincount(c, minlength=op.n)
-    for code_N, code_T, pan in [(0, 0, 0)] + SHAPES[:3]:
-        if pan:
-            op.layout_force(dims, code_N, code_T, pan)
+    for code_N, code_T, pan, panels in [(0, 0, 0, None)] + SHAPES[:3]:
+        if panels is not None:
+            op.layout_force(dims, code_N, code_T, pan, panels)
         y = H(op.layout_mat_vec_mul(dims, T(x))).astype(np.float64)
         z = H(op.layout_mat_vec_mul(dims, T(r), trans=True)).astype(np.float64)
         for got, trans, vec, lens in ((y, 0, x, lens_N), (z, 1, r, lens_T)):

@@ -3,15 +3,15 @@
 // Scattered gathers of x (A x) and of the residual (A^T r) are bound by the
 // L1/L2 request rate, so the solver works in a numbering that puts
 // neighbours in the same cache lines: voxels in 8x8x8 tiles of 4x4x4 bricks
-// (brick_index) and rays ordered by emitter and receiver position.  A^T is
-// further split into panels whose rays are staged in LDS (build_panels).
+// (brick_index) and rays ordered by emitter and receiver position.  A^T and
+// A get panel forms whose sources are staged in LDS (panels.hpp).
 // Inputs and outputs are permuted at the boundary of a solve only.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <numeric>
 
-#include "spmv_stream.hpp"
+#include "panels.hpp"
 
 namespace tv3 {
 
@@ -113,39 +113,60 @@ void permute_scatter(const float *in, const int32_t *map, float *out, int64_t n,
                            map, out, n);
 }
 
-// Largest ray list of a panel: its values take at most 48 KiB of LDS, which
-// leaves room for the 16 KiB product buffer of 4096-entry blocks within the
-// 64 KiB a workgroup may hold.
+// Largest source list of a panel: its values take at most 48 KiB of LDS,
+// within the 64 KiB a workgroup may hold.
 constexpr int kPanelSegCap = 12288;
 
-bool build_panels(tv3_ctx *ctx, const SparseRows &S, int unit, int pan_nnz, int blk_nnz,
-                  PanelT &P) {
-    const int64_t n = S.rows, m = S.cols, nnz = S.nnz;
+// Host copy of a CSR (here: the CSC of A, rows = internal voxels).
+struct HostCSR {
+    int64_t rows = 0, cols = 0, nnz = 0;
+    size_t vb = 0;
+    tv3_dtype vtype = TV3_F32;
+    std::vector<int32_t> ptr, ind;
+    std::vector<uint8_t> val;
+};
+
+static HostCSR download_csr(tv3_ctx *ctx, const SparseRows &S) {
+    HostCSR h;
+    h.rows = S.rows; h.cols = S.cols; h.nnz = S.nnz; h.vtype = S.vtype;
+    h.vb = dtype_size(S.vtype);
     hipStream_t s = ctx->stream;
-    const size_t vb = dtype_size(S.vtype);
-    TV3_REQUIRE(vb == 2 || S.vtype == TV3_F32, TV3_E_FORMAT, "panels need F32, F16 or BF16 weights");
-    std::vector<int32_t> ptr(n + 1), ind(nnz > 0 ? nnz : 1);
-    std::vector<uint8_t> val((nnz > 0 ? nnz : 1) * vb);
-    TV3_HIP(hipMemcpyAsync(ptr.data(), S.ptr.p, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost,
-                           s));
-    if (nnz > 0) {
-        TV3_HIP(hipMemcpyAsync(ind.data(), S.ind.p, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost,
-                               s));
-        TV3_HIP(hipMemcpyAsync(val.data(), S.val.p, vb * nnz, hipMemcpyDeviceToHost, s));
+    h.ptr.resize(S.rows + 1);
+    h.ind.resize(S.nnz > 0 ? S.nnz : 1);
+    h.val.resize((S.nnz > 0 ? S.nnz : 1) * h.vb);
+    TV3_HIP(hipMemcpyAsync(h.ptr.data(), S.ptr.p, sizeof(int32_t) * (S.rows + 1),
+                           hipMemcpyDeviceToHost, s));
+    if (S.nnz > 0) {
+        TV3_HIP(hipMemcpyAsync(h.ind.data(), S.ind.p, sizeof(int32_t) * S.nnz,
+                               hipMemcpyDeviceToHost, s));
+        TV3_HIP(hipMemcpyAsync(h.val.data(), S.val.p, h.vb * S.nnz, hipMemcpyDeviceToHost, s));
     }
     TV3_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+// Tiles of the voxel rows of A^T: panel p = rows [pan_row[p], pan_row[p+1])
+// (a unit of `unit` rows, split at brick boundaries where it holds more than
+// pan_nnz entries or more than kPanelSegCap distinct rays) and its sorted
+// distinct rays seg_ray[pan_seg[p] .. pan_seg[p+1]).
+struct PanelPlan {
+    std::vector<int32_t> pan_row{0}, pan_seg{0}, seg_ray;
+    int64_t max_seg = 0;
+};
+
+static bool plan_panels(const HostCSR &h, int unit, int pan_nnz, PanelPlan &pl) {
+    const int64_t n = h.rows, m = h.cols;
+    const std::vector<int32_t> &ptr = h.ptr, &ind = h.ind;
     const int64_t sub = unit >= 64 ? 64 : unit;   // split granularity: one brick
-    // pass 1: panel row ranges and their sorted ray lists
-    std::vector<int32_t> stamp(m > 0 ? m : 1, -1), local(m > 0 ? m : 1, 0);
-    std::vector<int32_t> pan_row{0}, pan_seg{0}, seg_ray, rays, fresh;
+    std::vector<int32_t> stamp(m > 0 ? m : 1, -1), rays, fresh;
     int32_t cur = 0;
-    int64_t cur_nnz = 0, max_seg = 0;
+    int64_t cur_nnz = 0;
     auto close = [&](int64_t row_end) {
         std::sort(rays.begin(), rays.end());
-        seg_ray.insert(seg_ray.end(), rays.begin(), rays.end());
-        max_seg = std::max<int64_t>(max_seg, (int64_t)rays.size());
-        pan_seg.push_back((int32_t)seg_ray.size());
-        pan_row.push_back((int32_t)row_end);
+        pl.seg_ray.insert(pl.seg_ray.end(), rays.begin(), rays.end());
+        pl.max_seg = std::max<int64_t>(pl.max_seg, (int64_t)rays.size());
+        pl.pan_seg.push_back((int32_t)pl.seg_ray.size());
+        pl.pan_row.push_back((int32_t)row_end);
         rays.clear();
         cur_nnz = 0;
         ++cur;
@@ -161,7 +182,7 @@ bool build_panels(tv3_ctx *ctx, const SparseRows &S, int unit, int pan_nnz, int 
                     const int32_t r = ind[k];
                     if (stamp[r] != cur) { stamp[r] = cur; fresh.push_back(r); }
                 }
-                const bool empty = (int64_t)pan_row.back() == c0;
+                const bool empty = (int64_t)pl.pan_row.back() == c0;
                 if (empty || (cur_nnz + cn <= pan_nnz &&
                               (int64_t)(rays.size() + fresh.size()) <= kPanelSegCap))
                     break;
@@ -174,85 +195,179 @@ bool build_panels(tv3_ctx *ctx, const SparseRows &S, int unit, int pan_nnz, int 
         }
         close(u1);
     }
-    const int64_t npan = (int64_t)pan_row.size() - 1;
-    // pass 2: local ray indices of the entries, row blocks inside panels
-    std::vector<uint16_t> sidx(nnz > 0 ? nnz : 1);
-    std::vector<int32_t> blk{0}, pan_blk{0};
-    std::vector<int64_t> pan_work(npan);
-    for (int64_t pn = 0; pn < npan; ++pn) {
-        for (int32_t j = pan_seg[pn]; j < pan_seg[pn + 1]; ++j) local[seg_ray[j]] = j - pan_seg[pn];
-        const int64_t r0 = pan_row[pn], r1 = pan_row[pn + 1];
-        for (int64_t k = ptr[r0]; k < ptr[r1]; ++k) sidx[k] = (uint16_t)local[ind[k]];
-        int64_t start = r0;
-        for (int64_t r = r0; r < r1; ++r)
-            if (r > start && ((int64_t)ptr[r + 1] - ptr[start] > blk_nnz || r - start >= 256)) {
-                blk.push_back((int32_t)r);
-                start = r;
-            }
-        blk.push_back((int32_t)r1);
-        pan_blk.push_back((int32_t)blk.size() - 1);
-        // cost: entries plus the staging gathers
-        pan_work[pn] = ((int64_t)ptr[r1] - ptr[r0]) + 4 * (pan_seg[pn + 1] - pan_seg[pn]);
+    return true;
+}
+
+// Pad, block and upload a panelled matrix.  Panel p holds rows [pan_row[p],
+// pan_row[p+1]) and sources src_off[p] .. src_off[p+1] (indices into *src,
+// or the index range itself when src is NULL); row r's entries are
+// rptr[r] .. rptr[r+1] with local source index lidx[k] and weight bytes
+// wb[k * vb ..].
+static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
+                        const std::vector<int32_t> &src_off, const std::vector<int32_t> *src,
+                        const std::vector<int32_t> &rptr, const std::vector<uint16_t> &lidx,
+                        const uint8_t *wb, size_t vb, tv3_dtype vt, int blk_nnz, Panels &P) {
+    hipStream_t s = ctx->stream;
+    const int64_t npan = (int64_t)pan_row.size() - 1, rows = pan_row.back();
+    std::vector<int32_t> pptr(rows + 1);
+    int64_t acc = 0;
+    for (int64_t r = 0; r < rows; ++r) {
+        pptr[r] = (int32_t)acc;
+        acc += ((int64_t)rptr[r + 1] - rptr[r] + 3) & ~int64_t(3);
     }
-    // blk holds each panel's boundaries once: first rows of all blocks, then n
-    std::vector<int32_t> bp(blk.size()), bq(blk.size());
-    for (size_t b = 0; b < blk.size(); ++b) bp[b] = ptr[blk[b]];
-    bq[0] = 0;
-    for (size_t b = 0; b + 1 < blk.size(); ++b) bq[b + 1] = bq[b] + ((bp[b + 1] - bp[b] + 3) & ~3);
-    const int64_t npad = bq.back();
+    TV3_REQUIRE(acc < (int64_t(1) << 31), TV3_E_UNSUPPORTED, "padded panel entries exceed 2^31");
+    pptr[rows] = (int32_t)acc;
+    std::vector<int32_t> blk{0}, pan_blk{0};
+    std::vector<int64_t> work(npan);
+    int max_src = 0;
+    for (int64_t p = 0; p < npan; ++p) {
+        const int64_t r0 = pan_row[p], r1 = pan_row[p + 1];
+        // a panel without rows (a tile no ray crosses, in the segment form of
+        // A) gets no row block at all
+        if (r1 > r0) {
+            int64_t start = r0;
+            for (int64_t r = r0; r < r1; ++r)
+                if (r > start &&
+                    ((int64_t)pptr[r + 1] - pptr[start] > blk_nnz || r - start >= 256)) {
+                    blk.push_back((int32_t)r);
+                    start = r;
+                }
+            blk.push_back((int32_t)r1);
+        }
+        pan_blk.push_back((int32_t)blk.size() - 1);
+        const int ns = src_off[p + 1] - src_off[p];
+        max_src = std::max(max_src, ns);
+        work[p] = ((int64_t)pptr[r1] - pptr[r0]) + 4 * (int64_t)ns;
+    }
     std::vector<uint32_t> pk;
     std::vector<uint16_t> ps;
     std::vector<float> pv;
-    if (vb == 2) pk.assign(npad > 0 ? npad : 1, 0u);
-    else { ps.assign(npad > 0 ? npad : 1, 0); pv.assign(npad > 0 ? npad : 1, 0.f); }
-    for (size_t b = 0; b + 1 < blk.size(); ++b)
-        for (int32_t k = bp[b]; k < bp[b + 1]; ++k) {
-            const int64_t q = bq[b] + (k - bp[b]);
+    const size_t np = acc > 0 ? (size_t)acc : 1;
+    if (vb == 2) pk.assign(np, 0u);
+    else { ps.assign(np, 0); pv.assign(np, 0.f); }
+    for (int64_t r = 0; r < rows; ++r)
+        for (int32_t k = rptr[r]; k < rptr[r + 1]; ++k) {
+            const int64_t q = (int64_t)pptr[r] + (k - rptr[r]);
             if (vb == 2) {
-                uint16_t wbits;
-                std::memcpy(&wbits, &val[(size_t)k * 2], 2);
-                pk[q] = (uint32_t)sidx[k] | ((uint32_t)wbits << 16);
+                uint16_t bits;
+                std::memcpy(&bits, wb + (size_t)k * 2, 2);
+                pk[q] = (uint32_t)lidx[k] | ((uint32_t)bits << 16);
             } else {
-                ps[q] = sidx[k];
-                std::memcpy(&pv[q], &val[(size_t)k * 4], 4);
+                ps[q] = lidx[k];
+                std::memcpy(&pv[q], wb + (size_t)k * 4, 4);
             }
         }
     std::vector<int32_t> order(npan);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return pan_work[a] > pan_work[b]; });
+                     [&](int32_t a, int32_t b) { return work[a] > work[b]; });
+    P.rows = rows;
     P.npan = npan;
-    P.nseg = (int64_t)seg_ray.size();
-    P.max_seg = (int)max_seg;
+    P.nsrc = src ? (int64_t)src->size() : 0;
+    P.max_src = max_src;
     P.blk_nnz = blk_nnz;
-    auto up = [&](DevBuf<int32_t> &d, const std::vector<int32_t> &h) {
-        d.alloc(h.empty() ? 1 : h.size());
-        if (!h.empty())
-            TV3_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(int32_t) * h.size(),
+    P.contiguous = src == nullptr;
+    P.vtype = vt;
+    auto up = [&](DevBuf<int32_t> &d, const std::vector<int32_t> &hv) {
+        d.alloc(hv.empty() ? 1 : hv.size());
+        if (!hv.empty())
+            TV3_HIP(hipMemcpyAsync(d.p, hv.data(), sizeof(int32_t) * hv.size(),
                                    hipMemcpyHostToDevice, s));
     };
     up(P.order, order);
-    up(P.pan_seg, pan_seg);
+    up(P.src_off, src_off);
     up(P.pan_blk, pan_blk);
-    up(P.seg_ray, seg_ray);
-    up(P.blk, blk);
-    up(P.bptr, bp);
-    up(P.bpad, bq);
-    P.vtype = S.vtype;
+    if (src) up(P.src, *src);
+    else P.src.release();
+    up(P.pptr, pptr);
+    const std::vector<BlkDesc> d = make_blk_desc(blk, pptr.data(), nullptr);
+    P.desc.alloc(d.size());
+    TV3_HIP(hipMemcpyAsync(P.desc.p, d.data(), sizeof(BlkDesc) * d.size(), hipMemcpyHostToDevice,
+                           s));
     if (vb == 2) {
         P.pk.alloc(pk.size());
+        P.ps.release();
+        P.pv.release();
         TV3_HIP(hipMemcpyAsync(P.pk.p, pk.data(), sizeof(uint32_t) * pk.size(),
                                hipMemcpyHostToDevice, s));
     } else {
-        P.sidx.alloc(ps.size());
-        P.pval.alloc(pv.size());
-        TV3_HIP(hipMemcpyAsync(P.sidx.p, ps.data(), sizeof(uint16_t) * ps.size(),
+        P.pk.release();
+        P.ps.alloc(ps.size());
+        P.pv.alloc(pv.size());
+        TV3_HIP(hipMemcpyAsync(P.ps.p, ps.data(), sizeof(uint16_t) * ps.size(),
                                hipMemcpyHostToDevice, s));
-        TV3_HIP(hipMemcpyAsync(P.pval.p, pv.data(), sizeof(float) * pv.size(),
+        TV3_HIP(hipMemcpyAsync(P.pv.p, pv.data(), sizeof(float) * pv.size(),
                                hipMemcpyHostToDevice, s));
     }
     TV3_HIP(hipStreamSynchronize(s));
-    return true;
+}
+
+// A^T in panel form: rows = voxels, sources = the tile's rays.
+static void build_panels_T(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, int blk_nnz,
+                           Panels &P) {
+    std::vector<int32_t> local(h.cols > 0 ? h.cols : 1, 0);
+    std::vector<uint16_t> lidx(h.nnz > 0 ? h.nnz : 1);
+    const int64_t npan = (int64_t)pl.pan_row.size() - 1;
+    for (int64_t p = 0; p < npan; ++p) {
+        for (int32_t j = pl.pan_seg[p]; j < pl.pan_seg[p + 1]; ++j)
+            local[pl.seg_ray[j]] = j - pl.pan_seg[p];
+        for (int64_t k = h.ptr[pl.pan_row[p]]; k < h.ptr[pl.pan_row[p + 1]]; ++k)
+            lidx[k] = (uint16_t)local[h.ind[k]];
+    }
+    pack_panels(ctx, pl.pan_row, pl.pan_seg, &pl.seg_ray, h.ptr, lidx, h.val.data(), h.vb,
+                h.vtype, blk_nnz, P);
+}
+
+// A in panel form: rows = ray segments (segment s = pan_seg[p] + position of
+// its ray in panel p's list), sources = the panel's voxel rows; and the
+// per-ray segment lists of the second pass.
+static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, int blk_nnz,
+                           Panels &P, SegSum &R) {
+    const int64_t npan = (int64_t)pl.pan_row.size() - 1, nseg = (int64_t)pl.seg_ray.size();
+    const int64_t m = h.cols;
+    std::vector<int32_t> local(m > 0 ? m : 1, 0), sptr(nseg + 1, 0);
+    // entries per segment
+    for (int64_t p = 0; p < npan; ++p) {
+        for (int32_t j = pl.pan_seg[p]; j < pl.pan_seg[p + 1]; ++j) local[pl.seg_ray[j]] = j;
+        for (int64_t k = h.ptr[pl.pan_row[p]]; k < h.ptr[pl.pan_row[p + 1]]; ++k)
+            ++sptr[local[h.ind[k]] + 1];
+    }
+    for (int64_t s = 0; s < nseg; ++s) sptr[s + 1] += sptr[s];
+    std::vector<int32_t> pos(sptr.begin(), sptr.end() - 1);
+    std::vector<uint16_t> lidx(h.nnz > 0 ? h.nnz : 1);
+    std::vector<uint8_t> wb((h.nnz > 0 ? h.nnz : 1) * h.vb);
+    for (int64_t p = 0; p < npan; ++p) {
+        for (int32_t j = pl.pan_seg[p]; j < pl.pan_seg[p + 1]; ++j) local[pl.seg_ray[j]] = j;
+        const int32_t v0 = pl.pan_row[p];
+        for (int32_t v = v0; v < pl.pan_row[p + 1]; ++v)
+            for (int32_t k = h.ptr[v]; k < h.ptr[v + 1]; ++k) {
+                const int32_t q = pos[local[h.ind[k]]]++;
+                lidx[q] = (uint16_t)(v - v0);
+                std::memcpy(&wb[(size_t)q * h.vb], &h.val[(size_t)k * h.vb], h.vb);
+            }
+    }
+    // panels of the segment matrix: segment ranges pan_seg, voxel ranges pan_row
+    pack_panels(ctx, pl.pan_seg, pl.pan_row, nullptr, sptr, lidx, wb.data(), h.vb, h.vtype,
+                blk_nnz, P);
+    // segments of each ray, in increasing segment number
+    std::vector<int32_t> rptr(m + 1, 0), rseg(nseg > 0 ? nseg : 1);
+    for (int64_t s = 0; s < nseg; ++s) ++rptr[pl.seg_ray[s] + 1];
+    for (int64_t r = 0; r < m; ++r) rptr[r + 1] += rptr[r];
+    std::vector<int32_t> rpos(rptr.begin(), rptr.end() - 1);
+    for (int64_t s = 0; s < nseg; ++s) rseg[rpos[pl.seg_ray[s]]++] = (int32_t)s;
+    hipStream_t st = ctx->stream;
+    R.rows = m;
+    R.nseg = nseg;
+    const double per_ray = m > 0 ? (double)nseg / (double)m : 0.0;
+    R.lanes = per_ray > 24 ? 16 : per_ray > 12 ? 8 : 4;
+    R.ptr.alloc(m + 1);
+    R.seg.alloc(rseg.size());
+    R.part.alloc(nseg > 0 ? nseg : 1);
+    TV3_HIP(hipMemcpyAsync(R.ptr.p, rptr.data(), sizeof(int32_t) * (m + 1), hipMemcpyHostToDevice,
+                           st));
+    TV3_HIP(hipMemcpyAsync(R.seg.p, rseg.data(), sizeof(int32_t) * rseg.size(),
+                           hipMemcpyHostToDevice, st));
+    TV3_HIP(hipStreamSynchronize(st));
 }
 
 // Average duration (ms) of `reps` launches of f on the context stream after
@@ -301,25 +416,40 @@ static float tune_shape(tv3_ctx *ctx, SparseRows &S, int64_t xlen) {
     return best;
 }
 
-// Panels of the CSC (tiles of 512 internal voxels, panels of at most 16384
-// entries) timed over row blocks of 1024/2048/4096 entries; kept when faster
-// than `best` (the tuned plain kernel's time).
-static bool tune_panels(tv3_ctx *ctx, FusedLayout &F, int64_t xlen, float best) {
+// Panel forms of A^T and A (tiles of 512 internal voxels, panels of at most
+// 16384 entries) timed over row blocks of 1024 and 2048 entries; each kept
+// when faster than the tuned plain kernel (t_A, t_T in ms).
+static void tune_panels(tv3_ctx *ctx, FusedLayout &F, float t_A, float t_T) {
     hipStream_t s = ctx->stream;
-    DevBuf<float> x(xlen > 0 ? xlen : 1), y(F.AT.rows > 0 ? F.AT.rows : 1);
-    TV3_HIP(hipMemsetAsync(x.p, 0, x.bytes(), s));
-    PanelT P;
-    int choice = 0;
-    for (int nb : {1024, 2048, 4096}) {
-        if (!build_panels(ctx, F.AT, 512, 16384, nb, P)) return false;
+    const int64_t n = F.AT.rows, m = F.A.rows;
+    const size_t vb = dtype_size(F.AT.vtype);
+    if (vb != 2 && F.AT.vtype != TV3_F32) return;
+    const HostCSR h = download_csr(ctx, F.AT);
+    PanelPlan pl;
+    if (!plan_panels(h, 512, 16384, pl)) return;
+    F.npan = (int64_t)pl.pan_row.size() - 1;
+    F.nseg = (int64_t)pl.seg_ray.size();
+    F.max_seg = (int)pl.max_seg;
+    DevBuf<float> xr(m > 0 ? m : 1), xv(n > 0 ? n : 1), y(n > m ? n : (m > 0 ? m : 1));
+    TV3_HIP(hipMemsetAsync(xr.p, 0, xr.bytes(), s));
+    TV3_HIP(hipMemsetAsync(xv.p, 0, xv.bytes(), s));
+    for (int nb : {1024, 2048}) {
+        Panels P;
+        build_panels_T(ctx, h, pl, nb, P);
         const float t = time_launches(
-            ctx, [&]() { launch_panel<float>(P, F.AT, x.p, EpiStore<float>{y.p}, s); });
-        if (t < best) { best = t; choice = nb; }
+            ctx, [&]() { launch_panel(P, xr.p, EpiStoreIf{y.p, nullptr}, s); });
+        if (t < t_T) { t_T = t; F.PT = std::move(P); F.panels_T = true; }
     }
-    if (!choice) return false;
-    if (choice != 4096) build_panels(ctx, F.AT, 512, 16384, choice, P);
-    F.PT = std::move(P);
-    return true;
+    for (int nb : {1024, 2048}) {
+        Panels P;
+        SegSum R;
+        build_panels_A(ctx, h, pl, nb, P, R);
+        const float t = time_launches(ctx, [&]() {
+            launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
+            launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
+        });
+        if (t < t_A) { t_A = t; F.PA = std::move(P); F.RS = std::move(R); F.panels_A = true; }
+    }
 }
 
 FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
@@ -371,9 +501,9 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
         throw tv3_error(TV3_E_FORMAT, "the fused solver needs F32, F16 or BF16 weights");
     TV3_HIP(hipGetLastError());
     build_transpose(ctx, N, L->AT);
-    tune_shape(ctx, N, n);
+    const float t_a = tune_shape(ctx, N, n);
     const float t_at = tune_shape(ctx, L->AT, m);
-    L->use_panels = tune_panels(ctx, *L, m, t_at);
+    tune_panels(ctx, *L, t_a, t_at);
     TV3_HIP(hipStreamSynchronize(s));
     op->fused = std::move(L);
     return *op->fused;
@@ -391,7 +521,7 @@ void layout_matvec(tv3_op *op, const int32_t dims[3], bool trans, const float *x
         permute_gather(yi.p, F.v_nat2int.p, y, n, s);
     } else {
         permute_gather(x, F.v_int2nat.p, xi.p, n, s);
-        launch_product<float>(F.A, xi.p, EpiStore<float>{yi.p}, s);
+        launch_forward<float>(F, xi.p, EpiStore<float>{yi.p}, s);
         permute_scatter(yi.p, F.r_int2nat.p, y, m, s);
     }
     TV3_HIP(hipGetLastError());
@@ -403,21 +533,30 @@ static void apply_code(tv3_ctx *ctx, SparseRows &S, int code) {
     else if (code < 0) build_row_blocks(ctx, S, -code);
 }
 
-void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz) {
+void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz,
+                  int panels) {
     FusedLayout &F = fused_layout(op, dims);
-    apply_code(op->ctx, F.A, code_N);
-    if (pan_nnz > 0) {
-        TV3_REQUIRE(code_T == -1024 || code_T == -2048 || code_T == -4096, TV3_E_ARG,
-                    "panels need a block size code of -1024, -2048 or -4096");
-        PanelT P;
-        TV3_REQUIRE(build_panels(op->ctx, F.AT, 512, pan_nnz, -code_T, P), TV3_E_UNSUPPORTED,
+    tv3_ctx *ctx = op->ctx;
+    const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0;
+    if (pa || pt) {
+        TV3_REQUIRE(pan_nnz > 0, TV3_E_ARG, "panels need pan_nnz > 0");
+        for (int c : {pa ? code_N : -1024, pt ? code_T : -1024})
+            TV3_REQUIRE(c == -1024 || c == -2048 || c == -4096, TV3_E_ARG,
+                        "panel block sizes are -1024, -2048 or -4096");
+        const HostCSR h = download_csr(ctx, F.AT);
+        PanelPlan pl;
+        TV3_REQUIRE(plan_panels(h, 512, pan_nnz, pl), TV3_E_UNSUPPORTED,
                     "a panel of this operator would need more rays than LDS holds");
-        F.PT = std::move(P);
-        F.use_panels = true;
-    } else {
-        F.use_panels = false;
-        apply_code(op->ctx, F.AT, code_T);
+        F.npan = (int64_t)pl.pan_row.size() - 1;
+        F.nseg = (int64_t)pl.seg_ray.size();
+        F.max_seg = (int)pl.max_seg;
+        if (pt) build_panels_T(ctx, h, pl, -code_T, F.PT);
+        if (pa) build_panels_A(ctx, h, pl, -code_N, F.PA, F.RS);
     }
+    F.panels_A = pa;
+    F.panels_T = pt;
+    if (!pa) apply_code(ctx, F.A, code_N);
+    if (!pt) apply_code(ctx, F.AT, code_T);
 }
 
 __global__ void k_fill(float *p, float v, int64_t n) {
@@ -439,7 +578,7 @@ double layout_time(tv3_op *op, const int32_t dims[3], bool trans, int reps) {
         ctx,
         [&]() {
             if (trans) launch_transposed<float>(F, x.p, EpiStore<float>{y.p}, s);
-            else launch_product<float>(F.A, x.p, EpiStore<float>{y.p}, s);
+            else launch_forward<float>(F, x.p, EpiStore<float>{y.p}, s);
         },
         reps > 0 ? reps : 1);
     return 1e3 * (double)ms;

@@ -1,0 +1,260 @@
+// panels.hpp -- LDS-staged panel products for gfx950.
+//
+// A^T r and A x over 128^3 ray matrices are bound by scattered gathers: every
+// entry of a row reads x at an unrelated index, and the L1 can look up only
+// about one cache line per clock.  The panel form (Panels) groups rows so
+// that a workgroup first brings the few thousand x values a panel reads into
+// LDS (one coalesced or lightly scattered pass), after which every entry
+// reads x from LDS through a 16-bit index.
+//
+//   A^T r: rows = voxels of an 8x8x8 tile, sources = the rays crossing it.
+//   A u:   pass 1 rows = ray segments (the part of one ray inside one tile),
+//          sources = the tile's 512 voxels (a contiguous range); pass 2 sums
+//          each ray's segment partials (k_seg_sum) and runs the epilogue.
+//
+// Phase 1 of a row block: each thread loads groups of 4 entries of one row
+// (16 bytes), multiplies with the staged x and sums the group in registers;
+// phase 2 sums each row's group sums (a quarter of the entries) from LDS.
+#pragma once
+
+#include <type_traits>
+
+#include "spmv_stream.hpp"
+
+namespace tv3 {
+
+// weight of a packed entry (bits 16..31)
+template <typename V> __device__ __forceinline__ float pk_weight(uint32_t u);
+template <> __device__ __forceinline__ float pk_weight<__half>(uint32_t u) {
+    return __half2float(__ushort_as_half((unsigned short)(u >> 16)));
+}
+template <> __device__ __forceinline__ float pk_weight<__hip_bfloat16>(uint32_t u) {
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+// sum of the 4 products of the entry group at padded position q
+template <typename V>
+__device__ __forceinline__ float group_sum(const uint32_t *pk, const uint16_t *, const float *,
+                                          int64_t q, const float *xs) {
+    const uint4 u = *reinterpret_cast<const uint4 *>(pk + q);
+    const float a = pk_weight<V>(u.x) * xs[u.x & 0xffffu];
+    const float b = pk_weight<V>(u.y) * xs[u.y & 0xffffu];
+    const float c = pk_weight<V>(u.z) * xs[u.z & 0xffffu];
+    const float d = pk_weight<V>(u.w) * xs[u.w & 0xffffu];
+    return (a + b) + (c + d);
+}
+template <>
+__device__ __forceinline__ float group_sum<float>(const uint32_t *, const uint16_t *ps,
+                                                 const float *pv, int64_t q, const float *xs) {
+    const uint2 u = *reinterpret_cast<const uint2 *>(ps + q);
+    const float4 w = *reinterpret_cast<const float4 *>(pv + q);
+    const float a = w.x * xs[u.x & 0xffffu], b = w.y * xs[u.x >> 16];
+    const float c = w.z * xs[u.y & 0xffffu], d = w.w * xs[u.y >> 16];
+    return (a + b) + (c + d);
+}
+
+// Epilogue that stores y[row] = sum unless the predicate *active is 0.
+struct EpiStoreIf {
+    float *y;
+    const int32_t *active;     // NULL: always
+    struct Pre {};
+    __device__ bool skip() const { return active && !*active; }
+    __device__ __forceinline__ Pre load(int) const { return {}; }
+    __device__ __forceinline__ void row(int r, float s, bool on, const Pre &) const {
+        if (on) y[r] = s;
+    }
+    __device__ __forceinline__ void finish() const {}
+};
+
+// One workgroup per panel (largest panels first, so the launch ends with
+// small ones).  Block descriptors: p0 / len = padded start / padded length.
+template <int NB, typename V, bool CONTIG, typename Epi>
+__global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order,
+                                               const int32_t *__restrict__ src_off,
+                                               const int32_t *__restrict__ pan_blk,
+                                               const int32_t *__restrict__ src,
+                                               const BlkDesc *__restrict__ desc,
+                                               const int32_t *__restrict__ pptr,
+                                               const uint32_t *__restrict__ pk,
+                                               const uint16_t *__restrict__ ps,
+                                               const float *__restrict__ pv,
+                                               const float *__restrict__ x, Epi epi) {
+    constexpr int NG = NB / 4;               // entry groups per block
+    extern __shared__ float xs[];            // the panel's source values
+    __shared__ float gs[NG];                 // group sums
+    if (epi.skip()) return;
+    const int t = threadIdx.x;
+    const int pan = order[blockIdx.x];
+    const int s0 = src_off[pan], ns = src_off[pan + 1] - s0;
+    const int bb = pan_blk[pan], b1 = pan_blk[pan + 1];
+    BlkDesc d = desc[bb];
+    // stage the sources: batches of 8 per thread, all loads in flight
+    for (int k0 = 0; k0 < ns; k0 += 8 * 256) {
+        float xv[8];
+        if (CONTIG) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xv[i] = x[s0 + min(k0 + i * 256 + t, ns - 1)];
+        } else {
+            int rv[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) rv[i] = src[s0 + min(k0 + i * 256 + t, ns - 1)];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xv[i] = x[rv[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (k0 + i * 256 + t < ns) xs[k0 + i * 256 + t] = xv[i];
+    }
+    __syncthreads();
+    for (int b = bb; b < b1; ++b) {
+        const BlkDesc dn = desc[b + 1];      // in flight while this block runs
+        if (d.len <= NB) {
+            const int row = block_row(d);
+            const int ga = (pptr[row] - d.p0) >> 2, ge = (pptr[row + 1] - d.p0) >> 2;
+            const auto pre = epi.load(row);
+            if (d.len > 0) {
+                const int glast = (d.len >> 2) - 1;
+                float v[NG / 256];
+#pragma unroll
+                for (int j = 0; j < NG / 256; ++j)
+                    v[j] = group_sum<V>(pk, ps, pv, (int64_t)d.p0 + 4 * min(j * 256 + t, glast),
+                                        xs);
+#pragma unroll
+                for (int j = 0; j < NG / 256; ++j) gs[j * 256 + t] = v[j];
+            }
+            __syncthreads();
+            block_rows<NG>(gs, d, ga, ge, epi, row, pre);
+        } else {
+            // one row longer than NB
+            const auto pre = epi.load(d.r0);
+            float s = 0;
+            for (int g = t; g < (d.len >> 2); g += 256)
+                s += group_sum<V>(pk, ps, pv, (int64_t)d.p0 + 4 * g, xs);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if ((t & 63) == 0) gs[t >> 6] = s;
+            __syncthreads();
+            if (t == 0) s = (gs[0] + gs[1]) + (gs[2] + gs[3]);
+            epi.row(d.r0, s, t == 0, pre);
+        }
+        __syncthreads();             // gs is rewritten by the next row block
+        d = dn;
+    }
+    epi.finish();
+}
+
+// Pass 2 of A x: L lanes per ray sum its segment partials (batches of 4 per
+// lane, loads clamped to the ray's last segment), then the epilogue.
+template <int L, typename Epi>
+__global__ __launch_bounds__(256) void k_seg_sum(int64_t rows, const int32_t *__restrict__ ptr,
+                                                 const int32_t *__restrict__ seg,
+                                                 const float *__restrict__ part, Epi epi,
+                                                 int64_t nlogical) {
+    if (epi.skip()) return;
+    const int64_t G = gridDim.x;
+    const int64_t chunk = (nlogical + G - 1) / G;
+    const int64_t w = xcd_remap(blockIdx.x, (int)G);
+    const int64_t l0 = w * chunk, l1 = min(nlogical, l0 + chunk);
+    const int lane = threadIdx.x & (L - 1);
+    for (int64_t lb = l0; lb < l1; ++lb) {
+        const int64_t row0 = (lb * 256 + threadIdx.x) / L;
+        const bool has = row0 < rows;
+        const int64_t row = has ? row0 : rows - 1;
+        const int beg = ptr[row], end = has ? ptr[row + 1] : beg;
+        const auto pre = epi.load((int)row);
+        float s0 = 0, s1 = 0;
+        for (int k0 = beg + lane; k0 < end; k0 += 4 * L) {
+            int c[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) c[i] = seg[min(k0 + i * L, end - 1)];
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = part[c[i]];
+            s0 += v[0] + (k0 + 2 * L < end ? v[2] : 0.f);
+            s1 += (k0 + L < end ? v[1] : 0.f) + (k0 + 3 * L < end ? v[3] : 0.f);
+        }
+        float s = s0 + s1;
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+        epi.row((int)row, s, has && lane == 0, pre);
+    }
+    epi.finish();
+}
+
+template <typename V, bool CONTIG, typename Epi>
+void launch_panel_v(const Panels &P, const float *x, const Epi &epi, hipStream_t s) {
+    const unsigned g = (unsigned)P.npan;
+    const size_t lds = sizeof(float) * (size_t)(P.max_src > 0 ? P.max_src : 1);
+    if (g == 0) return;
+#define TV3_PCASE(NBV)                                                                        \
+    case NBV:                                                                                 \
+        hipLaunchKernelGGL((k_panel<NBV, V, CONTIG, Epi>), dim3(g), dim3(256), lds, s,          \
+                           P.order.p, P.src_off.p, P.pan_blk.p, P.src.p, P.desc.p, P.pptr.p,  \
+                           P.pk.p, P.ps.p, P.pv.p, x, epi);                                   \
+        break;
+    switch (P.blk_nnz) {
+        TV3_PCASE(1024) TV3_PCASE(2048) TV3_PCASE(4096)
+    default: throw tv3_error(TV3_E_ARG, "panel block size must be 1024, 2048 or 4096");
+    }
+#undef TV3_PCASE
+}
+
+// Panel product y = M x with epilogue epi (FP32 accumulation).
+template <typename Epi>
+void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s) {
+    switch (P.vtype) {
+    case TV3_F32:
+        P.contiguous ? launch_panel_v<float, true, Epi>(P, x, epi, s)
+                     : launch_panel_v<float, false, Epi>(P, x, epi, s);
+        break;
+    case TV3_F16:
+        P.contiguous ? launch_panel_v<__half, true, Epi>(P, x, epi, s)
+                     : launch_panel_v<__half, false, Epi>(P, x, epi, s);
+        break;
+    case TV3_BF16:
+        P.contiguous ? launch_panel_v<__hip_bfloat16, true, Epi>(P, x, epi, s)
+                     : launch_panel_v<__hip_bfloat16, false, Epi>(P, x, epi, s);
+        break;
+    default: throw tv3_error(TV3_E_FORMAT, "panels need F32, F16 or BF16 weights");
+    }
+}
+
+template <typename Epi>
+void launch_seg_sum(const SegSum &R, const Epi &epi, hipStream_t s) {
+    const int L = R.lanes;
+    const int64_t nl = (R.rows * L + 255) / 256;
+    const unsigned g = (unsigned)(nl < kMaxProductGrid ? nl : kMaxProductGrid);
+    if (g == 0) return;
+#define TV3_SCASE(LL)                                                                     \
+    case LL:                                                                              \
+        hipLaunchKernelGGL((k_seg_sum<LL, Epi>), dim3(g), dim3(256), 0, s, R.rows, R.ptr.p, \
+                           R.seg.p, R.part.p, epi, nl);                                   \
+        break;
+    switch (L) {
+        TV3_SCASE(2) TV3_SCASE(4) TV3_SCASE(8) TV3_SCASE(16) TV3_SCASE(32)
+    default: throw tv3_error(TV3_E_ARG, "segment-sum lanes must be a power of two in [2, 32]");
+    }
+#undef TV3_SCASE
+}
+
+// A x of a fused layout (rows = rays) with epilogue epi; `active` (may be
+// NULL) predicates the first pass of the panel form like epi.skip().
+template <typename Acc, typename Epi>
+void launch_forward(const FusedLayout &F, const float *x, const Epi &epi, hipStream_t s,
+                    const int32_t *active = nullptr) {
+    if (F.panels_A) {
+        launch_panel(F.PA, x, EpiStoreIf{F.RS.part.p, active}, s);
+        launch_seg_sum(F.RS, epi, s);
+    } else {
+        launch_product<Acc, Epi>(F.A, x, epi, s);
+    }
+}
+
+// A^T r of a fused layout (rows = voxels) with epilogue epi.
+template <typename Acc, typename Epi>
+void launch_transposed(const FusedLayout &F, const float *x, const Epi &epi, hipStream_t s) {
+    if (F.panels_T) launch_panel(F.PT, x, epi, s);
+    else launch_product<Acc, Epi>(F.AT, x, epi, s);
+}
+
+}  // namespace tv3

@@ -37,7 +37,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "spmv_stream.hpp"
+#include "panels.hpp"
 #include "vec.hpp"
 
 namespace tv3 {
@@ -650,7 +650,10 @@ __global__ void k_set_dd(FCtl *C, const double *p, int nb) {
 }
 
 // ------------------------------------------------------------------ host side
-int product_blocks(const SparseRows &S) {
+// workgroups of the launch that runs the A x epilogue (block partial sums)
+int product_blocks(const FusedLayout &F) {
+    if (F.panels_A) return (int)((F.RS.rows * F.RS.lanes + 255) / 256);
+    const SparseRows &S = F.A;
     if (S.blk_nnz) return (int)S.nblk;
     const int L = S.lanes ? S.lanes : 32;
     return (int)((S.rows * L + 255) / 256);
@@ -692,7 +695,7 @@ struct Fused {
         tmp = tmpbuf.p;
         G = groups_of<B>(L);
         nbS = (int)stencil_grid(G);
-        nbN = product_blocks(F->A);
+        nbN = product_blocks(*F);
         nbM = 2048;
         const size_t tot = 3 * (size_t)nbS + 3 * (size_t)nbN + 7 * (size_t)nbS + (size_t)nbS +
                            (size_t)nbN + (size_t)nbS + nbM + nbM + nbM;
@@ -727,7 +730,8 @@ struct Fused {
         const dim3 gS(nbS);
         hipLaunchKernelGGL(k_descend<B>, gS, dim3(256), 0, s, L, G, U[c], d, sx, sy, sz, ox, oy,
                            oz, U[n_], ctl.p, pD);
-        launch_product<float>(F->A, U[n_], EpiRes{Au[n_], b, delta, Au[c], ctl.p, pN, {0, 0, 0}}, s);
+        launch_forward<float>(*F, U[n_], EpiRes{Au[n_], b, delta, Au[c], ctl.p, pN, {0, 0, 0}}, s,
+                              &ctl.p->active);
         launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p}, s);
         hipLaunchKernelGGL((k_stage2<3, 3>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pD, nbS, pN, nbN,
                            st2.p);
@@ -743,7 +747,7 @@ struct Fused {
 
     void tau_first(int fallback) {
         hipLaunchKernelGGL(k_dd_norm<B>, dim3(nbS), dim3(256), 0, s, L, G, d, pDd);
-        launch_product<float>(F->A, d, EpiNorm{ctl.p, pAd, {0}}, s);
+        launch_forward<float>(*F, d, EpiNorm{ctl.p, pAd, {0}}, s);
         hipLaunchKernelGGL(k_tau_first, dim3(1), dim3(1024), 0, s, ctl.p, pDd, nbS, pAd, nbN,
                            fallback);
     }
@@ -819,7 +823,8 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
         TV3_HIP(hipMemcpyAsync(S.U[0], S.Atb, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     }
     TV3_HIP(hipMemcpyAsync(S.rcd, S.U[0], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    launch_product<float>(FL.A, S.U[0], EpiRes{S.Au[0], S.b, S.delta, S.Au[0], S.ctl.p, S.pN, {0, 0, 0}}, s);
+    launch_forward<float>(FL, S.U[0],
+                          EpiRes{S.Au[0], S.b, S.delta, S.Au[0], S.ctl.p, S.pN, {0, 0, 0}}, s);
     launch_transposed<float>(FL, S.Au[0], EpiGrad{S.g[0], S.Atb, S.ctl.p}, s);
     hipLaunchKernelGGL(k_update_w<B>, dim3(S.nbS), dim3(256), 0, s, S.L, S.G, S.U[0], S.U[0], S.sx,
                        S.sy, S.sz, S.g[0], S.dts, S.d, S.ctl.p, S.pW);

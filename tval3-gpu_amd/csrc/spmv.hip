@@ -137,16 +137,38 @@ void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz) {
     }
     blk.push_back((int32_t)S.rows);
     S.nblk = (int64_t)blk.size() - 1;
-    std::vector<int32_t> bp(blk.size());
-    for (size_t b = 0; b < blk.size(); ++b) bp[b] = ptr[blk[b]];
-    S.blk.alloc(blk.size());
-    S.bptr.alloc(blk.size());
-    TV3_HIP(hipMemcpyAsync(S.blk.p, blk.data(), sizeof(int32_t) * blk.size(),
-                           hipMemcpyHostToDevice, ctx->stream));
-    TV3_HIP(hipMemcpyAsync(S.bptr.p, bp.data(), sizeof(int32_t) * bp.size(),
-                           hipMemcpyHostToDevice, ctx->stream));
+    const std::vector<BlkDesc> d = make_blk_desc(blk, ptr.data(), nullptr);
+    S.desc.alloc(d.size());
+    TV3_HIP(hipMemcpyAsync(S.desc.p, d.data(), sizeof(BlkDesc) * d.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
     TV3_HIP(hipStreamSynchronize(ctx->stream));
     S.blk_nnz = blk_nnz;
+}
+
+std::vector<BlkDesc> make_blk_desc(const std::vector<int32_t> &blk, const int32_t *ptr,
+                                   const std::vector<int32_t> *pad_start) {
+    const size_t nb = blk.empty() ? 0 : blk.size() - 1;
+    std::vector<BlkDesc> d(nb + 1);
+    for (size_t b = 0; b < nb; ++b) {
+        BlkDesc &e = d[b];
+        e.r0 = blk[b];
+        e.nr = blk[b + 1] - blk[b];
+        // kernels serve row r0 + min(lane group, nr - 1): a block needs a row
+        TV3_REQUIRE(e.nr > 0, TV3_E_FORMAT, "row block without rows");
+        e.p0 = ptr[blk[b]];
+        e.len = ptr[blk[b + 1]] - e.p0;
+        e.q0 = pad_start ? (*pad_start)[b] : e.p0;
+        int lg = 6;
+        while (lg > 0 && (1 << lg) * e.nr > 256) --lg;
+        e.lg = lg;
+        e.pad0 = e.pad1 = 0;
+    }
+    BlkDesc &z = d[nb];
+    z.r0 = nb ? blk[nb] : 0;
+    z.nr = z.len = z.lg = z.pad0 = z.pad1 = 0;
+    z.p0 = z.q0 = nb ? ptr[blk[nb]] : 0;
+    if (pad_start && nb) z.q0 = (*pad_start)[nb];
+    return d;
 }
 
 void spmv_csr(const SparseRows &S, const float *x, float *y, hipStream_t s, int lanes,

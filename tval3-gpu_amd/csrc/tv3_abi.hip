@@ -652,12 +652,12 @@ static void fill_layout_info(const FusedLayout &F, tv3_layout_info *info) {
     std::memset(info, 0, sizeof(*info));
     auto code = [](const SparseRows &S) { return S.blk_nnz ? -S.blk_nnz : S.lanes; };
     info->brick = F.B;
-    info->panels = F.use_panels ? 1 : 0;
-    info->code_N = code(F.A);
-    info->code_T = F.use_panels ? -F.PT.blk_nnz : code(F.AT);
-    info->npan = F.use_panels ? F.PT.npan : 0;
-    info->nseg = F.use_panels ? F.PT.nseg : 0;
-    info->max_seg = F.use_panels ? F.PT.max_seg : 0;
+    info->panels = (F.panels_A ? 1 : 0) | (F.panels_T ? 2 : 0);
+    info->code_N = F.panels_A ? -F.PA.blk_nnz : code(F.A);
+    info->code_T = F.panels_T ? -F.PT.blk_nnz : code(F.AT);
+    info->npan = F.npan;
+    info->nseg = F.nseg;
+    info->max_seg = F.max_seg;
 }
 
 static void check_dims(const tv3_op *op, const int32_t dims[3]) {
@@ -676,17 +676,17 @@ tv3_status tv3_op_prepare(tv3_op *op, const int32_t dims[3], tv3_layout_info *in
 }
 
 tv3_status tv3_op_layout_force(tv3_op *op, const int32_t dims[3], int32_t code_N, int32_t code_T,
-                               int32_t pan_nnz, tv3_layout_info *info) {
+                               int32_t pan_nnz, int32_t panels, tv3_layout_info *info) {
     if (!op) return TV3_E_ARG;
     auto ok = [](int L) {
         return L == 0 || (L >= 2 && L <= 64 && (L & (L - 1)) == 0) || L == -1024 || L == -2048 ||
                L == -4096;
     };
-    if (!ok(code_N) || !ok(code_T) || pan_nnz < 0) return TV3_E_ARG;
+    if (!ok(code_N) || !ok(code_T) || pan_nnz < 0 || panels < 0 || panels > 3) return TV3_E_ARG;
     return guard(op->ctx, [&] {
         check_dims(op, dims);
         set_device(op->ctx);
-        layout_force(op, dims, code_N, code_T, pan_nnz);
+        layout_force(op, dims, code_N, code_T, pan_nnz, panels);
         fill_layout_info(fused_layout(op, dims), info);
     });
 }

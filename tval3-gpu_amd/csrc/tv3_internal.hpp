@@ -102,6 +102,20 @@ inline size_t dtype_size(tv3_dtype t) {
 
 enum class OpKind { CSR, DENSE, DYNAMIC, HYBRID };
 
+// Row block of the row-blocked product kernels, read with one 32-byte
+// scalar load: rows [r0, r0 + nr), entries [p0, p0 + len) of the CSR arrays
+// (q0: the block's first entry in a padded entry array, = p0 when none), lg =
+// log2 of the lanes per row in phase 2 (the largest power of two <= 64 with
+// (1 << lg) * nr <= 256).
+struct BlkDesc {
+    int32_t r0, nr, p0, len, q0, lg, pad0, pad1;
+};
+
+// Descriptors of the row blocks whose first rows are blk[0 .. nb) (blk[nb] =
+// end), plus one empty sentinel so that block b + 1 can always be read.
+std::vector<BlkDesc> make_blk_desc(const std::vector<int32_t> &blk, const int32_t *ptr,
+                                   const std::vector<int32_t> *pad_start);
+
 // Compressed sparse rows of a matrix (also used for the CSC = CSR of A^T).
 struct SparseRows {
     int64_t rows = 0, cols = 0, nnz = 0;
@@ -113,8 +127,7 @@ struct SparseRows {
     // blk_nnz entries (or one longer row); blk[b] = first row of block b
     int blk_nnz = 0;       // 0: use the row-group kernel
     int64_t nblk = 0;
-    DevBuf<int32_t> blk;   // nblk + 1 first rows
-    DevBuf<int32_t> bptr;  // nblk + 1 first entries (ptr[blk[b]])
+    DevBuf<BlkDesc> desc;  // nblk + 1 (sentinel)
 };
 
 // Partition rows into blocks for the stream kernel (host greedy over rowptr).
@@ -158,50 +171,58 @@ __host__ __device__ __forceinline__ int64_t brick_index(int B, int64_t pq, int p
     return (tile << 9) + (sub << 6) + in_brick;
 }
 
-// A^T split into panels for the LDS-staged product.  A panel is a run of
-// consecutive internal voxel rows (an 8x8x8 tile or a brick-aligned part of
-// one) together with the sorted list of the distinct rays its entries touch;
-// the product stages those rays' values in LDS once and every entry of the
-// panel addresses them with a 16-bit local index.  Row blocks (at most
-// blk_nnz entries and 256 rows, or one longer row) never cross a panel.
-// Entries are stored per block, padded to a multiple of 4 so that a thread
-// loads 4 of them with one 16-byte access: 16-bit weights packed with their
-// index into one 32-bit word (index low, weight bits high), F32 weights in a
-// separate array.
-struct PanelT {
-    int64_t npan = 0, nseg = 0;
-    int max_seg = 0;       // largest ray list of a panel (LDS floats)
+// A sparse matrix M stored for the LDS-staged panel product y = M x.  Rows
+// are grouped in panels (consecutive row ranges); a panel reads x only at
+// its sources -- a sorted list of indices, or a contiguous index range --
+// which the product stages in LDS once; every entry addresses its source by
+// a 16-bit position in that list.  Each row's entries are padded with zero
+// weights to a multiple of 4 so that one 16-byte load brings a group of 4
+// entries of one row; 16-bit weights are packed with their index into one
+// 32-bit word (index low, weight bits high), F32 weights live in a separate
+// array.  Row blocks (at most blk_nnz padded entries and 256 rows, or one
+// longer row) never cross a panel.
+struct Panels {
+    int64_t rows = 0, npan = 0, nsrc = 0;
+    int max_src = 0;           // largest source list of a panel (LDS floats)
     int blk_nnz = 0;
+    bool contiguous = false;   // sources of panel p: x[src_off[p] .. src_off[p + 1])
     tv3_dtype vtype = TV3_F16;
-    DevBuf<int32_t> order;     // npan panels, decreasing entry count
-    DevBuf<int32_t> pan_seg;   // npan + 1 offsets into seg_ray
-    DevBuf<int32_t> pan_blk;   // npan + 1 offsets into blk
-    DevBuf<int32_t> seg_ray;   // nseg internal ray indices
-    DevBuf<int32_t> blk, bptr; // row blocks: first row, first CSC entry (+1 sentinel)
-    DevBuf<int32_t> bpad;      // row blocks: first padded entry (+1 sentinel)
+    DevBuf<int32_t> order;     // npan panels, decreasing work
+    DevBuf<int32_t> src_off;   // npan + 1 offsets into src (or index ranges)
+    DevBuf<int32_t> pan_blk;   // npan + 1 offsets into desc
+    DevBuf<int32_t> src;       // nsrc source indices (empty when contiguous)
+    DevBuf<int32_t> pptr;      // rows + 1 padded row starts
+    DevBuf<BlkDesc> desc;      // row blocks (+1 sentinel); p0, len in padded entries
     DevBuf<uint32_t> pk;       // padded packed entries (16-bit weights)
-    DevBuf<uint16_t> sidx;     // padded local ray indices (F32 weights)
-    DevBuf<float> pval;        // padded weights (F32 weights)
+    DevBuf<uint16_t> ps;       // padded local indices (F32 weights)
+    DevBuf<float> pv;          // padded weights (F32 weights)
 };
 
-// Build P from S (the CSC of A, rows = internal voxels, sorted columns): rows
-// grouped in units of `unit` rows (one tile), split so that a panel holds at
-// most pan_nnz entries where a unit has more.  Returns false (P untouched)
-// when a panel would need more than 65536 rays.
-bool build_panels(tv3_ctx *ctx, const SparseRows &S, int unit, int pan_nnz, int blk_nnz,
-                  PanelT &P);
+// Second pass of the panel form of A x: y_i = sum of the partial sums of the
+// ray segments of ray i (one segment per ray and tile it crosses).
+struct SegSum {
+    int64_t rows = 0, nseg = 0;
+    int lanes = 8;
+    DevBuf<int32_t> ptr, seg;  // rows + 1, nseg: segments of each ray
+    DevBuf<float> part;        // nseg partial sums (written by the first pass)
+};
 
 // The fused solver's internal layout of a sparse operator: voxels numbered by
 // brick_index (B = 8 when every dimension is a multiple of 8, else 4 when a
 // multiple of 4, else 1), rays in a locality order; A and its CSC in that
-// numbering, and the CSC's panels when use_panels.
+// numbering, and their panel forms where those were tuned in.
 struct FusedLayout {
     int32_t dims[3] = {0, 0, 0};
-
     int B = 1, nby = 0, nbx = 0;
     SparseRows A, AT;
-    PanelT PT;
-    bool use_panels = false;
+    // panel forms: A^T x directly (rows = voxels, sources = rays); A x as
+    // per-tile segment sums (rows = segments, sources = the tile's voxels)
+    // followed by the per-ray sum RS
+    Panels PT, PA;
+    SegSum RS;
+    bool panels_T = false, panels_A = false;
+    int64_t npan = 0, nseg = 0;
+    int max_seg = 0;
     DevBuf<int32_t> v_nat2int, v_int2nat;   // voxel maps (n)
     DevBuf<int32_t> r_int2nat;              // ray map (m): internal ray -> natural ray
 };
@@ -277,7 +298,8 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]);
 void layout_matvec(tv3_op *op, const int32_t dims[3], bool trans, const float *x, float *y,
                    hipStream_t s);
 // override the layout's launch shapes (tv3_op_layout_force)
-void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz);
+void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz,
+                  int panels);
 // average duration (us) of one layout product kernel launch
 double layout_time(tv3_op *op, const int32_t dims[3], bool trans, int reps);
 void permute_gather(const float *in, const int32_t *map, float *out, int64_t n, hipStream_t s);

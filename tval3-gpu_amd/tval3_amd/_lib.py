@@ -111,7 +111,7 @@ SIGNATURES = [
     ("tv3_shrink", C.c_int, [_P, C.c_int64, C.c_int, _P, _P, _P, _P, _P, _P, C.c_double, _P,
                              _P, _P, C.POINTER(C.c_double)]),
     ("tv3_op_prepare", C.c_int, [_P, _P, C.POINTER(LayoutInfo)]),
-    ("tv3_op_layout_force", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32,
+    ("tv3_op_layout_force", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(LayoutInfo)]),
     ("tv3_op_layout_matvec", C.c_int, [_P, _P, C.c_int, _P, _P, _P]),
     ("tv3_op_layout_time", C.c_int, [_P, _P, C.c_int, C.c_int32, C.POINTER(C.c_double)]),

@@ -150,13 +150,14 @@ class Operator:
         self.ctx.check(L.load().tv3_op_prepare(self.h, _vp(d), C.byref(info)))
         return {k: getattr(info, k) for k, _ in L.LayoutInfo._fields_ if k != "reserved"}
 
-    def layout_force(self, dims, code_N=0, code_T=0, pan_nnz=0):
-        """Override the layout's launch shapes (codes as set_lanes; pan_nnz > 0
-        with code_T in (-1024, -2048, -4096) rebuilds A^T panels)."""
+    def layout_force(self, dims, code_N=0, code_T=0, pan_nnz=16384, panels=0):
+        """Override the layout's launch shapes.  panels: bit 0 = A x, bit 1 =
+        A^T r in panel form (block sizes -code_N / -code_T, at most pan_nnz
+        entries per panel); other directions take set_lanes codes."""
         d = np.ascontiguousarray(dims, np.int32)
         info = L.LayoutInfo()
         self.ctx.check(L.load().tv3_op_layout_force(self.h, _vp(d), code_N, code_T, pan_nnz,
-                                                    C.byref(info)))
+                                                    panels, C.byref(info)))
         return {k: getattr(info, k) for k, _ in L.LayoutInfo._fields_ if k != "reserved"}
 
     def layout_mat_vec_mul(self, dims, x, y=None, trans=False):
