@@ -60,6 +60,22 @@ def bytes_AT(nnz, m, n, sv):
     return nnz * (sv + 4) + 4 * (n + 1) + 4 * m + 4 * n
 
 
+def measured_traffic(key, N, path):
+    """HBM bytes per launch of the dominant product kernel(s) from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, each scaled
+    by what the counter reads for a copy of known size), or None when no
+    summary for this workload exists."""
+    f = os.path.join(ROOT, "profiles", f"traffic_{N}_{path}.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        d = json.load(fh)
+    k = d.get("kernels", {}).get(key)
+    if not k:
+        return None
+    return (k["fetch_kib"] * d["fetch_factor"] + k["write_kib"] * d["write_factor"]) * 1024.0
+
+
 def time_spmv(op, trans, reps=50):
     """Average duration of one product, HIP events on the launch stream."""
     import torch
@@ -161,17 +177,46 @@ def main():
     gA, gAT = bA / tA, bAT / tAT
     log(f"SpMV A: {tA * 1e6:.1f} us {gA / 1e9:.0f} GB/s; A^T: {tAT * 1e6:.1f} us {gAT / 1e9:.0f} GB/s")
     if tAT >= tA:
-        dom_t, dom_b = tAT, bAT
+        dom_t, dom_b, dom_key = tAT, bAT, "AT"
         dom_name = ("A^T product, panel form (k_panel, rays staged in LDS)" if panels & 2
                     else "A^T product (CSC row blocks)")
     else:
-        dom_t, dom_b = tA, bA
+        dom_t, dom_b, dom_key = tA, bA, "A"
         dom_name = ("A product, panel form (k_panel segment sums + k_seg_sum)" if panels & 1
                     else "A product (CSR row blocks)")
+    traffic = measured_traffic(dom_key, N, args.path)
 
     extra = {}
     if not args.no_extra:
-        # reference-shaped FP32 path and the dynamic Siddon path, shorter runs
+        # equal rel-err: FP16 fused against the reference-shaped FP32 loop, both
+        # run to the default stopping rule (tol 1e-6, tol_inn 1e-3, maxcnt 10,
+        # maxit 1025)
+        conv = dict(mu=2.0 ** 9, mu0=2.0 ** 9, tv_mode="iso")
+        tf, _, inf16 = timed_solve(ctx, op, b_dev, dims,
+                                   api.default_opts(**dict(conv, solver=solver)), ut_dev)
+        tp, _, inp = timed_solve(ctx, A32, b_dev, dims,
+                                 api.default_opts(**dict(conv, solver="plain")), ut_dev)
+        extra["equal_error"] = {
+            "stopping": "tol 1e-6, tol_inn 1e-3, maxcnt 10, maxit 1025",
+            "fast": {"path": args.path, "rel_err": inf16["rel_error"],
+                     "inner": inf16["total_iters"], "outer": inf16["outer_iters"], "secs": tf,
+                     "inner_per_s": inf16["total_iters"] / tf,
+                     "outer_per_s": inf16["outer_iters"] / tf},
+            "plain_f32": {"rel_err": inp["rel_error"], "inner": inp["total_iters"],
+                          "outer": inp["outer_iters"], "secs": tp,
+                          "inner_per_s": inp["total_iters"] / tp,
+                          "outer_per_s": inp["outer_iters"] / tp},
+        }
+        ee = extra["equal_error"]
+        ee["err_diff"] = ee["fast"]["rel_err"] - ee["plain_f32"]["rel_err"]
+        ee["speedup_outer"] = ee["fast"]["outer_per_s"] / ee["plain_f32"]["outer_per_s"]
+        ee["pass"] = bool(ee["err_diff"] <= 1e-3 and ee["speedup_outer"] >= 1.8)
+        log(f"equal error: fast err {ee['fast']['rel_err']:.4e} ({ee['fast']['inner']} inner, "
+            f"{ee['fast']['outer']} outer, {tf:.3f}s); plain FP32 err {ee['plain_f32']['rel_err']:.4e} "
+            f"({ee['plain_f32']['inner']} inner, {ee['plain_f32']['outer']} outer, {tp:.3f}s); "
+            f"outer-iter speedup {ee['speedup_outer']:.2f}")
+        # the reference-shaped FP32 loop and the dynamic Siddon operator,
+        # fixed iteration counts
         kp = min(args.steps, 100)
         tp, _, ip = timed_solve(ctx, A32, b_dev, dims,
                                 api.default_opts(**dict(fixed, solver="plain", maxit=kp)), ut_dev)
@@ -203,8 +248,9 @@ def main():
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = {"value": args.cpu_iters / tc, "unit": "inner iter/s", "cores": cores,
                "kind": "port",
-               "sample": f"oracle FP32 TVAL3 (OpenMP) on the same 128^3 CSR, {args.cpu_iters} "
-                         f"inner iterations incl. setup ({tc:.1f}s)"}
+               "sample": f"oracle FP32 TVAL3 (C, OpenMP) on the same {N}^3 Siddon CSR "
+                         f"({nnz} nnz, FP32 weights), {args.cpu_iters} inner iterations "
+                         f"including its setup (scaleb, A^T b), {tc:.1f}s"}
         log(f"cpu: {cpu['value']:.3f} iter/s on {cores} threads")
 
     out = {
@@ -225,7 +271,7 @@ def main():
                    "voxels": n, "rays": m, "nnz": nnz, "weights": wdt, "path": args.path,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": dom_b / dom_t / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": dom_b / dom_t / PEAK_HBM, "traffic": None,
+                     "unit": "GB/s", "frac": dom_b / dom_t / PEAK_HBM, "traffic": traffic,
                      "kernel": dom_name},
         "cpu_baseline": cpu,
         "A_GBps": gA / 1e9,

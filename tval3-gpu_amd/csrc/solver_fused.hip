@@ -280,6 +280,107 @@ __global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *
     block_sum_store<7>(acc, part, gridDim.x);
 }
 
+// The same on one 8x8x8 tile per workgroup of the B = 8 numbering (512
+// threads, one voxel each): U is staged in LDS over the tile and a one-voxel
+// shell ([-1, 8]^3), V = DU - W is computed once for every voxel of the tile
+// and of its three lower faces (where D^T reads it), and D^T V reads V from
+// LDS.  Per voxel the arithmetic is k_update_w's, so d is identical.
+constexpr int kTileBox = 10;   // shell edge: voxels -1 .. 8
+
+__device__ __forceinline__ int box_slot(int a, int b, int c) {
+    return ((c + 1) * kTileBox + (b + 1)) * kTileBox + (a + 1);
+}
+
+__global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__restrict__ U,
+                                                       const float *__restrict__ Uc,
+                                                       const float *__restrict__ sx,
+                                                       const float *__restrict__ sy,
+                                                       const float *__restrict__ sz,
+                                                       const float *__restrict__ g,
+                                                       const float *__restrict__ dts,
+                                                       float *__restrict__ d, const FCtl *C,
+                                                       double *part) {
+    constexpr int NBOX = kTileBox * kTileBox * kTileBox;
+    __shared__ float su[NBOX], svx[NBOX], svy[NBOX], svz[NBOX];
+    if (!C->active) return;
+    const int t = threadIdx.x;
+    const int tile = blockIdx.x;                       // tiles ordered y, x, z
+    const int ty = tile % L.nby, rest = tile / L.nby;
+    const int tx = rest % L.nbx, tz = rest / L.nbx;
+    const int y0 = ty * 8, x0 = tx * 8, z0 = tz * 8;
+    const int64_t base = (int64_t)tile << 9;
+    auto gidx = [&](int a, int b, int c) {
+        int y = y0 + a, x = x0 + b, z = z0 + c;   // periodic wrap by at most one voxel
+        y = y < 0 ? y + L.p : (y >= L.p ? y - L.p : y);
+        x = x < 0 ? x + L.q : (x >= L.q ? x - L.q : x);
+        z = z < 0 ? z + L.r : (z >= L.r ? z - L.r : z);
+        return brick_index(8, L.pq, L.p, L.nby, L.nbx, y, x, z);
+    };
+    // tile voxel q (internal order) -> offsets (a, b, c) = (y, x, z) in the tile
+    auto tile_pos = [](int q, int &a, int &b, int &c) {
+        const int sub = q >> 6, w = q & 63;
+        a = ((sub & 1) << 2) | (w & 3);
+        b = (((sub >> 1) & 1) << 2) | ((w >> 2) & 3);
+        c = ((sub >> 2) << 2) | (w >> 4);
+    };
+    for (int k = t; k < NBOX; k += 512) {
+        const int a = k % kTileBox - 1, b = (k / kTileBox) % kTileBox - 1, c = k / 100 - 1;
+        su[k] = U[gidx(a, b, c)];
+    }
+    __syncthreads();
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
+    const bool iso = C->iso;
+    // V at the tile (q < 512) and at its lower faces y = -1, x = -1, z = -1
+    for (int q = t; q < 512 + 3 * 64; q += 512) {
+        int a, b, c;
+        int64_t gi;
+        if (q < 512) {
+            tile_pos(q, a, b, c);
+            gi = base + q;
+        } else {
+            const int f = (q - 512) >> 6, j = (q - 512) & 63, u = j & 7, v = j >> 3;
+            a = f == 0 ? -1 : u;
+            b = f == 0 ? u : (f == 1 ? -1 : v);
+            c = f == 2 ? -1 : v;
+            gi = gidx(a, b, c);
+        }
+        const int sl = box_slot(a, b, c);
+        const float u0 = su[sl], uyp = su[sl + 1], uxp = su[sl + kTileBox],
+                    uzp = su[sl + kTileBox * kTileBox];
+        const float s_x = sx[gi], s_y = sy[gi], s_z = sz[gi];
+        float wx, wy, wz;
+        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, beta, ib, iso, wx,
+                                 wy, wz);
+        const float vx = (uxp - u0) - wx, vy = (uyp - u0) - wy, vz = (uzp - u0) - wz;
+        svx[sl] = vx;
+        svy[sl] = vy;
+        svz[sl] = vz;
+        if (q < 512) {
+            acc[0] += l1;
+            acc[1] += (double)vx * vx + (double)vy * vy + (double)vz * vz;
+            acc[2] += (double)vx * s_x + (double)vy * s_y + (double)vz * s_z;
+        }
+    }
+    __syncthreads();
+    {
+        int a, b, c;
+        tile_pos(t, a, b, c);
+        const int sl = box_slot(a, b, c);
+        const int64_t i = base + t;
+        const float g2 = svy[sl - 1] - svy[sl] + svx[sl - kTileBox] - svx[sl] +
+                         svz[sl - kTileBox * kTileBox] - svz[sl];          // D^T V, :501
+        const float dn = mdb * g[i] + g2 + dts[i];                         // :915-918
+        const float uc = Uc[i], du = su[sl] - uc;
+        acc[3] += (double)du * du;
+        acc[4] += (double)du * (double)(dn - d[i]);
+        acc[5] += (double)uc * uc;
+        acc[6] += (double)dn * dn;
+        d[i] = dn;
+    }
+    block_sum_store<7, 8>(acc, part, gridDim.x);
+}
+
 // |D d|^2 for the first-iteration step length (:587-596)
 template <int B>
 __global__ __launch_bounds__(256) void k_dd_norm(Lay L, Groups G, const float *__restrict__ d,
@@ -667,7 +768,7 @@ struct Fused {
     hipStream_t s;
     Lay L;
     int64_t n, m;
-    int nbS, nbN, nbNd, nbM;
+    int nbS, nbN, nbNd, nbM, nbW;
     Groups G;
     DevBuf<float> pool;
     float *U[2], *g[2], *Au[2], *d, *sx, *sy, *sz, *ox, *oy, *oz, *dts, *Atb, *rcd, *b, *delta,
@@ -696,6 +797,8 @@ struct Fused {
         G = groups_of<B>(L);
         nbS = (int)stencil_grid(G);
         nbN = product_blocks(*F);
+        // update_W: one workgroup per 8x8x8 tile with B = 8 (nbW <= nbS)
+        nbW = B == 8 ? (int)(n / 512) : nbS;
         nbM = 2048;
         const size_t tot = 3 * (size_t)nbS + 3 * (size_t)nbN + 7 * (size_t)nbS + (size_t)nbS +
                            (size_t)nbN + (size_t)nbS + nbM + nbM + nbM;
@@ -738,11 +841,19 @@ struct Fused {
         hipLaunchKernelGGL(k_line_search, dim3(1), dim3(1024), 0, s, ctl.p, st2.p, mode);
         hipLaunchKernelGGL(k_apply_alpha, dim3(2048), dim3(256), 0, s, n, m, U[c], U[n_], g[c],
                            g[n_], Au[c], Au[n_], ctl.p);
-        hipLaunchKernelGGL(k_update_w<B>, gS, dim3(256), 0, s, L, G, U[n_], U[c], sx, sy, sz,
-                           g[n_], dts, d, ctl.p, pW);
-        hipLaunchKernelGGL((k_stage2<7, 0>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pW, nbS, pW, 0,
+        update_w(U[n_], U[c], g[n_]);
+        hipLaunchKernelGGL((k_stage2<7, 0>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pW, nbW, pW, 0,
                            st2.p);
         hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1024), 0, s, ctl.p, st2.p, hist.p);
+    }
+
+    void update_w(const float *Un, const float *Ucur, const float *gn) {
+        if (B == 8)
+            hipLaunchKernelGGL(k_update_w_tile, dim3(nbW), dim3(512), 0, s, L, Un, Ucur, sx, sy,
+                               sz, gn, dts, d, ctl.p, pW);
+        else
+            hipLaunchKernelGGL(k_update_w<B>, dim3(nbS), dim3(256), 0, s, L, G, Un, Ucur, sx, sy,
+                               sz, gn, dts, d, ctl.p, pW);
     }
 
     void tau_first(int fallback) {
@@ -826,9 +937,8 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     launch_forward<float>(FL, S.U[0],
                           EpiRes{S.Au[0], S.b, S.delta, S.Au[0], S.ctl.p, S.pN, {0, 0, 0}}, s);
     launch_transposed<float>(FL, S.Au[0], EpiGrad{S.g[0], S.Atb, S.ctl.p}, s);
-    hipLaunchKernelGGL(k_update_w<B>, dim3(S.nbS), dim3(256), 0, s, S.L, S.G, S.U[0], S.U[0], S.sx,
-                       S.sy, S.sz, S.g[0], S.dts, S.d, S.ctl.p, S.pW);
-    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pN, S.nbN, S.pW, S.nbS);
+    S.update_w(S.U[0], S.U[0], S.g[0]);
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pN, S.nbN, S.pW, S.nbW);
     TV3_HIP(hipGetLastError());
 
     S.capture(0);

@@ -8,12 +8,12 @@
 
 namespace tv3 {
 
-// Sum K per-thread doubles over a 256-thread block (wavefront butterflies,
-// then the 4 wavefront totals in a fixed order); thread 0 stores
-// partials[k*nblk + blockIdx.x].
-template <int K>
+// Sum K per-thread doubles over a workgroup of NW wavefronts (4: 256
+// threads, 8: 512) -- wavefront butterflies, then the wavefront totals in a
+// fixed order; thread 0 stores partials[k*nblk + blockIdx.x].
+template <int K, int NW = 4>
 __device__ __forceinline__ void block_sum_store(double (&acc)[K], double *partials, int nblk) {
-    __shared__ double sh[K][4];
+    __shared__ double sh[K][NW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -25,8 +25,11 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[K], double *partia
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            partials[(int64_t)k * nblk + blockIdx.x] = (sh[k][0] + sh[k][1]) + (sh[k][2] + sh[k][3]);
+        for (int k = 0; k < K; ++k) {
+            double v = (sh[k][0] + sh[k][1]) + (sh[k][2] + sh[k][3]);
+            if (NW == 8) v += (sh[k][4] + sh[k][5]) + (sh[k][6] + sh[k][7]);
+            partials[(int64_t)k * nblk + blockIdx.x] = v;
+        }
     }
 }
 
