@@ -116,6 +116,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 5 plain device copies of 256 MiB (reference for counters)")
+    ap.add_argument("--shapes", default="auto",
+                    help="fused layout launch shapes: 'auto' (timed on this GPU) or "
+                         "'code_N,code_T,pan_nnz,panels' as Operator.layout_force takes them "
+                         "(under a counter profiler the timings that pick shapes are skewed)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the solver has no "
@@ -146,6 +150,9 @@ def main():
     if solver == "fused":
         t0 = time.time()
         layout = op.prepare(dims)      # renumbering, panels, tuned shapes: setup, not timed
+        if args.shapes != "auto":
+            cn, ct, pn, pm = (int(v) for v in args.shapes.split(","))
+            layout = op.layout_force(dims, cn, ct, pn, pm)
         lanes = (layout["code_N"], layout["code_T"])
         log(f"layout: {layout} ({time.time() - t0:.1f}s)")
     else:
@@ -168,8 +175,8 @@ def main():
     sv = {"f32": 4, "f16": 2, "bf16": 2}[wdt]
     panels = layout["panels"] if layout else 0
     if layout:
-        tA = op.layout_time(dims, False, 50)
-        tAT = op.layout_time(dims, True, 50)
+        tA = op.layout_time(dims, False, 20)
+        tAT = op.layout_time(dims, True, 20)
     else:
         tA = time_spmv(op, False)
         tAT = time_spmv(op, True)

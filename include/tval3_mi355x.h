@@ -251,8 +251,11 @@ tv3_status tv3_op_layout_matvec(tv3_op *op, const int32_t dims[3], tv3_trans tra
                                 const float *x_dev, float *y_dev, void *stream);
 
 /* Average duration (microseconds) of one launch of the layout's A (TV3_N) or
- * A^T (TV3_T) product kernel over `reps` back-to-back launches on the
- * context stream (x = 1), timed with HIP events on that stream. */
+ * A^T (TV3_T) product (x = 1; for the panel form of A both of its kernels),
+ * timed with HIP events on the context stream around each of `reps`
+ * launches.  Before each launch a read of a 512 MiB buffer evicts the
+ * operator from the caches, as the solver's other kernels do between two
+ * products. */
 tv3_status tv3_op_layout_time(tv3_op *op, const int32_t dims[3], tv3_trans trans, int32_t reps,
                               double *us);
 

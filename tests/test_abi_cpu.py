@@ -84,3 +84,46 @@ def test_bf16_rounding_helper():
     back = (b.astype(np.uint32) << 16).view(np.float32)
     assert back[0] == 1.0 and back[2] == -2.5
     assert np.allclose(back, v, rtol=2 ** -8)
+
+
+def test_cpp_dropin_shim_compiles_and_links():
+    """include/tval3_gpu_3d.hpp: the reference-shaped C++ overload set over the
+    C ABI compiles (every overload instantiated, float and double) and links
+    against the HIP library.  No GPU call is made."""
+    from tval3_amd import _lib as L
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("library not built")
+    src = r'''
+#include "tval3_gpu_3d.hpp"
+template <class T> int use(bool run) {
+    mat_host<T> U(8, 8, 8), b(100), Ut(8, 8, 8), Ad(100 * 512);
+    sparse_mat_host<T> A(100, 512, 0, sparse_mat_csr);
+    geometry_host g;
+    tval3_options<T> o;
+    o.tv_mode = TV3_TV_ISO;
+    if (run) {
+        tval3_info<T> i1 = tval3_gpu_3d(U, A, b, o, Ut, true, 0, 1, nullptr);
+        tval3_info<T> i2 = tval3_gpu_3d(U, Ad, b, o, Ut);
+        tval3_info<T> i3 = tval3_gpu_3d(U, g, b, o, Ut);
+        tval3_info<T> i4 = tval3_gpu_3d(U, g, A, b, o, Ut);
+        tval3_operator op(A);
+        mat_vec_mul(1, op, (const T *)nullptr, (T *)nullptr, nullptr, true);
+        return i1.total_iters + i2.total_iters + i3.total_iters + i4.total_iters;
+    }
+    return 0;
+}
+int main(int argc, char **) { return use<float>(argc > 5) + use<double>(argc > 5); }
+'''
+    tmp = os.path.join(ROOT, "tests", "_shim_probe")
+    with open(tmp + ".cpp", "w") as f:
+        f.write(src)
+    try:
+        libdir = os.path.dirname(L.LIB_PATH)
+        subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-I",
+                               os.path.join(ROOT, "include"), tmp + ".cpp", "-o", tmp,
+                               "-L" + libdir, "-ltval3_mi355x", "-Wl,-rpath-link,/opt/rocm/lib",
+                               "-Wl,--allow-shlib-undefined"])
+    finally:
+        for p in (tmp, tmp + ".cpp"):
+            if os.path.exists(p):
+                os.remove(p)
