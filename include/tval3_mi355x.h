@@ -70,7 +70,10 @@ typedef struct {
     int32_t maxcnt, maxit;
     double c, gamma, gam, rate_gam;
     int32_t isreal, nonneg;
-    uint32_t numInitIterations;   /* accepted; only skipMulRatio == 0 is supported */
+    /* A^T-skip heuristic (3d/tval3_gpu_3d.cpp:398-468), plain solver only:
+     * skipMulRatio = 0 turns it off.  A skipped A^T product leaves g as
+     * A^T(last Au) - Atb (the reference subtracts Atb again from the stale g). */
+    uint32_t numInitIterations;
     double skipMulRatio, maxRelativeChange;
     int32_t tv_mode;              /* tv3_tv_mode                                    */
     int32_t solver;               /* tv3_solver_kind                                */

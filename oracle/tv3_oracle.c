@@ -34,6 +34,7 @@ void orc_opts_default(orc_opts *o) {
     o->maxcnt = 10; o->maxit = 1025;
     o->c = 1e-5; o->gamma = 0.6; o->gam = 0.9995; o->rate_gam = 0.9;
     o->nonneg = 1; o->tv_iso = 0;
+    o->numInitIterations = 0; o->skipMulRatio = 0.0; o->maxRelativeChange = 0.0;
 }
 
 int64_t orc_build_paths(int E, int R, int mps, const uint8_t *use_path,

@@ -42,6 +42,9 @@ typedef struct {
     double c, gamma, gam, rate_gam;
     int nonneg;
     int tv_iso;          /* 0 = anisotropic shrinkage (reference), 1 = isotropic */
+    /* A^T-skip heuristic (3d/tval3_gpu_3d.cpp:398-468); skipMulRatio = 0 off */
+    int numInitIterations;
+    double skipMulRatio, maxRelativeChange;
 } orc_opts;
 
 typedef struct {
@@ -49,6 +52,7 @@ typedef struct {
     double rel_chg, rel_error;
     int backtracks, fallbacks;
     int status;          /* 0 ok, 6 non-finite */
+    int n_matvec_T;      /* A^T products of the loop (A^T b excluded) */
 } orc_info;
 
 /* per inner iteration record (capacity = cap entries, count written) */

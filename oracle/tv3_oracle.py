@@ -37,13 +37,15 @@ class OrcOpts(C.Structure):
                 ("beta0", C.c_double), ("rate_cnt", C.c_double), ("tol", C.c_double),
                 ("tol_inn", C.c_double), ("maxcnt", C.c_int), ("maxit", C.c_int),
                 ("c", C.c_double), ("gamma", C.c_double), ("gam", C.c_double),
-                ("rate_gam", C.c_double), ("nonneg", C.c_int), ("tv_iso", C.c_int)]
+                ("rate_gam", C.c_double), ("nonneg", C.c_int), ("tv_iso", C.c_int),
+                ("numInitIterations", C.c_int), ("skipMulRatio", C.c_double),
+                ("maxRelativeChange", C.c_double)]
 
 
 class OrcInfo(C.Structure):
     _fields_ = [("total_iters", C.c_int), ("outer_iters", C.c_int), ("rel_chg", C.c_double),
                 ("rel_error", C.c_double), ("backtracks", C.c_int), ("fallbacks", C.c_int),
-                ("status", C.c_int)]
+                ("status", C.c_int), ("n_matvec_T", C.c_int)]
 
 
 _HIST_D = ["f", "lam1", "lam2", "lam3", "lam4", "lam5", "tau", "alpha", "relchg", "mu",

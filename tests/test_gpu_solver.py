@@ -83,6 +83,36 @@ def test_plain_fp32_csr_3d_iso_matches_oracle(ctx):
     assert info["n_matvec_N"] > info["total_iters"]
 
 
+@pytest.mark.parametrize("skip", [(0.5, 2, 0.0), (2.0, 3, 1e30), (1.0, 0, 0.05)])
+def test_plain_skip_heuristic_matches_oracle(ctx, skip):
+    """A^T-skip rule (3d/tval3_gpu_3d.cpp:398-468, skipped g kept as
+    A^T(last Au) - Atb): the FP32 plain solver takes the same skip decisions
+    as the FP32 oracle (same count of A^T products) and the same iterates."""
+    ratio, ninit, maxrel = skip
+    N = 32
+    u, A, b = problem_2d(N, seed=4)
+    op = api.DenseOperator(ctx, A)
+    kw = dict(maxit=100, tol=1e-4, tol_inn=1e-3, skipMulRatio=ratio, numInitIterations=ninit,
+              maxRelativeChange=maxrel)
+    o = api.default_opts(**kw)
+    U, info, hist = api.tval3_gpu_3d(ctx, op, b.astype(np.float32), (N, N, 1), o, Ut=u)
+    oop = orc.dense_op(A.astype(np.float32).astype(np.float64))
+    oo = oracle_opts(o)
+    oo.skipMulRatio, oo.numInitIterations, oo.maxRelativeChange = ratio, ninit, maxrel
+    Uo, io, ho, _ = orc.solve(oop, (N, N, 1), b.astype(np.float32), oo, Ut=u, dtype=np.float32)
+    print(f"skip {skip}: A^T products gpu {info['n_matvec_T']} oracle {io.n_matvec_T} of "
+          f"{io.total_iters} iterations; rel(gpu,orc32)={rel(U, Uo):.2e} err {info['rel_error']:.3e}")
+    assert info["total_iters"] == io.total_iters
+    # the GPU count also holds the product A^T b
+    assert info["n_matvec_T"] - 1 == io.n_matvec_T
+    if maxrel == 0.0:
+        assert io.n_matvec_T == io.total_iters + 1 + io.fallbacks    # nothing skipped
+    else:
+        assert io.n_matvec_T < io.total_iters                       # skips happened
+    assert (hist["cnt"] == ho["cnt"]).all()
+    assert rel(U, Uo) < 1e-4
+
+
 def test_plain_fp64_matches_fp64_oracle(ctx):
     N = 12
     u, g, dyn, A, oop, b = problem_3d(ctx, N, E=16, R=40)
@@ -137,10 +167,14 @@ def test_solver_error_paths(ctx):
     with pytest.raises(TV3Error) as e:
         api.tval3_gpu_3d(ctx, op, bb, (N, N, 1))
     assert e.value.status == 6
+    with pytest.raises(TV3Error) as e:      # skipping: plain solver only
+        api.tval3_gpu_3d(ctx, op, b.astype(np.float32), (N, N, 1),
+                         api.default_opts(skipMulRatio=0.5, solver="fused"))
+    assert e.value.status == 8
     with pytest.raises(TV3Error) as e:
         api.tval3_gpu_3d(ctx, op, b.astype(np.float32), (N, N, 1),
-                         api.default_opts(skipMulRatio=0.5))
-    assert e.value.status == 8
+                         api.default_opts(skipMulRatio=-1.0))
+    assert e.value.status == 1
     U, info, _ = api.tval3_gpu_3d(ctx, op, b.astype(np.float32), (N, N, 1),
                                   api.default_opts(maxit=0))
     assert info["total_iters"] == 0

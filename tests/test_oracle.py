@@ -314,6 +314,44 @@ def test_solve_noise_free_well_conditioned(orc):
     assert info.total_iters <= 600 and info.outer_iters <= o.maxcnt
 
 
+def test_skip_heuristic_never_skipping_equals_baseline(orc):
+    """A^T-skip rule (3d/tval3_gpu_3d.cpp:421-467) with maxRelativeChange = 0:
+    Au always changes, so every product is taken and the iterates equal the
+    run without the heuristic bit for bit, with the same product count."""
+    N = 12
+    u = wl.phantom_2d(N, 4, seed=21)
+    A = wl.dense_gaussian(90, N * N, seed=22)
+    b = wl.add_noise(A @ u, 1e-3, seed=23)
+    base = dict(maxit=60, tol_inn=1e-3)
+    Ub, ib, hb, _ = orc.solve(orc.dense_op(A), (N, N, 1), b, orc.default_opts(**base),
+                              dtype=np.float32)
+    Us, is_, hs, _ = orc.solve(orc.dense_op(A), (N, N, 1), b,
+                               orc.default_opts(**base, skipMulRatio=0.5, numInitIterations=2,
+                                                maxRelativeChange=0.0), dtype=np.float32)
+    assert (Ub == Us).all() and (hb["f"] == hs["f"]).all()
+    assert ib.n_matvec_T == is_.n_matvec_T == ib.total_iters + 1 + ib.fallbacks
+
+
+def test_skip_heuristic_skips_and_still_recovers(orc):
+    """With no change limit the rule skips on a fixed schedule: after the
+    forced first product and numInitIterations products, skipMulRatio = 2
+    gives one product then two skips.  A skipped product keeps g =
+    A^T(last Au) - Atb (the reference would subtract Atb a second time), so
+    the noise-free well-conditioned problem is still recovered."""
+    N = 16
+    u = wl.phantom_2d(N, 4, seed=7)
+    A = wl.dense_gaussian(N * N, N * N, seed=8)
+    b = A @ u
+    o = orc.default_opts(mu=2.0 ** 12, mu0=2.0 ** 12, beta=2.0 ** 7, beta0=2.0 ** 7,
+                         tol=1e-6, tol_inn=1e-4, maxit=600, skipMulRatio=2.0,
+                         numInitIterations=3, maxRelativeChange=1e30)
+    U, info, hist, st = orc.solve(orc.dense_op(A), (N, N, 1), b, o, Ut=u)
+    assert st == 0
+    calls = info.total_iters + 1 + info.fallbacks       # get_g calls
+    assert info.n_matvec_T < 0.6 * calls
+    assert info.rel_error < 5e-3, info.rel_error
+
+
 def test_solve_nonmonotone_invariants(orc):
     N = 16
     u = wl.phantom_2d(N, 5, seed=9)

@@ -16,8 +16,9 @@
 //    twice, as written);
 //  * every work vector is allocated once per solve (the reference allocates
 //    and frees 3 n-vectors inside get_g/update_g/update_W/update_mlp);
-//  * only skipMulRatio == 0 (the skip heuristic re-subtracts Atb from a
-//    stale g, :463-467 then :565-567).
+//  * the A^T-skip heuristic (:398-468) leaves a skipped g as A^T(last Au)
+//    - Atb; the reference subtracts Atb a second time from the stale g
+//    (:463-467 then :565-567).
 // This path is the parity anchor and the baseline the fused path is
 // measured against.
 #include <chrono>
@@ -41,14 +42,16 @@ struct PlainState {
     DevBuf<T> pool;
     T *delta, *sx, *sy, *sz, *Up, *dU, *uup, *Ux, *Uxp, *dUx, *Uy, *Uyp, *dUy, *Uz, *Uzp, *dUz,
         *Wx, *Wy, *Wz, *Atb, *Au, *Aup, *dAu, *d, *g, *gp, *dg, *g2, *g2p, *dg2, *Vx, *Vy, *Vz,
-        *dx, *dy, *dz, *Ad, *DtsAtd, *rcdU, *bs;
+        *dx, *dy, *dz, *Ad, *DtsAtd, *rcdU, *bs, *lastAu, *dlast;
     int64_t nN = 0, nT = 0;
+    int cur_init = 0;        // A^T-skip state (:155-157)
+    double cur_skip = 0.0;
 
     void alloc() {
         T **nv[] = {&sx, &sy, &sz, &Up, &dU, &uup, &Ux, &Uxp, &dUx, &Uy, &Uyp, &dUy, &Uz, &Uzp,
                     &dUz, &Wx, &Wy, &Wz, &Atb, &d, &g, &gp, &dg, &g2, &g2p, &dg2, &Vx, &Vy, &Vz,
                     &dx, &dy, &dz, &DtsAtd, &rcdU};
-        T **mv[] = {&delta, &Au, &Aup, &dAu, &Ad, &bs};
+        T **mv[] = {&delta, &Au, &Aup, &dAu, &Ad, &bs, &lastAu, &dlast};
         const size_t cn = sizeof(nv) / sizeof(nv[0]), cm = sizeof(mv) / sizeof(mv[0]);
         // pad each vector to 256 bytes
         auto pad = [](int64_t k) { return (k + 63) / 64 * 64; };
@@ -73,12 +76,34 @@ struct PlainState {
         auto c = lam35<T>(ctx, m, Au, bs, delta, s);
         lam3 = c[0]; lam5 = c[1];
     }
-    void get_g(const T *U) {
+    // :366-572; force: the initial and the fallback call (:1148, :880)
+    void get_g(const T *U, bool force) {
         A(false, U, Au);
-        A(true, Au, g);
+        bool mul = true;
+        if (o->skipMulRatio != 0.0) {
+            // relative change of Au since the last A^T product (:398-412)
+            axpy3<T>(m, dlast, -1.0, lastAu, Au, s);
+            const double dn = dot<T>(ctx, m, dlast, dlast, s), an = dot<T>(ctx, m, Au, Au, s);
+            const double rel = std::sqrt(dn) / std::sqrt(an);
+            if (force) {
+                cur_skip = 0.0;
+            } else if ((uint32_t)cur_init < o->numInitIterations) {
+                ++cur_init;
+                cur_skip = 0.0;
+            } else if (cur_skip < 1.0) {
+                cur_skip += o->skipMulRatio;
+            } else if (rel > o->maxRelativeChange) {
+                // Au moved too far: multiply
+            } else {
+                cur_skip -= 1.0;
+                mul = false;
+            }
+            if (mul) copy(lastAu, Au, m);
+        }
+        if (mul) A(true, Au, g);
         get_lams();
         launch_Dt<T>(dims, Vx, Vy, Vz, g2, s);
-        axpy3<T>(n, g, -1.0, Atb, g, s);
+        if (mul) axpy3<T>(n, g, -1.0, Atb, g, s);
         f = fval();
     }
     double get_tau(bool fst) {
@@ -94,11 +119,11 @@ struct PlainState {
         auto r = tau_bb<T>(ctx, n, g, gp, g2, g2p, uup, muDbeta, dg, dg2, s);
         return std::fabs(r[0] / r[1]);
     }
-    void descend(T *U, double tau) {
+    void descend(T *U, double tau, bool force) {
         axpy3<T>(n, U, -tau, d, U, s);
         if (o->nonneg) nonneg<T>(n, U, s);
         launch_D<T>(dims, U, Ux, Uy, Uz, s);
-        get_g(U);
+        get_g(U, force);
     }
     void update_g(double alpha, T *U) {
         axpy3<T>(n, g, alpha, dg, gp, s);
@@ -185,7 +210,7 @@ void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
     S.copy(S.rcdU, U, n);
     launch_D<T>(dims, U, S.Ux, S.Uy, S.Uz, s);
     S.shrinkage();
-    S.get_g(U);
+    S.get_g(U, true);
     S.gradient();
 
     double Q = 1, Qp, C = S.f, gam = o.gam;
@@ -201,7 +226,7 @@ void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
             S.copy(S.Up, U, n); S.copy(S.gp, S.g, n); S.copy(S.g2p, S.g2, n);
             S.copy(S.Aup, S.Au, m); S.copy(S.Uxp, S.Ux, n); S.copy(S.Uyp, S.Uy, n);
             S.copy(S.Uzp, S.Uz, n);
-            S.descend(U, tau);
+            S.descend(U, tau, false);
             double alpha = 1.0;
             axpy3<T>(n, S.dU, -1.0, S.Up, U, s);
             double c_armij = dot<T>(ctx, n, S.d, S.d, s) * tau * o.c * S.beta;
@@ -218,7 +243,7 @@ void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
                         gam *= o.rate_gam;
                         tau = S.get_tau(true);
                         S.copy(U, S.Up, n);
-                        S.descend(U, tau);
+                        S.descend(U, tau, true);
                         cnt = 6;
                         ++fallbacks;
                         break;

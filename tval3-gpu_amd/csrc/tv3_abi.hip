@@ -94,8 +94,10 @@ double to_double(const uint8_t *p, tv3_dtype t, int64_t k) {
 
 void validate_opts(const tv3_opts *o) {
     TV3_REQUIRE(o, TV3_E_ARG, "opts is NULL");
-    TV3_REQUIRE(o->skipMulRatio == 0.0, TV3_E_UNSUPPORTED,
-                "skipMulRatio != 0 (A^T skipping) is not supported");
+    TV3_REQUIRE(o->skipMulRatio >= 0.0 && o->maxRelativeChange >= 0.0, TV3_E_ARG,
+                "skip heuristic parameters must be non-negative");
+    TV3_REQUIRE(o->skipMulRatio == 0.0 || o->solver == TV3_SOLVER_PLAIN, TV3_E_UNSUPPORTED,
+                "skipMulRatio != 0 (A^T skipping) is supported by the plain solver only");
     TV3_REQUIRE(o->maxit >= 0 && o->maxcnt >= 0, TV3_E_ARG, "maxit/maxcnt must be >= 0");
     TV3_REQUIRE(o->mu > 0 && o->mu0 > 0 && o->beta > 0 && o->beta0 > 0, TV3_E_ARG,
                 "mu, mu0, beta, beta0 must be positive");
