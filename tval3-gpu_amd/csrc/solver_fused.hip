@@ -458,8 +458,9 @@ __global__ __launch_bounds__(256) void k_apply_alpha(int64_t n, int64_t m,
                                                      float *__restrict__ Aun, const FCtl *C) {
     if (!C->active || C->alpha == 1.0) return;
     const float a = (float)C->alpha;
-    const int64_t mx = n > m ? n : m;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mx; i += (int64_t)gridDim.x * 256) {
+    const int64_t mx = n > m ? n : m, G = (int64_t)gridDim.x * 256;
+#pragma unroll 4
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < mx; i += G) {
         if (i < n) {
             const float u = Uc[i], g = gc[i];
             Un[i] = u + a * (Un[i] - u);
@@ -587,9 +588,13 @@ template <int KA, int KB>
 __global__ __launch_bounds__(256) void k_stage2(const FCtl *C, int check, const double *pa, int nbA,
                                                 const double *pb, int nbB, double *out) {
     if (check && !C->active) return;
-    __shared__ double sh[4];
+    constexpr int K = KA + KB;
+    __shared__ double sh[K][4];
     const int j = blockIdx.x, t = threadIdx.x;
-    for (int k = 0; k < KA + KB; ++k) {
+    // every load of every array first (a thread has at most a few per array)
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
         const double *p = k < KA ? pa + (int64_t)k * nbA : pb + (int64_t)(k - KA) * nbB;
         const int nb = k < KA ? nbA : nbB;
         const int chunk = (nb + S2 - 1) / S2, lo = j * chunk, hi = min(nb, lo + chunk);
@@ -597,13 +602,30 @@ __global__ __launch_bounds__(256) void k_stage2(const FCtl *C, int check, const 
         int i = lo + t;
         for (; i + 256 < hi; i += 512) { a0 += p[i]; a1 += p[i + 256]; }
         if (i < hi) a0 += p[i];
-        double v = a0 + a1;
+        v[k] = a0 + a1;
+    }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((t & 63) == 0) sh[t >> 6] = v;
-        __syncthreads();
-        if (t == 0) out[(int64_t)k * S2 + j] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-        __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        double x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if ((t & 63) == 0) sh[k][t >> 6] = x;
+    }
+    __syncthreads();
+    if (t < K) out[(int64_t)t * S2 + j] = (sh[t][0] + sh[t][1]) + (sh[t][2] + sh[t][3]);
+}
+
+// Sums of the K arrays of S2 stage-2 values on one wavefront (lane j holds
+// entry j, butterfly tree): every lane ends with the K totals.
+template <int K>
+__device__ __forceinline__ void wave_sums(const double *p2, double *out) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double x = p2[(int64_t)k * S2 + t];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        out[k] = x;
     }
 }
 
@@ -613,12 +635,19 @@ __device__ __forceinline__ double fval(const FCtl *C, double l1, double l2, doub
 }
 
 // nonmonotone Armijo line search on f(alpha) (:839-890); mode 1 = accept
-__global__ void k_line_search(FCtl *C, const double *p2, int mode) {
+__device__ void line_search(FCtl *C, const double *a, const double *b, int mode);
+
+__global__ __launch_bounds__(64) void k_line_search(FCtl *C, const double *p2, int mode) {
     if (!C->active) return;
+    static_assert(S2 == 64, "one stage-2 value per lane");
     double a[3], b[3];
-    reduce_parts<3>(p2, S2, a);
-    reduce_parts<3>(p2 + 3 * S2, S2, b);
-    if (threadIdx.x != 0) return;
+    wave_sums<3>(p2, a);
+    wave_sums<3>(p2 + 3 * S2, b);
+    if (threadIdx.x == 0) line_search(C, a, b, mode);
+}
+
+// a = {|V_t|^2, <V_t,sigma>, <V_c,V_t>}, b = {|r_t|^2, <delta,r_t>, <r_c,r_t>}
+__device__ void line_search(FCtl *C, const double *a, const double *b, int mode) {
     const double l2t = a[0], l4t = a[1], x01 = a[2], l3t = b[0], l5t = b[1], r01 = b[2];
     const double l2c = C->lam2, l4c = C->lam4, l3c = C->lam3, l5c = C->lam5;
     auto at = [&](double al, double *o) {
@@ -655,11 +684,18 @@ __global__ void k_line_search(FCtl *C, const double *p2, int mode) {
     C->f = f;
 }
 
-__global__ void k_iter_end(FCtl *C, const double *p2, double *hist) {
+__device__ void iter_end(FCtl *C, const double *a, double *hist);
+
+__global__ __launch_bounds__(64) void k_iter_end(FCtl *C, const double *p2, double *hist) {
     if (!C->active) return;
     double a[7];
-    reduce_parts<7>(p2, S2, a);
-    if (threadIdx.x != 0) return;
+    wave_sums<7>(p2, a);
+    if (threadIdx.x == 0) iter_end(C, a, hist);
+}
+
+// f, C, Q, BB step, history and stopping tests after update_W (:965-1029,
+// :1176-1197, :663); a = the 7 sums of k_update_w.
+__device__ void iter_end(FCtl *C, const double *a, double *hist) {
     C->lam1 = a[0]; C->lam2 = a[1]; C->lam4 = a[2];
     const double ss = a[3], sy = a[4];
     C->nrmup2 = a[5];
@@ -838,13 +874,13 @@ struct Fused {
         launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p}, s);
         hipLaunchKernelGGL((k_stage2<3, 3>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pD, nbS, pN, nbN,
                            st2.p);
-        hipLaunchKernelGGL(k_line_search, dim3(1), dim3(1024), 0, s, ctl.p, st2.p, mode);
-        hipLaunchKernelGGL(k_apply_alpha, dim3(2048), dim3(256), 0, s, n, m, U[c], U[n_], g[c],
+        hipLaunchKernelGGL(k_line_search, dim3(1), dim3(64), 0, s, ctl.p, st2.p, mode);
+        hipLaunchKernelGGL(k_apply_alpha, dim3(512), dim3(256), 0, s, n, m, U[c], U[n_], g[c],
                            g[n_], Au[c], Au[n_], ctl.p);
         update_w(U[n_], U[c], g[n_]);
         hipLaunchKernelGGL((k_stage2<7, 0>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pW, nbW, pW, 0,
                            st2.p);
-        hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1024), 0, s, ctl.p, st2.p, hist.p);
+        hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(64), 0, s, ctl.p, st2.p, hist.p);
     }
 
     void update_w(const float *Un, const float *Ucur, const float *gn) {
