@@ -219,6 +219,33 @@ tv3_status tv3_shrink(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, const float *Ux
                       const float *Uy, const float *Uz, const float *sx, const float *sy,
                       const float *sz, double beta, float *Wx, float *Wy, float *Wz,
                       double *lam1);
+/* D and D^T in double precision (the double instantiation of the same
+ * kernels, as the reference's double overloads use). */
+tv3_status tv3_D_f64(tv3_ctx *ctx, const int32_t dims[3], const double *U_dev, double *Ux_dev,
+                     double *Uy_dev, double *Uz_dev, void *stream);
+tv3_status tv3_Dt_f64(tv3_ctx *ctx, const int32_t dims[3], const double *X_dev,
+                      const double *Y_dev, const double *Z_dev, double *res_dev, void *stream);
+/* The shrinkage of shrinkage_gpu (3d/tval3_gpu_3d.cpp:924-960) as a map of its
+ * argument V (= DU - sigma/beta there) at threshold t (= 1/beta there):
+ *   aniso  W_c = sign(V_c) max(|V_c| - t, 0),  s = sum_c max(|V_c| - t, 0)
+ *   iso    W = max(|V|_2 - t, 0) V/|V|_2,       s = max(|V|_2 - t, 0)
+ * per voxel; lam1 = sum(s).  s may be NULL.  dtype TV3_F32 or TV3_F64 (all
+ * pointers of that type, device memory); asynchronous on stream.
+ * mask_out (n bytes, may be NULL) receives the active set: aniso bit c set
+ * when |V_c| > t, iso bit 0 when |V|_2 > t.  mask_in (may be NULL) replaces
+ * those comparisons by a recorded active set: the smooth branch it selects
+ * is evaluated (active: the formulas above without the max; inactive: 0). */
+tv3_status tv3_shrink_apply(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, tv3_dtype dtype,
+                            const void *Vx, const void *Vy, const void *Vz, double t, void *Wx,
+                            void *Wy, void *Wz, void *s, const uint8_t *mask_in,
+                            uint8_t *mask_out, void *stream);
+/* Its vector-Jacobian product: gV = (dW/dV)^T gW + (ds/dV)^T gs (gs may be
+ * NULL), the inactive side taken on the threshold surface.  The reference
+ * has no adjoint; this is the backward of the autograd binding. */
+tv3_status tv3_shrink_adjoint(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, tv3_dtype dtype,
+                              const void *Vx, const void *Vy, const void *Vz, double t,
+                              const void *gWx, const void *gWy, const void *gWz, const void *gs,
+                              void *gVx, void *gVy, void *gVz, void *stream);
 
 /* ---- the fused solver's operator layout (no reference counterpart) ------ */
 typedef struct tv3_layout_info {

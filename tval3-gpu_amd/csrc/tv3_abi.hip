@@ -595,6 +595,87 @@ tv3_status tv3_shrink(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, const float *Ux
     });
 }
 
+tv3_status tv3_D_f64(tv3_ctx *ctx, const int32_t dims[3], const double *U, double *Ux,
+                     double *Uy, double *Uz, void *stream) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        dims_n(dims);
+        TV3_REQUIRE(U && Ux && Uy && Uz, TV3_E_ARG, "NULL argument");
+        set_device(ctx);
+        launch_D<double>(dims, U, Ux, Uy, Uz, (hipStream_t)stream);
+        TV3_HIP(hipGetLastError());
+    });
+}
+
+tv3_status tv3_Dt_f64(tv3_ctx *ctx, const int32_t dims[3], const double *X, const double *Y,
+                      const double *Z, double *res, void *stream) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        dims_n(dims);
+        TV3_REQUIRE(X && Y && Z && res, TV3_E_ARG, "NULL argument");
+        set_device(ctx);
+        launch_Dt<double>(dims, X, Y, Z, res, (hipStream_t)stream);
+        TV3_HIP(hipGetLastError());
+    });
+}
+
+tv3_status tv3_shrink_apply(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, tv3_dtype dtype,
+                            const void *Vx, const void *Vy, const void *Vz, double t, void *Wx,
+                            void *Wy, void *Wz, void *s, const uint8_t *mask_in,
+                            uint8_t *mask_out, void *stream) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(n >= 0, TV3_E_ARG, "n must be non-negative");
+        TV3_REQUIRE(n == 0 || (Vx && Vy && Vz && Wx && Wy && Wz), TV3_E_ARG, "NULL argument");
+        TV3_REQUIRE(mode == TV3_TV_ANISO || mode == TV3_TV_ISO, TV3_E_ARG, "bad tv mode");
+        TV3_REQUIRE(t >= 0 && std::isfinite(t), TV3_E_ARG, "threshold must be finite and >= 0");
+        set_device(ctx);
+        hipStream_t st = (hipStream_t)stream;
+        int iso = mode == TV3_TV_ISO;
+        if (dtype == TV3_F32)
+            shrink_apply<float>(n, iso, t, (const float *)Vx, (const float *)Vy,
+                                (const float *)Vz, (float *)Wx, (float *)Wy, (float *)Wz,
+                                (float *)s, mask_in, mask_out, st);
+        else if (dtype == TV3_F64)
+            shrink_apply<double>(n, iso, t, (const double *)Vx, (const double *)Vy,
+                                 (const double *)Vz, (double *)Wx, (double *)Wy, (double *)Wz,
+                                 (double *)s, mask_in, mask_out, st);
+        else
+            TV3_REQUIRE(false, TV3_E_ARG, "dtype must be TV3_F32 or TV3_F64");
+        TV3_HIP(hipGetLastError());
+    });
+}
+
+tv3_status tv3_shrink_adjoint(tv3_ctx *ctx, int64_t n, tv3_tv_mode mode, tv3_dtype dtype,
+                              const void *Vx, const void *Vy, const void *Vz, double t,
+                              const void *gWx, const void *gWy, const void *gWz, const void *gs,
+                              void *gVx, void *gVy, void *gVz, void *stream) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(n >= 0, TV3_E_ARG, "n must be non-negative");
+        TV3_REQUIRE(n == 0 || (Vx && Vy && Vz && gWx && gWy && gWz && gVx && gVy && gVz),
+                    TV3_E_ARG, "NULL argument");
+        TV3_REQUIRE(mode == TV3_TV_ANISO || mode == TV3_TV_ISO, TV3_E_ARG, "bad tv mode");
+        TV3_REQUIRE(t >= 0 && std::isfinite(t), TV3_E_ARG, "threshold must be finite and >= 0");
+        set_device(ctx);
+        hipStream_t st = (hipStream_t)stream;
+        int iso = mode == TV3_TV_ISO;
+        if (dtype == TV3_F32)
+            shrink_adjoint<float>(n, iso, t, (const float *)Vx, (const float *)Vy,
+                                  (const float *)Vz, (const float *)gWx, (const float *)gWy,
+                                  (const float *)gWz, (const float *)gs, (float *)gVx,
+                                  (float *)gVy, (float *)gVz, st);
+        else if (dtype == TV3_F64)
+            shrink_adjoint<double>(n, iso, t, (const double *)Vx, (const double *)Vy,
+                                   (const double *)Vz, (const double *)gWx, (const double *)gWy,
+                                   (const double *)gWz, (const double *)gs, (double *)gVx,
+                                   (double *)gVy, (double *)gVz, st);
+        else
+            TV3_REQUIRE(false, TV3_E_ARG, "dtype must be TV3_F32 or TV3_F64");
+        TV3_HIP(hipGetLastError());
+    });
+}
+
 static float time_spmv(tv3_ctx *ctx, const SparseRows &S, int L, const float *x, float *y) {
     hipStream_t s = ctx->stream;
     hipEvent_t a, b;

@@ -323,4 +323,14 @@ double shrink_host(tv3_ctx *ctx, int64_t n, int iso, const T *Ux, const T *Uy, c
                    const T *sx, const T *sy, const T *sz, double beta, T *Wx, T *Wy, T *Wz,
                    hipStream_t s);
 
+// shrinkage as a function of V = DU - sigma/beta and its VJP (autodiff.hip)
+template <typename T>
+void shrink_apply(int64_t n, int iso, double t, const T *Vx, const T *Vy, const T *Vz, T *Wx,
+                  T *Wy, T *Wz, T *s, const uint8_t *mask_in, uint8_t *mask_out,
+                  hipStream_t st);
+template <typename T>
+void shrink_adjoint(int64_t n, int iso, double t, const T *Vx, const T *Vy, const T *Vz,
+                    const T *gWx, const T *gWy, const T *gWz, const T *gs, T *gVx, T *gVy,
+                    T *gVz, hipStream_t st);
+
 }  // namespace tv3

@@ -110,6 +110,12 @@ SIGNATURES = [
     ("tv3_Dt", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     ("tv3_shrink", C.c_int, [_P, C.c_int64, C.c_int, _P, _P, _P, _P, _P, _P, C.c_double, _P,
                              _P, _P, C.POINTER(C.c_double)]),
+    ("tv3_D_f64", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    ("tv3_Dt_f64", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    ("tv3_shrink_apply", C.c_int, [_P, C.c_int64, C.c_int, C.c_int, _P, _P, _P, C.c_double,
+                                   _P, _P, _P, _P, _P, _P, _P]),
+    ("tv3_shrink_adjoint", C.c_int, [_P, C.c_int64, C.c_int, C.c_int, _P, _P, _P, C.c_double,
+                                     _P, _P, _P, _P, _P, _P, _P, _P]),
     ("tv3_op_prepare", C.c_int, [_P, _P, C.POINTER(LayoutInfo)]),
     ("tv3_op_layout_force", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(LayoutInfo)]),
