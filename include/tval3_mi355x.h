@@ -205,6 +205,37 @@ tv3_status tv3_solve_f64(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const i
                          const double *b_host, double *U_host, const double *Ut_host,
                          tv3_info *info, tv3_history *hist);
 
+/* ---- batched solves ------------------------------------------------------ */
+/* No reference counterpart (the reference solves one problem per call).
+ * nprob independent problems, each with its own sparse operator (CSR, one
+ * weight type for the whole batch), dims (3 per problem: dim_y, dim_x,
+ * dim_z), b and result U (device pointers, float), solved together with one
+ * set of options: every kernel of an iteration covers all problems, the
+ * iterations are graph-captured, each problem keeps its own scalars,
+ * decisions and stopping (its iterates equal a separate fused-schedule solve
+ * of that problem up to FP32 summation order).  Ut_dev (array or NULL,
+ * entries may be NULL) gives infos[k].rel_error; hists (array of nprob or
+ * NULL) receives each problem's history.  opts->solver is ignored; the
+ * A^T-skip heuristic is not supported (skipMulRatio must be 0).  binfo may be
+ * NULL; when it is not, lanes_N and lanes_T > 0 on entry fix the products'
+ * launch shapes instead of timing them. */
+typedef struct tv3_batch_info {
+    int32_t lanes_N, lanes_T;   /* lanes per row of the block-diagonal products (in/out) */
+    int32_t graph_iters;        /* iterations per graph launch                          */
+    int32_t graph_launches;
+    double secs;                /* GPU time from scaling b to the unscaled results      */
+    double loop_secs;           /* GPU time of the iterations                           */
+    double host_secs;           /* host wall time of the call, set-up included          */
+    int64_t total_iters;        /* inner iterations summed over the problems            */
+    int64_t nnz, voxels, rays;  /* batch totals (voxels, rays padded to 256 per problem) */
+} tv3_batch_info;
+
+tv3_status tv3_solve_batched(tv3_ctx *ctx, int32_t nprob, tv3_op *const *ops,
+                             const int32_t *dims, const tv3_opts *opts,
+                             const float *const *b_dev, float *const *U_dev,
+                             const float *const *Ut_dev, tv3_info *infos, tv3_history *hists,
+                             tv3_batch_info *binfo);
+
 /* ---- device building blocks (exposed for tests and autograd bindings) ---- */
 /* D (3d/tval3_gpu_3d.cpp:271-314): periodic forward differences.
  * Ux along dim_x (stride dim_y), Uy along dim_y (stride 1), Uz along dim_z. */

@@ -1,0 +1,177 @@
+"""Batched TVAL3 (tv3_solve_batched): many problems of uneven size in one set
+of launches per iteration.
+
+- No cross-talk: a problem solved inside a batch gives bitwise the result,
+  iteration count and history of the same problem solved alone (same launch
+  shapes), and graph size does not change results.
+- Each problem against the FP32 oracle with the envelope criterion of the
+  fused solver (the batched path runs the same algorithm with FP32 products
+  in its own summation order): rounding-level agreement at the start, then
+  within a small multiple of the oracle's own sensitivity to a 1e-7
+  perturbation of b, and the final error.
+- Backtracking, fallbacks and the outer loop inside a batch; error paths.
+"""
+import numpy as np
+import pytest
+
+import tv3_oracle as orc
+from tval3_amd import api, workload as wl
+from tval3_amd._lib import TV3Error
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available()
+    return api.Context(0)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def make(ctx, dims, E, R, seed, dtype="f32", noise=0.005):
+    import torch
+    u = wl.phantom_3d(dims=dims, seed=seed)
+    g = wl.transducer_geometry(E, R, dims=dims, seed=seed + 1)
+    dyn = api.DynamicOperator(ctx, g, "siddon")
+    A = dyn.materialize(dtype)
+    A32 = dyn.materialize("f32")
+    rp, c, v = A32.download_csr()
+    oop = orc.csr_op(A32.m, A32.n, rp, c, v)
+    b = wl.add_noise(orc.matvec(oop, 0, u), noise, seed=seed + 2).astype(np.float32)
+    return dict(dims=dims, u=u, A=A, oop=oop, b=b, bt=torch.from_numpy(b).cuda(),
+                ut=torch.from_numpy(u.astype(np.float32)).cuda())
+
+
+PROBS = [((16, 16, 16), 24, 64, 1), ((12, 10, 6), 16, 48, 4), ((8, 20, 12), 20, 40, 7),
+         ((24, 8, 16), 24, 56, 10)]
+
+
+@pytest.fixture(scope="module")
+def batch(ctx):
+    return [make(ctx, *pp) for pp in PROBS]
+
+
+def opts(**kw):
+    base = dict(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, tv_mode="iso", maxit=60)
+    base.update(kw)
+    return api.default_opts(**base)
+
+
+def solve(ctx, probs, o, lanes=(8, 8), history=True):
+    return api.tval3_gpu_3d_batched(ctx, [p["A"] for p in probs], [p["bt"] for p in probs],
+                                    [p["dims"] for p in probs], o, Uts=[p["ut"] for p in probs],
+                                    history=history, lanes=lanes)
+
+
+def test_batch_equals_solo_bitwise(ctx, batch):
+    o = opts()
+    Ub, ib, bi, hb = solve(ctx, batch, o)
+    print("batch info", bi)
+    for k, p in enumerate(batch):
+        Us, is_, _, hs = solve(ctx, [p], o)
+        assert ib[k]["total_iters"] == is_[0]["total_iters"] > 0
+        assert ib[k]["outer_iters"] == is_[0]["outer_iters"]
+        assert (hb[k]["cnt"] == hs[0]["cnt"]).all()
+        assert np.array_equal(hb[k]["f"], hs[0]["f"])
+        assert np.array_equal(Ub[k].cpu().numpy(), Us[0].cpu().numpy()), k
+    assert bi["total_iters"] == sum(i["total_iters"] for i in ib)
+
+
+def test_batch_graph_size_and_lanes(ctx, batch):
+    o1 = opts(graph_iters=1)
+    o8 = opts(graph_iters=8)
+    U1, i1, _, _ = solve(ctx, batch, o1, history=False)
+    U8, i8, b8, _ = solve(ctx, batch, o8, history=False)
+    for k in range(len(batch)):
+        assert np.array_equal(U1[k].cpu().numpy(), U8[k].cpu().numpy())
+        assert i1[k]["total_iters"] == i8[k]["total_iters"]
+    assert b8["graph_launches"] > 0
+    # other launch shapes: the same iterates up to FP32 summation order, which
+    # TVAL3's BB step amplifies over the iterations (rounding level at the
+    # start, final errors within the spread such small problems show)
+    _, i8h, _, h8 = solve(ctx, batch, o8)
+    Ut, it, bt, ht = solve(ctx, batch, o8, lanes=None)
+    print("tuned lanes", bt["lanes_N"], bt["lanes_T"])
+    assert bt["lanes_N"] in (2, 4, 8, 16, 32, 64) and bt["lanes_T"] in (2, 4, 8, 16, 32, 64)
+    for k in range(len(batch)):
+        f8, ft = h8[k]["f"][:4], ht[k]["f"][:4]
+        assert (np.abs(ft - f8) <= 1e-5 * np.abs(f8)).all()
+        assert abs(it[k]["rel_error"] - i8h[k]["rel_error"]) < 2e-2
+
+
+def first_divergence(h1, h2):
+    n = min(len(h1["cnt"]), len(h2["cnt"]))
+    for k in range(n):
+        if h1["cnt"][k] != h2["cnt"][k] or h1["outer"][k] != h2["outer"][k]:
+            return k
+    return n
+
+
+def test_batch_tracks_oracle(ctx, batch):
+    o = opts()
+    Ub, ib, _, hb = solve(ctx, batch, o)
+    oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=60, tv_iso=1)
+    for k, p in enumerate(batch):
+        Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
+        xi = np.random.default_rng(0).choice([-1.0, 1.0], size=p["b"].shape)
+        bp = (p["b"].astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+        Up, ip, hpp, _ = orc.solve(p["oop"], p["dims"], bp, oo, Ut=p["u"], dtype=np.float32)
+        kk = min(first_divergence(hb[k], ho), first_divergence(hpp, ho))
+        dev = np.abs(hb[k]["f"][:kk] - ho["f"][:kk]) / np.abs(ho["f"][:kk])
+        env = np.maximum.accumulate(np.abs(hpp["f"][:kk] - ho["f"][:kk]) / np.abs(ho["f"][:kk]))
+        Ug = Ub[k].cpu().numpy()
+        print(f"{p['dims']}: iters batch {ib[k]['total_iters']} oracle {io.total_iters}; "
+              f"decisions agree for {kk}; err batch {ib[k]['rel_error']:.4e} oracle "
+              f"{io.rel_error:.4e} perturbed {ip.rel_error:.4e}; rel(U batch, oracle) "
+              f"{rel(Ug, Uo):.2e} rel(U perturbed, oracle) {rel(Up, Uo):.2e}")
+        assert kk >= 10
+        assert (dev[:4] < 1e-5).all()
+        assert (dev <= 30 * env + 1e-6).all()
+        assert abs(ib[k]["rel_error"] - io.rel_error) <= max(3 * abs(ip.rel_error - io.rel_error),
+                                                             1e-3)
+        assert rel(Ug, Uo) <= max(10 * rel(Up, Uo), 1e-4)
+
+
+def test_batch_outer_loop_backtracking_fallback(ctx, batch):
+    # continuation over several outer iterations
+    o = opts(maxit=120, tol_inn=5e-3, mu0=2.0 ** 6, beta0=4.0)
+    Ub, ib, _, hb = solve(ctx, batch, o)
+    for k in range(len(batch)):
+        assert ib[k]["outer_iters"] >= 2
+        assert (np.diff(hb[k]["outer"]) >= 0).all()
+        assert hb[k]["mu"][-1] > hb[k]["mu"][0]
+    # Armijo constant 0, gamma 1: every failing test ends in the fallback
+    noisy = []
+    for p in batch[:2]:
+        import torch
+        b = wl.add_noise(p["b"].astype(np.float64), 0.05, seed=14).astype(np.float32)
+        q = dict(p)
+        q["bt"] = torch.from_numpy(b).cuda()
+        noisy.append(q)
+    o = opts(maxit=40, gamma=1.0, c=0.0, gam=0.0, tol_inn=1e-12)
+    Ub, ib, _, hb = solve(ctx, noisy, o)
+    for k in range(len(noisy)):
+        assert ib[k]["total_iters"] == 40
+        assert set(hb[k]["cnt"].tolist()) <= {0, 6}
+        assert ib[k]["fallbacks"] == int((hb[k]["cnt"] == 6).sum())
+    assert sum(i["fallbacks"] for i in ib) > 0
+
+
+def test_batch_errors(ctx, batch):
+    import torch
+    p16 = make(ctx, (8, 8, 8), 16, 32, 3, dtype="f16")
+    with pytest.raises(TV3Error):        # mixed weight types
+        solve(ctx, [batch[0], p16], opts())
+    with pytest.raises(TV3Error):        # dims do not match the operator
+        api.tval3_gpu_3d_batched(ctx, [batch[0]["A"]], [batch[0]["bt"]], [(16, 16, 15)], opts())
+    with pytest.raises(TV3Error):        # skip heuristic
+        solve(ctx, batch[:1], opts(skipMulRatio=0.5))
+    with pytest.raises(ValueError):
+        api.tval3_gpu_3d_batched(ctx, [batch[0]["A"]], [batch[0]["bt"][:-1]], [(16, 16, 16)])
+    dense = api.DenseOperator(ctx, np.ones((4, 512)))
+    with pytest.raises(TV3Error):        # dense operators are not batched this way
+        api.tval3_gpu_3d_batched(ctx, [dense], [torch.zeros(4, device="cuda")], [(8, 8, 8)])

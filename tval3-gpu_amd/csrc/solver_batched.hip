@@ -1,0 +1,1085 @@
+// solver_batched.hip -- many TVAL3 problems of uneven size solved together.
+//
+// Every problem follows the fused solver's algorithm (the reference loop
+// 3d/tval3_gpu_3d.cpp:1088-1345 with the quadratic line search and FP64
+// device scalars, tval3_step.hpp); what is batched is the work: each kernel
+// of an inner iteration covers every problem of the batch, so the number of
+// launches per iteration does not grow with the batch, and K iterations are
+// captured in one graph.
+//
+//  * Problems are concatenated: voxels in natural order (y fastest), rays in
+//    the operator's order; each problem's voxel and ray ranges start at a
+//    multiple of 256.  The system matrix is the block-diagonal concatenation
+//    of the problems' CSR matrices (global voxel indices), its CSC likewise.
+//  * A workgroup works inside one problem (256-element groups never cross a
+//    range start); it finds its problem by binary search over the range
+//    starts and returns at once when that problem is not active.
+//  * Each problem has its control block (FCtl) and its own buffer parity: the
+//    accepted U, g and Au alternate between two copies per problem, while the
+//    trial vectors -- input and outputs of the products -- are single.
+//  * Per-problem reductions: every workgroup stores its partial sums; one
+//    workgroup per problem sums its problem's slice in a fixed order and runs
+//    the scalar step of that problem.
+//  * Products: a row-group CSR kernel over the block-diagonal matrix, L lanes
+//    per row (L timed per direction at set-up), whose workgroups skip the rows
+//    of inactive problems.
+//
+// One inner iteration: descend (trial U_t, V statistics), A U_t, residual
+// statistics, A^T A U_t - Atb, line search, apply alpha (accepted copies),
+// update_W (d, BB sums), end of iteration -- 8 launches for any batch.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "spmv_stream.hpp"
+#include "tval3_step.hpp"
+#include "vec.hpp"
+
+namespace tv3 {
+
+namespace {
+
+constexpr int GB = 256;   // elements per group = threads per workgroup
+
+struct BDims {
+    int32_t p, q, r, pad;
+    int64_t v0, n;        // voxel range
+    int64_t r0, m;        // ray range
+};
+
+// problem holding group g: starts[k] <= g < starts[k + 1] (starts[P] = end)
+__device__ __forceinline__ int find_prob(const int32_t *__restrict__ starts, int P, int g) {
+    int lo = 0, hi = P;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (starts[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+struct Vox {
+    int y, x, z;
+    int p, q, r;
+    int64_t v0, pq;
+    __device__ __forceinline__ int64_t at(int yy, int xx, int zz) const {
+        return v0 + ((int64_t)zz * q + xx) * p + yy;
+    }
+};
+
+// voxel of thread t of group w (false: padding)
+__device__ __forceinline__ bool voxel_at(const BDims &dm, int64_t i, Vox &v) {
+    const int64_t li = i - dm.v0;
+    if (li >= dm.n) return false;
+    const uint32_t l = (uint32_t)li, p = (uint32_t)dm.p, q = (uint32_t)dm.q;
+    const uint32_t yx = l % (p * q);
+    v.y = (int)(yx % p); v.x = (int)(yx / p); v.z = (int)(l / (p * q));
+    v.p = dm.p; v.q = dm.q; v.r = dm.r;
+    v.v0 = dm.v0; v.pq = (int64_t)dm.p * dm.q;
+    return true;
+}
+
+__device__ __forceinline__ int nxt(int v, int n) { return v + 1 == n ? 0 : v + 1; }
+__device__ __forceinline__ int prv(int v, int n) { return v == 0 ? n - 1 : v - 1; }
+
+// Sum of part[k * nb + j] for j in [lo, hi), K arrays, in a fixed order
+// (256 threads, strided partial sums, then a fixed tree); every thread gets
+// the totals.
+template <int K>
+__device__ void seg_sums(const double *part, int nb, int lo, int hi, double *out) {
+    __shared__ double sh[K][GB];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double *p = part + (int64_t)k * nb;
+        double a0 = 0, a1 = 0;
+        int j = lo + t;
+        for (; j + GB < hi; j += 2 * GB) { a0 += p[j]; a1 += p[j + GB]; }
+        if (j < hi) a0 += p[j];
+        sh[k][t] = a0 + a1;
+    }
+    __syncthreads();
+    for (int off = GB / 2; off > 0; off >>= 1) {
+        if (t < off)
+#pragma unroll
+            for (int k = 0; k < K; ++k) sh[k][t] += sh[k][t + off];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = sh[k][0];
+    __syncthreads();   // sh is reused by the next call
+}
+
+enum Pred { P_ACTIVE = 0, P_TAU = 1, P_ALL = 2, P_OUTER = 3 };
+
+__device__ __forceinline__ bool pred_on(const FCtl &c, int pred) {
+    switch (pred) {
+    case P_ACTIVE: return c.active != 0;
+    case P_TAU: return c.need_tau != 0;
+    case P_OUTER: return c.outer_flag != 0;
+    default: return true;
+    }
+}
+
+// ------------------------------------------------------------------ products
+// y = M x over the block-diagonal CSR; L lanes per row; rows of a workgroup
+// lie in one problem.  SUB: y = M x - sub.
+template <int L, typename V, bool SUB>
+__global__ __launch_bounds__(256) void k_b_csr(int64_t rows, const int32_t *__restrict__ ptr,
+                                               const int32_t *__restrict__ ind,
+                                               const V *__restrict__ val,
+                                               const float *__restrict__ x, float *__restrict__ y,
+                                               const float *__restrict__ sub,
+                                               const int32_t *__restrict__ starts, int P,
+                                               const FCtl *__restrict__ C, int pred) {
+    const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t row0 = bid * (256 / L);
+    const int k = find_prob(starts, P, (int)(row0 / GB));
+    if (!pred_on(C[k], pred)) return;
+    const int64_t row = row0 + threadIdx.x / L;
+    const int lane = threadIdx.x & (L - 1);
+    if (row >= rows) return;          // whole groups leave together (L divides 256)
+    const int beg = ptr[row], end = ptr[row + 1];
+    float s = 0.f, s2 = 0.f;
+    int e = beg + lane;
+    for (; e + L < end; e += 2 * L) {
+        const int c0 = ind[e], c1 = ind[e + L];
+        const float w0 = ld_weight<V>(val, e), w1 = ld_weight<V>(val, e + L);
+        s = fmaf(w0, x[c0], s);
+        s2 = fmaf(w1, x[c1], s2);
+    }
+    if (e < end) s = fmaf(ld_weight<V>(val, e), x[ind[e]], s);
+    s += s2;
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+    if (lane == 0) y[row] = SUB ? s - sub[row] : s;
+}
+
+// ------------------------------------------------------------------ stencils
+// descend: U_t = max(U_c - tau d, 0) (:690-696); V0/V1 statistics.
+__global__ __launch_bounds__(256) void k_b_descend(const BDims *__restrict__ D,
+                                                   const int32_t *__restrict__ vst, int P,
+                                                   const FCtl *__restrict__ C,
+                                                   const float *__restrict__ U0,
+                                                   const float *__restrict__ U1,
+                                                   const float *__restrict__ d,
+                                                   const float *__restrict__ sx,
+                                                   const float *__restrict__ sy,
+                                                   const float *__restrict__ sz,
+                                                   const float *__restrict__ wsx,
+                                                   const float *__restrict__ wsy,
+                                                   const float *__restrict__ wsz,
+                                                   float *__restrict__ Un, double *part) {
+    const int k = find_prob(vst, P, blockIdx.x);
+    const FCtl &c = C[k];
+    if (!c.active) return;
+    const BDims dm = D[k];
+    const float *Uc = c.cur ? U1 : U0;
+    double acc[3] = {0, 0, 0};
+    const float tau = (float)c.tau, beta = (float)c.beta, ib = 1.f / beta;
+    const bool nn = c.nonneg, iso = c.iso, old = c.sigw_old;
+    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    Vox v;
+    if (voxel_at(dm, i, v)) {
+        const int64_t iy = v.at(nxt(v.y, v.p), v.x, v.z), ix = v.at(v.y, nxt(v.x, v.q), v.z),
+                      iz = v.at(v.y, v.x, nxt(v.z, v.r));
+        const float u0 = Uc[i], uy = Uc[iy], ux = Uc[ix], uz = Uc[iz];
+        auto step = [&](float u, float dv) {
+            float t = u - tau * dv;
+            return (nn && t < 0.f) ? 0.f : t;
+        };
+        const float t0 = step(u0, d[i]), ty = step(uy, d[iy]), tx = step(ux, d[ix]),
+                    tz = step(uz, d[iz]);
+        const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
+        float wx, wy, wz;
+        shrink3(ux - u0, uy - u0, uz - u0, old ? wsx[i] : s_x, old ? wsy[i] : s_y,
+                old ? wsz[i] : s_z, beta, ib, iso, wx, wy, wz);
+        const float v0x = (ux - u0) - wx, v0y = (uy - u0) - wy, v0z = (uz - u0) - wz;
+        const float v1x = (tx - t0) - wx, v1y = (ty - t0) - wy, v1z = (tz - t0) - wz;
+        acc[0] += (double)v1x * v1x + (double)v1y * v1y + (double)v1z * v1z;
+        acc[1] += (double)v1x * s_x + (double)v1y * s_y + (double)v1z * s_z;
+        acc[2] += (double)v0x * v1x + (double)v0y * v1y + (double)v0z * v1z;
+        Un[i] = t0;
+    }
+    block_sum_store<3>(acc, part, gridDim.x);
+}
+
+// update_W + D^T + gradient + BB sums (:965-1029, :896-919, :1017-1067).
+// same = 1: the new iterate is the current copy (set-up), else the other one.
+__global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
+                                                    const int32_t *__restrict__ vst, int P,
+                                                    const FCtl *__restrict__ C, int same,
+                                                    const float *__restrict__ U0,
+                                                    const float *__restrict__ U1,
+                                                    const float *__restrict__ sx,
+                                                    const float *__restrict__ sy,
+                                                    const float *__restrict__ sz,
+                                                    const float *__restrict__ g0,
+                                                    const float *__restrict__ g1,
+                                                    const float *__restrict__ dts,
+                                                    float *__restrict__ d, double *part) {
+    const int k = find_prob(vst, P, blockIdx.x);
+    const FCtl &c = C[k];
+    if (!c.active) return;
+    const BDims dm = D[k];
+    const int nw = same ? c.cur : c.cur ^ 1;
+    const float *U = nw ? U1 : U0, *Uc = c.cur ? U1 : U0, *g = nw ? g1 : g0;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const float beta = (float)c.beta, ib = 1.f / beta, mdb = (float)c.muDbeta;
+    const bool iso = c.iso;
+    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    Vox v;
+    if (voxel_at(dm, i, v)) {
+        const int y = v.y, x = v.x, z = v.z;
+        const int yp = nxt(y, v.p), ym = prv(y, v.p), xp = nxt(x, v.q), xm = prv(x, v.q),
+                  zp = nxt(z, v.r), zm = prv(z, v.r);
+        const float u0 = U[i];
+        const float uyp = U[v.at(yp, x, z)], uxp = U[v.at(y, xp, z)], uzp = U[v.at(y, x, zp)];
+        float wx, wy, wz;
+        const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
+        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, beta, ib, iso, wx,
+                                 wy, wz);
+        const float vx = (uxp - u0) - wx, vy = (uyp - u0) - wy, vz = (uzp - u0) - wz;
+        float vyb, vxb, vzb;
+        {
+            const int64_t j = v.at(ym, x, z);
+            const float uj = U[j];
+            const float dyj = u0 - uj, dxj = U[v.at(ym, xp, z)] - uj, dzj = U[v.at(ym, x, zp)] - uj;
+            float a, b, cc;
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, cc);
+            vyb = dyj - b;
+        }
+        {
+            const int64_t j = v.at(y, xm, z);
+            const float uj = U[j];
+            const float dxj = u0 - uj, dyj = U[v.at(yp, xm, z)] - uj, dzj = U[v.at(y, xm, zp)] - uj;
+            float a, b, cc;
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, cc);
+            vxb = dxj - a;
+        }
+        {
+            const int64_t j = v.at(y, x, zm);
+            const float uj = U[j];
+            const float dzj = u0 - uj, dxj = U[v.at(y, xp, zm)] - uj, dyj = U[v.at(yp, x, zm)] - uj;
+            float a, b, cc;
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, cc);
+            vzb = dzj - cc;
+        }
+        const float g2 = vyb - vy + vxb - vx + vzb - vz;            // D^T V, :501
+        const float dn = mdb * g[i] + g2 + dts[i];                   // :915-918
+        const float uc = Uc[i], du = u0 - uc;
+        acc[0] += l1;
+        acc[1] += (double)vx * vx + (double)vy * vy + (double)vz * vz;
+        acc[2] += (double)vx * s_x + (double)vy * s_y + (double)vz * s_z;
+        acc[3] += (double)du * du;
+        acc[4] += (double)du * (double)(dn - d[i]);
+        acc[5] += (double)uc * uc;
+        acc[6] += (double)dn * dn;
+        d[i] = dn;
+    }
+    block_sum_store<7>(acc, part, gridDim.x);
+}
+
+// |D d|^2 for the first-iteration step length (:587-596)
+__global__ __launch_bounds__(256) void k_b_dd_norm(const BDims *__restrict__ D,
+                                                   const int32_t *__restrict__ vst, int P,
+                                                   const FCtl *__restrict__ C,
+                                                   const float *__restrict__ d, double *part) {
+    const int k = find_prob(vst, P, blockIdx.x);
+    if (!C[k].need_tau) return;
+    const BDims dm = D[k];
+    double acc[1] = {0};
+    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    Vox v;
+    if (voxel_at(dm, i, v)) {
+        const float d0 = d[i];
+        const float a = d[v.at(nxt(v.y, v.p), v.x, v.z)] - d0,
+                    b = d[v.at(v.y, nxt(v.x, v.q), v.z)] - d0,
+                    c = d[v.at(v.y, v.x, nxt(v.z, v.r))] - d0;
+        acc[0] += (double)a * a + (double)b * b + (double)c * c;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// update_mlp, sigma part (:1036-1069): keeps the old sigma for the W of the
+// next descend and returns lam4 = <sigma_new, V>
+__global__ __launch_bounds__(256) void k_b_mlp_sigma(const BDims *__restrict__ D,
+                                                     const int32_t *__restrict__ vst, int P,
+                                                     const FCtl *__restrict__ C,
+                                                     const float *__restrict__ U0,
+                                                     const float *__restrict__ U1,
+                                                     float *__restrict__ sx, float *__restrict__ sy,
+                                                     float *__restrict__ sz, float *__restrict__ ox,
+                                                     float *__restrict__ oy, float *__restrict__ oz,
+                                                     double *part) {
+    const int k = find_prob(vst, P, blockIdx.x);
+    const FCtl &c = C[k];
+    if (!c.outer_flag) return;
+    const BDims dm = D[k];
+    const float *U = c.cur ? U1 : U0;
+    double acc[1] = {0};
+    const float beta = (float)c.beta, ib = 1.f / beta;
+    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    Vox v;
+    if (voxel_at(dm, i, v)) {
+        const float u0 = U[i];
+        const float dy = U[v.at(nxt(v.y, v.p), v.x, v.z)] - u0,
+                    dx = U[v.at(v.y, nxt(v.x, v.q), v.z)] - u0,
+                    dz = U[v.at(v.y, v.x, nxt(v.z, v.r))] - u0;
+        const float a = sx[i], b = sy[i], cc = sz[i];
+        float wx, wy, wz;
+        shrink3(dx, dy, dz, a, b, cc, beta, ib, c.iso, wx, wy, wz);
+        const float vx = dx - wx, vy = dy - wy, vz = dz - wz;
+        const float na = a - beta * vx, nb = b - beta * vy, nc = cc - beta * vz;
+        ox[i] = a; oy[i] = b; oz[i] = cc;
+        sx[i] = na; sy[i] = nb; sz[i] = nc;
+        acc[0] += (double)na * vx + (double)nb * vy + (double)nc * vz;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// outer update of d and DtsAtd (:1239-1246) without a stored g2
+__global__ __launch_bounds__(256) void k_b_outer_d(const BDims *__restrict__ D,
+                                                   const int32_t *__restrict__ vst, int P,
+                                                   const FCtl *__restrict__ C,
+                                                   float *__restrict__ d,
+                                                   const float *__restrict__ g0,
+                                                   const float *__restrict__ g1,
+                                                   float *__restrict__ dts, double *part) {
+    const int k = find_prob(vst, P, blockIdx.x);
+    const FCtl &c = C[k];
+    if (!c.outer_flag) return;
+    const BDims dm = D[k];
+    const float *g = c.cur ? g1 : g0;
+    double acc[1] = {0};
+    const float ratio = (float)c.ratio, mo = (float)c.mdb_old, mn = (float)c.muDbeta;
+    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (i - dm.v0 < dm.n) {
+        const float dv = d[i], gv = g[i];
+        const float g2 = dv - mo * gv - dts[i];
+        const float nd = ratio * dv;
+        const float dn = mn * gv + g2 + nd;
+        dts[i] = nd;
+        d[i] = dn;
+        acc[0] += (double)dn * dn;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// |U - rcdU|^2 then rcdU = U (:1208-1217)
+__global__ __launch_bounds__(256) void k_b_relout(const BDims *__restrict__ D,
+                                                  const int32_t *__restrict__ vst, int P,
+                                                  const FCtl *__restrict__ C,
+                                                  const float *__restrict__ U0,
+                                                  const float *__restrict__ U1,
+                                                  float *__restrict__ rcd, double *part) {
+    const int k = find_prob(vst, P, blockIdx.x);
+    const FCtl &c = C[k];
+    if (!c.outer_flag) return;
+    const BDims dm = D[k];
+    const float *U = c.cur ? U1 : U0;
+    double acc[1] = {0};
+    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (i - dm.v0 < dm.n) {
+        const float u = U[i], t = u - rcd[i];
+        acc[0] += (double)t * t;
+        rcd[i] = u;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// ------------------------------------------------------------------ rays
+// residual statistics of the trial: r1 = Au_t - b, r0 = Au_c - b:
+// |r1|^2, <delta, r1>, <r0, r1>
+__global__ __launch_bounds__(256) void k_b_res(const BDims *__restrict__ D,
+                                               const int32_t *__restrict__ rst, int P,
+                                               const FCtl *__restrict__ C, int pred,
+                                               const float *__restrict__ AuT,
+                                               const float *__restrict__ b,
+                                               const float *__restrict__ delta,
+                                               const float *__restrict__ Au0,
+                                               const float *__restrict__ Au1, double *part) {
+    const int k = find_prob(rst, P, blockIdx.x);
+    const FCtl &c = C[k];
+    if (!pred_on(c, pred)) return;
+    const BDims dm = D[k];
+    const float *Auc = c.cur ? Au1 : Au0;
+    double acc[3] = {0, 0, 0};
+    const int64_t r = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (r - dm.r0 < dm.m) {
+        const float bv = b[r], r1 = AuT[r] - bv, r0 = Auc[r] - bv;
+        acc[0] += (double)r1 * r1;
+        acc[1] += (double)delta[r] * r1;
+        acc[2] += (double)r0 * r1;
+    }
+    block_sum_store<3>(acc, part, gridDim.x);
+}
+
+// |A d|^2 (first-iteration step length)
+__global__ __launch_bounds__(256) void k_b_sqnorm(const BDims *__restrict__ D,
+                                                  const int32_t *__restrict__ rst, int P,
+                                                  const FCtl *__restrict__ C,
+                                                  const float *__restrict__ y, double *part) {
+    const int k = find_prob(rst, P, blockIdx.x);
+    if (!C[k].need_tau) return;
+    const BDims dm = D[k];
+    double acc[1] = {0};
+    const int64_t r = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (r - dm.r0 < dm.m) acc[0] = (double)y[r] * y[r];
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void k_b_mlp_delta(const BDims *__restrict__ D,
+                                                     const int32_t *__restrict__ rst, int P,
+                                                     const FCtl *__restrict__ C,
+                                                     const float *__restrict__ Au0,
+                                                     const float *__restrict__ Au1,
+                                                     const float *__restrict__ b,
+                                                     float *__restrict__ delta, double *part) {
+    const int k = find_prob(rst, P, blockIdx.x);
+    const FCtl &c = C[k];
+    if (!c.outer_flag) return;
+    const BDims dm = D[k];
+    const float *Au = c.cur ? Au1 : Au0;
+    double acc[1] = {0};
+    const float mu = (float)c.mu;
+    const int64_t r = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (r - dm.r0 < dm.m) {
+        const float rv = Au[r] - b[r];
+        const float dn = delta[r] - mu * rv;
+        delta[r] = dn;
+        acc[0] += (double)dn * rv;
+    }
+    block_sum_store<1>(acc, part, gridDim.x);
+}
+
+// accepted copies: X_new = X_c + alpha (X_t - X_c), X_t itself when alpha = 1
+// (voxel groups first, then ray groups)
+__global__ __launch_bounds__(256) void k_b_apply(const BDims *__restrict__ D,
+                                                 const int32_t *__restrict__ vst,
+                                                 const int32_t *__restrict__ rst, int P, int nbV,
+                                                 const FCtl *__restrict__ C,
+                                                 float *__restrict__ U0, float *__restrict__ U1,
+                                                 const float *__restrict__ T,
+                                                 float *__restrict__ g0, float *__restrict__ g1,
+                                                 const float *__restrict__ gT,
+                                                 float *__restrict__ Au0, float *__restrict__ Au1,
+                                                 const float *__restrict__ AuT) {
+    const bool vox = (int)blockIdx.x < nbV;
+    const int w = vox ? (int)blockIdx.x : (int)blockIdx.x - nbV;
+    const int k = find_prob(vox ? vst : rst, P, w);
+    const FCtl &c = C[k];
+    if (!c.active) return;
+    const BDims dm = D[k];
+    const float a = (float)c.alpha;
+    const bool one = c.alpha == 1.0;
+    const int64_t i = (int64_t)w * GB + threadIdx.x;
+    if (vox) {
+        if (i - dm.v0 >= dm.n) return;
+        const float *Uc = c.cur ? U1 : U0, *gc = c.cur ? g1 : g0;
+        float *Un = c.cur ? U0 : U1, *gn = c.cur ? g0 : g1;
+        const float t = T[i], gt = gT[i];
+        if (one) {
+            Un[i] = t;
+            gn[i] = gt;
+        } else {
+            const float u = Uc[i], g = gc[i];
+            Un[i] = u + a * (t - u);
+            gn[i] = g + a * (gt - g);
+        }
+    } else {
+        if (i - dm.r0 >= dm.m) return;
+        const float *Ac = c.cur ? Au1 : Au0;
+        float *An = c.cur ? Au0 : Au1;
+        const float t = AuT[i];
+        if (one) {
+            An[i] = t;
+        } else {
+            const float v = Ac[i];
+            An[i] = v + a * (t - v);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ per-problem scalar kernels
+// one 256-thread workgroup per problem
+__global__ __launch_bounds__(256) void k_b_search(FCtl *C, const int32_t *__restrict__ vst,
+                                                  const int32_t *__restrict__ rst, int nbV,
+                                                  int nbR, const double *pV, const double *pR) {
+    FCtl *c = C + blockIdx.x;
+    if (!c->active) return;
+    double a[3], b[3];
+    seg_sums<3>(pV, nbV, vst[blockIdx.x], vst[blockIdx.x + 1], a);
+    seg_sums<3>(pR, nbR, rst[blockIdx.x], rst[blockIdx.x + 1], b);
+    if (threadIdx.x == 0) line_search(c, a, b, c->mode);
+}
+
+__global__ __launch_bounds__(256) void k_b_iter_end(FCtl *C, const int32_t *__restrict__ vst,
+                                                    int nbV, const double *pW, double *hist,
+                                                    int64_t hist_stride) {
+    FCtl *c = C + blockIdx.x;
+    if (!c->active) return;
+    double a[7];
+    seg_sums<7>(pW, nbV, vst[blockIdx.x], vst[blockIdx.x + 1], a);
+    if (threadIdx.x == 0) {
+        iter_end(c, a, hist + (int64_t)blockIdx.x * hist_stride);
+        c->cur ^= 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_tau(FCtl *C, const int32_t *__restrict__ vst,
+                                               const int32_t *__restrict__ rst, int nbV, int nbR,
+                                               const double *pD, const double *pA) {
+    FCtl *c = C + blockIdx.x;
+    if (!c->need_tau) return;
+    double a[1], b[1];
+    seg_sums<1>(pD, nbV, vst[blockIdx.x], vst[blockIdx.x + 1], a);
+    seg_sums<1>(pA, nbR, rst[blockIdx.x], rst[blockIdx.x + 1], b);
+    if (threadIdx.x == 0) tau_first_step(c, a[0], b[0], c->need_fallback);
+}
+
+__global__ __launch_bounds__(256) void k_b_init(FCtl *C, const int32_t *__restrict__ vst,
+                                                const int32_t *__restrict__ rst, int nbV, int nbR,
+                                                const double *pN, const double *pW) {
+    FCtl *c = C + blockIdx.x;
+    double a[3], w[7];
+    seg_sums<3>(pN, nbR, rst[blockIdx.x], rst[blockIdx.x + 1], a);
+    seg_sums<7>(pW, nbV, vst[blockIdx.x], vst[blockIdx.x + 1], w);
+    if (threadIdx.x == 0) init_step(c, a, w);
+}
+
+__global__ __launch_bounds__(256) void k_b_relout_scalar(FCtl *C, const int32_t *__restrict__ vst,
+                                                         int nbV, const double *p) {
+    FCtl *c = C + blockIdx.x;
+    if (!c->outer_flag) return;
+    double a[1];
+    seg_sums<1>(p, nbV, vst[blockIdx.x], vst[blockIdx.x + 1], a);
+    if (threadIdx.x == 0) c->relout = sqrt(a[0]) / sqrt(c->nrmup2);
+}
+
+__global__ __launch_bounds__(256) void k_b_outer_scalars(FCtl *C, const int32_t *__restrict__ vst,
+                                                         const int32_t *__restrict__ rst, int nbV,
+                                                         int nbR, const double *p4,
+                                                         const double *p5) {
+    FCtl *c = C + blockIdx.x;
+    if (!c->outer_flag) return;
+    double a[1], b[1];
+    seg_sums<1>(p4, nbV, vst[blockIdx.x], vst[blockIdx.x + 1], a);
+    seg_sums<1>(p5, nbR, rst[blockIdx.x], rst[blockIdx.x + 1], b);
+    if (threadIdx.x == 0) outer_step(c, a[0], b[0]);
+}
+
+__global__ __launch_bounds__(256) void k_b_set_dd(FCtl *C, const int32_t *__restrict__ vst,
+                                                  int nbV, const double *p) {
+    FCtl *c = C + blockIdx.x;
+    if (!c->outer_flag) return;
+    double a[1];
+    seg_sums<1>(p, nbV, vst[blockIdx.x], vst[blockIdx.x + 1], a);
+    if (threadIdx.x == 0) {
+        c->dd = a[0];
+        c->outer_flag = 0;
+    }
+}
+
+// ------------------------------------------------------------------ set-up kernels
+__global__ void k_concat_ptr(const int32_t *__restrict__ ptr, int64_t rows, int64_t padded,
+                             int64_t off, int32_t *__restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j > padded) return;
+    out[j] = (int32_t)(off + (j <= rows ? ptr[j] : ptr[rows]));
+}
+
+__global__ void k_concat_ind(const int32_t *__restrict__ ind, int64_t nnz, int64_t off,
+                             int32_t *__restrict__ out) {
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < nnz;
+         j += (int64_t)gridDim.x * 256)
+        out[j] = (int32_t)(ind[j] + off);
+}
+
+__global__ void k_scale_into(const float *__restrict__ x, int64_t n, float s,
+                             float *__restrict__ y) {
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
+        y[j] = s * x[j];
+}
+
+__global__ void k_unscale(const float *__restrict__ x, int64_t n, float s, float *__restrict__ y) {
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
+        y[j] = x[j] / s;
+}
+
+inline int64_t pad256(int64_t v) { return (v + GB - 1) / GB * GB; }
+
+inline unsigned grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+// ------------------------------------------------------------------ host side
+struct Batch {
+    tv3_ctx *ctx;
+    hipStream_t s;
+    const tv3_opts *o;
+    int P = 0;
+    std::vector<BDims> dims;
+    int64_t N = 0, M = 0;            // padded totals
+    int nbV = 0, nbR = 0;            // groups
+    tv3_dtype vt = TV3_F32;
+    SparseRows A, AT;                // block-diagonal CSR of A and of A^T
+    int lanes_N = 0, lanes_T = 0;
+    DevBuf<BDims> ddims;
+    DevBuf<int32_t> vst, rst;
+    DevBuf<float> pool;
+    float *U[2], *T, *g[2], *gT, *d, *sx, *sy, *sz, *ox, *oy, *oz, *dts, *Atb, *rcd;
+    float *Au[2], *AuT, *b, *delta;
+    DevBuf<double> parts, hist;
+    double *pD, *pN, *pW, *pDd, *pAd, *p4, *p5, *pOD, *pRO;
+    int64_t hist_stride = 1;
+    DevBuf<FCtl> ctl;
+    PinnedBuf<FCtl> hctl;
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 8;
+
+    ~Batch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+
+    void read_ctl() {
+        TV3_HIP(hipMemcpyAsync(hctl.p, ctl.p, sizeof(FCtl) * P, hipMemcpyDeviceToHost, s));
+        TV3_HIP(hipStreamSynchronize(s));
+    }
+    void write_ctl() {
+        TV3_HIP(hipMemcpyAsync(ctl.p, hctl.p, sizeof(FCtl) * P, hipMemcpyHostToDevice, s));
+    }
+
+    template <bool SUB>
+    void product(const SparseRows &S, int L, const float *x, float *y, const float *sub,
+                 const int32_t *starts, int pred) {
+        const int64_t rpb = 256 / L;
+        const unsigned grid = (unsigned)((S.rows + rpb - 1) / rpb);
+#define TV3_BCSR(LL, VV)                                                                        \
+    hipLaunchKernelGGL((k_b_csr<LL, VV, SUB>), dim3(grid), dim3(256), 0, s, S.rows, S.ptr.p,    \
+                       S.ind.p, (const VV *)S.val.p, x, y, sub, starts, P, ctl.p, pred)
+#define TV3_BCSR_V(LL)                                                                          \
+    switch (S.vtype) {                                                                          \
+    case TV3_F32: TV3_BCSR(LL, float); break;                                                   \
+    case TV3_F16: TV3_BCSR(LL, __half); break;                                                  \
+    case TV3_BF16: TV3_BCSR(LL, __hip_bfloat16); break;                                         \
+    default: throw tv3_error(TV3_E_FORMAT, "batched products take F32, F16 or BF16 weights");   \
+    }
+        switch (L) {
+        case 2: TV3_BCSR_V(2); break;
+        case 4: TV3_BCSR_V(4); break;
+        case 8: TV3_BCSR_V(8); break;
+        case 16: TV3_BCSR_V(16); break;
+        case 32: TV3_BCSR_V(32); break;
+        case 64: TV3_BCSR_V(64); break;
+        default: throw tv3_error(TV3_E_ARG, "lanes per row must be a power of two in [2, 64]");
+        }
+#undef TV3_BCSR_V
+#undef TV3_BCSR
+    }
+    void prod_N(const float *x, float *y, int pred) {
+        product<false>(A, lanes_N, x, y, nullptr, rst.p, pred);
+    }
+    void prod_T(const float *x, float *y, const float *sub, int pred) {
+        if (sub) product<true>(AT, lanes_T, x, y, sub, vst.p, pred);
+        else product<false>(AT, lanes_T, x, y, nullptr, vst.p, pred);
+    }
+
+    // one inner iteration of every active problem
+    void iteration() {
+        const dim3 bV(nbV), bR(nbR), bP(P), t(256);
+        hipLaunchKernelGGL(k_b_descend, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, U[0], U[1], d, sx,
+                           sy, sz, ox, oy, oz, T, pD);
+        prod_N(T, AuT, P_ACTIVE);
+        hipLaunchKernelGGL(k_b_res, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, (int)P_ACTIVE, AuT, b,
+                           delta, Au[0], Au[1], pN);
+        prod_T(AuT, gT, Atb, P_ACTIVE);
+        hipLaunchKernelGGL(k_b_search, bP, t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, pD, pN);
+        hipLaunchKernelGGL(k_b_apply, dim3(nbV + nbR), t, 0, s, ddims.p, vst.p, rst.p, P, nbV,
+                           ctl.p, U[0], U[1], T, g[0], g[1], gT, Au[0], Au[1], AuT);
+        hipLaunchKernelGGL(k_b_update_w, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, 0, U[0], U[1],
+                           sx, sy, sz, g[0], g[1], dts, d, pW);
+        hipLaunchKernelGGL(k_b_iter_end, bP, t, 0, s, ctl.p, vst.p, nbV, pW, hist.p,
+                           hist_stride);
+    }
+
+    // first-iteration step lengths of the problems that need one
+    void tau_pass() {
+        const dim3 bV(nbV), bR(nbR), t(256);
+        hipLaunchKernelGGL(k_b_dd_norm, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, d, pDd);
+        prod_N(d, AuT, P_TAU);
+        hipLaunchKernelGGL(k_b_sqnorm, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, AuT, pAd);
+        hipLaunchKernelGGL(k_b_tau, dim3(P), t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, pDd, pAd);
+    }
+
+    void capture() {
+        hipGraph_t gr;
+        TV3_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < graph_iters; ++k) iteration();
+        TV3_HIP(hipStreamEndCapture(s, &gr));
+        TV3_HIP(hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0));
+        TV3_HIP(hipGraphDestroy(gr));
+    }
+
+    // average time (ms) of a product launch with L lanes, every problem on
+    float time_product(bool trans, int L, const float *x, float *y) {
+        hipEvent_t a, e;
+        TV3_HIP(hipEventCreate(&a)); TV3_HIP(hipEventCreate(&e));
+        auto run = [&]() {
+            if (trans) product<false>(AT, L, x, y, nullptr, vst.p, P_ALL);
+            else product<false>(A, L, x, y, nullptr, rst.p, P_ALL);
+        };
+        run();
+        TV3_HIP(hipEventRecord(a, s));
+        for (int r = 0; r < 3; ++r) run();
+        TV3_HIP(hipEventRecord(e, s));
+        TV3_HIP(hipEventSynchronize(e));
+        float ms = 0;
+        TV3_HIP(hipEventElapsedTime(&ms, a, e));
+        (void)hipEventDestroy(a); (void)hipEventDestroy(e);
+        return ms / 3;
+    }
+
+    void tune() {
+        const int cand[] = {2, 4, 8, 16, 32, 64};
+        float bestN = 1e30f, bestT = 1e30f;
+        for (int L : cand) {
+            const float tn = time_product(false, L, U[0], AuT);
+            if (tn < bestN) { bestN = tn; lanes_N = L; }
+            const float tt = time_product(true, L, AuT, gT);
+            if (tt < bestT) { bestT = tt; lanes_T = L; }
+        }
+    }
+};
+
+void concat_matrix(tv3_ctx *ctx, hipStream_t s, const std::vector<const SparseRows *> &parts,
+                   const std::vector<int64_t> &row0, const std::vector<int64_t> &col0,
+                   int64_t rows_total, int64_t cols_total, SparseRows &out) {
+    const int P = (int)parts.size();
+    int64_t nnz = 0;
+    for (auto *S : parts) nnz += S->nnz;
+    TV3_REQUIRE(nnz < ((int64_t)1 << 31), TV3_E_SHAPE,
+                "the batch holds more than 2^31 - 1 entries (int32 CSR)");
+    out.rows = rows_total; out.cols = cols_total; out.nnz = nnz; out.vtype = parts[0]->vtype;
+    const size_t vs = dtype_size(out.vtype);
+    out.ptr.alloc(rows_total + 1);
+    out.ind.alloc(nnz > 0 ? nnz : 1);
+    out.val.alloc((nnz > 0 ? nnz : 1) * vs);
+    int64_t off = 0;
+    for (int k = 0; k < P; ++k) {
+        const SparseRows &S = *parts[k];
+        const int64_t padded = (k + 1 < P ? row0[k + 1] : rows_total) - row0[k];
+        hipLaunchKernelGGL(k_concat_ptr, dim3((unsigned)((padded + 256) / 256)), dim3(256), 0, s,
+                           S.ptr.p, S.rows, padded, off, out.ptr.p + row0[k]);
+        if (S.nnz) {
+            hipLaunchKernelGGL(k_concat_ind, dim3(grid_for(S.nnz)), dim3(256), 0, s, S.ind.p,
+                               S.nnz, col0[k], out.ind.p + off);
+            TV3_HIP(hipMemcpyAsync(out.val.p + off * vs, S.val.p, S.nnz * vs,
+                                   hipMemcpyDeviceToDevice, s));
+        }
+        off += S.nnz;
+    }
+    (void)ctx;
+    TV3_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3,
+                   const tv3_opts &o, const float *const *b_dev, float *const *U_dev,
+                   const float *const *Ut_dev, tv3_info *infos, tv3_history *hists,
+                   tv3_batch_info *binfo) {
+    auto t_host0 = std::chrono::steady_clock::now();
+    Batch B;
+    B.ctx = ctx; B.s = ctx->stream; B.o = &o; B.P = P;
+    hipStream_t s = B.s;
+    B.graph_iters = o.graph_iters > 0 ? o.graph_iters : 8;
+    // ranges
+    B.dims.resize(P);
+    std::vector<int32_t> vst(P + 1), rst(P + 1);
+    std::vector<int64_t> v0(P), r0(P);
+    int64_t N = 0, M = 0;
+    for (int k = 0; k < P; ++k) {
+        tv3_op *op = ops[k];
+        TV3_REQUIRE(op, TV3_E_ARG, "NULL operator in the batch");
+        TV3_REQUIRE(op->kind == OpKind::CSR, TV3_E_FORMAT,
+                    "batched solves take sparse (CSR) operators");
+        TV3_REQUIRE(op->A.vtype == ops[0]->A.vtype, TV3_E_FORMAT,
+                    "every operator of a batch must have the same weight type");
+        const int32_t *dk = dims3 + 3 * k;
+        TV3_REQUIRE(dk[0] > 0 && dk[1] > 0 && dk[2] > 0, TV3_E_ARG,
+                    "dims must be three positive sizes");
+        const int64_t n = (int64_t)dk[0] * dk[1] * dk[2];
+        TV3_REQUIRE(n == op->n, TV3_E_SHAPE, "prod(dims) must equal the columns of A");
+        TV3_REQUIRE(n < ((int64_t)1 << 31) && op->m < ((int64_t)1 << 31), TV3_E_SHAPE,
+                    "problem too large for the batched solver");
+        BDims &bd = B.dims[k];
+        bd.p = dk[0]; bd.q = dk[1]; bd.r = dk[2]; bd.pad = 0;
+        bd.v0 = N; bd.n = n; bd.r0 = M; bd.m = op->m;
+        v0[k] = N; r0[k] = M;
+        vst[k] = (int32_t)(N / GB); rst[k] = (int32_t)(M / GB);
+        N += pad256(n);
+        M += pad256(std::max<int64_t>(op->m, 1));
+    }
+    TV3_REQUIRE(N / GB < ((int64_t)1 << 31) && M / GB < ((int64_t)1 << 31), TV3_E_SHAPE,
+                "batch too large");
+    vst[P] = (int32_t)(N / GB); rst[P] = (int32_t)(M / GB);
+    B.N = N; B.M = M; B.nbV = (int)(N / GB); B.nbR = (int)(M / GB);
+    B.ddims.alloc(P);
+    B.vst.alloc(P + 1);
+    B.rst.alloc(P + 1);
+    TV3_HIP(hipMemcpyAsync(B.ddims.p, B.dims.data(), sizeof(BDims) * P, hipMemcpyHostToDevice, s));
+    TV3_HIP(hipMemcpyAsync(B.vst.p, vst.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice, s));
+    TV3_HIP(hipMemcpyAsync(B.rst.p, rst.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice, s));
+    // block-diagonal matrices
+    {
+        std::vector<const SparseRows *> pa(P), pt(P);
+        for (int k = 0; k < P; ++k) { pa[k] = &ops[k]->A; pt[k] = &ops[k]->AT; }
+        concat_matrix(ctx, s, pa, r0, v0, M, N, B.A);
+        concat_matrix(ctx, s, pt, v0, r0, N, M, B.AT);
+    }
+    // vectors
+    const int nv = 2 + 1 + 2 + 1 + 1 + 3 + 3 + 1 + 1 + 1, mv = 2 + 1 + 1 + 1;
+    B.pool.alloc((size_t)nv * N + (size_t)mv * M);
+    TV3_HIP(hipMemsetAsync(B.pool.p, 0, B.pool.bytes(), s));
+    {
+        float *p = B.pool.p;
+        float **nvp[] = {&B.U[0], &B.U[1], &B.T, &B.g[0], &B.g[1], &B.gT, &B.d, &B.sx, &B.sy,
+                         &B.sz, &B.ox, &B.oy, &B.oz, &B.dts, &B.Atb, &B.rcd};
+        for (float **q : nvp) { *q = p; p += N; }
+        float **mvp[] = {&B.Au[0], &B.Au[1], &B.AuT, &B.b, &B.delta};
+        for (float **q : mvp) { *q = p; p += M; }
+    }
+    {
+        const size_t nbV = B.nbV, nbR = B.nbR;
+        B.parts.alloc(3 * nbV + 3 * nbR + 7 * nbV + nbV + nbR + nbV + nbR + nbV + nbV);
+        double *q = B.parts.p;
+        B.pD = q; q += 3 * nbV;
+        B.pN = q; q += 3 * nbR;
+        B.pW = q; q += 7 * nbV;
+        B.pDd = q; q += nbV;
+        B.pAd = q; q += nbR;
+        B.p4 = q; q += nbV;
+        B.p5 = q; q += nbR;
+        B.pOD = q; q += nbV;
+        B.pRO = q; q += nbV;
+    }
+    B.hist_stride = (int64_t)std::max(o.maxit, 1) * HF;
+    B.hist.alloc((size_t)P * B.hist_stride);
+    B.ctl.alloc(P);
+    B.hctl.alloc(P);
+
+    hipEvent_t ev0, ev1, evl0, evl1;
+    TV3_HIP(hipEventCreate(&ev0)); TV3_HIP(hipEventCreate(&ev1));
+    TV3_HIP(hipEventCreate(&evl0)); TV3_HIP(hipEventCreate(&evl1));
+
+    // launch shapes of the two products (every problem active), unless given
+    if (binfo && binfo->lanes_N > 0 && binfo->lanes_T > 0) {
+        B.lanes_N = binfo->lanes_N;
+        B.lanes_T = binfo->lanes_T;
+    } else {
+        B.tune();
+    }
+    TV3_HIP(hipEventRecord(ev0, s));
+
+    // scaleb (:225-266) per problem on a host copy of b
+    std::vector<float> scl(P, 1.f);
+    for (int k = 0; k < P; ++k) {
+        const int64_t m = B.dims[k].m;
+        std::vector<float> bh(m);
+        TV3_HIP(hipMemcpyAsync(bh.data(), b_dev[k], sizeof(float) * m, hipMemcpyDeviceToHost, s));
+        TV3_HIP(hipStreamSynchronize(s));
+        float bmin = bh[0], bmax = bmin, amin = std::fabs(bmin), amax = amin;
+        for (float v : bh) {
+            TV3_REQUIRE(std::isfinite(v), TV3_E_NONFINITE, "b holds NaN or Inf");
+            const float a = std::fabs(v);
+            if (a < amin) { bmin = v; amin = a; }
+            if (a > amax) { bmax = v; amax = a; }
+        }
+        const float dif = std::fabs(bmax - bmin);
+        if (dif < 0.5f) scl[k] = 0.5f / dif;
+        else if (dif > 1.5f) scl[k] = 1.5f / dif;
+        hipLaunchKernelGGL(k_scale_into, dim3(grid_for(m)), dim3(256), 0, s, b_dev[k], m, scl[k],
+                           B.b + B.dims[k].r0);
+    }
+
+    FCtl *H = B.hctl.p;
+    std::memset(H, 0, sizeof(FCtl) * P);
+    for (int k = 0; k < P; ++k) {
+        FCtl &c = H[k];
+        c.mu = std::min(o.mu0, o.mu); c.beta = std::min(o.beta0, o.beta);
+        c.muDbeta = c.mu / c.beta;
+        c.c = o.c; c.gamma = o.gamma; c.rate_gam = o.rate_gam; c.tol_inn = o.tol_inn;
+        c.gam_opt = o.gam; c.gam = o.gam; c.rate_cnt = o.rate_cnt; c.mu_max = o.mu;
+        c.beta_max = o.beta; c.maxit = o.maxit; c.nonneg = o.nonneg;
+        c.iso = o.tv_mode == TV3_TV_ISO; c.hist_cap = o.maxit; c.active = 1; c.alpha = 1;
+    }
+    B.write_ctl();
+
+    // initialisation (:1128-1153), every problem at once
+    B.prod_T(B.b, B.Atb, nullptr, P_ALL);
+    for (int k = 0; k < P; ++k) {
+        const BDims &bd = B.dims[k];
+        if (o.warm_start)
+            hipLaunchKernelGGL(k_scale_into, dim3(grid_for(bd.n)), dim3(256), 0, s, U_dev[k], bd.n,
+                               scl[k], B.U[0] + bd.v0);
+    }
+    if (!o.warm_start)
+        TV3_HIP(hipMemcpyAsync(B.U[0], B.Atb, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
+    TV3_HIP(hipMemcpyAsync(B.rcd, B.U[0], sizeof(float) * N, hipMemcpyDeviceToDevice, s));
+    B.prod_N(B.U[0], B.Au[0], P_ALL);
+    hipLaunchKernelGGL(k_b_res, dim3(B.nbR), dim3(256), 0, s, B.ddims.p, B.rst.p, P, B.ctl.p,
+                       (int)P_ALL, B.Au[0], B.b, B.delta, B.Au[0], B.Au[1], B.pN);
+    B.prod_T(B.Au[0], B.g[0], B.Atb, P_ALL);
+    hipLaunchKernelGGL(k_b_update_w, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P, B.ctl.p,
+                       1, B.U[0], B.U[1], B.sx, B.sy, B.sz, B.g[0], B.g[1], B.dts, B.d, B.pW);
+    hipLaunchKernelGGL(k_b_init, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p, B.rst.p, B.nbV,
+                       B.nbR, B.pN, B.pW);
+    TV3_HIP(hipGetLastError());
+    B.capture();
+    TV3_HIP(hipEventRecord(evl0, s));
+    if (o.maxcnt <= 0 || o.maxit <= 0) {     // the outer loop of :1166 does not run
+        B.read_ctl();
+        for (int k = 0; k < P; ++k) { H[k].done = 1; H[k].active = 0; H[k].need_tau = 0; }
+        B.write_ctl();
+    }
+
+    // per-problem loop state lives in the control blocks; the host only
+    // classifies problems after each launch and runs the outer updates
+    int graph_launches = 0;
+    B.read_ctl();
+    for (;;) {
+        bool any_tau = false, any_active = false, any_outer = false;
+        for (int k = 0; k < P; ++k) {
+            FCtl &c = H[k];
+            if (c.done) continue;
+            if (c.need_tau) { any_tau = true; continue; }
+            if (c.active) { any_active = true; continue; }
+            if (c.nonfinite) { c.done = 1; continue; }
+            // the inner loop has ended (:1170-1200): outer update
+            c.outer += 1;
+            c.outer_flag = 1;
+            any_outer = true;
+        }
+        if (any_outer) {
+            B.write_ctl();
+            hipLaunchKernelGGL(k_b_relout, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P,
+                               B.ctl.p, B.U[0], B.U[1], B.rcd, B.pRO);
+            hipLaunchKernelGGL(k_b_relout_scalar, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p,
+                               B.nbV, B.pRO);
+            B.read_ctl();
+            for (int k = 0; k < P; ++k) {
+                FCtl &c = H[k];
+                if (!c.outer_flag) continue;
+                if (c.relout < o.tol) {          // :1218
+                    c.outer_flag = 0;
+                    c.done = 1;
+                }
+            }
+            B.write_ctl();
+            hipLaunchKernelGGL(k_b_mlp_sigma, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P,
+                               B.ctl.p, B.U[0], B.U[1], B.sx, B.sy, B.sz, B.ox, B.oy, B.oz, B.p4);
+            hipLaunchKernelGGL(k_b_mlp_delta, dim3(B.nbR), dim3(256), 0, s, B.ddims.p, B.rst.p, P,
+                               B.ctl.p, B.Au[0], B.Au[1], B.b, B.delta, B.p5);
+            hipLaunchKernelGGL(k_b_outer_scalars, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p,
+                               B.rst.p, B.nbV, B.nbR, B.p4, B.p5);
+            hipLaunchKernelGGL(k_b_outer_d, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P,
+                               B.ctl.p, B.d, B.g[0], B.g[1], B.dts, B.pOD);
+            hipLaunchKernelGGL(k_b_set_dd, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p, B.nbV,
+                               B.pOD);
+            B.read_ctl();
+            for (int k = 0; k < P; ++k) {      // loop condition of :1166
+                FCtl &c = H[k];
+                if (c.done) continue;
+                if (c.outer >= o.maxcnt || c.total >= o.maxit) {
+                    c.done = 1;
+                    c.active = 0;
+                    c.need_tau = 0;
+                }
+            }
+            B.write_ctl();
+            continue;
+        }
+        if (any_tau) {                 // first iteration of an inner loop or a fallback
+            B.tau_pass();
+            B.iteration();
+        } else if (any_active) {
+            TV3_HIP(hipGraphLaunch(B.graph, s));
+            ++graph_launches;
+        } else {
+            break;
+        }
+        B.read_ctl();
+    }
+    TV3_HIP(hipEventRecord(evl1, s));
+    for (int k = 0; k < P; ++k) {
+        const BDims &bd = B.dims[k];
+        const float *Uk = H[k].cur ? B.U[1] : B.U[0];
+        hipLaunchKernelGGL(k_unscale, dim3(grid_for(bd.n)), dim3(256), 0, s, Uk + bd.v0, bd.n,
+                           scl[k], U_dev[k]);
+    }
+    TV3_HIP(hipEventRecord(ev1, s));
+    TV3_HIP(hipStreamSynchronize(s));
+    float ms = 0, msl = 0;
+    TV3_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    TV3_HIP(hipEventElapsedTime(&msl, evl0, evl1));
+    int64_t tot = 0;
+    std::vector<double> hh;
+    for (int k = 0; k < P; ++k) {
+        const FCtl &c = H[k];
+        tv3_info &inf = infos[k];
+        std::memset(&inf, 0, sizeof(inf));
+        inf.total_iters = c.total;
+        inf.outer_iters = c.outer;
+        inf.rel_chg = c.relout;
+        inf.secs = ms / 1000.0;
+        inf.loop_secs = msl / 1000.0;
+        inf.backtracks = c.backtracks;
+        inf.fallbacks = c.fallbacks;
+        inf.n_matvec_N = 1 + (int64_t)c.total + c.inner_loops + 2 * (int64_t)c.fallbacks;
+        inf.n_matvec_T = 2 + (int64_t)c.total + c.fallbacks;
+        inf.status = c.nonfinite ? TV3_E_NONFINITE : TV3_OK;
+        tot += c.total;
+        if (Ut_dev && Ut_dev[k]) {
+            auto e = diff_norm<float>(ctx, B.dims[k].n, Ut_dev[k], U_dev[k], s);
+            inf.rel_error = std::sqrt(e[0]) / std::sqrt(e[1]);
+        }
+        if (hists && hists[k].cap > 0) {
+            tv3_history &h = hists[k];
+            const int cnt = std::min(c.total, h.cap);
+            hh.assign((size_t)std::max(cnt, 1) * HF, 0.0);
+            if (cnt > 0)
+                TV3_HIP(hipMemcpy(hh.data(), B.hist.p + (int64_t)k * B.hist_stride,
+                                  sizeof(double) * HF * cnt, hipMemcpyDeviceToHost));
+            for (int j = 0; j < cnt; ++j) {
+                const double *q = &hh[(size_t)j * HF];
+                h.f[j] = q[0]; h.lam1[j] = q[1]; h.lam2[j] = q[2]; h.lam3[j] = q[3];
+                h.lam4[j] = q[4]; h.lam5[j] = q[5]; h.tau[j] = q[6]; h.alpha[j] = q[7];
+                h.relchg[j] = q[8]; h.mu[j] = q[9]; h.beta[j] = q[10]; h.C[j] = q[11];
+                h.cnt[j] = (int32_t)q[12]; h.outer[j] = (int32_t)q[13];
+            }
+            h.count = cnt;
+        }
+    }
+    if (binfo) {
+        std::memset(binfo, 0, sizeof(*binfo));
+        binfo->lanes_N = B.lanes_N;
+        binfo->lanes_T = B.lanes_T;
+        binfo->graph_iters = B.graph_iters;
+        binfo->graph_launches = graph_launches;
+        binfo->secs = ms / 1000.0;
+        binfo->loop_secs = msl / 1000.0;
+        binfo->host_secs =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t_host0).count();
+        binfo->total_iters = tot;
+        binfo->nnz = B.A.nnz;
+        binfo->voxels = N;
+        binfo->rays = M;
+    }
+    (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1);
+    (void)hipEventDestroy(evl0); (void)hipEventDestroy(evl1);
+}
+
+}  // namespace tv3

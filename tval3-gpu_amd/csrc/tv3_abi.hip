@@ -556,6 +556,31 @@ tv3_status tv3_solve_f64(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const i
     return solve_host<double>(ctx, op, opts, dims, b, U, Ut, info, hist);
 }
 
+tv3_status tv3_solve_batched(tv3_ctx *ctx, int32_t nprob, tv3_op *const *ops,
+                             const int32_t *dims, const tv3_opts *opts,
+                             const float *const *b_dev, float *const *U_dev,
+                             const float *const *Ut_dev, tv3_info *infos, tv3_history *hists,
+                             tv3_batch_info *binfo) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        validate_opts(opts);
+        TV3_REQUIRE(opts->skipMulRatio == 0.0, TV3_E_UNSUPPORTED,
+                    "skipMulRatio != 0 is supported by the plain solver only");
+        TV3_REQUIRE(nprob > 0 && ops && dims && b_dev && U_dev && infos, TV3_E_ARG,
+                    "NULL argument or empty batch");
+        for (int32_t k = 0; k < nprob; ++k) {
+            TV3_REQUIRE(ops[k] && b_dev[k] && U_dev[k], TV3_E_ARG, "NULL entry in the batch");
+            TV3_REQUIRE(ops[k]->ctx == ctx, TV3_E_ARG, "operator of another context");
+        }
+        if (binfo && (binfo->lanes_N > 0 || binfo->lanes_T > 0))
+            for (int32_t L : {binfo->lanes_N, binfo->lanes_T})
+                TV3_REQUIRE(L >= 2 && L <= 64 && (L & (L - 1)) == 0, TV3_E_ARG,
+                            "lanes per row must be powers of two in [2, 64]");
+        set_device(ctx);
+        solve_batched(ctx, nprob, ops, dims, *opts, b_dev, U_dev, Ut_dev, infos, hists, binfo);
+    });
+}
+
 tv3_status tv3_D(tv3_ctx *ctx, const int32_t dims[3], const float *U, float *Ux, float *Uy,
                  float *Uz, void *stream) {
     if (!ctx) return TV3_E_ARG;

@@ -312,6 +312,10 @@ void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
 void solve_fused(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims[3],
                  const float *b_dev, float *U_dev, const float *Ut_dev, tv3_info *info,
                  tv3_history *hist);
+void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3,
+                   const tv3_opts &o, const float *const *b_dev, float *const *U_dev,
+                   const float *const *Ut_dev, tv3_info *infos, tv3_history *hists,
+                   tv3_batch_info *binfo);
 
 // stencil building blocks (vec.hip), float and double
 template <typename T>

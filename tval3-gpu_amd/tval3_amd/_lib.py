@@ -78,6 +78,16 @@ class LayoutInfo(C.Structure):
                 ("max_seg", C.c_int32), ("reserved", C.c_int32)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("lanes_N", C.c_int32), ("lanes_T", C.c_int32), ("graph_iters", C.c_int32),
+                ("graph_launches", C.c_int32), ("secs", C.c_double), ("loop_secs", C.c_double),
+                ("host_secs", C.c_double), ("total_iters", C.c_int64), ("nnz", C.c_int64),
+                ("voxels", C.c_int64), ("rays", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # (name, restype, argtypes) of every exported entry point of the header
 SIGNATURES = [
     ("tv3_abi_version", C.c_int32, []),
@@ -106,6 +116,8 @@ SIGNATURES = [
                                    C.POINTER(History)]),
     ("tv3_solve_f64", C.c_int, [_P, _P, C.POINTER(Opts), _P, _P, _P, _P, C.POINTER(Info),
                                 C.POINTER(History)]),
+    ("tv3_solve_batched", C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(Opts), _P, _P, _P, _P, _P,
+                                    C.POINTER(BatchInfo)]),
     ("tv3_D", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     ("tv3_Dt", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     ("tv3_shrink", C.c_int, [_P, C.c_int64, C.c_int, _P, _P, _P, _P, _P, _P, C.c_double, _P,

@@ -362,3 +362,54 @@ def tval3_gpu_3d(ctx, A, b, dims, opts=None, Ut=None, U0=None, history=True, his
     if hp is not None:
         hist = {k: v[:hp.count].copy() for k, v in harr.items()}
     return U, info.as_dict(), hist
+
+
+def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
+                         hist_cap=4096, lanes=None):
+    """Solve many problems together (tv3_solve_batched): ops a list of sparse
+    operators (one weight type), bs their b (float32 torch tensors on the
+    device), dims a list of (dim_y, dim_x, dim_z).  Every kernel of an
+    iteration covers all problems; each problem keeps its own scalars and
+    stopping.  lanes = (lanes_N, lanes_T) fixes the products' launch shapes
+    (default: timed).  Returns (list of U tensors, list of info dicts, batch
+    info dict, list of history dicts or None)."""
+    import torch
+    o = opts if opts is not None else default_opts()
+    P = len(ops)
+    if not (len(bs) == len(dims) == P) or P == 0:
+        raise ValueError("ops, bs and dims must be non-empty lists of equal length")
+    for k, (op, b) in enumerate(zip(ops, bs)):
+        if b.dtype != torch.float32 or not b.is_contiguous() or b.numel() != op.m:
+            raise ValueError(f"b[{k}] must be a contiguous float32 tensor of {op.m} entries")
+    n = [int(np.prod(dk)) for dk in dims]
+    if o.warm_start:
+        raise ValueError("use the U0 of tval3_gpu_3d for warm starts; batched solves start "
+                         "from A^T b")
+    Us = [torch.zeros(nk, dtype=torch.float32, device=bs[0].device) for nk in n]
+    Ph = C.c_void_p * P
+    opsa = Ph(*[op.h for op in ops])
+    d = (C.c_int32 * (3 * P))(*[int(v) for dk in dims for v in dk])
+    ba = Ph(*[b.data_ptr() for b in bs])
+    ua = Ph(*[u.data_ptr() for u in Us])
+    uta = None
+    if Uts is not None:
+        uta = Ph(*[(ut.data_ptr() if ut is not None else None) for ut in Uts])
+    infos = (L.Info * P)()
+    hp = None
+    keep = []
+    if history:
+        hp = (L.History * P)()
+        for k in range(P):
+            h, arrs = _history(hist_cap)
+            hp[k] = h
+            keep.append(arrs)
+    bi = L.BatchInfo()
+    if lanes is not None:
+        bi.lanes_N, bi.lanes_T = lanes
+    torch.cuda.synchronize()
+    ctx.check(L.load().tv3_solve_batched(ctx.h, P, opsa, d, C.byref(o), ba, ua, uta, infos,
+                                         hp, C.byref(bi)))
+    hists = None
+    if history:
+        hists = [{k: v[:hp[j].count].copy() for k, v in keep[j].items()} for j in range(P)]
+    return Us, [infos[k].as_dict() for k in range(P)], bi.as_dict(), hists

@@ -1,0 +1,158 @@
+"""Batched TVAL3 on a batch of uneven problems (one persistent set of launches
+per iteration, graph-captured iterations).
+
+    python -m tval3_amd.batchbench [--problems 40] [--maxit 100] [--seed 1000]
+                                   [--no-fp64] [--no-sequential]
+
+Workload: `problems` independent problems, problem i seeded with
+seed + 1000 i: every volume edge drawn uniformly from {32, 48, 64, 80, 96}
+(per axis), a slowness phantom, E x R rays on the sphere of radius 1.1 with
+E x R nearest to 0.25 x voxels at E:R = 1:4, Siddon weights in a BF16 CSR
+built on the device, b = A u_true + 0.5% noise; isotropic TV, mu = 2^9.
+
+Reports the products' launch shapes picked by the timing autotuner,
+aggregate inner iterations per second of the batch, the same problems solved
+one after another (the batched solver on one problem at a time, and the
+fused solver), and per problem the error against u_true and against the FP64
+plain solve of the same problem (tv3_solve_f64, same weights).  Prints one
+JSON object.
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+EDGES = (32, 48, 64, 80, 96)
+
+
+def problem_shapes(count, seed):
+    """(dims, E, R) of each problem: edges uniform over EDGES per axis; E x R
+    nearest to n/4 with R = 4E."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng(seed + 1000 * i)
+        dims = tuple(int(v) for v in rng.choice(EDGES, size=3))
+        n = dims[0] * dims[1] * dims[2]
+        E = max(1, int(round(np.sqrt(n / 16.0))))
+        out.append((dims, E, 4 * E, seed + 1000 * i))
+    return out
+
+
+def build(ctx, shapes, dtype="bf16"):
+    import torch
+
+    from . import api, workload as wl
+    probs = []
+    for dims, E, R, sd in shapes:
+        u = wl.phantom_3d(dims=dims, seed=sd)
+        g = wl.transducer_geometry(E, R, dims=dims, seed=sd + 1)
+        A = api.DynamicOperator(ctx, g, "siddon").materialize(dtype)
+        ut = torch.tensor(u, dtype=torch.float32, device="cuda")
+        Au = A.mat_vec_mul(ut)
+        b = wl.add_noise(Au.double().cpu().numpy(), 0.005, seed=sd + 2).astype(np.float32)
+        probs.append(dict(dims=dims, E=E, R=R, A=A, u=u, ut=ut, b=b,
+                          bt=torch.from_numpy(b).cuda()))
+    return probs
+
+
+def run(problems=40, maxit=100, seed=1000, fp64=True, sequential=True, log=print):
+    import torch
+
+    from . import api
+    ctx = api.Context(0)
+    t0 = time.time()
+    shapes = problem_shapes(problems, seed)
+    probs = build(ctx, shapes)
+    torch.cuda.synchronize()
+    nvox = sum(int(np.prod(p["dims"])) for p in probs)
+    nnz = sum(p["A"].nnz for p in probs)
+    rays = sum(p["A"].m for p in probs)
+    log(f"{problems} problems: {nvox} voxels, {rays} rays, {nnz} nnz (BF16), built in "
+        f"{time.time() - t0:.1f}s")
+    o = api.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tv_mode="iso", maxit=maxit)
+    ops = [p["A"] for p in probs]
+    bs = [p["bt"] for p in probs]
+    dims = [p["dims"] for p in probs]
+    uts = [p["ut"] for p in probs]
+    # warm-up (tunes the launch shapes), then the timed batch
+    _, _, bi0, _ = api.tval3_gpu_3d_batched(ctx, ops, bs, dims, o, Uts=uts)
+    lanes = (bi0["lanes_N"], bi0["lanes_T"])
+    Ub, ib, bi, _ = api.tval3_gpu_3d_batched(ctx, ops, bs, dims, o, Uts=uts, lanes=lanes)
+    out = dict(workload=dict(problems=problems, voxels=nvox, rays=rays, nnz=nnz, weights="bf16",
+                             maxit=maxit, seed=seed, shapes=[list(p["dims"]) for p in probs]),
+               tuned=dict(lanes_N=bi0["lanes_N"], lanes_T=bi0["lanes_T"]),
+               batch=dict(loop_s=bi["loop_secs"], total_iters=bi["total_iters"],
+                          iter_per_s=bi["total_iters"] / bi["loop_secs"],
+                          graph_launches=bi["graph_launches"], secs=bi["secs"],
+                          host_secs=bi["host_secs"]))
+    log(f"batch: {bi['total_iters']} inner iterations in {bi['loop_secs']:.3f}s -> "
+        f"{out['batch']['iter_per_s']:.0f} iter/s (lanes {lanes})")
+    if sequential:
+        t_solo, it_solo = 0.0, 0
+        for p in probs:
+            _, i1, b1, _ = api.tval3_gpu_3d_batched(ctx, [p["A"]], [p["bt"]], [p["dims"]], o,
+                                                    lanes=lanes)
+            t_solo += b1["loop_secs"]
+            it_solo += b1["total_iters"]
+        out["one_at_a_time"] = dict(loop_s=t_solo, total_iters=it_solo,
+                                    iter_per_s=it_solo / t_solo)
+        of = api.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tv_mode="iso", maxit=maxit,
+                              solver="fused")
+        t_f, it_f = 0.0, 0
+        for p in probs:
+            p["A"].prepare(p["dims"])
+            _, inf, _ = api.tval3_gpu_3d(ctx, p["A"], p["bt"], p["dims"], of, history=False)
+            t_f += inf["loop_secs"]
+            it_f += inf["total_iters"]
+        out["fused_one_at_a_time"] = dict(loop_s=t_f, total_iters=it_f, iter_per_s=it_f / t_f)
+        log(f"one at a time: batched solver {out['one_at_a_time']['iter_per_s']:.0f} iter/s, "
+            f"fused solver {out['fused_one_at_a_time']['iter_per_s']:.0f} iter/s")
+    per = []
+    for k, p in enumerate(probs):
+        rec = dict(dims=list(p["dims"]), iters=ib[k]["total_iters"], outer=ib[k]["outer_iters"],
+                   err_utrue=ib[k]["rel_error"])
+        if fp64:
+            # the BF16 weights exactly, as F32 (FP64 products take F32 or F64 weights)
+            rp, col, val = p["A"].download_csr()
+            A32 = api.SparseOperator(ctx, p["A"].m, p["A"].n, rp, col, val, dtype="f32")
+            o64 = api.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tv_mode="iso", maxit=maxit)
+            U64, i64, _ = api.tval3_gpu_3d(ctx, A32, p["b"].astype(np.float64), p["dims"],
+                                           o64, Ut=p["u"], history=False)
+            A32.close()
+            Ug = Ub[k].double().cpu().numpy()
+            rec["err_fp64"] = float(np.linalg.norm(Ug - U64) / np.linalg.norm(U64))
+            rec["err_utrue_fp64"] = i64["rel_error"]
+            rec["iters_fp64"] = i64["total_iters"]
+        per.append(rec)
+    out["per_problem"] = per
+    e = np.array([r["err_utrue"] for r in per])
+    out["err_utrue_median"] = float(np.median(e))
+    if fp64:
+        e64 = np.array([r["err_fp64"] for r in per])
+        d = np.array([r["err_utrue"] - r["err_utrue_fp64"] for r in per])
+        out["err_fp64_median"] = float(np.median(e64))
+        out["err_fp64_max"] = float(e64.max())
+        out["err_utrue_minus_fp64_max"] = float(np.abs(d).max())
+        log(f"error vs FP64 solve: median {out['err_fp64_median']:.2e} max "
+            f"{out['err_fp64_max']:.2e}; error vs u_true: batch median "
+            f"{out['err_utrue_median']:.4f}, |batch - FP64| max "
+            f"{out['err_utrue_minus_fp64_max']:.2e}")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--problems", type=int, default=40)
+    ap.add_argument("--maxit", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-sequential", action="store_true")
+    a = ap.parse_args(argv)
+    out = run(a.problems, a.maxit, a.seed, not a.no_fp64, not a.no_sequential,
+              log=lambda s: print(s, flush=True))
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
