@@ -452,15 +452,11 @@ static void tune_panels(tv3_ctx *ctx, FusedLayout &F, float t_A, float t_T) {
     }
 }
 
-FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
+void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L) {
     TV3_REQUIRE(op->kind == OpKind::CSR, TV3_E_UNSUPPORTED,
                 "the fused solver needs a sparse operator (materialize a dynamic operator first)");
-    if (op->fused && op->fused->dims[0] == dims[0] && op->fused->dims[1] == dims[1] &&
-        op->fused->dims[2] == dims[2])
-        return *op->fused;
     tv3_ctx *ctx = op->ctx;
     hipStream_t s = ctx->stream;
-    std::unique_ptr<FusedLayout> L(new FusedLayout());
     for (int a = 0; a < 3; ++a) L->dims[a] = dims[a];
     const int p = dims[0], q = dims[1], r = dims[2];
     const int64_t n = (int64_t)p * q * r, m = op->m;
@@ -501,6 +497,18 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
         throw tv3_error(TV3_E_FORMAT, "the fused solver needs F32, F16 or BF16 weights");
     TV3_HIP(hipGetLastError());
     build_transpose(ctx, N, L->AT);
+}
+
+FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
+    if (op->fused && op->fused->dims[0] == dims[0] && op->fused->dims[1] == dims[1] &&
+        op->fused->dims[2] == dims[2])
+        return *op->fused;
+    tv3_ctx *ctx = op->ctx;
+    hipStream_t s = ctx->stream;
+    std::unique_ptr<FusedLayout> L(new FusedLayout());
+    renumber_operator(op, dims, L.get());
+    SparseRows &N = L->A;
+    const int64_t n = N.cols, m = N.rows;
     const float t_a = tune_shape(ctx, N, n);
     const float t_at = tune_shape(ctx, L->AT, m);
     tune_panels(ctx, *L, t_a, t_at);

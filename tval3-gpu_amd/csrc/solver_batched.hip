@@ -7,10 +7,11 @@
 // launches per iteration does not grow with the batch, and K iterations are
 // captured in one graph.
 //
-//  * Problems are concatenated: voxels in natural order (y fastest), rays in
-//    the operator's order; each problem's voxel and ray ranges start at a
-//    multiple of 256.  The system matrix is the block-diagonal concatenation
-//    of the problems' CSR matrices (global voxel indices), its CSC likewise.
+//  * Problems are concatenated, each in the fused solver's numbering (voxels
+//    in 8^3 tiles of 4^3 bricks where its dims allow, rays in a locality
+//    order); each problem's voxel and ray ranges start at a multiple of 256.
+//    The system matrix is the block-diagonal concatenation of the problems'
+//    renumbered CSR matrices (global voxel indices), its CSC likewise.
 //  * A workgroup works inside one problem (256-element groups never cross a
 //    range start); it finds its problem by binary search over the range
 //    starts and returns at once when that problem is not active.
@@ -44,7 +45,8 @@ namespace {
 constexpr int GB = 256;   // elements per group = threads per workgroup
 
 struct BDims {
-    int32_t p, q, r, pad;
+    int32_t p, q, r;
+    int32_t B, nby, nbx;  // voxel numbering of the problem (brick_index)
     int64_t v0, n;        // voxel range
     int64_t r0, m;        // ray range
 };
@@ -61,22 +63,40 @@ __device__ __forceinline__ int find_prob(const int32_t *__restrict__ starts, int
 
 struct Vox {
     int y, x, z;
-    int p, q, r;
+    int p, q, r, B, nby, nbx;
     int64_t v0, pq;
     __device__ __forceinline__ int64_t at(int yy, int xx, int zz) const {
-        return v0 + ((int64_t)zz * q + xx) * p + yy;
+        return v0 + brick_index(B, pq, p, nby, nbx, yy, xx, zz);
     }
 };
 
-// voxel of thread t of group w (false: padding)
+// voxel (y, x, z) of internal index i of the batch (false: padding); the
+// inverse of brick_index for the problem's numbering
 __device__ __forceinline__ bool voxel_at(const BDims &dm, int64_t i, Vox &v) {
     const int64_t li = i - dm.v0;
     if (li >= dm.n) return false;
-    const uint32_t l = (uint32_t)li, p = (uint32_t)dm.p, q = (uint32_t)dm.q;
-    const uint32_t yx = l % (p * q);
-    v.y = (int)(yx % p); v.x = (int)(yx / p); v.z = (int)(l / (p * q));
-    v.p = dm.p; v.q = dm.q; v.r = dm.r;
+    const uint32_t l = (uint32_t)li;
+    v.p = dm.p; v.q = dm.q; v.r = dm.r; v.B = dm.B; v.nby = dm.nby; v.nbx = dm.nbx;
     v.v0 = dm.v0; v.pq = (int64_t)dm.p * dm.q;
+    if (dm.B == 8) {
+        const uint32_t tile = l >> 9, sub = (l >> 6) & 7, ib = l & 63;
+        const uint32_t ty = tile % (uint32_t)dm.nby, rest = tile / (uint32_t)dm.nby;
+        const uint32_t tx = rest % (uint32_t)dm.nbx, tz = rest / (uint32_t)dm.nbx;
+        v.y = (int)(ty * 8 + ((sub & 1) << 2) + (ib & 3));
+        v.x = (int)(tx * 8 + (((sub >> 1) & 1) << 2) + ((ib >> 2) & 3));
+        v.z = (int)(tz * 8 + ((sub >> 2) << 2) + (ib >> 4));
+    } else if (dm.B == 4) {
+        const uint32_t brick = l >> 6, ib = l & 63;
+        const uint32_t by = brick % (uint32_t)dm.nby, rest = brick / (uint32_t)dm.nby;
+        const uint32_t bx = rest % (uint32_t)dm.nbx, bz = rest / (uint32_t)dm.nbx;
+        v.y = (int)(by * 4 + (ib & 3));
+        v.x = (int)(bx * 4 + ((ib >> 2) & 3));
+        v.z = (int)(bz * 4 + (ib >> 4));
+    } else {
+        const uint32_t p = (uint32_t)dm.p, q = (uint32_t)dm.q;
+        const uint32_t yx = l % (p * q);
+        v.y = (int)(yx % p); v.x = (int)(yx / p); v.z = (int)(l / (p * q));
+    }
     return true;
 }
 
@@ -597,15 +617,18 @@ __global__ void k_concat_ind(const int32_t *__restrict__ ind, int64_t nnz, int64
         out[j] = (int32_t)(ind[j] + off);
 }
 
-__global__ void k_scale_into(const float *__restrict__ x, int64_t n, float s,
-                             float *__restrict__ y) {
+// out[i] = s in[map[i]] (natural -> internal order, scaled)
+__global__ void k_gather_scale(const float *__restrict__ in, const int32_t *__restrict__ map,
+                               int64_t n, float s, float *__restrict__ out) {
     for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
-        y[j] = s * x[j];
+        out[j] = s * in[map[j]];
 }
 
-__global__ void k_unscale(const float *__restrict__ x, int64_t n, float s, float *__restrict__ y) {
+// out[map[i]] = in[i] / s (internal -> natural order, unscaled, :1275-1277)
+__global__ void k_scatter_unscale(const float *__restrict__ in, const int32_t *__restrict__ map,
+                                  int64_t n, float s, float *__restrict__ out) {
     for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
-        y[j] = x[j] / s;
+        out[map[j]] = in[j] / s;
 }
 
 inline int64_t pad256(int64_t v) { return (v + GB - 1) / GB * GB; }
@@ -816,7 +839,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
         TV3_REQUIRE(n < ((int64_t)1 << 31) && op->m < ((int64_t)1 << 31), TV3_E_SHAPE,
                     "problem too large for the batched solver");
         BDims &bd = B.dims[k];
-        bd.p = dk[0]; bd.q = dk[1]; bd.r = dk[2]; bd.pad = 0;
+        bd.p = dk[0]; bd.q = dk[1]; bd.r = dk[2];
         bd.v0 = N; bd.n = n; bd.r0 = M; bd.m = op->m;
         v0[k] = N; r0[k] = M;
         vst[k] = (int32_t)(N / GB); rst[k] = (int32_t)(M / GB);
@@ -830,15 +853,28 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
     B.ddims.alloc(P);
     B.vst.alloc(P + 1);
     B.rst.alloc(P + 1);
-    TV3_HIP(hipMemcpyAsync(B.ddims.p, B.dims.data(), sizeof(BDims) * P, hipMemcpyHostToDevice, s));
     TV3_HIP(hipMemcpyAsync(B.vst.p, vst.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice, s));
     TV3_HIP(hipMemcpyAsync(B.rst.p, rst.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice, s));
-    // block-diagonal matrices
+    // each problem in the fused solver's numbering (8^3 tiles of 4^3 bricks
+    // where the dims allow, rays in a locality order), then the block-diagonal
+    // matrices; the renumbered copies are released once concatenated
+    std::vector<std::unique_ptr<FusedLayout>> lay(P);
     {
         std::vector<const SparseRows *> pa(P), pt(P);
-        for (int k = 0; k < P; ++k) { pa[k] = &ops[k]->A; pt[k] = &ops[k]->AT; }
+        for (int k = 0; k < P; ++k) {
+            lay[k].reset(new FusedLayout());
+            renumber_operator(ops[k], dims3 + 3 * k, lay[k].get());
+            BDims &bd = B.dims[k];
+            bd.B = lay[k]->B; bd.nby = lay[k]->nby; bd.nbx = lay[k]->nbx;
+            pa[k] = &lay[k]->A;
+            pt[k] = &lay[k]->AT;
+        }
+        TV3_HIP(hipMemcpyAsync(B.ddims.p, B.dims.data(), sizeof(BDims) * P, hipMemcpyHostToDevice,
+                               s));
         concat_matrix(ctx, s, pa, r0, v0, M, N, B.A);
         concat_matrix(ctx, s, pt, v0, r0, N, M, B.AT);
+        TV3_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < P; ++k) { lay[k]->A = SparseRows(); lay[k]->AT = SparseRows(); }
     }
     // vectors
     const int nv = 2 + 1 + 2 + 1 + 1 + 3 + 3 + 1 + 1 + 1, mv = 2 + 1 + 1 + 1;
@@ -901,8 +937,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
         const float dif = std::fabs(bmax - bmin);
         if (dif < 0.5f) scl[k] = 0.5f / dif;
         else if (dif > 1.5f) scl[k] = 1.5f / dif;
-        hipLaunchKernelGGL(k_scale_into, dim3(grid_for(m)), dim3(256), 0, s, b_dev[k], m, scl[k],
-                           B.b + B.dims[k].r0);
+        hipLaunchKernelGGL(k_gather_scale, dim3(grid_for(m)), dim3(256), 0, s, b_dev[k],
+                           lay[k]->r_int2nat.p, m, scl[k], B.b + B.dims[k].r0);
     }
 
     FCtl *H = B.hctl.p;
@@ -923,8 +959,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
     for (int k = 0; k < P; ++k) {
         const BDims &bd = B.dims[k];
         if (o.warm_start)
-            hipLaunchKernelGGL(k_scale_into, dim3(grid_for(bd.n)), dim3(256), 0, s, U_dev[k], bd.n,
-                               scl[k], B.U[0] + bd.v0);
+            hipLaunchKernelGGL(k_gather_scale, dim3(grid_for(bd.n)), dim3(256), 0, s, U_dev[k],
+                               lay[k]->v_int2nat.p, bd.n, scl[k], B.U[0] + bd.v0);
     }
     if (!o.warm_start)
         TV3_HIP(hipMemcpyAsync(B.U[0], B.Atb, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
@@ -1017,8 +1053,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
     for (int k = 0; k < P; ++k) {
         const BDims &bd = B.dims[k];
         const float *Uk = H[k].cur ? B.U[1] : B.U[0];
-        hipLaunchKernelGGL(k_unscale, dim3(grid_for(bd.n)), dim3(256), 0, s, Uk + bd.v0, bd.n,
-                           scl[k], U_dev[k]);
+        hipLaunchKernelGGL(k_scatter_unscale, dim3(grid_for(bd.n)), dim3(256), 0, s, Uk + bd.v0,
+                           lay[k]->v_int2nat.p, bd.n, scl[k], U_dev[k]);
     }
     TV3_HIP(hipEventRecord(ev1, s));
     TV3_HIP(hipStreamSynchronize(s));

@@ -294,6 +294,9 @@ void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
 
 // internal layout (layout.hip)
 FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]);
+// the layout's numbering only: voxel and ray maps, A and its CSC renumbered
+// (no panel forms, no tuned launch shapes)
+void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L);
 // y = A x or A^T x with the layout's kernels, natural order in and out
 void layout_matvec(tv3_op *op, const int32_t dims[3], bool trans, const float *x, float *y,
                    hipStream_t s);
