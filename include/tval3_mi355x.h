@@ -184,6 +184,15 @@ tv3_status tv3_matvec(tv3_op *op, tv3_trans trans, const float *x_dev, float *y_
                       void *stream);
 tv3_status tv3_matvec_f64(tv3_op *op, tv3_trans trans, const double *x_dev, double *y_dev,
                           void *stream);
+/* Several products at once, Y[:, j] = op X[:, j] (or op^T X[:, j]) for
+ * j < nrhs; X and Y column-major device arrays (column j at X + j ldx).  The
+ * reference multiplies a dense matrix by several vectors as one GEMM for its
+ * multi-layer case (2d/mat_vec_mul.h:91-99).  A dense BF16 operator whose
+ * sizes are multiples of 256 runs them on the matrix cores, 64 columns per
+ * product (X split into two BF16 halves, FP32 accumulation); any other
+ * operator applies tv3_matvec per column.  Asynchronous on stream. */
+tv3_status tv3_matvec_multi(tv3_op *op, tv3_trans trans, int32_t nrhs, const float *X_dev,
+                            int64_t ldx, float *Y_dev, int64_t ldy, void *stream);
 
 /* ---- solver ------------------------------------------------------------- */
 /* tval3_gpu_3d(U, A, b, opts, Ut, pagelocked, mainGPU, numGPUs, gpuArray)
@@ -207,9 +216,12 @@ tv3_status tv3_solve_f64(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const i
 
 /* ---- batched solves ------------------------------------------------------ */
 /* No reference counterpart (the reference solves one problem per call).
- * nprob independent problems, each with its own sparse operator (CSR, one
- * weight type for the whole batch), dims (3 per problem: dim_y, dim_x,
- * dim_z), b and result U (device pointers, float), solved together with one
+ * nprob independent problems, each with its sparse operator (CSR, one
+ * weight type for the sparse ones) or a dense BF16 operator that several
+ * problems may share (their products then run as one matrix-core product
+ * with up to 64 right-hand sides; rows and columns multiples of 256), dims
+ * (3 per problem: dim_y, dim_x, dim_z), b and result U (device pointers,
+ * float), solved together with one
  * set of options: every kernel of an iteration covers all problems, the
  * iterations are graph-captured, each problem keeps its own scalars,
  * decisions and stopping (its iterates equal a separate fused-schedule solve
@@ -217,8 +229,9 @@ tv3_status tv3_solve_f64(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const i
  * entries may be NULL) gives infos[k].rel_error; hists (array of nprob or
  * NULL) receives each problem's history.  opts->solver is ignored; the
  * A^T-skip heuristic is not supported (skipMulRatio must be 0).  binfo may be
- * NULL; when it is not, lanes_N and lanes_T > 0 on entry fix the products'
- * launch shapes instead of timing them. */
+ * NULL; when it is not, lanes_N and lanes_T > 0 on entry fix the sparse
+ * products' launch shapes and splits_N, splits_T > 0 the dense products' K
+ * splits instead of timing them (out: the shapes used, the first group's). */
 typedef struct tv3_batch_info {
     int32_t lanes_N, lanes_T;   /* lanes per row of the block-diagonal products (in/out) */
     int32_t graph_iters;        /* iterations per graph launch                          */
@@ -228,6 +241,10 @@ typedef struct tv3_batch_info {
     double host_secs;           /* host wall time of the call, set-up included          */
     int64_t total_iters;        /* inner iterations summed over the problems            */
     int64_t nnz, voxels, rays;  /* batch totals (voxels, rays padded to 256 per problem) */
+    int32_t dense_groups;       /* shared dense operators (matrix-core products)        */
+    int32_t splits_N, splits_T; /* K splits of the dense products (in/out, like lanes)  */
+    int32_t reserved;
+    int64_t dense_values;       /* entries of the shared dense operators                */
 } tv3_batch_info;
 
 tv3_status tv3_solve_batched(tv3_ctx *ctx, int32_t nprob, tv3_op *const *ops,

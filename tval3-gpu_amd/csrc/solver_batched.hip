@@ -621,14 +621,14 @@ __global__ void k_concat_ind(const int32_t *__restrict__ ind, int64_t nnz, int64
 __global__ void k_gather_scale(const float *__restrict__ in, const int32_t *__restrict__ map,
                                int64_t n, float s, float *__restrict__ out) {
     for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
-        out[j] = s * in[map[j]];
+        out[j] = s * in[map ? map[j] : j];
 }
 
 // out[map[i]] = in[i] / s (internal -> natural order, unscaled, :1275-1277)
 __global__ void k_scatter_unscale(const float *__restrict__ in, const int32_t *__restrict__ map,
                                   int64_t n, float s, float *__restrict__ out) {
     for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
-        out[map[j]] = in[j] / s;
+        out[map ? map[j] : j] = in[j] / s;
 }
 
 inline int64_t pad256(int64_t v) { return (v + GB - 1) / GB * GB; }
@@ -662,6 +662,17 @@ struct Batch {
     PinnedBuf<FCtl> hctl;
     hipGraphExec_t graph = nullptr;
     int graph_iters = 8;
+    // problems sharing one dense BF16 operator: their vectors are consecutive,
+    // so G of them form an n x G (m x G) column-major block and a product is
+    // one matrix-core multi-RHS product
+    struct DGroup {
+        tv3_op *op;
+        int k0, G;                 // internal problems k0 .. k0 + G - 1
+        int64_t v0, r0, ldn, ldm;  // first ranges and per-problem strides
+        int sN = 1, sT = 1;        // K splits of the two products
+    };
+    std::vector<DGroup> groups;
+    DevBuf<float> gpart;           // split-K partials of the group products
 
     ~Batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -702,12 +713,30 @@ struct Batch {
 #undef TV3_BCSR_V
 #undef TV3_BCSR
     }
+    // dense groups: every column is computed (the outputs are scratch, read
+    // only for the problems whose predicate holds)
+    void group_N(const DGroup &g, const float *x, float *y, int splits) {
+        for (int c = 0; c < g.G; c += 64)
+            gemm_rhs((const uint16_t *)g.op->dense_rm.p, g.op->n, g.op->m, g.op->n,
+                     x + g.v0 + c * g.ldn, g.ldn, std::min(64, g.G - c), y + g.r0 + c * g.ldm,
+                     g.ldm, nullptr, splits, gpart.p, s);
+    }
+    void group_T(const DGroup &g, const float *x, float *y, const float *sub, int splits) {
+        for (int c = 0; c < g.G; c += 64)
+            gemm_rhs((const uint16_t *)g.op->dense.p, g.op->lda, g.op->n, g.op->m,
+                     x + g.r0 + c * g.ldm, g.ldm, std::min(64, g.G - c), y + g.v0 + c * g.ldn,
+                     g.ldn, sub ? sub + g.v0 + c * g.ldn : nullptr, splits, gpart.p, s);
+    }
     void prod_N(const float *x, float *y, int pred) {
-        product<false>(A, lanes_N, x, y, nullptr, rst.p, pred);
+        if (A.rows) product<false>(A, lanes_N, x, y, nullptr, rst.p, pred);
+        for (const DGroup &g : groups) group_N(g, x, y, g.sN);
     }
     void prod_T(const float *x, float *y, const float *sub, int pred) {
-        if (sub) product<true>(AT, lanes_T, x, y, sub, vst.p, pred);
-        else product<false>(AT, lanes_T, x, y, nullptr, vst.p, pred);
+        if (AT.rows) {
+            if (sub) product<true>(AT, lanes_T, x, y, sub, vst.p, pred);
+            else product<false>(AT, lanes_T, x, y, nullptr, vst.p, pred);
+        }
+        for (const DGroup &g : groups) group_T(g, x, y, sub, g.sT);
     }
 
     // one inner iteration of every active problem
@@ -766,6 +795,7 @@ struct Batch {
     }
 
     void tune() {
+        if (!A.rows) { lanes_N = lanes_T = 8; return; }
         const int cand[] = {2, 4, 8, 16, 32, 64};
         float bestN = 1e30f, bestT = 1e30f;
         for (int L : cand) {
@@ -775,6 +805,40 @@ struct Batch {
             if (tt < bestT) { bestT = tt; lanes_T = L; }
         }
     }
+
+    template <typename F>
+    float time_ms(F &&f) {
+        hipEvent_t a, e;
+        TV3_HIP(hipEventCreate(&a)); TV3_HIP(hipEventCreate(&e));
+        f();
+        TV3_HIP(hipEventRecord(a, s));
+        for (int r = 0; r < 3; ++r) f();
+        TV3_HIP(hipEventRecord(e, s));
+        TV3_HIP(hipEventSynchronize(e));
+        float ms = 0;
+        TV3_HIP(hipEventElapsedTime(&ms, a, e));
+        (void)hipEventDestroy(a); (void)hipEventDestroy(e);
+        return ms / 3;
+    }
+
+    // K splits of each group's two products, timed (the scratch holds the
+    // largest candidate)
+    void tune_groups() {
+        for (DGroup &g : groups) {
+            float bN = 1e30f, bT = 1e30f;
+            for (int S = 1; S <= kMaxSplits; S *= 2) {
+                if (gemm_rhs_shape_ok(g.op->m, g.op->n, S)) {
+                    const float t = time_ms([&] { group_N(g, U[0], AuT, S); });
+                    if (t < bN) { bN = t; g.sN = S; }
+                }
+                if (gemm_rhs_shape_ok(g.op->n, g.op->m, S)) {
+                    const float t = time_ms([&] { group_T(g, AuT, gT, nullptr, S); });
+                    if (t < bT) { bT = t; g.sT = S; }
+                }
+            }
+        }
+    }
+    static constexpr int kMaxSplits = 32;
 };
 
 void concat_matrix(tv3_ctx *ctx, hipStream_t s, const std::vector<const SparseRows *> &parts,
@@ -810,27 +874,53 @@ void concat_matrix(tv3_ctx *ctx, hipStream_t s, const std::vector<const SparseRo
 
 }  // namespace
 
-void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3,
-                   const tv3_opts &o, const float *const *b_dev, float *const *U_dev,
-                   const float *const *Ut_dev, tv3_info *infos, tv3_history *hists,
+void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *dims_in,
+                   const tv3_opts &o, const float *const *b_in, float *const *U_in,
+                   const float *const *Ut_in, tv3_info *infos, tv3_history *hists,
                    tv3_batch_info *binfo) {
     auto t_host0 = std::chrono::steady_clock::now();
     Batch B;
     B.ctx = ctx; B.s = ctx->stream; B.o = &o; B.P = P;
     hipStream_t s = B.s;
     B.graph_iters = o.graph_iters > 0 ? o.graph_iters : 8;
+    // internal order: sparse problems first, then the problems of each shared
+    // dense operator consecutively; orig[k] = caller's index of problem k
+    std::vector<int> orig;
+    {
+        std::vector<int> dense;
+        for (int k = 0; k < P; ++k) {
+            tv3_op *op = ops_in[k];
+            TV3_REQUIRE(op, TV3_E_ARG, "NULL operator in the batch");
+            if (op->kind == OpKind::CSR) orig.push_back(k);
+            else if (op->kind == OpKind::DENSE) dense.push_back(k);
+            else
+                throw tv3_error(TV3_E_FORMAT,
+                                "batched solves take sparse (CSR) or dense operators");
+        }
+        std::stable_sort(dense.begin(), dense.end(), [&](int a, int b) {
+            return ops_in[a] < ops_in[b];
+        });
+        orig.insert(orig.end(), dense.begin(), dense.end());
+    }
+    std::vector<tv3_op *> ops(P);
+    std::vector<int32_t> dims3v(3 * P);
+    for (int k = 0; k < P; ++k) {
+        ops[k] = ops_in[orig[k]];
+        for (int a = 0; a < 3; ++a) dims3v[3 * k + a] = dims_in[3 * orig[k] + a];
+    }
+    const int32_t *dims3 = dims3v.data();
+    int ncsr = 0;
+    while (ncsr < P && ops[ncsr]->kind == OpKind::CSR) ++ncsr;
     // ranges
     B.dims.resize(P);
     std::vector<int32_t> vst(P + 1), rst(P + 1);
     std::vector<int64_t> v0(P), r0(P);
-    int64_t N = 0, M = 0;
+    int64_t N = 0, M = 0, Ncsr = 0, Mcsr = 0;
     for (int k = 0; k < P; ++k) {
         tv3_op *op = ops[k];
-        TV3_REQUIRE(op, TV3_E_ARG, "NULL operator in the batch");
-        TV3_REQUIRE(op->kind == OpKind::CSR, TV3_E_FORMAT,
-                    "batched solves take sparse (CSR) operators");
-        TV3_REQUIRE(op->A.vtype == ops[0]->A.vtype, TV3_E_FORMAT,
-                    "every operator of a batch must have the same weight type");
+        if (k < ncsr)
+            TV3_REQUIRE(op->A.vtype == ops[0]->A.vtype, TV3_E_FORMAT,
+                        "every sparse operator of a batch must have the same weight type");
         const int32_t *dk = dims3 + 3 * k;
         TV3_REQUIRE(dk[0] > 0 && dk[1] > 0 && dk[2] > 0, TV3_E_ARG,
                     "dims must be three positive sizes");
@@ -840,11 +930,13 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
                     "problem too large for the batched solver");
         BDims &bd = B.dims[k];
         bd.p = dk[0]; bd.q = dk[1]; bd.r = dk[2];
+        bd.B = 1; bd.nby = 0; bd.nbx = 0;
         bd.v0 = N; bd.n = n; bd.r0 = M; bd.m = op->m;
         v0[k] = N; r0[k] = M;
         vst[k] = (int32_t)(N / GB); rst[k] = (int32_t)(M / GB);
         N += pad256(n);
         M += pad256(std::max<int64_t>(op->m, 1));
+        if (k + 1 == ncsr) { Ncsr = N; Mcsr = M; }
     }
     TV3_REQUIRE(N / GB < ((int64_t)1 << 31) && M / GB < ((int64_t)1 << 31), TV3_E_SHAPE,
                 "batch too large");
@@ -855,13 +947,15 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
     B.rst.alloc(P + 1);
     TV3_HIP(hipMemcpyAsync(B.vst.p, vst.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice, s));
     TV3_HIP(hipMemcpyAsync(B.rst.p, rst.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice, s));
-    // each problem in the fused solver's numbering (8^3 tiles of 4^3 bricks
+    // sparse problems in the fused solver's numbering (8^3 tiles of 4^3 bricks
     // where the dims allow, rays in a locality order), then the block-diagonal
-    // matrices; the renumbered copies are released once concatenated
+    // matrices over the sparse prefix; the renumbered copies are released
+    // once concatenated.  Dense problems keep the natural order.
     std::vector<std::unique_ptr<FusedLayout>> lay(P);
-    {
-        std::vector<const SparseRows *> pa(P), pt(P);
-        for (int k = 0; k < P; ++k) {
+    if (ncsr > 0) {
+        std::vector<const SparseRows *> pa(ncsr), pt(ncsr);
+        std::vector<int64_t> v0c(v0.begin(), v0.begin() + ncsr), r0c(r0.begin(), r0.begin() + ncsr);
+        for (int k = 0; k < ncsr; ++k) {
             lay[k].reset(new FusedLayout());
             renumber_operator(ops[k], dims3 + 3 * k, lay[k].get());
             BDims &bd = B.dims[k];
@@ -869,12 +963,38 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
             pa[k] = &lay[k]->A;
             pt[k] = &lay[k]->AT;
         }
-        TV3_HIP(hipMemcpyAsync(B.ddims.p, B.dims.data(), sizeof(BDims) * P, hipMemcpyHostToDevice,
-                               s));
-        concat_matrix(ctx, s, pa, r0, v0, M, N, B.A);
-        concat_matrix(ctx, s, pt, v0, r0, N, M, B.AT);
+        concat_matrix(ctx, s, pa, r0c, v0c, Mcsr, N, B.A);
+        concat_matrix(ctx, s, pt, v0c, r0c, Ncsr, M, B.AT);
         TV3_HIP(hipStreamSynchronize(s));
-        for (int k = 0; k < P; ++k) { lay[k]->A = SparseRows(); lay[k]->AT = SparseRows(); }
+        for (int k = 0; k < ncsr; ++k) { lay[k]->A = SparseRows(); lay[k]->AT = SparseRows(); }
+    }
+    TV3_HIP(hipMemcpyAsync(B.ddims.p, B.dims.data(), sizeof(BDims) * P, hipMemcpyHostToDevice, s));
+    // dense groups
+    size_t gscratch = 0;
+    for (int k = ncsr; k < P;) {
+        int e = k;
+        while (e < P && ops[e] == ops[k]) ++e;
+        tv3_op *op = ops[k];
+        TV3_REQUIRE(op->dtype == TV3_BF16, TV3_E_FORMAT,
+                    "dense operators of a batch must hold BF16 values (matrix-core products)");
+        TV3_REQUIRE(op->m % 256 == 0 && op->n % 256 == 0 && op->lda % 8 == 0, TV3_E_SHAPE,
+                    "dense operators of a batch need rows and columns multiples of 256");
+        dense_rowmajor(op, s);
+        Batch::DGroup g;
+        g.op = op; g.k0 = k; g.G = e - k;
+        g.v0 = v0[k]; g.r0 = r0[k]; g.ldn = pad256(op->n); g.ldm = pad256(op->m);
+        B.groups.push_back(g);
+        gscratch = std::max({gscratch, gemm_rhs_scratch(op->m, Batch::kMaxSplits),
+                             gemm_rhs_scratch(op->n, Batch::kMaxSplits)});
+        k = e;
+    }
+    if (gscratch) B.gpart.alloc(gscratch);
+    std::vector<const float *> b_dev(P), Ut_dev(P);
+    std::vector<float *> U_dev(P);
+    for (int k = 0; k < P; ++k) {
+        b_dev[k] = b_in[orig[k]];
+        U_dev[k] = U_in[orig[k]];
+        Ut_dev[k] = Ut_in ? Ut_in[orig[k]] : nullptr;
     }
     // vectors
     const int nv = 2 + 1 + 2 + 1 + 1 + 3 + 3 + 1 + 1 + 1, mv = 2 + 1 + 1 + 1;
@@ -918,6 +1038,19 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
     } else {
         B.tune();
     }
+    if (binfo && binfo->splits_N > 0 && binfo->splits_T > 0) {
+        for (auto &g : B.groups) {
+            TV3_REQUIRE(gemm_rhs_shape_ok(g.op->m, g.op->n, binfo->splits_N) &&
+                            gemm_rhs_shape_ok(g.op->n, g.op->m, binfo->splits_T) &&
+                            binfo->splits_N <= Batch::kMaxSplits &&
+                            binfo->splits_T <= Batch::kMaxSplits,
+                        TV3_E_ARG, "K splits must divide the dense products' K into 64-multiples");
+            g.sN = binfo->splits_N;
+            g.sT = binfo->splits_T;
+        }
+    } else {
+        B.tune_groups();
+    }
     TV3_HIP(hipEventRecord(ev0, s));
 
     // scaleb (:225-266) per problem on a host copy of b
@@ -938,7 +1071,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
         if (dif < 0.5f) scl[k] = 0.5f / dif;
         else if (dif > 1.5f) scl[k] = 1.5f / dif;
         hipLaunchKernelGGL(k_gather_scale, dim3(grid_for(m)), dim3(256), 0, s, b_dev[k],
-                           lay[k]->r_int2nat.p, m, scl[k], B.b + B.dims[k].r0);
+                           lay[k] ? lay[k]->r_int2nat.p : nullptr, m, scl[k],
+                           B.b + B.dims[k].r0);
     }
 
     FCtl *H = B.hctl.p;
@@ -960,7 +1094,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
         const BDims &bd = B.dims[k];
         if (o.warm_start)
             hipLaunchKernelGGL(k_gather_scale, dim3(grid_for(bd.n)), dim3(256), 0, s, U_dev[k],
-                               lay[k]->v_int2nat.p, bd.n, scl[k], B.U[0] + bd.v0);
+                               lay[k] ? lay[k]->v_int2nat.p : nullptr, bd.n, scl[k],
+                               B.U[0] + bd.v0);
     }
     if (!o.warm_start)
         TV3_HIP(hipMemcpyAsync(B.U[0], B.Atb, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
@@ -1054,7 +1189,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
         const BDims &bd = B.dims[k];
         const float *Uk = H[k].cur ? B.U[1] : B.U[0];
         hipLaunchKernelGGL(k_scatter_unscale, dim3(grid_for(bd.n)), dim3(256), 0, s, Uk + bd.v0,
-                           lay[k]->v_int2nat.p, bd.n, scl[k], U_dev[k]);
+                           lay[k] ? lay[k]->v_int2nat.p : nullptr, bd.n, scl[k], U_dev[k]);
     }
     TV3_HIP(hipEventRecord(ev1, s));
     TV3_HIP(hipStreamSynchronize(s));
@@ -1065,7 +1200,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
     std::vector<double> hh;
     for (int k = 0; k < P; ++k) {
         const FCtl &c = H[k];
-        tv3_info &inf = infos[k];
+        tv3_info &inf = infos[orig[k]];
         std::memset(&inf, 0, sizeof(inf));
         inf.total_iters = c.total;
         inf.outer_iters = c.outer;
@@ -1078,12 +1213,12 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
         inf.n_matvec_T = 2 + (int64_t)c.total + c.fallbacks;
         inf.status = c.nonfinite ? TV3_E_NONFINITE : TV3_OK;
         tot += c.total;
-        if (Ut_dev && Ut_dev[k]) {
+        if (Ut_dev[k]) {
             auto e = diff_norm<float>(ctx, B.dims[k].n, Ut_dev[k], U_dev[k], s);
             inf.rel_error = std::sqrt(e[0]) / std::sqrt(e[1]);
         }
-        if (hists && hists[k].cap > 0) {
-            tv3_history &h = hists[k];
+        if (hists && hists[orig[k]].cap > 0) {
+            tv3_history &h = hists[orig[k]];
             const int cnt = std::min(c.total, h.cap);
             hh.assign((size_t)std::max(cnt, 1) * HF, 0.0);
             if (cnt > 0)
@@ -1103,6 +1238,11 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
         std::memset(binfo, 0, sizeof(*binfo));
         binfo->lanes_N = B.lanes_N;
         binfo->lanes_T = B.lanes_T;
+        binfo->dense_groups = (int32_t)B.groups.size();
+        if (!B.groups.empty()) {
+            binfo->splits_N = B.groups[0].sN;
+            binfo->splits_T = B.groups[0].sT;
+        }
         binfo->graph_iters = B.graph_iters;
         binfo->graph_launches = graph_launches;
         binfo->secs = ms / 1000.0;
@@ -1111,6 +1251,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops, const int32_t *dims3
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t_host0).count();
         binfo->total_iters = tot;
         binfo->nnz = B.A.nnz;
+        for (const auto &g : B.groups) binfo->dense_values += g.op->m * g.op->n;
         binfo->voxels = N;
         binfo->rays = M;
     }

@@ -517,6 +517,49 @@ tv3_status tv3_matvec_f64(tv3_op *op, tv3_trans trans, const double *x, double *
     });
 }
 
+tv3_status tv3_matvec_multi(tv3_op *op, tv3_trans trans, int32_t nrhs, const float *X,
+                            int64_t ldx, float *Y, int64_t ldy, void *stream) {
+    if (!op) return TV3_E_ARG;
+    return guard(op->ctx, [&] {
+        TV3_REQUIRE(trans == TV3_N || trans == TV3_T, TV3_E_ARG, "bad trans");
+        TV3_REQUIRE(nrhs >= 0 && (nrhs == 0 || (X && Y)), TV3_E_ARG, "NULL argument");
+        const bool T = trans == TV3_T;
+        const int64_t xl = T ? op->m : op->n, yl = T ? op->n : op->m;
+        TV3_REQUIRE(ldx >= xl && ldy >= yl, TV3_E_ARG, "leading dimension below the vector length");
+        set_device(op->ctx);
+        hipStream_t s = (hipStream_t)stream;
+        const int64_t R = yl, K = xl;
+        const bool mfma = op->kind == OpKind::DENSE && op->dtype == TV3_BF16 && R % 256 == 0 &&
+                          K % 64 == 0 && op->lda % 8 == 0 && ldx % 4 == 0;
+        if (!mfma) {
+            for (int32_t j = 0; j < nrhs; ++j) op_apply(op, T, X + j * ldx, Y + j * ldy, s);
+            TV3_HIP(hipGetLastError());
+            return;
+        }
+        // K splits: enough workgroups to fill the device, K a multiple of 64 splits
+        int splits = 1;
+        while ((R / gemm_rhs_rows()) * splits < 1024 && gemm_rhs_shape_ok(R, K, splits * 2))
+            splits *= 2;
+        const size_t need = gemm_rhs_scratch(R, splits) * sizeof(float);
+        if (op->ctx->scratch.bytes() < need) op->ctx->scratch.alloc(need);
+        float *part = reinterpret_cast<float *>(op->ctx->scratch.p);
+        const uint16_t *M;
+        int64_t ldm;
+        if (!T) {
+            dense_rowmajor(op, s);
+            M = (const uint16_t *)op->dense_rm.p;
+            ldm = op->n;
+        } else {
+            M = (const uint16_t *)op->dense.p;
+            ldm = op->lda;
+        }
+        for (int32_t c = 0; c < nrhs; c += 64)
+            gemm_rhs(M, ldm, R, K, X + c * ldx, ldx, std::min(64, nrhs - c), Y + c * ldy, ldy,
+                     nullptr, splits, part, s);
+        TV3_HIP(hipGetLastError());
+    });
+}
+
 static void solve_device_impl(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts,
                               const int32_t dims[3], const float *b, float *U, const float *Ut,
                               tv3_info *info, tv3_history *hist) {

@@ -250,6 +250,7 @@ struct tv3_op {
     tv3_dtype dtype = TV3_F32;
     tv3::DevBuf<uint8_t> dense;
     int64_t lda = 0;
+    tv3::DevBuf<uint8_t> dense_rm;   // BF16 row-major copy (multi-RHS products), on demand
     // DYNAMIC / HYBRID
     tv3::DynGeom geo;
     // CSR built from a geometry: the locality order of its rays (may be empty)
@@ -278,6 +279,16 @@ void dense_gemv(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,
                 bool acc64 = false);
 void dense_gemv(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
                 bool acc64 = true);
+
+// dense multi-right-hand-side products on the matrix cores (dense_mfma.hip):
+// Y = M X (- sub), M row-major BF16 R x K, X/Y column-major, G <= 64
+int gemm_rhs_rows();   // rows per workgroup (R must be a multiple)
+bool gemm_rhs_shape_ok(int64_t R, int64_t K, int splits);
+void gemm_rhs(const uint16_t *M, int64_t ldm, int64_t R, int64_t K, const float *X, int64_t ldx,
+              int G, float *Y, int64_t ldy, const float *sub, int splits, float *part,
+              hipStream_t s);
+size_t gemm_rhs_scratch(int64_t R, int splits);   // floats
+void dense_rowmajor(tv3_op *op, hipStream_t s);     // builds op->dense_rm once
 
 // dynamic (dynamic.hip)
 void dynamic_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,

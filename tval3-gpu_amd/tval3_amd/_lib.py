@@ -82,7 +82,9 @@ class BatchInfo(C.Structure):
     _fields_ = [("lanes_N", C.c_int32), ("lanes_T", C.c_int32), ("graph_iters", C.c_int32),
                 ("graph_launches", C.c_int32), ("secs", C.c_double), ("loop_secs", C.c_double),
                 ("host_secs", C.c_double), ("total_iters", C.c_int64), ("nnz", C.c_int64),
-                ("voxels", C.c_int64), ("rays", C.c_int64)]
+                ("voxels", C.c_int64), ("rays", C.c_int64), ("dense_groups", C.c_int32),
+                ("splits_N", C.c_int32), ("splits_T", C.c_int32), ("reserved", C.c_int32),
+                ("dense_values", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -110,6 +112,7 @@ SIGNATURES = [
     ("tv3_op_csr_download", C.c_int, [_P, _P, _P, _P]),
     ("tv3_matvec", C.c_int, [_P, C.c_int, _P, _P, _P]),
     ("tv3_matvec_f64", C.c_int, [_P, C.c_int, _P, _P, _P]),
+    ("tv3_matvec_multi", C.c_int, [_P, C.c_int, C.c_int32, _P, C.c_int64, _P, C.c_int64, _P]),
     ("tv3_solve", C.c_int, [_P, _P, C.POINTER(Opts), _P, _P, _P, _P, C.POINTER(Info),
                             C.POINTER(History)]),
     ("tv3_solve_device", C.c_int, [_P, _P, C.POINTER(Opts), _P, _P, _P, _P, C.POINTER(Info),

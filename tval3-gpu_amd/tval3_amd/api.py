@@ -134,6 +134,24 @@ class Operator:
         self.ctx.check(f(self.h, 1 if trans else 0, _vp(x), _vp(y), _torch_stream()))
         return y
 
+    def mat_mat_mul(self, X, Y=None, trans=False):
+        """Y[:, j] = A X[:, j] (or A^T X[:, j]) for the columns of X, a float32
+        (length, nrhs) tensor with columns contiguous (X.t() contiguous, e.g.
+        torch.empty(nrhs, length).t()); on torch's current stream."""
+        import torch
+        xl, yl = (self.m, self.n) if trans else (self.n, self.m)
+        assert X.dtype == torch.float32 and X.dim() == 2 and X.shape[0] == xl
+        assert X.stride(0) == 1, "columns of X must be contiguous"
+        nrhs = X.shape[1]
+        if Y is None:
+            Y = torch.empty(nrhs, yl, dtype=torch.float32, device=X.device).t()
+        assert Y.shape == (yl, nrhs) and Y.stride(0) == 1
+        self.ctx.check(L.load().tv3_matvec_multi(self.h, 1 if trans else 0, nrhs, _vp(X),
+                                                 X.stride(1) if nrhs > 1 else xl, _vp(Y),
+                                                 Y.stride(1) if nrhs > 1 else yl,
+                                                 _torch_stream()))
+        return Y
+
     def materialize(self, dtype="f32"):
         h = C.c_void_p()
         self.ctx.check(L.load().tv3_op_materialize(self.h, L.DTYPES[dtype], C.byref(h)))
@@ -365,13 +383,15 @@ def tval3_gpu_3d(ctx, A, b, dims, opts=None, Ut=None, U0=None, history=True, his
 
 
 def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
-                         hist_cap=4096, lanes=None):
+                         hist_cap=4096, lanes=None, splits=None):
     """Solve many problems together (tv3_solve_batched): ops a list of sparse
     operators (one weight type), bs their b (float32 torch tensors on the
     device), dims a list of (dim_y, dim_x, dim_z).  Every kernel of an
     iteration covers all problems; each problem keeps its own scalars and
-    stopping.  lanes = (lanes_N, lanes_T) fixes the products' launch shapes
-    (default: timed).  Returns (list of U tensors, list of info dicts, batch
+    stopping.  Problems sharing one dense BF16 operator run its products as
+    one matrix-core product.  lanes = (lanes_N, lanes_T) fixes the sparse
+    products' launch shapes, splits = (splits_N, splits_T) the dense
+    products' K splits (default: timed).  Returns (list of U tensors, list of info dicts, batch
     info dict, list of history dicts or None)."""
     import torch
     o = opts if opts is not None else default_opts()
@@ -406,6 +426,8 @@ def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
     bi = L.BatchInfo()
     if lanes is not None:
         bi.lanes_N, bi.lanes_T = lanes
+    if splits is not None:
+        bi.splits_N, bi.splits_T = splits
     torch.cuda.synchronize()
     ctx.check(L.load().tv3_solve_batched(ctx.h, P, opsa, d, C.byref(o), ba, ua, uta, infos,
                                          hp, C.byref(bi)))
