@@ -233,7 +233,8 @@ tv3_status tv3_solve_f64(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const i
  * products' launch shapes and splits_N, splits_T > 0 the dense products' K
  * splits instead of timing them (out: the shapes used, the first group's). */
 typedef struct tv3_batch_info {
-    int32_t lanes_N, lanes_T;   /* lanes per row of the block-diagonal products (in/out) */
+    int32_t lanes_N, lanes_T;   /* shapes of the block-diagonal products (in/out): lanes
+                                   per row, or -NB for row blocks of NB entries */
     int32_t graph_iters;        /* iterations per graph launch                          */
     int32_t graph_launches;
     double secs;                /* GPU time from scaling b to the unscaled results      */

@@ -7,8 +7,8 @@ of launches per iteration.
 - Each problem against the FP32 oracle with the envelope criterion of the
   fused solver (the batched path runs the same algorithm with FP32 products
   in its own summation order): rounding-level agreement at the start, then
-  within a small multiple of the oracle's own sensitivity to a 1e-7
-  perturbation of b, and the final error.
+  within a small multiple of the oracle's own sensitivity to 1e-7
+  perturbations of b (three seeds), and the final error.
 - Backtracking, fallbacks and the outer loop inside a batch; error paths.
 """
 import numpy as np
@@ -67,12 +67,15 @@ def solve(ctx, probs, o, lanes=(8, 8), history=True):
                                     history=history, lanes=lanes)
 
 
-def test_batch_equals_solo_bitwise(ctx, batch):
+@pytest.mark.parametrize("lanes", [(8, 8), (-2048, -1024)])
+def test_batch_equals_solo_bitwise(ctx, batch, lanes):
+    """row-group products (8 lanes per row) and row-block products (blocks of
+    2048 / 1024 entries, which must start at every problem's first row)"""
     o = opts()
-    Ub, ib, bi, hb = solve(ctx, batch, o)
+    Ub, ib, bi, hb = solve(ctx, batch, o, lanes=lanes)
     print("batch info", bi)
     for k, p in enumerate(batch):
-        Us, is_, _, hs = solve(ctx, [p], o)
+        Us, is_, _, hs = solve(ctx, [p], o, lanes=lanes)
         assert ib[k]["total_iters"] == is_[0]["total_iters"] > 0
         assert ib[k]["outer_iters"] == is_[0]["outer_iters"]
         assert (hb[k]["cnt"] == hs[0]["cnt"]).all()
@@ -96,7 +99,8 @@ def test_batch_graph_size_and_lanes(ctx, batch):
     _, i8h, _, h8 = solve(ctx, batch, o8)
     Ut, it, bt, ht = solve(ctx, batch, o8, lanes=None)
     print("tuned lanes", bt["lanes_N"], bt["lanes_T"])
-    assert bt["lanes_N"] in (2, 4, 8, 16, 32, 64) and bt["lanes_T"] in (2, 4, 8, 16, 32, 64)
+    shapes = (2, 4, 8, 16, 32, 64, -1024, -2048, -4096)
+    assert bt["lanes_N"] in shapes and bt["lanes_T"] in shapes
     for k in range(len(batch)):
         f8, ft = h8[k]["f"][:4], ht[k]["f"][:4]
         assert (np.abs(ft - f8) <= 1e-5 * np.abs(f8)).all()
@@ -117,9 +121,16 @@ def test_batch_tracks_oracle(ctx, batch):
     oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=60, tv_iso=1)
     for k, p in enumerate(batch):
         Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
-        xi = np.random.default_rng(0).choice([-1.0, 1.0], size=p["b"].shape)
-        bp = (p["b"].astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
-        Up, ip, hpp, _ = orc.solve(p["oop"], p["dims"], bp, oo, Ut=p["u"], dtype=np.float32)
+        # the oracle's own sensitivity: three rounding-level perturbations of b
+        # (the first sets the trajectory envelope; the spread of the final
+        # error is the largest of the three)
+        runs = []
+        for seed in (0, 1, 2):
+            xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=p["b"].shape)
+            bp = (p["b"].astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+            runs.append(orc.solve(p["oop"], p["dims"], bp, oo, Ut=p["u"], dtype=np.float32))
+        Up, ip, hpp, _ = runs[0]
+        spread = max(abs(r[1].rel_error - io.rel_error) for r in runs)
         kk = min(first_divergence(hb[k], ho), first_divergence(hpp, ho))
         dev = np.abs(hb[k]["f"][:kk] - ho["f"][:kk]) / np.abs(ho["f"][:kk])
         env = np.maximum.accumulate(np.abs(hpp["f"][:kk] - ho["f"][:kk]) / np.abs(ho["f"][:kk]))
@@ -131,9 +142,12 @@ def test_batch_tracks_oracle(ctx, batch):
         assert kk >= 10
         assert (dev[:4] < 1e-5).all()
         assert (dev <= 30 * env + 1e-6).all()
-        assert abs(ib[k]["rel_error"] - io.rel_error) <= max(3 * abs(ip.rel_error - io.rel_error),
-                                                             1e-3)
-        assert rel(Ug, Uo) <= max(10 * rel(Up, Uo), 1e-4)
+        # U within a small multiple of the oracle's own spread; the final
+        # error then differs by at most |U - U_oracle| / |u_true| (triangle
+        # inequality), which can exceed the error spread of the perturbed runs
+        assert rel(Ug, Uo) <= max(10 * max(rel(r[0], Uo) for r in runs), 1e-4)
+        bound = rel(Ug, Uo) * np.linalg.norm(Uo) / np.linalg.norm(p["u"])
+        assert abs(ib[k]["rel_error"] - io.rel_error) <= max(3 * spread, 1e-3, 1.01 * bound)
 
 
 def test_batch_outer_loop_backtracking_fallback(ctx, batch):

@@ -21,9 +21,11 @@
 //  * Per-problem reductions: every workgroup stores its partial sums; one
 //    workgroup per problem sums its problem's slice in a fixed order and runs
 //    the scalar step of that problem.
-//  * Products: a row-group CSR kernel over the block-diagonal matrix, L lanes
-//    per row (L timed per direction at set-up), whose workgroups skip the rows
-//    of inactive problems.
+//  * Products: the fused solver's CSR kernels over the block-diagonal matrix
+//    -- row groups of L lanes per row, or row blocks of NB entries staged in
+//    LDS (shape timed per direction at set-up) -- whose row groups and blocks
+//    skip the rows of inactive problems; problems sharing a dense BF16
+//    operator run as one matrix-core product with up to 64 right-hand sides.
 //
 // One inner iteration: descend (trial U_t, V statistics), A U_t, residual
 // statistics, A^T A U_t - Atb, line search, apply alpha (accepted copies),
@@ -143,38 +145,28 @@ __device__ __forceinline__ bool pred_on(const FCtl &c, int pred) {
 }
 
 // ------------------------------------------------------------------ products
-// y = M x over the block-diagonal CSR; L lanes per row; rows of a workgroup
-// lie in one problem.  SUB: y = M x - sub.
-template <int L, typename V, bool SUB>
-__global__ __launch_bounds__(256) void k_b_csr(int64_t rows, const int32_t *__restrict__ ptr,
-                                               const int32_t *__restrict__ ind,
-                                               const V *__restrict__ val,
-                                               const float *__restrict__ x, float *__restrict__ y,
-                                               const float *__restrict__ sub,
-                                               const int32_t *__restrict__ starts, int P,
-                                               const FCtl *__restrict__ C, int pred) {
-    const int64_t bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t row0 = bid * (256 / L);
-    const int k = find_prob(starts, P, (int)(row0 / GB));
-    if (!pred_on(C[k], pred)) return;
-    const int64_t row = row0 + threadIdx.x / L;
-    const int lane = threadIdx.x & (L - 1);
-    if (row >= rows) return;          // whole groups leave together (L divides 256)
-    const int beg = ptr[row], end = ptr[row + 1];
-    float s = 0.f, s2 = 0.f;
-    int e = beg + lane;
-    for (; e + L < end; e += 2 * L) {
-        const int c0 = ind[e], c1 = ind[e + L];
-        const float w0 = ld_weight<V>(val, e), w1 = ld_weight<V>(val, e + L);
-        s = fmaf(w0, x[c0], s);
-        s2 = fmaf(w1, x[c1], s2);
+// Epilogue of the block-diagonal products (the fused solver's product
+// kernels, spmv_stream.hpp): y = M x (- sub); a row block or group of a
+// problem whose predicate is off is skipped (blocks never cross a problem's
+// first row, which is a multiple of 256).
+struct EpiBatch {
+    float *y;
+    const float *sub;
+    const int32_t *starts;
+    int P;
+    const FCtl *C;
+    int pred;
+    struct Pre { float s; };
+    __device__ bool skip() const { return false; }
+    __device__ __forceinline__ bool skip_rows(int row) const {
+        return !pred_on(C[find_prob(starts, P, row / GB)], pred);
     }
-    if (e < end) s = fmaf(ld_weight<V>(val, e), x[ind[e]], s);
-    s += s2;
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
-    if (lane == 0) y[row] = SUB ? s - sub[row] : s;
-}
+    __device__ __forceinline__ Pre load(int r) const { return {sub ? sub[r] : 0.f}; }
+    __device__ __forceinline__ void row(int r, float s, bool active, const Pre &p) const {
+        if (active) y[r] = sub ? s - p.s : s;
+    }
+    __device__ __forceinline__ void finish() const {}
+};
 
 // ------------------------------------------------------------------ stencils
 // descend: U_t = max(U_c - tau d, 0) (:690-696); V0/V1 statistics.
@@ -686,32 +678,9 @@ struct Batch {
         TV3_HIP(hipMemcpyAsync(ctl.p, hctl.p, sizeof(FCtl) * P, hipMemcpyHostToDevice, s));
     }
 
-    template <bool SUB>
-    void product(const SparseRows &S, int L, const float *x, float *y, const float *sub,
+    void product(const SparseRows &S, const float *x, float *y, const float *sub,
                  const int32_t *starts, int pred) {
-        const int64_t rpb = 256 / L;
-        const unsigned grid = (unsigned)((S.rows + rpb - 1) / rpb);
-#define TV3_BCSR(LL, VV)                                                                        \
-    hipLaunchKernelGGL((k_b_csr<LL, VV, SUB>), dim3(grid), dim3(256), 0, s, S.rows, S.ptr.p,    \
-                       S.ind.p, (const VV *)S.val.p, x, y, sub, starts, P, ctl.p, pred)
-#define TV3_BCSR_V(LL)                                                                          \
-    switch (S.vtype) {                                                                          \
-    case TV3_F32: TV3_BCSR(LL, float); break;                                                   \
-    case TV3_F16: TV3_BCSR(LL, __half); break;                                                  \
-    case TV3_BF16: TV3_BCSR(LL, __hip_bfloat16); break;                                         \
-    default: throw tv3_error(TV3_E_FORMAT, "batched products take F32, F16 or BF16 weights");   \
-    }
-        switch (L) {
-        case 2: TV3_BCSR_V(2); break;
-        case 4: TV3_BCSR_V(4); break;
-        case 8: TV3_BCSR_V(8); break;
-        case 16: TV3_BCSR_V(16); break;
-        case 32: TV3_BCSR_V(32); break;
-        case 64: TV3_BCSR_V(64); break;
-        default: throw tv3_error(TV3_E_ARG, "lanes per row must be a power of two in [2, 64]");
-        }
-#undef TV3_BCSR_V
-#undef TV3_BCSR
+        launch_product<float>(S, x, EpiBatch{y, sub, starts, P, ctl.p, pred}, s);
     }
     // dense groups: every column is computed (the outputs are scratch, read
     // only for the problems whose predicate holds)
@@ -728,14 +697,11 @@ struct Batch {
                      g.ldn, sub ? sub + g.v0 + c * g.ldn : nullptr, splits, gpart.p, s);
     }
     void prod_N(const float *x, float *y, int pred) {
-        if (A.rows) product<false>(A, lanes_N, x, y, nullptr, rst.p, pred);
+        if (A.rows) product(A, x, y, nullptr, rst.p, pred);
         for (const DGroup &g : groups) group_N(g, x, y, g.sN);
     }
     void prod_T(const float *x, float *y, const float *sub, int pred) {
-        if (AT.rows) {
-            if (sub) product<true>(AT, lanes_T, x, y, sub, vst.p, pred);
-            else product<false>(AT, lanes_T, x, y, nullptr, vst.p, pred);
-        }
+        if (AT.rows) product(AT, x, y, sub, vst.p, pred);
         for (const DGroup &g : groups) group_T(g, x, y, sub, g.sT);
     }
 
@@ -775,37 +741,6 @@ struct Batch {
         TV3_HIP(hipGraphDestroy(gr));
     }
 
-    // average time (ms) of a product launch with L lanes, every problem on
-    float time_product(bool trans, int L, const float *x, float *y) {
-        hipEvent_t a, e;
-        TV3_HIP(hipEventCreate(&a)); TV3_HIP(hipEventCreate(&e));
-        auto run = [&]() {
-            if (trans) product<false>(AT, L, x, y, nullptr, vst.p, P_ALL);
-            else product<false>(A, L, x, y, nullptr, rst.p, P_ALL);
-        };
-        run();
-        TV3_HIP(hipEventRecord(a, s));
-        for (int r = 0; r < 3; ++r) run();
-        TV3_HIP(hipEventRecord(e, s));
-        TV3_HIP(hipEventSynchronize(e));
-        float ms = 0;
-        TV3_HIP(hipEventElapsedTime(&ms, a, e));
-        (void)hipEventDestroy(a); (void)hipEventDestroy(e);
-        return ms / 3;
-    }
-
-    void tune() {
-        if (!A.rows) { lanes_N = lanes_T = 8; return; }
-        const int cand[] = {2, 4, 8, 16, 32, 64};
-        float bestN = 1e30f, bestT = 1e30f;
-        for (int L : cand) {
-            const float tn = time_product(false, L, U[0], AuT);
-            if (tn < bestN) { bestN = tn; lanes_N = L; }
-            const float tt = time_product(true, L, AuT, gT);
-            if (tt < bestT) { bestT = tt; lanes_T = L; }
-        }
-    }
-
     template <typename F>
     float time_ms(F &&f) {
         hipEvent_t a, e;
@@ -819,6 +754,42 @@ struct Batch {
         TV3_HIP(hipEventElapsedTime(&ms, a, e));
         (void)hipEventDestroy(a); (void)hipEventDestroy(e);
         return ms / 3;
+    }
+
+    // launch shape of one block-diagonal product: the row-group kernel with
+    // L lanes per row (code L) or the row-block kernel with NB-entry blocks
+    // (code -NB), timed with every problem on
+    int tune_matrix(SparseRows &S, const float *x, float *y, const int32_t *starts,
+                    const std::vector<int64_t> &row_starts) {
+        float best = 1e30f;
+        int code = 8;
+        for (int L : {2, 4, 8, 16, 32, 64}) {
+            S.blk_nnz = 0;
+            S.lanes = L;
+            const float t = time_ms([&] { product(S, x, y, nullptr, starts, P_ALL); });
+            if (t < best) { best = t; code = L; }
+        }
+        for (int NB : {1024, 2048, 4096}) {
+            build_row_blocks(ctx, S, NB, &row_starts);
+            const float t = time_ms([&] { product(S, x, y, nullptr, starts, P_ALL); });
+            if (t < best) { best = t; code = -NB; }
+        }
+        set_code(S, code, row_starts);
+        return code;
+    }
+    void set_code(SparseRows &S, int code, const std::vector<int64_t> &row_starts) {
+        if (code > 0) {
+            S.blk_nnz = 0;
+            S.lanes = code;
+        } else if (S.blk_nnz != -code) {
+            build_row_blocks(ctx, S, -code, &row_starts);
+        }
+    }
+    std::vector<int64_t> a_starts, at_starts;   // first rows of the problems
+    void tune() {
+        if (!A.rows) { lanes_N = lanes_T = 8; return; }
+        lanes_N = tune_matrix(A, U[0], AuT, rst.p, a_starts);
+        lanes_T = tune_matrix(AT, AuT, gT, vst.p, at_starts);
     }
 
     // K splits of each group's two products, timed (the scratch holds the
@@ -965,6 +936,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         }
         concat_matrix(ctx, s, pa, r0c, v0c, Mcsr, N, B.A);
         concat_matrix(ctx, s, pt, v0c, r0c, Ncsr, M, B.AT);
+        B.a_starts = r0c;
+        B.at_starts = v0c;
         TV3_HIP(hipStreamSynchronize(s));
         for (int k = 0; k < ncsr; ++k) { lay[k]->A = SparseRows(); lay[k]->AT = SparseRows(); }
     }
@@ -1032,9 +1005,13 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
     TV3_HIP(hipEventCreate(&evl0)); TV3_HIP(hipEventCreate(&evl1));
 
     // launch shapes of the two products (every problem active), unless given
-    if (binfo && binfo->lanes_N > 0 && binfo->lanes_T > 0) {
+    if (binfo && binfo->lanes_N != 0 && binfo->lanes_T != 0) {
         B.lanes_N = binfo->lanes_N;
         B.lanes_T = binfo->lanes_T;
+        if (B.A.rows) {
+            B.set_code(B.A, B.lanes_N, B.a_starts);
+            B.set_code(B.AT, B.lanes_T, B.at_starts);
+        }
     } else {
         B.tune();
     }

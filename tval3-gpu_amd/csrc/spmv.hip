@@ -119,7 +119,8 @@ static void launch_f64(const SparseRows &S, const double *x, double *y, hipStrea
     TV3_HIP(hipGetLastError());
 }
 
-void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz) {
+void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz,
+                      const std::vector<int64_t> *starts) {
     std::vector<int32_t> ptr(S.rows + 1);
     TV3_HIP(hipMemcpyAsync(ptr.data(), S.ptr.p, sizeof(int32_t) * (S.rows + 1),
                            hipMemcpyDeviceToHost, ctx->stream));
@@ -127,10 +128,14 @@ void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz) {
     std::vector<int32_t> blk;
     blk.reserve(S.nnz / (blk_nnz / 2) + S.rows / 256 + 2);
     int64_t start = 0;
+    size_t si = 0;   // next forced block start
     blk.push_back(0);
     for (int64_t r = 0; r < S.rows; ++r) {
+        while (starts && si < starts->size() && (*starts)[si] < r) ++si;
+        const bool forced = starts && si < starts->size() && (*starts)[si] == r;
         // close the current block before row r if adding it would overflow
-        if (r > start && ((int64_t)ptr[r + 1] - ptr[start] > blk_nnz || r - start >= 256)) {
+        if (r > start &&
+            (forced || (int64_t)ptr[r + 1] - ptr[start] > blk_nnz || r - start >= 256)) {
             blk.push_back((int32_t)r);
             start = r;
         }

@@ -19,6 +19,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "tv3_internal.hpp"
 
@@ -80,6 +81,16 @@ struct EpiStore {
     __device__ bool skip() const { return false; }
 };
 
+// An epilogue may also define skip_rows(first_row): true skips a whole row
+// block (the stream kernel) or row group (the row-group kernel) -- the
+// batched solver's way of leaving out the rows of finished problems.  Row
+// blocks and groups must then not cross the boundaries it tests.
+template <typename E, typename = void>
+struct HasSkipRows : std::false_type {};
+template <typename E>
+struct HasSkipRows<E, std::void_t<decltype(std::declval<const E &>().skip_rows(0))>>
+    : std::true_type {};
+
 // Phase 2 and the epilogue of one row block whose products are in prod.
 template <int NB, typename Acc, typename Epi>
 __device__ __forceinline__ void block_rows(const Acc *prod, const BlkDesc &d, int a, int e,
@@ -124,6 +135,12 @@ __global__ __launch_bounds__(256) void k_csr_stream(const BlkDesc *__restrict__ 
     BlkDesc d = desc[b0];
     for (int b = b0; b < b1; ++b) {
         const BlkDesc dn = desc[b + 1];      // in flight while this block runs
+        if constexpr (HasSkipRows<Epi>::value) {
+            if (epi.skip_rows(d.r0)) {       // uniform over the workgroup
+                d = dn;
+                continue;
+            }
+        }
         if (d.len <= NB) {
             const int row = block_row(d);
             const int a = ptr[row] - d.p0, e = ptr[row + 1] - d.p0;
@@ -182,6 +199,9 @@ __global__ __launch_bounds__(256) void k_csr_rows(int64_t rows, const int32_t *_
     const int64_t l0 = w * chunk, l1 = min(nlogical, l0 + chunk);
     const int lane = threadIdx.x & (L - 1);
     for (int64_t lb = l0; lb < l1; ++lb) {
+        if constexpr (HasSkipRows<Epi>::value) {
+            if (epi.skip_rows((int)(lb * (256 / L)))) continue;   // uniform
+        }
         const int64_t row0 = (lb * 256 + threadIdx.x) / L;
         const bool has = row0 < rows;
         const int64_t row = has ? row0 : rows - 1;

@@ -615,10 +615,13 @@ tv3_status tv3_solve_batched(tv3_ctx *ctx, int32_t nprob, tv3_op *const *ops,
             TV3_REQUIRE(ops[k] && b_dev[k] && U_dev[k], TV3_E_ARG, "NULL entry in the batch");
             TV3_REQUIRE(ops[k]->ctx == ctx, TV3_E_ARG, "operator of another context");
         }
-        if (binfo && (binfo->lanes_N > 0 || binfo->lanes_T > 0))
+        if (binfo && (binfo->lanes_N != 0 || binfo->lanes_T != 0))
             for (int32_t L : {binfo->lanes_N, binfo->lanes_T})
-                TV3_REQUIRE(L >= 2 && L <= 64 && (L & (L - 1)) == 0, TV3_E_ARG,
-                            "lanes per row must be powers of two in [2, 64]");
+                TV3_REQUIRE((L >= 2 && L <= 64 && (L & (L - 1)) == 0) || L == -1024 ||
+                                L == -2048 || L == -4096,
+                            TV3_E_ARG,
+                            "launch shapes: lanes per row (a power of two in [2, 64]) or "
+                            "-1024, -2048, -4096 (row blocks of that many entries)");
         set_device(ctx);
         solve_batched(ctx, nprob, ops, dims, *opts, b_dev, U_dev, Ut_dev, infos, hists, binfo);
     });

@@ -130,8 +130,10 @@ struct SparseRows {
     DevBuf<BlkDesc> desc;  // nblk + 1 (sentinel)
 };
 
-// Partition rows into blocks for the stream kernel (host greedy over rowptr).
-void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz);
+// Partition rows into blocks for the stream kernel (host greedy over rowptr);
+// starts (sorted, optional): rows where a block must begin.
+void build_row_blocks(tv3_ctx *ctx, SparseRows &S, int blk_nnz,
+                      const std::vector<int64_t> *starts = nullptr);
 
 struct DynGeom {
     int32_t E = 0, R = 0, mps = 0, rv_x = 0, rv_y = 0, rv_z = 0;

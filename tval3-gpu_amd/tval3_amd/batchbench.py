@@ -180,8 +180,6 @@ def run(problems=40, maxit=100, seed=1000, fp64=True, sequential=True, dense=8, 
             o64 = api.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tv_mode="iso", maxit=maxit)
             U64, i64, _ = api.tval3_gpu_3d(ctx, A32, p["b"].astype(np.float64), p["dims"],
                                            o64, Ut=p["u"], history=False)
-            if not p.get("dense"):
-                A32.close()
             Ug = Ub[k].double().cpu().numpy()
             rec["err_fp64"] = float(np.linalg.norm(Ug - U64) / np.linalg.norm(U64))
             rec["err_utrue_fp64"] = i64["rel_error"]
@@ -192,6 +190,8 @@ def run(problems=40, maxit=100, seed=1000, fp64=True, sequential=True, dense=8, 
                                            history=False)
             rec["plain32_err_fp64"] = float(np.linalg.norm(U32 - U64) / np.linalg.norm(U64))
             rec["plain32_err_utrue"] = i32["rel_error"]
+            if not p.get("dense"):
+                A32.close()
         per.append(rec)
     out["per_problem"] = per
     e = np.array([r["err_utrue"] for r in per])
