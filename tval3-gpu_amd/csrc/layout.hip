@@ -361,12 +361,15 @@ static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
     const double per_ray = m > 0 ? (double)nseg / (double)m : 0.0;
     R.lanes = per_ray > 24 ? 16 : per_ray > 12 ? 8 : 4;
     R.ptr.alloc(m + 1);
-    R.seg.alloc(rseg.size());
+    R.seg.alloc(rseg.size() > 0 ? rseg.size() : 1);   // the sum kernel reads seg[0] of empty rays
     R.part.alloc(nseg > 0 ? nseg : 1);
     TV3_HIP(hipMemcpyAsync(R.ptr.p, rptr.data(), sizeof(int32_t) * (m + 1), hipMemcpyHostToDevice,
                            st));
-    TV3_HIP(hipMemcpyAsync(R.seg.p, rseg.data(), sizeof(int32_t) * rseg.size(),
-                           hipMemcpyHostToDevice, st));
+    if (rseg.empty())
+        TV3_HIP(hipMemsetAsync(R.seg.p, 0, sizeof(int32_t), st));   // a valid index into part
+    else
+        TV3_HIP(hipMemcpyAsync(R.seg.p, rseg.data(), sizeof(int32_t) * rseg.size(),
+                               hipMemcpyHostToDevice, st));
     TV3_HIP(hipStreamSynchronize(st));
 }
 

@@ -145,38 +145,66 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
 
 // Pass 2 of A x: L lanes per ray sum its segment partials (batches of 4 per
 // lane, loads clamped to the ray's last segment), then the epilogue.
-template <int L, typename Epi>
+template <int L, int NLB, typename Epi>
 __global__ __launch_bounds__(256) void k_seg_sum(int64_t rows, const int32_t *__restrict__ ptr,
                                                  const int32_t *__restrict__ seg,
                                                  const float *__restrict__ part, Epi epi,
                                                  int64_t nlogical) {
+    // A workgroup owns NLB consecutive logical blocks (256 / L rays each).
+    // The three dependent loads of a ray -- its segment range, the segment
+    // indices, the partial sums -- are issued for all NLB blocks together, so
+    // the workgroup waits for three memory round trips instead of 3 NLB.
     if (epi.skip()) return;
-    const int64_t G = gridDim.x;
-    const int64_t chunk = (nlogical + G - 1) / G;
-    const int64_t w = xcd_remap(blockIdx.x, (int)G);
-    const int64_t l0 = w * chunk, l1 = min(nlogical, l0 + chunk);
+    const int64_t w = xcd_remap(blockIdx.x, (int)gridDim.x);
     const int lane = threadIdx.x & (L - 1);
-    for (int64_t lb = l0; lb < l1; ++lb) {
+    int64_t row[NLB];
+    bool has[NLB];
+    int beg[NLB], end[NLB];
+    typename Epi::Pre pre[NLB];
+#pragma unroll
+    for (int j = 0; j < NLB; ++j) {
+        const int64_t lb = w * NLB + j;
         const int64_t row0 = (lb * 256 + threadIdx.x) / L;
-        const bool has = row0 < rows;
-        const int64_t row = has ? row0 : rows - 1;
-        const int beg = ptr[row], end = has ? ptr[row + 1] : beg;
-        const auto pre = epi.load((int)row);
-        float s0 = 0, s1 = 0;
-        for (int k0 = beg + lane; k0 < end; k0 += 4 * L) {
-            int c[4];
+        has[j] = lb < nlogical && row0 < rows;
+        row[j] = has[j] ? row0 : rows - 1;
+        beg[j] = ptr[row[j]];
+        end[j] = has[j] ? ptr[row[j] + 1] : beg[j];
+        pre[j] = epi.load((int)row[j]);
+    }
+    // first batch of 4 segments per lane of every block (loads clamped into
+    // the ray's range, masked after the read)
+    int c[NLB][4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) c[i] = seg[min(k0 + i * L, end - 1)];
-            float v[4];
+    for (int j = 0; j < NLB; ++j) {
+        const int last = end[j] > 0 ? end[j] - 1 : 0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = part[c[i]];
-            s0 += v[0] + (k0 + 2 * L < end ? v[2] : 0.f);
-            s1 += (k0 + L < end ? v[1] : 0.f) + (k0 + 3 * L < end ? v[3] : 0.f);
+        for (int i = 0; i < 4; ++i) c[j][i] = seg[min(beg[j] + lane + i * L, last)];
+    }
+    float v[NLB][4];
+#pragma unroll
+    for (int j = 0; j < NLB; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[j][i] = part[c[j][i]];
+#pragma unroll
+    for (int j = 0; j < NLB; ++j) {
+        const int k0 = beg[j] + lane;
+        float s0 = (k0 < end[j] ? v[j][0] : 0.f) + (k0 + 2 * L < end[j] ? v[j][2] : 0.f);
+        float s1 = (k0 + L < end[j] ? v[j][1] : 0.f) + (k0 + 3 * L < end[j] ? v[j][3] : 0.f);
+        // rays with more than 4 L segments
+        for (int k = k0 + 4 * L; k < end[j]; k += 4 * L) {
+            int cc[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cc[i] = seg[min(k + i * L, end[j] - 1)];
+            float vv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) vv[i] = part[cc[i]];
+            s0 += vv[0] + (k + 2 * L < end[j] ? vv[2] : 0.f);
+            s1 += (k + L < end[j] ? vv[1] : 0.f) + (k + 3 * L < end[j] ? vv[3] : 0.f);
         }
-        float s = s0 + s1;
+        float sum = s0 + s1;
 #pragma unroll
-        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
-        epi.row((int)row, s, has && lane == 0, pre);
+        for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+        epi.row((int)row[j], sum, has[j] && lane == 0, pre[j]);
     }
     epi.finish();
 }
@@ -221,14 +249,17 @@ void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s
 
 template <typename Epi>
 void launch_seg_sum(const SegSum &R, const Epi &epi, hipStream_t s) {
+    constexpr int NLB = 2;   // logical blocks per workgroup
     const int L = R.lanes;
     const int64_t nl = (R.rows * L + 255) / 256;
-    const unsigned g = (unsigned)(nl < kMaxProductGrid ? nl : kMaxProductGrid);
+    const int64_t ng = (nl + NLB - 1) / NLB;
+    TV3_REQUIRE(ng < ((int64_t)1 << 31), TV3_E_SHAPE, "too many rays for one launch");
+    const unsigned g = (unsigned)ng;
     if (g == 0) return;
 #define TV3_SCASE(LL)                                                                     \
     case LL:                                                                              \
-        hipLaunchKernelGGL((k_seg_sum<LL, Epi>), dim3(g), dim3(256), 0, s, R.rows, R.ptr.p, \
-                           R.seg.p, R.part.p, epi, nl);                                   \
+        hipLaunchKernelGGL((k_seg_sum<LL, NLB, Epi>), dim3(g), dim3(256), 0, s, R.rows,     \
+                           R.ptr.p, R.seg.p, R.part.p, epi, nl);                          \
         break;
     switch (L) {
         TV3_SCASE(2) TV3_SCASE(4) TV3_SCASE(8) TV3_SCASE(16) TV3_SCASE(32)

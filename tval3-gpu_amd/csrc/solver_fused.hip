@@ -286,32 +286,39 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__res
         b = (((sub >> 1) & 1) << 2) | ((w >> 2) & 3);
         c = ((sub >> 2) << 2) | (w >> 4);
     };
-    for (int k = t; k < NBOX; k += 512) {
+    // every global load of the workgroup first (one memory round trip): the
+    // U box ([-1, 8]^3, two values per thread), sigma at the tile voxel q0 = t
+    // and at the lower-face voxel q1 = 512 + t (t < 192), and g, dts, Uc, d
+    // of voxel t; indices are clamped instead of guarded
+    auto box_g = [&](int k) {
         const int a = k % kTileBox - 1, b = (k / kTileBox) % kTileBox - 1, c = k / 100 - 1;
-        su[k] = U[gidx(a, b, c)];
-    }
+        return gidx(a, b, c);
+    };
+    const int k1 = t + 512 < NBOX ? t + 512 : NBOX - 1;
+    const float ub0 = U[box_g(t)], ub1 = U[box_g(k1)];
+    int a0, b0, c0;
+    tile_pos(t, a0, b0, c0);
+    const int64_t i = base + t;
+    const int q1 = t < 192 ? t : 191;                 // face item 512 + q1
+    const int f = q1 >> 6, j = q1 & 63, fu = j & 7, fv = j >> 3;
+    const int a1 = f == 0 ? -1 : fu, b1 = f == 0 ? fu : (f == 1 ? -1 : fv),
+              c1 = f == 2 ? -1 : fv;
+    const int64_t gi1 = gidx(a1, b1, c1);
+    const float sx0 = sx[i], sy0 = sy[i], sz0 = sz[i];
+    const float sx1 = sx[gi1], sy1 = sy[gi1], sz1 = sz[gi1];
+    const float gv = g[i], dv = dts[i], ucv = Uc[i], dold = d[i];
+    su[t] = ub0;
+    if (t + 512 < NBOX) su[t + 512] = ub1;
     __syncthreads();
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
     const bool iso = C->iso;
-    // V at the tile (q < 512) and at its lower faces y = -1, x = -1, z = -1
-    for (int q = t; q < 512 + 3 * 64; q += 512) {
-        int a, b, c;
-        int64_t gi;
-        if (q < 512) {
-            tile_pos(q, a, b, c);
-            gi = base + q;
-        } else {
-            const int f = (q - 512) >> 6, j = (q - 512) & 63, u = j & 7, v = j >> 3;
-            a = f == 0 ? -1 : u;
-            b = f == 0 ? u : (f == 1 ? -1 : v);
-            c = f == 2 ? -1 : v;
-            gi = gidx(a, b, c);
-        }
+    // V at the tile voxel and (t < 192) at a voxel of the lower faces
+    // y = -1, x = -1, z = -1, where D^T reads it
+    auto vat = [&](int a, int b, int c, float s_x, float s_y, float s_z, bool tile_voxel) {
         const int sl = box_slot(a, b, c);
         const float u0 = su[sl], uyp = su[sl + 1], uxp = su[sl + kTileBox],
                     uzp = su[sl + kTileBox * kTileBox];
-        const float s_x = sx[gi], s_y = sy[gi], s_z = sz[gi];
         float wx, wy, wz;
         const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, beta, ib, iso, wx,
                                  wy, wz);
@@ -319,25 +326,24 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__res
         svx[sl] = vx;
         svy[sl] = vy;
         svz[sl] = vz;
-        if (q < 512) {
+        if (tile_voxel) {
             acc[0] += l1;
             acc[1] += (double)vx * vx + (double)vy * vy + (double)vz * vz;
             acc[2] += (double)vx * s_x + (double)vy * s_y + (double)vz * s_z;
         }
-    }
+    };
+    vat(a0, b0, c0, sx0, sy0, sz0, true);
+    if (t < 192) vat(a1, b1, c1, sx1, sy1, sz1, false);
     __syncthreads();
     {
-        int a, b, c;
-        tile_pos(t, a, b, c);
-        const int sl = box_slot(a, b, c);
-        const int64_t i = base + t;
+        const int sl = box_slot(a0, b0, c0);
         const float g2 = svy[sl - 1] - svy[sl] + svx[sl - kTileBox] - svx[sl] +
                          svz[sl - kTileBox * kTileBox] - svz[sl];          // D^T V, :501
-        const float dn = mdb * g[i] + g2 + dts[i];                         // :915-918
-        const float uc = Uc[i], du = su[sl] - uc;
+        const float dn = mdb * gv + g2 + dv;                                // :915-918
+        const float du = su[sl] - ucv;
         acc[3] += (double)du * du;
-        acc[4] += (double)du * (double)(dn - d[i]);
-        acc[5] += (double)uc * uc;
+        acc[4] += (double)du * (double)(dn - dold);
+        acc[5] += (double)ucv * ucv;
         acc[6] += (double)dn * dn;
         d[i] = dn;
     }
