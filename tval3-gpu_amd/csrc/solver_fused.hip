@@ -575,9 +575,7 @@ __global__ __launch_bounds__(256) void k_stage2(const FCtl *C, int check, const 
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        double x = v[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        const double x = wave_sum(v[k]);
         if ((t & 63) == 0) sh[k][t >> 6] = x;
     }
     __syncthreads();
@@ -585,16 +583,13 @@ __global__ __launch_bounds__(256) void k_stage2(const FCtl *C, int check, const 
 }
 
 // Sums of the K arrays of S2 stage-2 values on one wavefront (lane j holds
-// entry j, butterfly tree): every lane ends with the K totals.
+// entry j, summed by wave_sum): every lane ends with the K totals.
 template <int K>
 __device__ __forceinline__ void wave_sums(const double *p2, double *out) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        double x = p2[(int64_t)k * S2 + t];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        out[k] = x;
+        out[k] = wave_sum(p2[(int64_t)k * S2 + t]);
     }
 }
 

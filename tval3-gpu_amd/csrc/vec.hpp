@@ -4,22 +4,22 @@
 
 #include <array>
 
+#include "lanes.hpp"
 #include "tv3_internal.hpp"
 
 namespace tv3 {
 
 // Sum K per-thread doubles over a workgroup of NW wavefronts (4: 256
-// threads, 8: 512) -- wavefront butterflies, then the wavefront totals in a
-// fixed order; thread 0 stores partials[k*nblk + blockIdx.x].
+// threads, 8: 512) -- wavefront sums (wave_sum), then the wavefront totals
+// in a fixed order; thread 0 stores partials[k*nblk + blockIdx.x].  Every
+// thread of the workgroup calls it.
 template <int K, int NW = 4>
 __device__ __forceinline__ void block_sum_store(double (&acc)[K], double *partials, int nblk) {
     __shared__ double sh[K][NW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        double v = acc[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        const double v = wave_sum(acc[k]);
         if (lane == 0) sh[k][wave] = v;
     }
     __syncthreads();
