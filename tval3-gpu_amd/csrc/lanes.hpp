@@ -32,19 +32,33 @@ __device__ __forceinline__ double lane_f64(double x, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Sum of a float over aligned groups of L lanes (L a power of two, 1..64);
-// every lane returns its group's sum.  Groups above 16 lanes finish with
-// lane exchanges across rows.
-template <int L>
-__device__ __forceinline__ float group_sum_f32(float v) {
-    static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L: power of two up to 64");
-    if constexpr (L >= 2) v += dpp_f32<0xB1>(v);
-    if constexpr (L >= 4) v += dpp_f32<0x4E>(v);
-    if constexpr (L >= 8) v += dpp_f32<0x141>(v);
-    if constexpr (L >= 16) v += dpp_f32<0x140>(v);
-    if constexpr (L >= 32) v += __shfl_xor(v, 16, 64);
-    if constexpr (L >= 64) v += __shfl_xor(v, 32, 64);
+__device__ __forceinline__ float lane_xor(float x, int off) { return __shfl_xor(x, off, 64); }
+__device__ __forceinline__ double lane_xor(double x, int off) { return __shfl_xor(x, off, 64); }
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) { return dpp_f32<CTRL>(x); }
+template <int CTRL>
+__device__ __forceinline__ double dpp(double x) { return dpp_f64<CTRL>(x); }
+
+// Sum of a float or double over aligned groups of L lanes (L a power of two,
+// 1..64, uniform over the wavefront); every lane returns its group's sum.
+// Groups above 16 lanes finish with lane exchanges across rows.  A group's
+// lanes must all be active (whole groups may be inactive).
+template <typename T>
+__device__ __forceinline__ T lane_sum(T v, int L) {
+    if (L >= 2) v += dpp<0xB1>(v);
+    if (L >= 4) v += dpp<0x4E>(v);
+    if (L >= 8) v += dpp<0x141>(v);
+    if (L >= 16) v += dpp<0x140>(v);
+    if (L >= 32) v += lane_xor(v, 16);
+    if (L >= 64) v += lane_xor(v, 32);
     return v;
+}
+
+template <int L, typename T>
+__device__ __forceinline__ T lane_sum(T v) {
+    static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L: power of two up to 64");
+    return lane_sum(v, L);
 }
 
 // Sum of a double over the 64 lanes of a wavefront: the four row sums of

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
             float s = 0;
             for (int g = t; g < (d.len >> 2); g += 256)
                 s += group_sum<V>(pk, ps, pv, (int64_t)d.p0 + 4 * g, xs);
-            s = group_sum_f32<64>(s);
+            s = lane_sum<64>(s);
             if ((t & 63) == 0) gs[t >> 6] = s;
             __syncthreads();
             if (t == 0) s = (gs[0] + gs[1]) + (gs[2] + gs[3]);
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_seg_sum(int64_t rows, const int32_t *__
             s0 += vv[0] + (k + 2 * L < end[j] ? vv[2] : 0.f);
             s1 += (k + L < end[j] ? vv[1] : 0.f) + (k + 3 * L < end[j] ? vv[3] : 0.f);
         }
-        const float sum = group_sum_f32<L>(s0 + s1);
+        const float sum = lane_sum<L>(s0 + s1);
         epi.row((int)row[j], sum, has[j] && lane == 0, pre[j]);
     }
     epi.finish();

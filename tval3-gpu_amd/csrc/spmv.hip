@@ -51,9 +51,7 @@ __global__ __launch_bounds__(256) void k_csr_f32(int64_t rows, const int32_t *__
         s2 = fma(w1, (Acc)x[c1], s2);
     }
     if (k < end) s = fma((Acc)ldw<V>(val, k), (Acc)x[ind[k]], s);
-    s += s2;
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+    s = lane_sum<L>(s + s2);
     if (lane == 0) y[row] = (float)s;
 }
 
@@ -69,8 +67,7 @@ __global__ __launch_bounds__(256) void k_csr_f64(int64_t rows, const int32_t *__
     const int beg = ptr[row], end = ptr[row + 1];
     double s = 0.0;
     for (int k = beg + lane; k < end; k += L) s = fma(ldw64(val, k), x[ind[k]], s);
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+    s = lane_sum<L>(s);
     if (lane == 0) y[row] = s;
 }
 

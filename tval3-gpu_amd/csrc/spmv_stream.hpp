@@ -21,6 +21,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "lanes.hpp"
 #include "tv3_internal.hpp"
 
 namespace tv3 {
@@ -101,11 +102,7 @@ __device__ __forceinline__ void block_rows(const Acc *prod, const BlkDesc &d, in
     const bool has = rr < d.nr;
     Acc s = 0;
     if (has) s = row_sum<NB>(prod, a, e, lane, L);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        Acc o = __shfl_xor(s, off, 64);
-        if (off < L) s += o;
-    }
+    s = lane_sum(s, L);
     epi.row(row, s, has && lane == 0, pre);
 }
 
@@ -167,8 +164,7 @@ __global__ __launch_bounds__(256) void k_csr_stream(const BlkDesc *__restrict__ 
             Acc s = 0;
             for (int k = d.p0 + t; k < d.p0 + d.len; k += 256)
                 s += (Acc)ld_weight<V>(val, k) * (Acc)x[ind[k]];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            s = lane_sum<64>(s);
             if ((t & 63) == 0) prod[t >> 6] = s;
             __syncthreads();
             if (t == 0) s = (prod[0] + prod[1]) + (prod[2] + prod[3]);
@@ -223,9 +219,7 @@ __global__ __launch_bounds__(256) void k_csr_rows(int64_t rows, const int32_t *_
             s += (Acc)w[0] * (Acc)x[c[0]] + (Acc)w[2] * (Acc)x[c[2]];
             s2 += (Acc)w[1] * (Acc)x[c[1]] + (Acc)w[3] * (Acc)x[c[3]];
         }
-        s += s2;
-#pragma unroll
-        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+        s = lane_sum<L>(s + s2);
         epi.row((int)row, s, has && lane == 0, pre);
     }
     epi.finish();
