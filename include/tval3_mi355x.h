@@ -70,8 +70,8 @@ typedef struct {
     int32_t maxcnt, maxit;
     double c, gamma, gam, rate_gam;
     int32_t isreal, nonneg;
-    /* A^T-skip heuristic (3d/tval3_gpu_3d.cpp:398-468), plain solver only:
-     * skipMulRatio = 0 turns it off.  A skipped A^T product leaves g as
+    /* A^T-skip heuristic (3d/tval3_gpu_3d.cpp:398-468), plain and fused
+     * solvers (not batched solves): skipMulRatio = 0 turns it off.  A skipped A^T product leaves g as
      * A^T(last Au) - Atb (the reference subtracts Atb again from the stale g). */
     uint32_t numInitIterations;
     double skipMulRatio, maxRelativeChange;
@@ -100,6 +100,8 @@ typedef struct {
     double *f, *lam1, *lam2, *lam3, *lam4, *lam5, *tau, *alpha, *relchg, *mu, *beta, *C;
     int32_t *cnt;     /* backtracking steps; 6 = the cnt==5 fallback */
     int32_t *outer;
+    int32_t *mulT;    /* 1: the iteration's get_g took the A^T product, 0: the
+                       * A^T-skip heuristic skipped it; NULL to omit */
 } tv3_history;
 
 /* Transducer geometry of a dynamic operator (geometry_host / geometry_device,
@@ -165,6 +167,9 @@ tv3_status tv3_op_hybrid_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weig
  * reference expects host-built matrices). */
 tv3_status tv3_op_materialize(tv3_op *dynamic_op, tv3_dtype val_type, tv3_op **out_csr);
 
+/* Frees the operator's device memory after the work queued on its device
+ * has finished.  It does not touch the context: an operator may be destroyed
+ * before or after the context that created it. */
 void tv3_op_destroy(tv3_op *op);
 tv3_status tv3_op_info(const tv3_op *op, int64_t *m, int64_t *n, int64_t *nnz);
 

@@ -61,6 +61,7 @@ typedef struct {
     double *f, *lam1, *lam2, *lam3, *lam4, *lam5, *tau, *alpha, *relchg, *mu, *beta, *C;
     int *cnt;            /* backtracking steps taken (6 = fallback) */
     int *outer;          /* outer index of the iteration */
+    int *mulT;           /* 1: the iteration's get_g multiplied by A^T; NULL to omit */
 } orc_history;
 
 void orc_opts_default(orc_opts *o);

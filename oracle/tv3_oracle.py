@@ -55,7 +55,8 @@ _HIST_D = ["f", "lam1", "lam2", "lam3", "lam4", "lam5", "tau", "alpha", "relchg"
 class OrcHistory(C.Structure):
     _fields_ = [("cap", C.c_int), ("count", C.c_int)] + \
                [(k, C.POINTER(C.c_double)) for k in _HIST_D] + \
-               [("cnt", C.POINTER(C.c_int)), ("outer", C.POINTER(C.c_int))]
+               [("cnt", C.POINTER(C.c_int)), ("outer", C.POINTER(C.c_int)),
+                ("mulT", C.POINTER(C.c_int))]
 
 
 _P = C.c_void_p
@@ -289,10 +290,12 @@ def solve(op, dims, b, opts=None, Ut=None, dtype=np.float64, hist_cap=4096):
     harr = {k: np.zeros(hist_cap, np.float64) for k in _HIST_D}
     harr["cnt"] = np.zeros(hist_cap, np.int32)
     harr["outer"] = np.zeros(hist_cap, np.int32)
+    harr["mulT"] = np.zeros(hist_cap, np.int32)
     for k in _HIST_D:
         setattr(h, k, harr[k].ctypes.data_as(C.POINTER(C.c_double)))
     h.cnt = harr["cnt"].ctypes.data_as(C.POINTER(C.c_int))
     h.outer = harr["outer"].ctypes.data_as(C.POINTER(C.c_int))
+    h.mulT = harr["mulT"].ctypes.data_as(C.POINTER(C.c_int))
     st = getattr(lib(), "orc_solve" + _sfx(dtype))(C.byref(op.s), C.byref(o), p, q, r, _ptr(b),
                                                    _ptr(U), _ptr(ut), C.byref(info),
                                                    C.byref(h))

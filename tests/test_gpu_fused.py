@@ -45,9 +45,12 @@ def opts(**kw):
 
 
 def first_divergence(h1, h2):
+    """first iteration whose decisions differ: backtracking count, outer
+    index, A^T product taken or skipped"""
     n = min(len(h1["cnt"]), len(h2["cnt"]))
     for k in range(n):
-        if h1["cnt"][k] != h2["cnt"][k] or h1["outer"][k] != h2["outer"][k]:
+        if (h1["cnt"][k] != h2["cnt"][k] or h1["outer"][k] != h2["outer"][k]
+                or h1["mulT"][k] != h2["mulT"][k]):
             return k
     return n
 
@@ -119,20 +122,22 @@ def test_fused_backtracking_and_fallback(ctx):
 def test_fused_fp16_weights_equal_error(ctx):
     """FP16 ray weights, FP32 accumulation: the error against u_true matches
     the FP32 path's within 1e-3 or within the spread the FP32 path itself
-    shows when b is perturbed at rounding level (final TVAL3 errors are that
-    sensitive on small problems)."""
-    import torch
+    shows when b is perturbed at rounding level (three perturbations; final
+    TVAL3 errors are that sensitive on small problems, and the launch shapes,
+    timed at set-up, fix the products' summation order per run)."""
     dims = (16, 16, 16)
     u, A16, A32, oop, b = setup(ctx, dims, 64, 128, seed=3, dtype="f16")
     o = api.default_opts(**opts(maxit=80), solver="fused")
     U16, i16, _ = api.tval3_gpu_3d(ctx, A16, b, dims, o, Ut=u)
     U32, i32, _ = api.tval3_gpu_3d(ctx, A32, b, dims, o, Ut=u)
-    xi = np.random.default_rng(0).choice([-1.0, 1.0], size=b.shape)
-    bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
-    Up, ip, _ = api.tval3_gpu_3d(ctx, A32, bp, dims, o, Ut=u)
-    spread = abs(ip["rel_error"] - i32["rel_error"])
+    errs = []
+    for seed in (0, 1, 2):
+        xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=b.shape)
+        bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+        errs.append(api.tval3_gpu_3d(ctx, A32, bp, dims, o, Ut=u)[1]["rel_error"])
+    spread = max(abs(e - i32["rel_error"]) for e in errs)
     print(f"fp16 err {i16['rel_error']:.4e} fp32 err {i32['rel_error']:.4e} "
-          f"fp32 perturbed {ip['rel_error']:.4e}")
+          f"fp32 perturbed {', '.join(f'{e:.4e}' for e in errs)}")
     assert abs(i16["rel_error"] - i32["rel_error"]) <= max(1e-3, 3 * spread)
 
 
@@ -148,3 +153,73 @@ def test_fused_device_inputs_and_warm_start(ctx):
     U3, i3, _ = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**opts(maxit=20), solver="fused"),
                                  U0=U2)
     assert i3["total_iters"] == 20
+
+
+def _skip_kw(skip):
+    ratio, ninit, maxrel = skip
+    return dict(skipMulRatio=ratio, numInitIterations=ninit, maxRelativeChange=maxrel)
+
+
+@pytest.mark.parametrize("skip", [(2.0, 3, 1e30), (1.0, 0, 0.05)])
+def test_fused_skip_heuristic_tracks_oracle(ctx, skip):
+    """A^T-skip heuristic (3d/tval3_gpu_3d.cpp:398-468; a skipped product
+    leaves g as it was) in the fused solver against the FP32 oracle running
+    the same rule: the same decisions (A^T product taken or skipped,
+    backtracking, outer index) for at least 10 iterations, rounding-level
+    agreement at the start and the oracle's own perturbation envelope after,
+    the same number of A^T products whenever both runs take the same
+    iterations, products actually skipped, and the final error."""
+    dims = (16, 16, 16)
+    u, A, A32, oop, b = setup(ctx, dims, 24, 64)
+    kw = _skip_kw(skip)
+    o = opts(maxit=60, **kw)
+    Uf, inf_, hf = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**o, solver="fused"), Ut=u)
+    oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=60, tv_iso=1)
+    for k, v in kw.items():
+        setattr(oo, k, v)
+    Uo, io, ho, _ = orc.solve(oop, dims, b, oo, Ut=u, dtype=np.float32)
+    # the oracle's own sensitivity: three rounding-level perturbations of b
+    # (with skipped products the dynamics are rougher, and one perturbation
+    # can underestimate the drift by two orders of magnitude)
+    runs = []
+    for s_ in (0, 1, 2):
+        xi = np.random.default_rng(s_).choice([-1.0, 1.0], size=b.shape)
+        bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+        runs.append(orc.solve(oop, dims, bp, oo, Ut=u, dtype=np.float32))
+    k = min([first_divergence(hf, ho)] + [first_divergence(r[2], ho) for r in runs])
+    dev_f = np.abs(hf["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k])
+    env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k]))
+                  for r in runs], axis=0)
+    ip = runs[0][1]
+    spread = max(abs(r[1].rel_error - io.rel_error) for r in runs)
+    same = (inf_["total_iters"] == io.total_iters and inf_["fallbacks"] == io.fallbacks
+            and np.array_equal(hf["outer"], ho["outer"]))
+    print(f"skip {skip}: iters fused {inf_['total_iters']} oracle {io.total_iters}; A^T "
+          f"products fused {inf_['n_matvec_T'] - 1} oracle {io.n_matvec_T}; decisions agree for "
+          f"{k}; err fused {inf_['rel_error']:.4e} oracle {io.rel_error:.4e} perturbed "
+          f"{ip.rel_error:.4e}; max f dev {dev_f.max():.1e} envelope {env.max():.1e}")
+    assert io.n_matvec_T < io.total_iters                 # the rule skipped products
+    assert inf_["n_matvec_T"] - 1 < inf_["total_iters"]
+    assert k >= 10
+    assert (dev_f[:4] < 1e-5).all()
+    assert (dev_f <= 30 * env + 1e-6).all()
+    if same:
+        # the fused count holds A^T b besides the oracle's products
+        assert inf_["n_matvec_T"] - 1 == io.n_matvec_T
+    assert abs(inf_["rel_error"] - io.rel_error) <= max(3 * spread, 1e-3)
+
+
+def test_fused_skip_rule_that_never_skips_changes_nothing(ctx):
+    """maxRelativeChange = 0: every get_g multiplies (Au always moves), so the
+    run with the heuristic on equals the run without it bit for bit, with the
+    same product count."""
+    dims = (12, 10, 6)
+    u, A, A32, oop, b = setup(ctx, dims, 24, 64, seed=7)
+    o0 = api.default_opts(**opts(maxit=50), solver="fused")
+    o1 = api.default_opts(**opts(maxit=50, **_skip_kw((0.5, 2, 0.0))), solver="fused")
+    U0, i0, h0 = api.tval3_gpu_3d(ctx, A, b, dims, o0)
+    U1, i1, h1 = api.tval3_gpu_3d(ctx, A, b, dims, o1)
+    assert np.array_equal(U0, U1)
+    assert np.array_equal(h0["f"], h1["f"])
+    assert i0["total_iters"] == i1["total_iters"]
+    assert i0["n_matvec_T"] == i1["n_matvec_T"]

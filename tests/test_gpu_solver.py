@@ -167,9 +167,9 @@ def test_solver_error_paths(ctx):
     with pytest.raises(TV3Error) as e:
         api.tval3_gpu_3d(ctx, op, bb, (N, N, 1))
     assert e.value.status == 6
-    with pytest.raises(TV3Error) as e:      # skipping: plain solver only
+    with pytest.raises(TV3Error) as e:      # the fused solver takes CSR operators
         api.tval3_gpu_3d(ctx, op, b.astype(np.float32), (N, N, 1),
-                         api.default_opts(skipMulRatio=0.5, solver="fused"))
+                         api.default_opts(solver="fused"))
     assert e.value.status == 8
     with pytest.raises(TV3Error) as e:
         api.tval3_gpu_3d(ctx, op, b.astype(np.float32), (N, N, 1),

@@ -352,6 +352,38 @@ def test_skip_heuristic_skips_and_still_recovers(orc):
     assert info.rel_error < 5e-3, info.rel_error
 
 
+@pytest.mark.parametrize("ratio,period", [(2.0, [1, 0, 0]), (1.0, [1, 0]), (0.5, [1, 1, 0])])
+def test_skip_heuristic_schedule_closed_form(orc, ratio, period):
+    """maxRelativeChange = 1e30: the decisions follow the counters alone
+    (3d/tval3_gpu_3d.cpp:437-466).  The skip value starts at 0 after the
+    forced first product; each of the first numInitIterations calls
+    multiplies; then a call multiplies while the value is below 1 (adding the
+    ratio) and otherwise skips (subtracting 1).  Ratio 2 gives 1,0,0 repeated,
+    ratio 1 gives 1,0 and ratio 0.5 gives 1,1,0.  A fallback (cnt 6) is a
+    forced product that resets the value, after a rejected trial that also
+    counted: the closed form is checked on the rows before the first one,
+    which cover at least a full period after the initial products; the row
+    of a fallback multiplies."""
+    N = 16
+    u = wl.phantom_2d(N, 4, seed=7)
+    A = wl.dense_gaussian(N * N, N * N, seed=8)
+    b = wl.add_noise(A @ u, 1e-3, seed=9)
+    ninit = 3
+    o = orc.default_opts(maxit=40, tol_inn=1e-9, maxcnt=1, skipMulRatio=ratio,
+                         numInitIterations=ninit, maxRelativeChange=1e30)
+    U, info, hist, st = orc.solve(orc.dense_op(A), (N, N, 1), b, o, dtype=np.float32)
+    assert st == 0
+    fb = np.flatnonzero(hist["cnt"] == 6)
+    k = int(fb[0]) if len(fb) else len(hist["cnt"])
+    assert k >= ninit + len(period) + 1, k
+    want = [1] * ninit + [period[j % len(period)] for j in range(k - ninit)]
+    assert hist["mulT"][:k].tolist() == want
+    assert (hist["mulT"][fb] == 1).all()
+    # every decision is counted: the forced initial product, one per row, one
+    # more per rejected trial that multiplied
+    assert info.n_matvec_T >= 1 + int(hist["mulT"].sum())
+
+
 def test_solve_nonmonotone_invariants(orc):
     N = 16
     u = wl.phantom_2d(N, 5, seed=9)

@@ -373,30 +373,41 @@ static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
     TV3_HIP(hipStreamSynchronize(st));
 }
 
-// Average duration (ms) of `reps` launches of f on the context stream after
-// one untimed launch.
-template <typename F>
-static float time_launches(tv3_ctx *ctx, F &&f, int reps = 4) {
-    hipStream_t s = ctx->stream;
-    hipEvent_t e0, e1;
-    TV3_HIP(hipEventCreate(&e0));
-    TV3_HIP(hipEventCreate(&e1));
-    f();
-    TV3_HIP(hipEventRecord(e0, s));
-    for (int r = 0; r < reps; ++r) f();
-    TV3_HIP(hipEventRecord(e1, s));
-    TV3_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    TV3_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return ms / reps;
-}
+// Median duration (ms) of `reps` launches of f on the context stream, each
+// timed alone after a 512 MiB write that evicts the operator from the
+// 256 MiB memory-side cache, as the solver's other kernels do between two
+// products (back-to-back launches would time a cache-resident operator and
+// can rank the shapes differently); one untimed launch first.
+struct ColdTimer {
+    tv3_ctx *ctx;
+    DevBuf<uint8_t> flush;
+    explicit ColdTimer(tv3_ctx *c) : ctx(c), flush(size_t(512) << 20) {}
+    template <typename F>
+    float operator()(F &&f, int reps = 5) {
+        hipStream_t s = ctx->stream;
+        std::vector<hipEvent_t> ev(2 * reps);
+        for (auto &e : ev) TV3_HIP(hipEventCreate(&e));
+        f();
+        for (int r = 0; r < reps; ++r) {
+            TV3_HIP(hipMemsetAsync(flush.p, r & 0xFF, flush.bytes(), s));
+            TV3_HIP(hipEventRecord(ev[2 * r], s));
+            f();
+            TV3_HIP(hipEventRecord(ev[2 * r + 1], s));
+        }
+        TV3_HIP(hipEventSynchronize(ev[2 * reps - 1]));
+        std::vector<float> t(reps);
+        for (int r = 0; r < reps; ++r)
+            TV3_HIP(hipEventElapsedTime(&t[r], ev[2 * r], ev[2 * r + 1]));
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        std::sort(t.begin(), t.end());
+        return t[reps / 2];
+    }
+};
 
 // Autotuner: time the product of S over the candidate launch shapes (row
 // blocks of 1024/2048/4096 entries, row groups of 8..64 lanes) and keep the
 // fastest.  Runs once per operator and volume shape.  Returns its time (ms).
-static float tune_shape(tv3_ctx *ctx, SparseRows &S, int64_t xlen) {
+static float tune_shape(tv3_ctx *ctx, ColdTimer &time_launches, SparseRows &S, int64_t xlen) {
     hipStream_t s = ctx->stream;
     DevBuf<float> x(xlen > 0 ? xlen : 1), y(S.rows > 0 ? S.rows : 1);
     TV3_HIP(hipMemsetAsync(x.p, 0, x.bytes(), s));
@@ -405,13 +416,13 @@ static float tune_shape(tv3_ctx *ctx, SparseRows &S, int64_t xlen) {
     int choice = 0;
     for (int nb : {1024, 2048, 4096}) {
         build_row_blocks(ctx, S, nb);
-        const float t = time_launches(ctx, run);
+        const float t = time_launches(run);
         if (t < best) { best = t; choice = -nb; }
     }
     for (int L : {8, 16, 32, 64}) {
         S.blk_nnz = 0;
         S.lanes = L;
-        const float t = time_launches(ctx, run);
+        const float t = time_launches(run);
         if (t < best) { best = t; choice = L; }
     }
     if (choice < 0) build_row_blocks(ctx, S, -choice);
@@ -422,7 +433,8 @@ static float tune_shape(tv3_ctx *ctx, SparseRows &S, int64_t xlen) {
 // Panel forms of A^T and A (tiles of 512 internal voxels, panels of at most
 // 16384 entries) timed over row blocks of 1024 and 2048 entries; each kept
 // when faster than the tuned plain kernel (t_A, t_T in ms).
-static void tune_panels(tv3_ctx *ctx, FusedLayout &F, float t_A, float t_T) {
+static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, float t_A,
+                        float t_T) {
     hipStream_t s = ctx->stream;
     const int64_t n = F.AT.rows, m = F.A.rows;
     const size_t vb = dtype_size(F.AT.vtype);
@@ -439,15 +451,15 @@ static void tune_panels(tv3_ctx *ctx, FusedLayout &F, float t_A, float t_T) {
     for (int nb : {1024, 2048}) {
         Panels P;
         build_panels_T(ctx, h, pl, nb, P);
-        const float t = time_launches(
-            ctx, [&]() { launch_panel(P, xr.p, EpiStoreIf{y.p, nullptr}, s); });
+        const float t =
+            time_launches([&]() { launch_panel(P, xr.p, EpiStoreIf{y.p, nullptr}, s); });
         if (t < t_T) { t_T = t; F.PT = std::move(P); F.panels_T = true; }
     }
     for (int nb : {1024, 2048}) {
         Panels P;
         SegSum R;
         build_panels_A(ctx, h, pl, nb, P, R);
-        const float t = time_launches(ctx, [&]() {
+        const float t = time_launches([&]() {
             launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
             launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
         });
@@ -512,9 +524,10 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
     renumber_operator(op, dims, L.get());
     SparseRows &N = L->A;
     const int64_t n = N.cols, m = N.rows;
-    const float t_a = tune_shape(ctx, N, n);
-    const float t_at = tune_shape(ctx, L->AT, m);
-    tune_panels(ctx, *L, t_a, t_at);
+    ColdTimer timer(ctx);
+    const float t_a = tune_shape(ctx, timer, N, n);
+    const float t_at = tune_shape(ctx, timer, L->AT, m);
+    tune_panels(ctx, timer, *L, t_a, t_at);
     TV3_HIP(hipStreamSynchronize(s));
     op->fused = std::move(L);
     return *op->fused;

@@ -1207,6 +1207,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
                 h.lam4[j] = q[4]; h.lam5[j] = q[5]; h.tau[j] = q[6]; h.alpha[j] = q[7];
                 h.relchg[j] = q[8]; h.mu[j] = q[9]; h.beta[j] = q[10]; h.C[j] = q[11];
                 h.cnt[j] = (int32_t)q[12]; h.outer[j] = (int32_t)q[13];
+                if (h.mulT) h.mulT[j] = (int32_t)q[14];
             }
             h.count = cnt;
         }

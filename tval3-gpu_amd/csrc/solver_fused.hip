@@ -512,14 +512,46 @@ struct EpiGrad {         // g_n = s - Atb
     float *g;
     const float *Atb;
     const FCtl *C;
+    int gated;                       // 1: only when get_g's skip decision multiplies
     struct Pre { float a; };         // Atb of the row
-    __device__ bool skip() const { return !C->active; }
+    __device__ bool skip() const { return !C->active || (gated && !C->mul_T); }
     __device__ __forceinline__ Pre load(int r) const { return {Atb[r]}; }
     __device__ __forceinline__ void row(int r, float s, bool active, const Pre &p) const {
         if (active) g[r] = s - p.a;
     }
     __device__ __forceinline__ void finish() const {}
 };
+
+// A^T-skip heuristic (:398-468): |Au_n - last Au|^2 and |Au_n|^2
+__global__ __launch_bounds__(256) void k_skip_stats(int64_t m, const float *__restrict__ Aun,
+                                                    const float *__restrict__ last,
+                                                    const FCtl *C, double *part) {
+    if (!C->active) return;
+    double acc[2] = {0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * 256) {
+        const float a = Aun[i], dl = a - last[i];
+        acc[0] += (double)dl * dl;
+        acc[1] += (double)a * a;
+    }
+    block_sum_store<2>(acc, part, gridDim.x);
+}
+
+// after the gated A^T product: a product makes Au_n the last Au; a skipped
+// one leaves g as it was (g_n = g_c), as the reference's get_g does
+__global__ __launch_bounds__(256) void k_skip_after(int64_t n, int64_t m,
+                                                    const float *__restrict__ gc,
+                                                    float *__restrict__ gn,
+                                                    const float *__restrict__ Aun,
+                                                    float *__restrict__ last, const FCtl *C) {
+    if (!C->active) return;
+    const int64_t G = (int64_t)gridDim.x * 256;
+    if (C->mul_T) {
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += G) last[i] = Aun[i];
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += G) gn[i] = gc[i];
+    }
+}
 
 // ------------------------------------------------------------------ scalar kernels
 // Sum K partial arrays of nb entries each in a fixed order (1024 threads, four
@@ -627,6 +659,13 @@ __global__ void k_init_scalars(FCtl *C, const double *pN, int nbN, const double 
     if (threadIdx.x == 0) init_step(C, a, b);
 }
 
+__global__ void k_skip_decide(FCtl *C, const double *p, int nb, int force) {
+    if (!C->active) return;
+    double a[2];
+    reduce_parts<2>(p, nb, a);
+    if (threadIdx.x == 0) skip_decide(C, a[0], a[1], force);
+}
+
 __global__ void k_relout_scalar(FCtl *C, const double *p, int nb) {
     double a[1];
     reduce_parts<1>(p, nb, a);
@@ -669,10 +708,11 @@ struct Fused {
     Groups G;
     DevBuf<float> pool;
     float *U[2], *g[2], *Au[2], *d, *sx, *sy, *sz, *ox, *oy, *oz, *dts, *Atb, *rcd, *b, *delta,
-        *tmp;
+        *lastAu, *tmp;
+    bool skipping = false;           // A^T-skip heuristic on (skipMulRatio != 0)
     DevBuf<float> tmpbuf;
     DevBuf<double> parts, hist, st2;
-    double *pD, *pN, *pW, *pDd, *pAd, *pS4, *pM5, *pOD, *pRO;
+    double *pD, *pN, *pW, *pDd, *pAd, *pS4, *pM5, *pOD, *pRO, *pK;
     DevBuf<FCtl> ctl;
     PinnedBuf<FCtl> hctl;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -680,14 +720,14 @@ struct Fused {
 
     void alloc() {
         auto pad = [](int64_t k) { return (k + 63) / 64 * 64; };
-        const int nv = 2 + 2 + 1 + 3 + 3 + 1 + 1 + 1 + 1, mv = 2 + 2 + 1;
+        const int nv = 2 + 2 + 1 + 3 + 3 + 1 + 1 + 1 + 1, mv = 2 + 2 + 1 + 1;
         pool.alloc((size_t)nv * pad(n) + (size_t)mv * pad(m));
         TV3_HIP(hipMemsetAsync(pool.p, 0, pool.bytes(), s));
         float *p = pool.p;
         float **nvp[] = {&U[0], &U[1], &g[0], &g[1], &d, &sx, &sy, &sz, &ox, &oy, &oz, &dts, &Atb,
                          &rcd};
         for (float **q : nvp) { *q = p; p += pad(n); }
-        float **mvp[] = {&Au[0], &Au[1], &b, &delta};
+        float **mvp[] = {&Au[0], &Au[1], &b, &delta, &lastAu};
         for (float **q : mvp) { *q = p; p += pad(m); }
         tmpbuf.alloc(n > m ? n : m);
         tmp = tmpbuf.p;
@@ -698,7 +738,7 @@ struct Fused {
         nbW = B == 8 ? (int)(n / 512) : nbS;
         nbM = 2048;
         const size_t tot = 3 * (size_t)nbS + 3 * (size_t)nbN + 7 * (size_t)nbS + (size_t)nbS +
-                           (size_t)nbN + (size_t)nbS + nbM + nbM + nbM;
+                           (size_t)nbN + (size_t)nbS + nbM + nbM + nbM + 2 * (size_t)nbM;
         parts.alloc(tot);
         double *q = parts.p;
         pD = q; q += 3 * (size_t)nbS;
@@ -710,6 +750,7 @@ struct Fused {
         pM5 = q; q += nbM;
         pOD = q; q += nbM;
         pRO = q; q += nbM;
+        pK = q; q += 2 * (size_t)nbM;
         st2.alloc(16 * S2);
         ctl.alloc(1);
         hctl.alloc(1);
@@ -732,7 +773,16 @@ struct Fused {
                            oz, U[n_], ctl.p, pD);
         launch_forward<float>(*F, U[n_], EpiRes{Au[n_], b, delta, Au[c], ctl.p, pN, {0, 0, 0}}, s,
                               &ctl.p->active);
-        launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p}, s);
+        if (skipping) {                // get_g's A^T-skip decision (:398-468)
+            hipLaunchKernelGGL(k_skip_stats, dim3(nbM), dim3(256), 0, s, m, Au[n_], lastAu, ctl.p,
+                               pK);
+            hipLaunchKernelGGL(k_skip_decide, dim3(1), dim3(1024), 0, s, ctl.p, pK, nbM, mode);
+            launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p, 1}, s);
+            hipLaunchKernelGGL(k_skip_after, dim3(nbM), dim3(256), 0, s, n, m, g[c], g[n_], Au[n_],
+                               lastAu, ctl.p);
+        } else {
+            launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p, 0}, s);
+        }
         hipLaunchKernelGGL((k_stage2<3, 3>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pD, nbS, pN, nbN,
                            st2.p);
         hipLaunchKernelGGL(k_line_search, dim3(1), dim3(64), 0, s, ctl.p, st2.p, mode);
@@ -820,6 +870,9 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     H.gam_opt = o.gam; H.gam = o.gam; H.rate_cnt = o.rate_cnt; H.mu_max = o.mu; H.beta_max = o.beta;
     H.maxit = o.maxit; H.nonneg = o.nonneg; H.iso = o.tv_mode == TV3_TV_ISO;
     H.hist_cap = o.maxit; H.active = 1; H.alpha = 1;
+    S.skipping = o.skipMulRatio != 0.0;
+    H.skip_ratio = o.skipMulRatio; H.max_rel = o.maxRelativeChange;
+    H.n_init = (int32_t)std::min<uint32_t>(o.numInitIterations, INT32_MAX);
     S.write_ctl();
 
     // initialisation (:1128-1153)
@@ -833,7 +886,9 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     TV3_HIP(hipMemcpyAsync(S.rcd, S.U[0], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     launch_forward<float>(FL, S.U[0],
                           EpiRes{S.Au[0], S.b, S.delta, S.Au[0], S.ctl.p, S.pN, {0, 0, 0}}, s);
-    launch_transposed<float>(FL, S.Au[0], EpiGrad{S.g[0], S.Atb, S.ctl.p}, s);
+    launch_transposed<float>(FL, S.Au[0], EpiGrad{S.g[0], S.Atb, S.ctl.p, 0}, s);
+    if (S.skipping)                  // the initial get_g multiplies (forced): last Au
+        TV3_HIP(hipMemcpyAsync(S.lastAu, S.Au[0], sizeof(float) * m, hipMemcpyDeviceToDevice, s));
     S.update_w(S.U[0], S.U[0], S.g[0]);
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pN, S.nbN, S.pW, S.nbW);
     TV3_HIP(hipGetLastError());
@@ -901,7 +956,8 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     // per completed iteration 1 A + 1 A^T; per inner loop one A d; per
     // fallback the rejected trial (A, A^T) and one A d; setup A^T b, A U, A^T
     info->n_matvec_N = 1 + (int64_t)H.total + H.inner_loops + 2 * (int64_t)H.fallbacks;
-    info->n_matvec_T = 2 + (int64_t)H.total + H.fallbacks;
+    // with skipping: A^T b, the initial product and the products get_g took
+    info->n_matvec_T = S.skipping ? 2 + (int64_t)H.n_T : 2 + (int64_t)H.total + H.fallbacks;
     info->status = H.nonfinite ? TV3_E_NONFINITE : TV3_OK;
     info->rel_error = 0;
     if (Ut_dev) {
@@ -919,6 +975,7 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
             hist->lam4[k] = h[4]; hist->lam5[k] = h[5]; hist->tau[k] = h[6]; hist->alpha[k] = h[7];
             hist->relchg[k] = h[8]; hist->mu[k] = h[9]; hist->beta[k] = h[10]; hist->C[k] = h[11];
             hist->cnt[k] = (int32_t)h[12]; hist->outer[k] = (int32_t)h[13];
+            if (hist->mulT) hist->mulT[k] = (int32_t)h[14];
         }
         hist->count = cnt;
     }

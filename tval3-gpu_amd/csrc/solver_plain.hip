@@ -46,6 +46,7 @@ struct PlainState {
     int64_t nN = 0, nT = 0;
     int cur_init = 0;        // A^T-skip state (:155-157)
     double cur_skip = 0.0;
+    bool last_mul = true;    // the last get_g multiplied by A^T
 
     void alloc() {
         T **nv[] = {&sx, &sy, &sz, &Up, &dU, &uup, &Ux, &Uxp, &dUx, &Uy, &Uyp, &dUy, &Uz, &Uzp,
@@ -104,6 +105,7 @@ struct PlainState {
         get_lams();
         launch_Dt<T>(dims, Vx, Vy, Vz, g2, s);
         if (mul) axpy3<T>(n, g, -1.0, Atb, g, s);
+        last_mul = mul;
         f = fval();
     }
     double get_tau(bool fst) {
@@ -268,6 +270,7 @@ void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
                 hist->tau[k] = tau; hist->alpha[k] = alpha; hist->relchg[k] = RelChg;
                 hist->mu[k] = S.mu; hist->beta[k] = S.beta; hist->C[k] = C;
                 hist->cnt[k] = cnt; hist->outer[k] = outer;
+                if (hist->mulT) hist->mulT[k] = S.last_mul;
             }
             if (!std::isfinite(S.f)) { nonfinite = true; break; }
             if (RelChg < o.tol_inn && !fst) break;

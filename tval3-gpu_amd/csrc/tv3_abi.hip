@@ -16,6 +16,10 @@ thread_local std::string g_create_err;
 
 template <typename F>
 tv3_status guard(tv3_ctx *ctx, F &&f) {
+    // a HIP error left in the thread's last-error slot by a call outside the
+    // library (its status already returned to that caller) is not this
+    // call's: clear it so launch checks below report only their own errors
+    (void)hipGetLastError();
     try {
         f();
         return TV3_OK;
@@ -96,8 +100,6 @@ void validate_opts(const tv3_opts *o) {
     TV3_REQUIRE(o, TV3_E_ARG, "opts is NULL");
     TV3_REQUIRE(o->skipMulRatio >= 0.0 && o->maxRelativeChange >= 0.0, TV3_E_ARG,
                 "skip heuristic parameters must be non-negative");
-    TV3_REQUIRE(o->skipMulRatio == 0.0 || o->solver == TV3_SOLVER_PLAIN, TV3_E_UNSUPPORTED,
-                "skipMulRatio != 0 (A^T skipping) is supported by the plain solver only");
     TV3_REQUIRE(o->maxit >= 0 && o->maxcnt >= 0, TV3_E_ARG, "maxit/maxcnt must be >= 0");
     TV3_REQUIRE(o->mu > 0 && o->mu0 > 0 && o->beta > 0 && o->beta0 > 0, TV3_E_ARG,
                 "mu, mu0, beta, beta0 must be positive");
@@ -269,6 +271,7 @@ tv3_status tv3_op_csr_create(tv3_ctx *ctx, int64_t m, int64_t n, int64_t nnz,
         set_device(ctx);
         std::unique_ptr<tv3_op> op(new tv3_op());
         op->ctx = ctx;
+        op->device = ctx->device;
         op->kind = OpKind::CSR;
         op->m = m; op->n = n;
         hipStream_t s = ctx->stream;
@@ -308,6 +311,7 @@ tv3_status tv3_op_dense_create(tv3_ctx *ctx, int64_t m, int64_t n, const void *A
         set_device(ctx);
         std::unique_ptr<tv3_op> op(new tv3_op());
         op->ctx = ctx;
+        op->device = ctx->device;
         op->kind = OpKind::DENSE;
         op->m = m; op->n = n; op->lda = lda; op->dtype = vt;
         upload_bytes(op->dense, A, (size_t)lda * n * dtype_size(vt), ctx->stream);
@@ -404,6 +408,7 @@ tv3_status tv3_op_dynamic_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_wei
         set_device(ctx);
         std::unique_ptr<tv3_op> op(new tv3_op());
         op->ctx = ctx;
+        op->device = ctx->device;
         op->kind = OpKind::DYNAMIC;
         fill_geometry(ctx, geom, mode, op.get());
         *out = op.release();
@@ -419,6 +424,7 @@ tv3_status tv3_op_hybrid_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weig
         set_device(ctx);
         std::unique_ptr<tv3_op> op(new tv3_op());
         op->ctx = ctx;
+        op->device = ctx->device;
         op->kind = OpKind::HYBRID;
         fill_geometry(ctx, geom, mode, op.get());
         SparseRows A;
@@ -439,6 +445,7 @@ tv3_status tv3_op_materialize(tv3_op *dyn, tv3_dtype vt, tv3_op **out) {
         set_device(ctx);
         std::unique_ptr<tv3_op> op(new tv3_op());
         op->ctx = ctx;
+        op->device = ctx->device;
         op->kind = OpKind::CSR;
         op->m = dyn->m; op->n = dyn->n;
         dynamic_materialize(dyn, vt, op->A);
@@ -450,8 +457,8 @@ tv3_status tv3_op_materialize(tv3_op *dyn, tv3_dtype vt, tv3_op **out) {
 
 void tv3_op_destroy(tv3_op *op) {
     if (!op) return;
-    (void)hipSetDevice(op->ctx->device);
-    (void)hipStreamSynchronize(op->ctx->stream);
+    (void)hipSetDevice(op->device);
+    (void)hipDeviceSynchronize();
     delete op;
 }
 
@@ -608,7 +615,7 @@ tv3_status tv3_solve_batched(tv3_ctx *ctx, int32_t nprob, tv3_op *const *ops,
     return guard(ctx, [&] {
         validate_opts(opts);
         TV3_REQUIRE(opts->skipMulRatio == 0.0, TV3_E_UNSUPPORTED,
-                    "skipMulRatio != 0 is supported by the plain solver only");
+                    "skipMulRatio != 0 is not supported by batched solves");
         TV3_REQUIRE(nprob > 0 && ops && dims && b_dev && U_dev && infos, TV3_E_ARG,
                     "NULL argument or empty batch");
         for (int32_t k = 0; k < nprob; ++k) {

@@ -244,6 +244,7 @@ struct tv3_ctx {
 
 struct tv3_op {
     tv3_ctx *ctx = nullptr;
+    int device = 0;                  // device of ctx (destruction does not read ctx)
     tv3::OpKind kind = tv3::OpKind::CSR;
     int64_t m = 0, n = 0;
     // CSR / HYBRID: A rows and A^T rows (the CSC)

@@ -29,9 +29,14 @@ struct FCtl {
     int32_t outer_flag;   // the outer update runs for this problem
     int32_t done;         // the problem has finished
     int32_t outer;        // outer iterations completed
+    // A^T-skip heuristic (:398-468), fused solver only
+    double skip_ratio, max_rel, cur_skip;
+    int32_t n_init, cur_init;
+    int32_t mul_T;        // the current get_g multiplies by A^T
+    int32_t n_T;          // A^T products taken by get_g inside iterations
 };
 
-constexpr int HF = 15;   // history fields per iteration
+constexpr int HF = 15;   // history fields per iteration (the last: A^T product taken)
 
 // W = shrink(DU - sigma/beta) of one voxel (3d/tval3_gpu_3d_kernels.cu:1249-1300
 // and the isotropic variant); returns the lam1 contribution
@@ -124,7 +129,8 @@ __device__ inline void iter_end(FCtl *C, const double *a, double *hist) {
         double *h = hist + (int64_t)k * HF;
         h[0] = C->f; h[1] = C->lam1; h[2] = C->lam2; h[3] = C->lam3; h[4] = C->lam4;
         h[5] = C->lam5; h[6] = C->tau; h[7] = C->alpha; h[8] = C->relchg; h[9] = C->mu;
-        h[10] = C->beta; h[11] = C->C; h[12] = C->cnt; h[13] = C->inner_loops - 1; h[14] = 0;
+        h[10] = C->beta; h[11] = C->C; h[12] = C->cnt; h[13] = C->inner_loops - 1;
+        h[14] = C->skip_ratio != 0.0 ? C->mul_T : 1;
     }
     C->total = k + 1;
     C->tau = fabs(ss / sy);                                          // BB step, :663
@@ -160,6 +166,31 @@ __device__ inline void init_step(FCtl *C, const double *a, const double *w) {
     C->total = 0; C->fst = 1; C->active = 1; C->inner_done = 0; C->need_fallback = 0;
     C->nonfinite = 0; C->sigw_old = 0; C->cnt = 0; C->alpha = 1; C->backtracks = 0;
     C->fallbacks = 0; C->inner_loops = 1; C->mode = 0; C->need_tau = 1;
+}
+
+// A^T-skip decision of get_g (:398-468) from |Au - last Au|^2 and |Au|^2:
+// products when forced (the initial and the fallback call), for the first
+// n_init calls, while the skip counter is below 1 (it then grows by the
+// ratio) and when Au moved by more than max_rel since the last product;
+// otherwise skip and count down.
+__device__ inline void skip_decide(FCtl *C, double dn, double an, int force) {
+    const double rel = sqrt(dn) / sqrt(an);
+    int mul = 1;
+    if (force) {
+        C->cur_skip = 0.0;
+    } else if (C->cur_init < C->n_init) {
+        ++C->cur_init;
+        C->cur_skip = 0.0;
+    } else if (C->cur_skip < 1.0) {
+        C->cur_skip += C->skip_ratio;
+    } else if (rel > C->max_rel) {
+        // Au moved too far: multiply
+    } else {
+        C->cur_skip -= 1.0;
+        mul = 0;
+    }
+    C->mul_T = mul;
+    C->n_T += mul;
 }
 
 // update_mlp bookkeeping and continuation (:1034-1083, :1222-1249)

@@ -59,7 +59,8 @@ HIST_D = ["f", "lam1", "lam2", "lam3", "lam4", "lam5", "tau", "alpha", "relchg",
 class History(C.Structure):
     _fields_ = [("cap", C.c_int32), ("count", C.c_int32)] + \
                [(k, C.POINTER(C.c_double)) for k in HIST_D] + \
-               [("cnt", C.POINTER(C.c_int32)), ("outer", C.POINTER(C.c_int32))]
+               [("cnt", C.POINTER(C.c_int32)), ("outer", C.POINTER(C.c_int32)),
+                ("mulT", C.POINTER(C.c_int32))]
 
 
 class Geometry(C.Structure):

@@ -333,10 +333,12 @@ def _history(cap):
     arrs = {k: np.zeros(cap, np.float64) for k in L.HIST_D}
     arrs["cnt"] = np.zeros(cap, np.int32)
     arrs["outer"] = np.zeros(cap, np.int32)
+    arrs["mulT"] = np.zeros(cap, np.int32)
     for k in L.HIST_D:
         setattr(h, k, arrs[k].ctypes.data_as(C.POINTER(C.c_double)))
     h.cnt = arrs["cnt"].ctypes.data_as(C.POINTER(C.c_int32))
     h.outer = arrs["outer"].ctypes.data_as(C.POINTER(C.c_int32))
+    h.mulT = arrs["mulT"].ctypes.data_as(C.POINTER(C.c_int32))
     return h, arrs
 
 
