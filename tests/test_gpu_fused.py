@@ -119,26 +119,40 @@ def test_fused_backtracking_and_fallback(ctx):
     assert np.allclose(h["f"][:4], hp["f"][:4], rtol=1e-5)
 
 
-def test_fused_fp16_weights_equal_error(ctx):
-    """FP16 ray weights, FP32 accumulation: the error against u_true matches
-    the FP32 path's within 1e-3 or within the spread the FP32 path itself
-    shows when b is perturbed at rounding level (three perturbations; final
-    TVAL3 errors are that sensitive on small problems, and the launch shapes,
-    timed at set-up, fix the products' summation order per run)."""
+def test_fused_fp16_weights_track_oracle(ctx):
+    """FP16 ray weights, FP32 accumulation: the fused solve with FP16 weights
+    against the FP32 oracle solving the same FP16-rounded matrix, with the
+    envelope criterion of test_fused_tracks_oracle.  (Rounding the weights to
+    FP16 changes the problem by up to 2^-11 per weight, far more than the
+    rounding-level perturbations of that criterion, so the FP16 run is not
+    compared with the FP32-weight run here; equal error of the two is a
+    criterion of the full-size benchmark.)"""
     dims = (16, 16, 16)
     u, A16, A32, oop, b = setup(ctx, dims, 64, 128, seed=3, dtype="f16")
+    rp, c, v = A16.download_csr()
+    assert np.array_equal(v, v.astype(np.float16).astype(np.float64))
+    oop16 = orc.csr_op(A16.m, A16.n, rp, c, v)
     o = api.default_opts(**opts(maxit=80), solver="fused")
-    U16, i16, _ = api.tval3_gpu_3d(ctx, A16, b, dims, o, Ut=u)
-    U32, i32, _ = api.tval3_gpu_3d(ctx, A32, b, dims, o, Ut=u)
-    errs = []
-    for seed in (0, 1, 2):
-        xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=b.shape)
+    U16, i16, h16 = api.tval3_gpu_3d(ctx, A16, b, dims, o, Ut=u)
+    oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=80, tv_iso=1)
+    Uo, io, ho, _ = orc.solve(oop16, dims, b, oo, Ut=u, dtype=np.float32)
+    runs = []
+    for s_ in (0, 1, 2):
+        xi = np.random.default_rng(s_).choice([-1.0, 1.0], size=b.shape)
         bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
-        errs.append(api.tval3_gpu_3d(ctx, A32, bp, dims, o, Ut=u)[1]["rel_error"])
-    spread = max(abs(e - i32["rel_error"]) for e in errs)
-    print(f"fp16 err {i16['rel_error']:.4e} fp32 err {i32['rel_error']:.4e} "
-          f"fp32 perturbed {', '.join(f'{e:.4e}' for e in errs)}")
-    assert abs(i16["rel_error"] - i32["rel_error"]) <= max(1e-3, 3 * spread)
+        runs.append(orc.solve(oop16, dims, bp, oo, Ut=u, dtype=np.float32))
+    k = min([first_divergence(h16, ho)] + [first_divergence(r[2], ho) for r in runs])
+    dev_f = np.abs(h16["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k])
+    env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k]))
+                  for r in runs], axis=0)
+    spread = max(abs(r[1].rel_error - io.rel_error) for r in runs)
+    print(f"fp16 weights: iters fused {i16['total_iters']} oracle {io.total_iters}; decisions "
+          f"agree for {k}; err fused {i16['rel_error']:.4e} oracle {io.rel_error:.4e} "
+          f"(spread {spread:.1e}); max f dev {dev_f.max():.1e} envelope {env.max():.1e}")
+    assert k >= 10
+    assert (dev_f[:4] < 1e-5).all()
+    assert (dev_f <= 30 * env + 1e-6).all()
+    assert abs(i16["rel_error"] - io.rel_error) <= max(3 * spread, 1e-3)
 
 
 def test_fused_device_inputs_and_warm_start(ctx):

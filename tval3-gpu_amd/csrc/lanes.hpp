@@ -71,4 +71,18 @@ __device__ __forceinline__ double wave_sum(double v) {
     return (lane_f64(v, 0) + lane_f64(v, 16)) + (lane_f64(v, 32) + lane_f64(v, 48));
 }
 
+// Keeps loads issued ahead of the branch that follows: an empty asm taking
+// their values on the path that otherwise does not use them, so the compiler
+// cannot sink them past the branch (the other path waits for them only if it
+// is taken).
+template <typename T>
+__device__ __forceinline__ void keep_one(const T &v) {
+    asm volatile("" ::"v"(v));
+}
+
+template <typename... T>
+__device__ __forceinline__ void keep_issued(const T &...v) {
+    (keep_one(v), ...);
+}
+
 }  // namespace tv3

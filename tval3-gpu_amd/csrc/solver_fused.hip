@@ -77,12 +77,11 @@ static Groups groups_of(const Lay &L) {
 }
 
 template <int B>
-__device__ __forceinline__ bool voxel_of(const Lay &L, const Groups &G, int64_t grp, int &y, int &x,
+__device__ __forceinline__ bool voxel_of(const Lay &L, const Groups &G, int grp, int &y, int &x,
                                          int &z) {
     const int t = threadIdx.x;
-    const int gy = (int)(grp % G.gy);
-    const int64_t rest = grp / G.gy;
-    const int gx = (int)(rest % G.gx), gz = (int)(rest / G.gx);
+    const int gy = grp % G.gy, rest = grp / G.gy;
+    const int gx = rest % G.gx, gz = rest / G.gx;
     if (B == 8) {
         const int lane = t & 63;
         y = gy * 8 + ((t >> 4) & 4) + (lane & 3);
@@ -109,12 +108,13 @@ static unsigned stencil_grid(const Groups &G) {
     return (unsigned)(G.count < kMaxStencilGrid ? G.count : kMaxStencilGrid);
 }
 
-// range of groups of this workgroup
-__device__ __forceinline__ void group_range(int64_t count, int64_t &g0, int64_t &g1) {
-    const int64_t Gd = gridDim.x, chunk = (count + Gd - 1) / Gd;
-    const int64_t w = xcd_remap(blockIdx.x, (int)Gd);
+// range of groups of this workgroup (group counts below 2^31, checked at
+// set-up: 32-bit index arithmetic)
+__device__ __forceinline__ void group_range(int64_t count, int &g0, int &g1) {
+    const int Gd = gridDim.x, cnt = (int)count, chunk = (cnt + Gd - 1) / Gd;
+    const int w = xcd_remap(blockIdx.x, Gd);
     g0 = w * chunk;
-    g1 = min(count, g0 + chunk);
+    g1 = min(cnt, g0 + chunk);
 }
 
 __device__ __forceinline__ int nxt(int v, int n) { return v + 1 == n ? 0 : v + 1; }
@@ -133,30 +133,42 @@ __global__ __launch_bounds__(256) void k_descend(Lay L, Groups G, const float *_
                                                  const float *__restrict__ wsz,
                                                  float *__restrict__ Un, const FCtl *C,
                                                  double *part) {
-    if (!C->active) return;
+    // control scalars and the data loads before the graph's stop flag is
+    // tested (the loads are in flight while it arrives)
+    const int active = C->active;
     double acc[3] = {0, 0, 0};
     const float tau = (float)C->tau, beta = (float)C->beta, ib = 1.f / beta;
     const bool nn = C->nonneg, iso = C->iso;
     const bool old = C->sigw_old;
-    int64_t g0, g1;
+    int g0, g1;
     group_range(G.count, g0, g1);
-    for (int64_t grp = g0; grp < g1; ++grp) {
+    // with B = 8 every voxel of a group is valid and the grid has one group
+    // per workgroup: the flag is tested in the first pass, after its loads
+    // (tied to grp == g0 so that it stays there); other B test it first
+    if constexpr (B != 8) {
+        if (!active) return;
+    }
+    for (int grp = g0; grp < g1; ++grp) {
         int y, x, z;
         if (!voxel_of<B>(L, G, grp, y, x, z)) continue;
         const int64_t i = vidx<B>(L, y, x, z);
         const int64_t iy = vidx<B>(L, nxt(y, L.p), x, z), ix = vidx<B>(L, y, nxt(x, L.q), z),
                       iz = vidx<B>(L, y, x, nxt(z, L.r));
         const float u0 = Uc[i], uy = Uc[iy], ux = Uc[ix], uz = Uc[iz];
+        const float d0 = d[i], dy = d[iy], dx = d[ix], dz = d[iz];
+        const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
+        const float w_x = old ? wsx[i] : s_x, w_y = old ? wsy[i] : s_y, w_z = old ? wsz[i] : s_z;
+        if (B == 8 && grp == g0 && !active) {
+            keep_issued(u0, uy, ux, uz, d0, dy, dx, dz, s_x, s_y, s_z, w_x, w_y, w_z);
+            return;
+        }
         auto step = [&](float u, float dv) {
             float t = u - tau * dv;
             return (nn && t < 0.f) ? 0.f : t;
         };
-        const float t0 = step(u0, d[i]), ty = step(uy, d[iy]), tx = step(ux, d[ix]),
-                    tz = step(uz, d[iz]);
-        const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
+        const float t0 = step(u0, d0), ty = step(uy, dy), tx = step(ux, dx), tz = step(uz, dz);
         float wx, wy, wz;
-        shrink3(ux - u0, uy - u0, uz - u0, old ? wsx[i] : s_x, old ? wsy[i] : s_y,
-                old ? wsz[i] : s_z, beta, ib, iso, wx, wy, wz);
+        shrink3(ux - u0, uy - u0, uz - u0, w_x, w_y, w_z, beta, ib, iso, wx, wy, wz);
         const float v0x = (ux - u0) - wx, v0y = (uy - u0) - wy, v0z = (uz - u0) - wz;
         const float v1x = (tx - t0) - wx, v1y = (ty - t0) - wy, v1z = (tz - t0) - wz;
         acc[0] += (double)v1x * v1x + (double)v1y * v1y + (double)v1z * v1z;
@@ -164,6 +176,7 @@ __global__ __launch_bounds__(256) void k_descend(Lay L, Groups G, const float *_
         acc[2] += (double)v0x * v1x + (double)v0y * v1y + (double)v0z * v1z;
         Un[i] = t0;
     }
+    if (!active) return;
     block_sum_store<3>(acc, part, gridDim.x);
 }
 
@@ -182,9 +195,9 @@ __global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
     const bool iso = C->iso;
-    int64_t g0, g1;
+    int g0, g1;
     group_range(G.count, g0, g1);
-    for (int64_t grp = g0; grp < g1; ++grp) {
+    for (int grp = g0; grp < g1; ++grp) {
         int y, x, z;
         if (!voxel_of<B>(L, G, grp, y, x, z)) continue;
         const int yp = nxt(y, L.p), ym = prv(y, L.p), xp = nxt(x, L.q), xm = prv(x, L.q),
@@ -265,7 +278,11 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__res
                                                        double *part) {
     constexpr int NBOX = kTileBox * kTileBox * kTileBox;
     __shared__ float su[NBOX], svx[NBOX], svy[NBOX], svz[NBOX];
-    if (!C->active) return;
+    // control scalars, then every data load, then the graph's stop flag: the
+    // loads are in flight while the flag arrives
+    const int active = C->active;
+    const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
+    const bool iso = C->iso;
     const int t = threadIdx.x;
     const int tile = blockIdx.x;                       // tiles ordered y, x, z
     const int ty = tile % L.nby, rest = tile / L.nby;
@@ -307,12 +324,14 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__res
     const float sx0 = sx[i], sy0 = sy[i], sz0 = sz[i];
     const float sx1 = sx[gi1], sy1 = sy[gi1], sz1 = sz[gi1];
     const float gv = g[i], dv = dts[i], ucv = Uc[i], dold = d[i];
+    if (!active) {
+        keep_issued(ub0, ub1, sx0, sy0, sz0, sx1, sy1, sz1, gv, dv, ucv, dold);
+        return;
+    }
     su[t] = ub0;
     if (t + 512 < NBOX) su[t + 512] = ub1;
     __syncthreads();
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
-    const bool iso = C->iso;
     // V at the tile voxel and (t < 192) at a voxel of the lower faces
     // y = -1, x = -1, z = -1, where D^T reads it
     auto vat = [&](int a, int b, int c, float s_x, float s_y, float s_z, bool tile_voxel) {
@@ -355,9 +374,9 @@ template <int B>
 __global__ __launch_bounds__(256) void k_dd_norm(Lay L, Groups G, const float *__restrict__ d,
                                                  double *part) {
     double acc[1] = {0};
-    int64_t g0, g1;
+    int g0, g1;
     group_range(G.count, g0, g1);
-    for (int64_t grp = g0; grp < g1; ++grp) {
+    for (int grp = g0; grp < g1; ++grp) {
         int y, x, z;
         if (!voxel_of<B>(L, G, grp, y, x, z)) continue;
         const int64_t i = vidx<B>(L, y, x, z);
@@ -380,9 +399,9 @@ __global__ __launch_bounds__(256) void k_mlp_sigma(Lay L, Groups G, const float 
                                                    const FCtl *C, double *part) {
     double acc[1] = {0};
     const float beta = (float)C->beta, ib = 1.f / beta;
-    int64_t g0, g1;
+    int g0, g1;
     group_range(G.count, g0, g1);
-    for (int64_t grp = g0; grp < g1; ++grp) {
+    for (int grp = g0; grp < g1; ++grp) {
         int y, x, z;
         if (!voxel_of<B>(L, G, grp, y, x, z)) continue;
         const int64_t i = vidx<B>(L, y, x, z);
@@ -732,6 +751,8 @@ struct Fused {
         tmpbuf.alloc(n > m ? n : m);
         tmp = tmpbuf.p;
         G = groups_of<B>(L);
+        TV3_REQUIRE(G.count < (int64_t(1) << 31), TV3_E_SHAPE,
+                    "volume too large for the fused solver's voxel groups");
         nbS = (int)stencil_grid(G);
         nbN = product_blocks(*F);
         // update_W: one workgroup per 8x8x8 tile with B = 8 (nbW <= nbS)
