@@ -122,8 +122,8 @@ def test_batch_tracks_oracle(ctx, batch):
     for k, p in enumerate(batch):
         Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
         # the oracle's own sensitivity: three rounding-level perturbations of b
-        # (the first sets the trajectory envelope; the spread of the final
-        # error is the largest of the three)
+        # (trajectory envelope and spread of the final error: the largest of
+        # the three; one perturbation can underestimate the drift tenfold)
         runs = []
         for seed in (0, 1, 2):
             xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=p["b"].shape)
@@ -131,9 +131,10 @@ def test_batch_tracks_oracle(ctx, batch):
             runs.append(orc.solve(p["oop"], p["dims"], bp, oo, Ut=p["u"], dtype=np.float32))
         Up, ip, hpp, _ = runs[0]
         spread = max(abs(r[1].rel_error - io.rel_error) for r in runs)
-        kk = min(first_divergence(hb[k], ho), first_divergence(hpp, ho))
+        kk = min([first_divergence(hb[k], ho)] + [first_divergence(r[2], ho) for r in runs])
         dev = np.abs(hb[k]["f"][:kk] - ho["f"][:kk]) / np.abs(ho["f"][:kk])
-        env = np.maximum.accumulate(np.abs(hpp["f"][:kk] - ho["f"][:kk]) / np.abs(ho["f"][:kk]))
+        env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:kk] - ho["f"][:kk])
+                                            / np.abs(ho["f"][:kk])) for r in runs], axis=0)
         Ug = Ub[k].cpu().numpy()
         print(f"{p['dims']}: iters batch {ib[k]['total_iters']} oracle {io.total_iters}; "
               f"decisions agree for {kk}; err batch {ib[k]['rel_error']:.4e} oracle "

@@ -237,3 +237,53 @@ def test_fused_skip_rule_that_never_skips_changes_nothing(ctx):
     assert np.array_equal(h0["f"], h1["f"])
     assert i0["total_iters"] == i1["total_iters"]
     assert i0["n_matvec_T"] == i1["n_matvec_T"]
+
+
+def test_fused_tiny_gradients_stay_finite(ctx):
+    """Regression: the isotropic shrinkage takes 1/|b| from the hardware
+    reciprocal square root, which flushes denormal inputs to zero.  A warm
+    start whose neighbour differences are ~1e-21 (|b|^2 denormal) must give
+    W = 0 there, not Inf/NaN; the solve then proceeds as from a zero start."""
+    dims = (16, 16, 16)
+    u, A, A32, oop, b = setup(ctx, dims, 24, 64, seed=13)
+    U0 = (1e-21 * np.random.default_rng(5).standard_normal(int(np.prod(dims)))).astype(np.float32)
+    assert (np.abs(np.diff(U0)) ** 2 < np.finfo(np.float32).tiny).mean() > 0.5
+    for mode in ("iso", "aniso"):
+        o = api.default_opts(**opts(maxit=20, tv_mode=mode), solver="fused")
+        U, info, h = api.tval3_gpu_3d(ctx, A, b, dims, o, U0=U0, Ut=u)
+        assert info["status"] == 0 and info["total_iters"] > 0
+        assert np.isfinite(U).all() and np.isfinite(h["f"]).all()
+
+
+@pytest.mark.parametrize("code_N,code_T,panels", [(8, 8, 0), (-1024, -1024, 0), (-1024, -1024, 1),
+                                                  (-1024, -1024, 2), (-2048, -2048, 3)])
+def test_fused_every_product_form_tracks_oracle(ctx, code_N, code_T, panels):
+    """The fused solve with each form of its products forced (row groups, row
+    blocks, the panel form of A x -- whose second pass stores the line-search
+    partial sums per workgroup -- and of A^T r): rounding-level agreement
+    with the FP32 oracle at the start and the oracle's own perturbation
+    envelope after (regression: a pass-2 grid that differed from the count
+    its partial sums are read with gave wrong line-search sums)."""
+    dims = (16, 16, 16)
+    u, A, A32, oop, b = setup(ctx, dims, 24, 64, seed=17)
+    info = A.layout_force(dims, code_N, code_T, 16384, panels)
+    assert info["panels"] == panels
+    o = opts(maxit=30)
+    Uf, inf_, hf = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**o, solver="fused"), Ut=u)
+    oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=30, tv_iso=1)
+    Uo, io, ho, _ = orc.solve(oop, dims, b, oo, Ut=u, dtype=np.float32)
+    runs = []
+    for s_ in (0, 1, 2):
+        xi = np.random.default_rng(s_).choice([-1.0, 1.0], size=b.shape)
+        bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+        runs.append(orc.solve(oop, dims, bp, oo, Ut=u, dtype=np.float32))
+    k = min([first_divergence(hf, ho)] + [first_divergence(r[2], ho) for r in runs])
+    dev_f = np.abs(hf["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k])
+    env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k]))
+                  for r in runs], axis=0)
+    print(f"forms {code_N},{code_T},{panels}: decisions agree for {k}; max f dev "
+          f"{dev_f.max():.1e} envelope {env.max():.1e}; err {inf_['rel_error']:.4e} oracle "
+          f"{io.rel_error:.4e}")
+    assert k >= 10
+    assert (dev_f[:4] < 1e-5).all()
+    assert (dev_f <= 30 * env + 1e-6).all()

@@ -245,14 +245,24 @@ void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s
     }
 }
 
+constexpr int kSegSumNLB = 2;   // logical blocks (256 / L rays each) per pass-2 workgroup
+
+// Logical blocks and workgroups of pass 2 of A x.  The workgroup count is
+// also the number of block partials its epilogue stores: every reader of
+// those partials takes it from here.
+inline int64_t seg_sum_logical(const SegSum &R) { return (R.rows * R.lanes + 255) / 256; }
+inline int seg_sum_blocks(const SegSum &R) {
+    const int64_t ng = (seg_sum_logical(R) + kSegSumNLB - 1) / kSegSumNLB;
+    TV3_REQUIRE(ng < ((int64_t)1 << 31), TV3_E_SHAPE, "too many rays for one launch");
+    return (int)ng;
+}
+
 template <typename Epi>
 void launch_seg_sum(const SegSum &R, const Epi &epi, hipStream_t s) {
-    constexpr int NLB = 2;   // logical blocks per workgroup
+    constexpr int NLB = kSegSumNLB;
     const int L = R.lanes;
-    const int64_t nl = (R.rows * L + 255) / 256;
-    const int64_t ng = (nl + NLB - 1) / NLB;
-    TV3_REQUIRE(ng < ((int64_t)1 << 31), TV3_E_SHAPE, "too many rays for one launch");
-    const unsigned g = (unsigned)ng;
+    const int64_t nl = seg_sum_logical(R);
+    const unsigned g = (unsigned)seg_sum_blocks(R);
     if (g == 0) return;
 #define TV3_SCASE(LL)                                                                     \
     case LL:                                                                              \

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void k_b_descend(const BDims *__restrict__ D,
         const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
         float wx, wy, wz;
         shrink3(ux - u0, uy - u0, uz - u0, old ? wsx[i] : s_x, old ? wsy[i] : s_y,
-                old ? wsz[i] : s_z, beta, ib, iso, wx, wy, wz);
+                old ? wsz[i] : s_z, ib, iso, wx, wy, wz);
         const float v0x = (ux - u0) - wx, v0y = (uy - u0) - wy, v0z = (uz - u0) - wz;
         const float v1x = (tx - t0) - wx, v1y = (ty - t0) - wy, v1z = (tz - t0) - wz;
         acc[0] += (double)v1x * v1x + (double)v1y * v1y + (double)v1z * v1z;
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
         const float uyp = U[v.at(yp, x, z)], uxp = U[v.at(y, xp, z)], uzp = U[v.at(y, x, zp)];
         float wx, wy, wz;
         const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
-        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, beta, ib, iso, wx,
+        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, ib, iso, wx,
                                  wy, wz);
         const float vx = (uxp - u0) - wx, vy = (uyp - u0) - wy, vz = (uzp - u0) - wz;
         float vyb, vxb, vzb;
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
             const float uj = U[j];
             const float dyj = u0 - uj, dxj = U[v.at(ym, xp, z)] - uj, dzj = U[v.at(ym, x, zp)] - uj;
             float a, b, cc;
-            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, cc);
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, cc);
             vyb = dyj - b;
         }
         {
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
             const float uj = U[j];
             const float dxj = u0 - uj, dyj = U[v.at(yp, xm, z)] - uj, dzj = U[v.at(y, xm, zp)] - uj;
             float a, b, cc;
-            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, cc);
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, cc);
             vxb = dxj - a;
         }
         {
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
             const float uj = U[j];
             const float dzj = u0 - uj, dxj = U[v.at(y, xp, zm)] - uj, dyj = U[v.at(yp, x, zm)] - uj;
             float a, b, cc;
-            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, cc);
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, cc);
             vzb = dzj - cc;
         }
         const float g2 = vyb - vy + vxb - vx + vzb - vz;            // D^T V, :501
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_b_mlp_sigma(const BDims *__restrict__ D
                     dz = U[v.at(v.y, v.x, nxt(v.z, v.r))] - u0;
         const float a = sx[i], b = sy[i], cc = sz[i];
         float wx, wy, wz;
-        shrink3(dx, dy, dz, a, b, cc, beta, ib, c.iso, wx, wy, wz);
+        shrink3(dx, dy, dz, a, b, cc, ib, c.iso, wx, wy, wz);
         const float vx = dx - wx, vy = dy - wy, vz = dz - wz;
         const float na = a - beta * vx, nb = b - beta * vy, nc = cc - beta * vz;
         ox[i] = a; oy[i] = b; oz[i] = cc;

@@ -34,6 +34,7 @@
 //  * vectors are FP32 in the layout of layout.hip, products accumulate in
 //    FP32, all inner products / norms accumulate in FP64 (block partials
 //    reduced in a fixed order), all scalars are FP64.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -47,6 +48,7 @@ namespace {
 
 struct Lay {
     int p, q, r, nby, nbx;
+    int nbz;             // tiles along z (B = 8)
     int64_t pq;
 };
 
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(256) void k_descend(Lay L, Groups G, const float *_
         };
         const float t0 = step(u0, d0), ty = step(uy, dy), tx = step(ux, dx), tz = step(uz, dz);
         float wx, wy, wz;
-        shrink3(ux - u0, uy - u0, uz - u0, w_x, w_y, w_z, beta, ib, iso, wx, wy, wz);
+        shrink3(ux - u0, uy - u0, uz - u0, w_x, w_y, w_z, ib, iso, wx, wy, wz);
         const float v0x = (ux - u0) - wx, v0y = (uy - u0) - wy, v0z = (uz - u0) - wz;
         const float v1x = (tx - t0) - wx, v1y = (ty - t0) - wy, v1z = (tz - t0) - wz;
         acc[0] += (double)v1x * v1x + (double)v1y * v1y + (double)v1z * v1z;
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *
         // V at i
         float wx, wy, wz;
         const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
-        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, beta, ib, iso, wx,
+        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, ib, iso, wx,
                                  wy, wz);
         const float vx = (uxp - u0) - wx, vy = (uyp - u0) - wy, vz = (uzp - u0) - wz;
         // V_a at i - e_a (needs the full DU there)
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *
             const float dyj = u0 - uj, dxj = U[vidx<B>(L, ym, xp, z)] - uj,
                         dzj = U[vidx<B>(L, ym, x, zp)] - uj;
             float a, b, c;
-            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, c);
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, c);
             vyb = dyj - b;
         }
         {
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *
             const float dxj = u0 - uj, dyj = U[vidx<B>(L, yp, xm, z)] - uj,
                         dzj = U[vidx<B>(L, y, xm, zp)] - uj;
             float a, b, c;
-            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, c);
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, c);
             vxb = dxj - a;
         }
         {
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *
             const float dzj = u0 - uj, dxj = U[vidx<B>(L, y, xp, zm)] - uj,
                         dyj = U[vidx<B>(L, yp, x, zm)] - uj;
             float a, b, c;
-            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], beta, ib, iso, a, b, c);
+            shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, c);
             vzb = dzj - c;
         }
         const float g2 = vyb - vy + vxb - vx + vzb - vz;            // D^T V, :501
@@ -267,7 +269,8 @@ __device__ __forceinline__ int box_slot(int a, int b, int c) {
     return ((c + 1) * kTileBox + (b + 1)) * kTileBox + (a + 1);
 }
 
-__global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__restrict__ U,
+__global__ __launch_bounds__(512) void k_update_w_tile(Lay L, int ntiles, int tpw,
+                                                       const float *__restrict__ U,
                                                        const float *__restrict__ Uc,
                                                        const float *__restrict__ sx,
                                                        const float *__restrict__ sy,
@@ -278,24 +281,17 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__res
                                                        double *part) {
     constexpr int NBOX = kTileBox * kTileBox * kTileBox;
     __shared__ float su[NBOX], svx[NBOX], svy[NBOX], svz[NBOX];
-    // control scalars, then every data load, then the graph's stop flag: the
-    // loads are in flight while the flag arrives
+    // control scalars, then the first tile's data loads, then the graph's
+    // stop flag: the loads are in flight while the flag arrives
     const int active = C->active;
     const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
     const bool iso = C->iso;
     const int t = threadIdx.x;
-    const int tile = blockIdx.x;                       // tiles ordered y, x, z
-    const int ty = tile % L.nby, rest = tile / L.nby;
-    const int tx = rest % L.nbx, tz = rest / L.nbx;
-    const int y0 = ty * 8, x0 = tx * 8, z0 = tz * 8;
-    const int64_t base = (int64_t)tile << 9;
-    auto gidx = [&](int a, int b, int c) {
-        int y = y0 + a, x = x0 + b, z = z0 + c;   // periodic wrap by at most one voxel
-        y = y < 0 ? y + L.p : (y >= L.p ? y - L.p : y);
-        x = x < 0 ? x + L.q : (x >= L.q ? x - L.q : x);
-        z = z < 0 ? z + L.r : (z >= L.r ? z - L.r : z);
-        return brick_index(8, L.pq, L.p, L.nby, L.nbx, y, x, z);
-    };
+    // a workgroup owns tpw consecutive tiles (ordered y, x, z: consecutive
+    // tiles are y neighbours, one tile's upper shell is the next one's face),
+    // consecutive workgroups on one XCD
+    const int tile0 = xcd_remap(blockIdx.x, gridDim.x) * tpw;
+    const int tile1 = min(ntiles, tile0 + tpw);
     // tile voxel q (internal order) -> offsets (a, b, c) = (y, x, z) in the tile
     auto tile_pos = [](int q, int &a, int &b, int &c) {
         const int sub = q >> 6, w = q & 63;
@@ -303,44 +299,66 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__res
         b = (((sub >> 1) & 1) << 2) | ((w >> 2) & 3);
         c = ((sub >> 2) << 2) | (w >> 4);
     };
-    // every global load of the workgroup first (one memory round trip): the
-    // U box ([-1, 8]^3, two values per thread), sigma at the tile voxel q0 = t
-    // and at the lower-face voxel q1 = 512 + t (t < 192), and g, dts, Uc, d
-    // of voxel t; indices are clamped instead of guarded
-    auto box_g = [&](int k) {
-        const int a = k % kTileBox - 1, b = (k / kTileBox) % kTileBox - 1, c = k / 100 - 1;
-        return gidx(a, b, c);
-    };
+    // per thread, for every tile: the U box ([-1, 8]^3) entries t and
+    // k1 = t + 512, sigma at the tile voxel q0 = t and at the lower-face voxel
+    // q1 = 512 + t (t < 192), and g, dts, Uc, d of voxel t; indices clamped
+    // instead of guarded.  A box or face voxel (a, b, c) lies in the tile
+    // shifted by (a >> 3, b >> 3, c >> 3) in {-1, 0, 1}^3 (periodic wrap of
+    // the tile coordinates: every dimension is a multiple of 8) at the
+    // in-tile position of (a & 7, b & 7, c & 7), both fixed per thread.
     const int k1 = t + 512 < NBOX ? t + 512 : NBOX - 1;
-    const float ub0 = U[box_g(t)], ub1 = U[box_g(k1)];
     int a0, b0, c0;
     tile_pos(t, a0, b0, c0);
-    const int64_t i = base + t;
     const int q1 = t < 192 ? t : 191;                 // face item 512 + q1
     const int f = q1 >> 6, j = q1 & 63, fu = j & 7, fv = j >> 3;
     const int a1 = f == 0 ? -1 : fu, b1 = f == 0 ? fu : (f == 1 ? -1 : fv),
               c1 = f == 2 ? -1 : fv;
-    const int64_t gi1 = gidx(a1, b1, c1);
-    const float sx0 = sx[i], sy0 = sy[i], sz0 = sz[i];
-    const float sx1 = sx[gi1], sy1 = sy[gi1], sz1 = sz[gi1];
-    const float gv = g[i], dv = dts[i], ucv = Uc[i], dold = d[i];
-    if (!active) {
-        keep_issued(ub0, ub1, sx0, sy0, sz0, sx1, sy1, sz1, gv, dv, ucv, dold);
-        return;
+    struct Where { int da, db, dc, off; };
+    auto where = [](int a, int b, int c) {
+        const int ai = a & 7, bi = b & 7, ci = c & 7;
+        const int sub = ((ci >> 2) << 2) | ((bi >> 2) << 1) | (ai >> 2);
+        return Where{a >> 3, b >> 3, c >> 3,
+                     (sub << 6) + ((ci & 3) << 4) + ((bi & 3) << 2) + (ai & 3)};
+    };
+    auto box_where = [&](int k) {
+        return where(k % kTileBox - 1, (k / kTileBox) % kTileBox - 1, k / 100 - 1);
+    };
+    const Where w0 = box_where(t), w1 = box_where(k1), wf = where(a1, b1, c1);
+    struct Loads { float ub[2], s0[3], s1[3], gv, dv, ucv, dold; };
+    auto load = [&](int tile, Loads &r) {
+        const int ty = tile % L.nby, rest = tile / L.nby;
+        const int tx = rest % L.nbx, tz = rest / L.nbx;
+        auto gidx = [&](const Where &w) {
+            int y = ty + w.da, x = tx + w.db, z = tz + w.dc;
+            y = y < 0 ? y + L.nby : (y >= L.nby ? y - L.nby : y);
+            x = x < 0 ? x + L.nbx : (x >= L.nbx ? x - L.nbx : x);
+            z = z < 0 ? z + L.nbz : (z >= L.nbz ? z - L.nbz : z);
+            return ((int64_t)((z * L.nbx + x) * L.nby + y) << 9) + w.off;
+        };
+        r.ub[0] = U[gidx(w0)];
+        r.ub[1] = U[gidx(w1)];
+        const int64_t i = ((int64_t)tile << 9) + t, gi1 = gidx(wf);
+        r.s0[0] = sx[i]; r.s0[1] = sy[i]; r.s0[2] = sz[i];
+        r.s1[0] = sx[gi1]; r.s1[1] = sy[gi1]; r.s1[2] = sz[gi1];
+        r.gv = g[i]; r.dv = dts[i]; r.ucv = Uc[i]; r.dold = d[i];
+    };
+    Loads cur;
+    load(tile0 < tile1 ? tile0 : ntiles - 1, cur);
+    if (!active || tile0 >= tile1) {
+        keep_issued(cur.ub[0], cur.ub[1], cur.s0[0], cur.s0[1], cur.s0[2], cur.s1[0], cur.s1[1],
+                    cur.s1[2], cur.gv, cur.dv, cur.ucv, cur.dold);
+        if (!active) return;
     }
-    su[t] = ub0;
-    if (t + 512 < NBOX) su[t + 512] = ub1;
-    __syncthreads();
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    // V at the tile voxel and (t < 192) at a voxel of the lower faces
-    // y = -1, x = -1, z = -1, where D^T reads it
-    auto vat = [&](int a, int b, int c, float s_x, float s_y, float s_z, bool tile_voxel) {
+    // V at a voxel of the tile or (t < 192) of its lower faces y = -1,
+    // x = -1, z = -1, where D^T reads it
+    auto vat = [&](int a, int b, int c, const float *s, bool tile_voxel) {
         const int sl = box_slot(a, b, c);
         const float u0 = su[sl], uyp = su[sl + 1], uxp = su[sl + kTileBox],
                     uzp = su[sl + kTileBox * kTileBox];
         float wx, wy, wz;
-        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, beta, ib, iso, wx,
-                                 wy, wz);
+        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s[0], s[1], s[2], ib, iso,
+                                 wx, wy, wz);
         const float vx = (uxp - u0) - wx, vy = (uyp - u0) - wy, vz = (uzp - u0) - wz;
         svx[sl] = vx;
         svy[sl] = vy;
@@ -348,23 +366,36 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, const float *__res
         if (tile_voxel) {
             acc[0] += l1;
             acc[1] += (double)vx * vx + (double)vy * vy + (double)vz * vz;
-            acc[2] += (double)vx * s_x + (double)vy * s_y + (double)vz * s_z;
+            acc[2] += (double)vx * s[0] + (double)vy * s[1] + (double)vz * s[2];
         }
     };
-    vat(a0, b0, c0, sx0, sy0, sz0, true);
-    if (t < 192) vat(a1, b1, c1, sx1, sy1, sz1, false);
-    __syncthreads();
-    {
-        const int sl = box_slot(a0, b0, c0);
-        const float g2 = svy[sl - 1] - svy[sl] + svx[sl - kTileBox] - svx[sl] +
-                         svz[sl - kTileBox * kTileBox] - svz[sl];          // D^T V, :501
-        const float dn = mdb * gv + g2 + dv;                                // :915-918
-        const float du = su[sl] - ucv;
-        acc[3] += (double)du * du;
-        acc[4] += (double)du * (double)(dn - dold);
-        acc[5] += (double)ucv * ucv;
-        acc[6] += (double)dn * dn;
-        d[i] = dn;
+    for (int tile = tile0; tile < tile1; ++tile) {
+        su[t] = cur.ub[0];
+        if (t + 512 < NBOX) su[t + 512] = cur.ub[1];
+        // the next tile's loads fly while this one is computed
+        Loads nxt;
+        const bool more = tile + 1 < tile1;
+        if (more) load(tile + 1, nxt);
+        __syncthreads();
+        vat(a0, b0, c0, cur.s0, true);
+        if (t < 192) vat(a1, b1, c1, cur.s1, false);
+        __syncthreads();
+        {
+            const int sl = box_slot(a0, b0, c0);
+            const float g2 = svy[sl - 1] - svy[sl] + svx[sl - kTileBox] - svx[sl] +
+                             svz[sl - kTileBox * kTileBox] - svz[sl];      // D^T V, :501
+            const float dn = mdb * cur.gv + g2 + cur.dv;                    // :915-918
+            const float du = su[sl] - cur.ucv;
+            acc[3] += (double)du * du;
+            acc[4] += (double)du * (double)(dn - cur.dold);
+            acc[5] += (double)cur.ucv * cur.ucv;
+            acc[6] += (double)dn * dn;
+            d[((int64_t)tile << 9) + t] = dn;
+        }
+        if (more) {
+            __syncthreads();        // su and sv are rewritten for the next tile
+            cur = nxt;
+        }
     }
     block_sum_store<7, 8>(acc, part, gridDim.x);
 }
@@ -411,7 +442,7 @@ __global__ __launch_bounds__(256) void k_mlp_sigma(Lay L, Groups G, const float 
                     dz = U[vidx<B>(L, y, x, nxt(z, L.r))] - u0;
         const float a = sx[i], b = sy[i], c = sz[i];
         float wx, wy, wz;
-        shrink3(dx, dy, dz, a, b, c, beta, ib, C->iso, wx, wy, wz);
+        shrink3(dx, dy, dz, a, b, c, ib, C->iso, wx, wy, wz);
         const float vx = dx - wx, vy = dy - wy, vz = dz - wz;
         const float na = a - beta * vx, nb = b - beta * vy, nc = c - beta * vz;
         ox[i] = a; oy[i] = b; oz[i] = c;
@@ -708,7 +739,7 @@ __global__ void k_set_dd(FCtl *C, const double *p, int nb) {
 // ------------------------------------------------------------------ host side
 // workgroups of the launch that runs the A x epilogue (block partial sums)
 int product_blocks(const FusedLayout &F) {
-    if (F.panels_A) return (int)((F.RS.rows * F.RS.lanes + 255) / 256);
+    if (F.panels_A) return seg_sum_blocks(F.RS);
     const SparseRows &S = F.A;
     if (S.blk_nnz) return (int)S.nblk;
     const int L = S.lanes ? S.lanes : 32;
@@ -724,6 +755,7 @@ struct Fused {
     Lay L;
     int64_t n, m;
     int nbS, nbN, nbNd, nbM, nbW;
+    int ntiles = 0, tpw = 1;         // 8x8x8 tiles, tiles per update_W workgroup
     Groups G;
     DevBuf<float> pool;
     float *U[2], *g[2], *Au[2], *d, *sx, *sy, *sz, *ox, *oy, *oz, *dts, *Atb, *rcd, *b, *delta,
@@ -755,8 +787,16 @@ struct Fused {
                     "volume too large for the fused solver's voxel groups");
         nbS = (int)stencil_grid(G);
         nbN = product_blocks(*F);
-        // update_W: one workgroup per 8x8x8 tile with B = 8 (nbW <= nbS)
-        nbW = B == 8 ? (int)(n / 512) : nbS;
+        // update_W with B = 8: tpw consecutive 8x8x8 tiles per workgroup, as
+        // many workgroups as are resident at once (one round, nbW <= nbS)
+        ntiles = (int)(n / 512);
+        if (B == 8) {
+            int per_cu = 0;
+            TV3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_w_tile, 512, 0));
+            const int resident = std::max(1, per_cu) * std::max(1, ctx->num_cus);
+            tpw = std::max(1, (ntiles + resident - 1) / resident);
+        }
+        nbW = B == 8 ? (ntiles + tpw - 1) / tpw : nbS;
         nbM = 2048;
         const size_t tot = 3 * (size_t)nbS + 3 * (size_t)nbN + 7 * (size_t)nbS + (size_t)nbS +
                            (size_t)nbN + (size_t)nbS + nbM + nbM + nbM + 2 * (size_t)nbM;
@@ -817,8 +857,8 @@ struct Fused {
 
     void update_w(const float *Un, const float *Ucur, const float *gn) {
         if (B == 8)
-            hipLaunchKernelGGL(k_update_w_tile, dim3(nbW), dim3(512), 0, s, L, Un, Ucur, sx, sy,
-                               sz, gn, dts, d, ctl.p, pW);
+            hipLaunchKernelGGL(k_update_w_tile, dim3(nbW), dim3(512), 0, s, L, ntiles, tpw, Un,
+                               Ucur, sx, sy, sz, gn, dts, d, ctl.p, pW);
         else
             hipLaunchKernelGGL(k_update_w<B>, dim3(nbS), dim3(256), 0, s, L, G, Un, Ucur, sx, sy,
                                sz, gn, dts, d, ctl.p, pW);
@@ -853,7 +893,7 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     S.ctx = ctx; S.F = &FL; S.o = &o; S.s = ctx->stream;
     hipStream_t s = S.s;
     S.L.p = FL.dims[0]; S.L.q = FL.dims[1]; S.L.r = FL.dims[2];
-    S.L.nby = FL.nby; S.L.nbx = FL.nbx;
+    S.L.nby = FL.nby; S.L.nbx = FL.nbx; S.L.nbz = S.L.r / 8;
     S.L.pq = (int64_t)S.L.p * S.L.q;
     S.n = (int64_t)S.L.p * S.L.q * S.L.r;
     S.m = FL.A.rows;

@@ -39,11 +39,14 @@ struct FCtl {
 constexpr int HF = 15;   // history fields per iteration (the last: A^T product taken)
 
 // W = shrink(DU - sigma/beta) of one voxel (3d/tval3_gpu_3d_kernels.cu:1249-1300
-// and the isotropic variant); returns the lam1 contribution
+// and the isotropic variant); returns the lam1 contribution.  ib = 1/beta:
+// sigma/beta is formed as sigma * ib, the quotient itself when beta is a
+// power of two (the default beta0 and continuation rate keep it one); the
+// isotropic |b| and scale (|b| - ib)/|b| use the hardware reciprocal square
+// root (about 1 ulp).  The plain solver keeps the quotients of the oracle.
 __device__ __forceinline__ float shrink3(float ux, float uy, float uz, float sx, float sy, float sz,
-                                        float beta, float ib, bool iso, float &wx, float &wy,
-                                        float &wz) {
-    const float bx = ux - sx / beta, by = uy - sy / beta, bz = uz - sz / beta;
+                                        float ib, bool iso, float &wx, float &wy, float &wz) {
+    const float bx = fmaf(-sx, ib, ux), by = fmaf(-sy, ib, uy), bz = fmaf(-sz, ib, uz);
     if (!iso) {
         const float tx = fmaxf(fabsf(bx) - ib, 0.f), ty = fmaxf(fabsf(by) - ib, 0.f),
                     tz = fmaxf(fabsf(bz) - ib, 0.f);
@@ -52,10 +55,14 @@ __device__ __forceinline__ float shrink3(float ux, float uy, float uz, float sx,
         wz = bz >= 0 ? tz : -tz;
         return tx + ty + tz;
     }
-    const float nrm = sqrtf(bx * bx + by * by + bz * bz);
-    const float sh = nrm - ib;
+    // 1/|b| of max(|b|^2, ib^2): below ib the shrink is 0 either way, and the
+    // hardware reciprocal square root flushes denormal inputs to zero (an
+    // infinite 1/|b| would turn W into Inf/NaN)
+    const float n2 = bx * bx + by * by + bz * bz;
+    const float r = __builtin_amdgcn_rsqf(fmaxf(n2, ib * ib));
+    const float sh = n2 * r - ib;
     if (sh > 0.f) {
-        const float f = sh / nrm;
+        const float f = sh * r;
         wx = bx * f; wy = by * f; wz = bz * f;
         return sh;
     }
