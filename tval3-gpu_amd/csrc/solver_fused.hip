@@ -269,6 +269,39 @@ __device__ __forceinline__ int box_slot(int a, int b, int c) {
     return ((c + 1) * kTileBox + (b + 1)) * kTileBox + (a + 1);
 }
 
+// Tiles of B = 8: tile voxel q (internal order) -> offsets (a, b, c) =
+// (y, x, z) in the tile.
+__device__ __forceinline__ void tile_pos(int q, int &a, int &b, int &c) {
+    const int sub = q >> 6, w = q & 63;
+    a = ((sub & 1) << 2) | (w & 3);
+    b = (((sub >> 1) & 1) << 2) | ((w >> 2) & 3);
+    c = ((sub >> 2) << 2) | (w >> 4);
+}
+
+// A voxel at offsets (a, b, c) in [-1, 8]^3 from a tile's corner lies in the
+// tile shifted by (a >> 3, b >> 3, c >> 3) in {-1, 0, 1}^3 (periodic wrap of
+// the tile coordinates: every dimension is a multiple of 8) at the in-tile
+// position of (a & 7, b & 7, c & 7); both parts are fixed per thread.
+struct TileOff {
+    int da, db, dc, off;
+};
+
+__device__ __forceinline__ TileOff tile_off(int a, int b, int c) {
+    const int ai = a & 7, bi = b & 7, ci = c & 7;
+    const int sub = ((ci >> 2) << 2) | ((bi >> 2) << 1) | (ai >> 2);
+    return TileOff{a >> 3, b >> 3, c >> 3, (sub << 6) + ((ci & 3) << 4) + ((bi & 3) << 2) + (ai & 3)};
+}
+
+// internal index of that voxel for the tile at tile coordinates (ty, tx, tz)
+__device__ __forceinline__ int64_t tile_gidx(const Lay &L, int ty, int tx, int tz,
+                                             const TileOff &w) {
+    int y = ty + w.da, x = tx + w.db, z = tz + w.dc;
+    y = y < 0 ? y + L.nby : (y >= L.nby ? y - L.nby : y);
+    x = x < 0 ? x + L.nbx : (x >= L.nbx ? x - L.nbx : x);
+    z = z < 0 ? z + L.nbz : (z >= L.nbz ? z - L.nbz : z);
+    return ((int64_t)((z * L.nbx + x) * L.nby + y) << 9) + w.off;
+}
+
 __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, int ntiles, int tpw,
                                                        const float *__restrict__ U,
                                                        const float *__restrict__ Uc,
@@ -292,20 +325,10 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, int ntiles, int tp
     // consecutive workgroups on one XCD
     const int tile0 = xcd_remap(blockIdx.x, gridDim.x) * tpw;
     const int tile1 = min(ntiles, tile0 + tpw);
-    // tile voxel q (internal order) -> offsets (a, b, c) = (y, x, z) in the tile
-    auto tile_pos = [](int q, int &a, int &b, int &c) {
-        const int sub = q >> 6, w = q & 63;
-        a = ((sub & 1) << 2) | (w & 3);
-        b = (((sub >> 1) & 1) << 2) | ((w >> 2) & 3);
-        c = ((sub >> 2) << 2) | (w >> 4);
-    };
     // per thread, for every tile: the U box ([-1, 8]^3) entries t and
     // k1 = t + 512, sigma at the tile voxel q0 = t and at the lower-face voxel
     // q1 = 512 + t (t < 192), and g, dts, Uc, d of voxel t; indices clamped
-    // instead of guarded.  A box or face voxel (a, b, c) lies in the tile
-    // shifted by (a >> 3, b >> 3, c >> 3) in {-1, 0, 1}^3 (periodic wrap of
-    // the tile coordinates: every dimension is a multiple of 8) at the
-    // in-tile position of (a & 7, b & 7, c & 7), both fixed per thread.
+    // instead of guarded
     const int k1 = t + 512 < NBOX ? t + 512 : NBOX - 1;
     int a0, b0, c0;
     tile_pos(t, a0, b0, c0);
@@ -313,31 +336,17 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, int ntiles, int tp
     const int f = q1 >> 6, j = q1 & 63, fu = j & 7, fv = j >> 3;
     const int a1 = f == 0 ? -1 : fu, b1 = f == 0 ? fu : (f == 1 ? -1 : fv),
               c1 = f == 2 ? -1 : fv;
-    struct Where { int da, db, dc, off; };
-    auto where = [](int a, int b, int c) {
-        const int ai = a & 7, bi = b & 7, ci = c & 7;
-        const int sub = ((ci >> 2) << 2) | ((bi >> 2) << 1) | (ai >> 2);
-        return Where{a >> 3, b >> 3, c >> 3,
-                     (sub << 6) + ((ci & 3) << 4) + ((bi & 3) << 2) + (ai & 3)};
+    auto box_off = [](int k) {
+        return tile_off(k % kTileBox - 1, (k / kTileBox) % kTileBox - 1, k / 100 - 1);
     };
-    auto box_where = [&](int k) {
-        return where(k % kTileBox - 1, (k / kTileBox) % kTileBox - 1, k / 100 - 1);
-    };
-    const Where w0 = box_where(t), w1 = box_where(k1), wf = where(a1, b1, c1);
+    const TileOff w0 = box_off(t), w1 = box_off(k1), wf = tile_off(a1, b1, c1);
     struct Loads { float ub[2], s0[3], s1[3], gv, dv, ucv, dold; };
     auto load = [&](int tile, Loads &r) {
         const int ty = tile % L.nby, rest = tile / L.nby;
         const int tx = rest % L.nbx, tz = rest / L.nbx;
-        auto gidx = [&](const Where &w) {
-            int y = ty + w.da, x = tx + w.db, z = tz + w.dc;
-            y = y < 0 ? y + L.nby : (y >= L.nby ? y - L.nby : y);
-            x = x < 0 ? x + L.nbx : (x >= L.nbx ? x - L.nbx : x);
-            z = z < 0 ? z + L.nbz : (z >= L.nbz ? z - L.nbz : z);
-            return ((int64_t)((z * L.nbx + x) * L.nby + y) << 9) + w.off;
-        };
-        r.ub[0] = U[gidx(w0)];
-        r.ub[1] = U[gidx(w1)];
-        const int64_t i = ((int64_t)tile << 9) + t, gi1 = gidx(wf);
+        r.ub[0] = U[tile_gidx(L, ty, tx, tz, w0)];
+        r.ub[1] = U[tile_gidx(L, ty, tx, tz, w1)];
+        const int64_t i = ((int64_t)tile << 9) + t, gi1 = tile_gidx(L, ty, tx, tz, wf);
         r.s0[0] = sx[i]; r.s0[1] = sy[i]; r.s0[2] = sz[i];
         r.s1[0] = sx[gi1]; r.s1[1] = sy[gi1]; r.s1[2] = sz[gi1];
         r.gv = g[i]; r.dv = dts[i]; r.ucv = Uc[i]; r.dold = d[i];
@@ -398,6 +407,96 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, int ntiles, int tp
         }
     }
     block_sum_store<7, 8>(acc, part, gridDim.x);
+}
+
+// descend on tiles of B = 8 (:690-696): U_t = max(U_c - tau d, 0) and the
+// line-search statistics of k_descend.  U_c and d are staged in LDS on the
+// tile and its upper shell ([0, 8]^3: the +1 neighbours D takes); a
+// workgroup walks tpw consecutive tiles with the next tile's loads in flight.
+constexpr int kDBox = 9;
+
+__global__ __launch_bounds__(512) void k_descend_tile(Lay L, int ntiles, int tpw,
+                                                      const float *__restrict__ Uc,
+                                                      const float *__restrict__ d,
+                                                      const float *__restrict__ sx,
+                                                      const float *__restrict__ sy,
+                                                      const float *__restrict__ sz,
+                                                      const float *__restrict__ wsx,
+                                                      const float *__restrict__ wsy,
+                                                      const float *__restrict__ wsz,
+                                                      float *__restrict__ Un, const FCtl *C,
+                                                      double *part) {
+    constexpr int NB = kDBox * kDBox * kDBox;        // 729
+    constexpr int NSHELL = NB - 512;                  // 217 upper-shell voxels
+    __shared__ float su[NB], sd[NB];
+    const int active = C->active;
+    const float tau = (float)C->tau, beta = (float)C->beta, ib = 1.f / beta;
+    const bool nn = C->nonneg, iso = C->iso, old = C->sigw_old;
+    const int t = threadIdx.x;
+    const int tile0 = xcd_remap(blockIdx.x, gridDim.x) * tpw;
+    const int tile1 = min(ntiles, tile0 + tpw);
+    int a0, b0, c0;
+    tile_pos(t, a0, b0, c0);
+    const int sl = (c0 * kDBox + b0) * kDBox + a0;
+    // shell voxel e = t (t < 217): the points of [0, 8]^3 with c = 8 (81),
+    // then b = 8, c < 8 (72), then a = 8, b < 8, c < 8 (64)
+    const int e = t < NSHELL ? t : NSHELL - 1;
+    int ae, be, ce;
+    if (e < 81) { ce = 8; be = e / 9; ae = e % 9; }
+    else if (e < 153) { be = 8; ce = (e - 81) / 9; ae = (e - 81) % 9; }
+    else { ae = 8; ce = (e - 153) >> 3; be = (e - 153) & 7; }
+    const int sle = (ce * kDBox + be) * kDBox + ae;
+    const TileOff we = tile_off(ae, be, ce);
+    struct Loads { float u, dd, ue, de, s[3], w[3]; };
+    auto load = [&](int tile, Loads &r) {
+        const int ty = tile % L.nby, rest = tile / L.nby;
+        const int tx = rest % L.nbx, tz = rest / L.nbx;
+        const int64_t i = ((int64_t)tile << 9) + t, ie = tile_gidx(L, ty, tx, tz, we);
+        r.u = Uc[i]; r.dd = d[i]; r.ue = Uc[ie]; r.de = d[ie];
+        r.s[0] = sx[i]; r.s[1] = sy[i]; r.s[2] = sz[i];
+        if (old) { r.w[0] = wsx[i]; r.w[1] = wsy[i]; r.w[2] = wsz[i]; }
+        else { r.w[0] = r.s[0]; r.w[1] = r.s[1]; r.w[2] = r.s[2]; }
+    };
+    Loads cur;
+    load(tile0 < tile1 ? tile0 : ntiles - 1, cur);
+    if (!active || tile0 >= tile1) {
+        keep_issued(cur.u, cur.dd, cur.ue, cur.de, cur.s[0], cur.s[1], cur.s[2], cur.w[0],
+                    cur.w[1], cur.w[2]);
+        if (!active) return;
+    }
+    double acc[3] = {0, 0, 0};
+    auto step = [&](float u, float dv) {
+        const float r = u - tau * dv;
+        return (nn && r < 0.f) ? 0.f : r;
+    };
+    for (int tile = tile0; tile < tile1; ++tile) {
+        su[sl] = cur.u;
+        sd[sl] = cur.dd;
+        if (t < NSHELL) {
+            su[sle] = cur.ue;
+            sd[sle] = cur.de;
+        }
+        Loads nxt;
+        const bool more = tile + 1 < tile1;
+        if (more) load(tile + 1, nxt);
+        __syncthreads();
+        const float u0 = su[sl], uy = su[sl + 1], ux = su[sl + kDBox], uz = su[sl + kDBox * kDBox];
+        const float t0 = step(u0, sd[sl]), ty = step(uy, sd[sl + 1]),
+                    tx = step(ux, sd[sl + kDBox]), tz = step(uz, sd[sl + kDBox * kDBox]);
+        float wx, wy, wz;
+        shrink3(ux - u0, uy - u0, uz - u0, cur.w[0], cur.w[1], cur.w[2], ib, iso, wx, wy, wz);
+        const float v0x = (ux - u0) - wx, v0y = (uy - u0) - wy, v0z = (uz - u0) - wz;
+        const float v1x = (tx - t0) - wx, v1y = (ty - t0) - wy, v1z = (tz - t0) - wz;
+        acc[0] += (double)v1x * v1x + (double)v1y * v1y + (double)v1z * v1z;
+        acc[1] += (double)v1x * cur.s[0] + (double)v1y * cur.s[1] + (double)v1z * cur.s[2];
+        acc[2] += (double)v0x * v1x + (double)v0y * v1y + (double)v0z * v1z;
+        Un[((int64_t)tile << 9) + t] = t0;
+        if (more) {
+            __syncthreads();        // su and sd are rewritten for the next tile
+            cur = nxt;
+        }
+    }
+    block_sum_store<3, 8>(acc, part, gridDim.x);
 }
 
 // |D d|^2 for the first-iteration step length (:587-596)
@@ -754,8 +853,8 @@ struct Fused {
     hipStream_t s;
     Lay L;
     int64_t n, m;
-    int nbS, nbN, nbNd, nbM, nbW;
-    int ntiles = 0, tpw = 1;         // 8x8x8 tiles, tiles per update_W workgroup
+    int nbS, nbN, nbNd, nbM, nbW, nbD;
+    int ntiles = 0, tpw = 1, tpwD = 1;   // 8x8x8 tiles, tiles per update_W / descend workgroup
     Groups G;
     DevBuf<float> pool;
     float *U[2], *g[2], *Au[2], *d, *sx, *sy, *sz, *ox, *oy, *oz, *dts, *Atb, *rcd, *b, *delta,
@@ -791,12 +890,17 @@ struct Fused {
         // many workgroups as are resident at once (one round, nbW <= nbS)
         ntiles = (int)(n / 512);
         if (B == 8) {
-            int per_cu = 0;
-            TV3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_w_tile, 512, 0));
-            const int resident = std::max(1, per_cu) * std::max(1, ctx->num_cus);
-            tpw = std::max(1, (ntiles + resident - 1) / resident);
+            auto tiles_per_wg = [&](const void *kernel) {
+                int per_cu = 0;
+                TV3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 512, 0));
+                const int resident = std::max(1, per_cu) * std::max(1, ctx->num_cus);
+                return std::max(1, (ntiles + resident - 1) / resident);
+            };
+            tpw = tiles_per_wg(reinterpret_cast<const void *>(k_update_w_tile));
+            tpwD = tiles_per_wg(reinterpret_cast<const void *>(k_descend_tile));
         }
         nbW = B == 8 ? (ntiles + tpw - 1) / tpw : nbS;
+        nbD = B == 8 ? (ntiles + tpwD - 1) / tpwD : nbS;
         nbM = 2048;
         const size_t tot = 3 * (size_t)nbS + 3 * (size_t)nbN + 7 * (size_t)nbS + (size_t)nbS +
                            (size_t)nbN + (size_t)nbS + nbM + nbM + nbM + 2 * (size_t)nbM;
@@ -830,8 +934,12 @@ struct Fused {
     void iteration(int c, int mode) {
         const int n_ = 1 - c;
         const dim3 gS(nbS);
-        hipLaunchKernelGGL(k_descend<B>, gS, dim3(256), 0, s, L, G, U[c], d, sx, sy, sz, ox, oy,
-                           oz, U[n_], ctl.p, pD);
+        if (B == 8)
+            hipLaunchKernelGGL(k_descend_tile, dim3(nbD), dim3(512), 0, s, L, ntiles, tpwD, U[c], d,
+                               sx, sy, sz, ox, oy, oz, U[n_], ctl.p, pD);
+        else
+            hipLaunchKernelGGL(k_descend<B>, gS, dim3(256), 0, s, L, G, U[c], d, sx, sy, sz, ox,
+                               oy, oz, U[n_], ctl.p, pD);
         launch_forward<float>(*F, U[n_], EpiRes{Au[n_], b, delta, Au[c], ctl.p, pN, {0, 0, 0}}, s,
                               &ctl.p->active);
         if (skipping) {                // get_g's A^T-skip decision (:398-468)
@@ -844,7 +952,7 @@ struct Fused {
         } else {
             launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p, 0}, s);
         }
-        hipLaunchKernelGGL((k_stage2<3, 3>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pD, nbS, pN, nbN,
+        hipLaunchKernelGGL((k_stage2<3, 3>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pD, nbD, pN, nbN,
                            st2.p);
         hipLaunchKernelGGL(k_line_search, dim3(1), dim3(64), 0, s, ctl.p, st2.p, mode);
         hipLaunchKernelGGL(k_apply_alpha, dim3(512), dim3(256), 0, s, n, m, U[c], U[n_], g[c],
