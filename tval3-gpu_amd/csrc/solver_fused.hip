@@ -729,65 +729,65 @@ __device__ void reduce_parts(const double *part, int nb, double *out) {
         for (int k = 0; k < K; ++k) out[k] = sh[k][0];
 }
 
-// Second reduction stage: K = KA + KB partial arrays (nbA resp. nbB entries)
-// -> K arrays of S2 entries, workgroup j summing a contiguous slice in a fixed
-// order.  Keeps the single-workgroup scalar kernels off long load chains.
-constexpr int S2 = 64;
-
+// Totals of KA partial arrays of nbA entries and KB arrays of nbB entries on
+// one workgroup of 1024 threads, every load issued up front: thread t takes
+// entries t + 1024 u (u < 4) of each array per pass of 4096 entries (one
+// pass for nb <= 4096, one memory round trip), then wavefront sums (DPP) and
+// the 16 wavefront totals on wavefront 0, in a fixed order.  Every lane of
+// wavefront 0 returns the K totals.
 template <int KA, int KB>
-__global__ __launch_bounds__(256) void k_stage2(const FCtl *C, int check, const double *pa, int nbA,
-                                                const double *pb, int nbB, double *out) {
-    if (check && !C->active) return;
+__device__ __forceinline__ void reduce_wg(const double *pa, int nbA, const double *pb, int nbB,
+                                          double *out) {
     constexpr int K = KA + KB;
-    __shared__ double sh[K][4];
-    const int j = blockIdx.x, t = threadIdx.x;
-    // every load of every array first (a thread has at most a few per array)
+    __shared__ double sh[K][16];
+    const int t = threadIdx.x;
     double v[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const double *p = k < KA ? pa + (int64_t)k * nbA : pb + (int64_t)(k - KA) * nbB;
-        const int nb = k < KA ? nbA : nbB;
-        const int chunk = (nb + S2 - 1) / S2, lo = j * chunk, hi = min(nb, lo + chunk);
-        double a0 = 0, a1 = 0;
-        int i = lo + t;
-        for (; i + 256 < hi; i += 512) { a0 += p[i]; a1 += p[i + 256]; }
-        if (i < hi) a0 += p[i];
-        v[k] = a0 + a1;
+    for (int k = 0; k < K; ++k) v[k] = 0.0;
+    const int nbm = nbA > nbB ? nbA : nbB;
+    for (int j0 = 0; j0 < nbm; j0 += 4096) {
+        double x[4][K];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * 1024 + t;
+#pragma unroll
+            for (int k = 0; k < KA; ++k) x[u][k] = j < nbA ? pa[(int64_t)k * nbA + j] : 0.0;
+#pragma unroll
+            for (int k = 0; k < KB; ++k) x[u][KA + k] = j < nbB ? pb[(int64_t)k * nbB + j] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += (x[0][k] + x[1][k]) + (x[2][k] + x[3][k]);
     }
+    const int lane = t & 63, wave = t >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double x = wave_sum(v[k]);
-        if ((t & 63) == 0) sh[k][t >> 6] = x;
+        const double w = wave_sum(v[k]);
+        if (lane == 0) sh[k][wave] = w;
     }
     __syncthreads();
-    if (t < K) out[(int64_t)t * S2 + j] = (sh[t][0] + sh[t][1]) + (sh[t][2] + sh[t][3]);
-}
-
-// Sums of the K arrays of S2 stage-2 values on one wavefront (lane j holds
-// entry j, summed by wave_sum): every lane ends with the K totals.
-template <int K>
-__device__ __forceinline__ void wave_sums(const double *p2, double *out) {
-    const int t = threadIdx.x;
+    if (t < 64) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        out[k] = wave_sum(p2[(int64_t)k * S2 + t]);
+        for (int k = 0; k < K; ++k) out[k] = wave_sum(lane < 16 ? sh[k][lane] : 0.0);
     }
 }
 
-__global__ __launch_bounds__(64) void k_line_search(FCtl *C, const double *p2, int mode) {
+// Armijo line search from the block partials of descend (pa) and of the A
+// product's epilogue (pb) directly, one workgroup
+__global__ __launch_bounds__(1024) void k_line_search_parts(FCtl *C, const double *pa, int nbA,
+                                                            const double *pb, int nbB, int mode) {
     if (!C->active) return;
-    static_assert(S2 == 64, "one stage-2 value per lane");
-    double a[3], b[3];
-    wave_sums<3>(p2, a);
-    wave_sums<3>(p2 + 3 * S2, b);
-    if (threadIdx.x == 0) line_search(C, a, b, mode);
+    double s[6];
+    reduce_wg<3, 3>(pa, nbA, pb, nbB, s);
+    if (threadIdx.x == 0) line_search(C, s, s + 3, mode);
 }
 
-__global__ __launch_bounds__(64) void k_iter_end(FCtl *C, const double *p2, double *hist) {
+// end of an inner iteration from update_W's block partials directly
+__global__ __launch_bounds__(1024) void k_iter_end_parts(FCtl *C, const double *p, int nb,
+                                                         double *hist) {
     if (!C->active) return;
-    double a[7];
-    wave_sums<7>(p2, a);
-    if (threadIdx.x == 0) iter_end(C, a, hist);
+    double s[7];
+    reduce_wg<7, 0>(p, nb, nullptr, 0, s);
+    if (threadIdx.x == 0) iter_end(C, s, hist);
 }
 
 // first-iteration step length tau = |dd / (|Dd|^2 + mu/beta |Ad|^2)| (:583-625);
@@ -861,7 +861,7 @@ struct Fused {
         *lastAu, *tmp;
     bool skipping = false;           // A^T-skip heuristic on (skipMulRatio != 0)
     DevBuf<float> tmpbuf;
-    DevBuf<double> parts, hist, st2;
+    DevBuf<double> parts, hist;
     double *pD, *pN, *pW, *pDd, *pAd, *pS4, *pM5, *pOD, *pRO, *pK;
     DevBuf<FCtl> ctl;
     PinnedBuf<FCtl> hctl;
@@ -916,7 +916,6 @@ struct Fused {
         pOD = q; q += nbM;
         pRO = q; q += nbM;
         pK = q; q += 2 * (size_t)nbM;
-        st2.alloc(16 * S2);
         ctl.alloc(1);
         hctl.alloc(1);
         hist.alloc((size_t)(o->maxit > 0 ? o->maxit : 1) * HF);
@@ -952,15 +951,12 @@ struct Fused {
         } else {
             launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p, 0}, s);
         }
-        hipLaunchKernelGGL((k_stage2<3, 3>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pD, nbD, pN, nbN,
-                           st2.p);
-        hipLaunchKernelGGL(k_line_search, dim3(1), dim3(64), 0, s, ctl.p, st2.p, mode);
+        hipLaunchKernelGGL(k_line_search_parts, dim3(1), dim3(1024), 0, s, ctl.p, pD, nbD, pN, nbN,
+                           mode);
         hipLaunchKernelGGL(k_apply_alpha, dim3(512), dim3(256), 0, s, n, m, U[c], U[n_], g[c],
                            g[n_], Au[c], Au[n_], ctl.p);
         update_w(U[n_], U[c], g[n_]);
-        hipLaunchKernelGGL((k_stage2<7, 0>), dim3(S2), dim3(256), 0, s, ctl.p, 1, pW, nbW, pW, 0,
-                           st2.p);
-        hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(64), 0, s, ctl.p, st2.p, hist.p);
+        hipLaunchKernelGGL(k_iter_end_parts, dim3(1), dim3(1024), 0, s, ctl.p, pW, nbW, hist.p);
     }
 
     void update_w(const float *Un, const float *Ucur, const float *gn) {
