@@ -139,6 +139,7 @@ struct tval3_options {
     tv3_tv_mode tv_mode = TV3_TV_ANISO;
     tv3_solver_kind solver = TV3_SOLVER_PLAIN;
     int graph_iters = 0;
+    int tiles_per_wg = 0;
 };
 
 template <class T>
@@ -187,6 +188,7 @@ tv3_opts to_opts(const tval3_options<T> &o) {
     r.numInitIterations = o.numInitIterations; r.skipMulRatio = o.skipMulRatio;
     r.maxRelativeChange = o.maxRelativeChange;
     r.tv_mode = o.tv_mode; r.solver = o.solver; r.graph_iters = o.graph_iters;
+    r.tiles_per_wg = o.tiles_per_wg;
     return r;
 }
 

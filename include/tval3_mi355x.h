@@ -80,6 +80,8 @@ typedef struct {
     int32_t warm_start;           /* 1: U on entry is the initial guess, else A^T b */
     int32_t graph_iters;          /* fused: inner iterations per graph launch, 0 = auto */
     int32_t verbose;
+    int32_t tiles_per_wg;         /* fused, 8x8x8 tiles: tiles per stencil workgroup,
+                                   * 0 = auto (as many workgroups as are resident)  */
 } tv3_opts;
 
 /* tval3_info (2d/tval3_types.h:37-43; 3-D adds outer_secs, 3d/tval3_gpu_3d.cpp:1507) */

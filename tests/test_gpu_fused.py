@@ -287,3 +287,28 @@ def test_fused_every_product_form_tracks_oracle(ctx, code_N, code_T, panels):
     assert k >= 10
     assert (dev_f[:4] < 1e-5).all()
     assert (dev_f <= 30 * env + 1e-6).all()
+
+
+def test_fused_tiles_per_workgroup(ctx):
+    """The tile kernels (update_W, descend) walk consecutive 8x8x8 tiles per
+    workgroup with the next tile's loads in flight.  At test sizes the
+    automatic choice is one tile per workgroup, so several are forced here,
+    up to one workgroup for all tiles: the per-voxel arithmetic is the same
+    and only the grouping of the FP64 block partials changes, so the
+    decisions are the same and f agrees to FP64 rounding for the first
+    iterations, then the run stays within FP32 rounding drift."""
+    dims = (16, 16, 24)                      # 2 x 2 x 3 tiles
+    u, A, A32, oop, b = setup(ctx, dims, 24, 64, seed=19)
+    o = opts(maxit=30)
+    U1, i1, h1 = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**o, solver="fused"), Ut=u)
+    for tpw in (2, 3, 5, 12):
+        Uk, ik, hk = api.tval3_gpu_3d(ctx, A, b, dims,
+                                      api.default_opts(**o, solver="fused", tiles_per_wg=tpw),
+                                      Ut=u)
+        k = first_divergence(hk, h1)
+        dev = np.abs(hk["f"][:k] - h1["f"][:k]) / np.abs(h1["f"][:k])
+        print(f"tiles per workgroup {tpw}: decisions agree for {k}; f dev first 10 "
+              f"{dev[:10].max():.1e}, all {dev.max():.1e}; rel(U) {rel(Uk, U1):.1e}")
+        assert k >= 20
+        assert (dev[:10] <= 1e-9).all()
+        assert abs(ik["rel_error"] - i1["rel_error"]) <= 1e-3

@@ -898,6 +898,7 @@ struct Fused {
             };
             tpw = tiles_per_wg(reinterpret_cast<const void *>(k_update_w_tile));
             tpwD = tiles_per_wg(reinterpret_cast<const void *>(k_descend_tile));
+            if (o->tiles_per_wg > 0) tpw = tpwD = o->tiles_per_wg;
         }
         nbW = B == 8 ? (ntiles + tpw - 1) / tpw : nbS;
         nbD = B == 8 ? (ntiles + tpwD - 1) / tpwD : nbS;

@@ -101,6 +101,7 @@ void validate_opts(const tv3_opts *o) {
     TV3_REQUIRE(o->skipMulRatio >= 0.0 && o->maxRelativeChange >= 0.0, TV3_E_ARG,
                 "skip heuristic parameters must be non-negative");
     TV3_REQUIRE(o->maxit >= 0 && o->maxcnt >= 0, TV3_E_ARG, "maxit/maxcnt must be >= 0");
+    TV3_REQUIRE(o->tiles_per_wg >= 0, TV3_E_ARG, "tiles_per_wg must be >= 0");
     TV3_REQUIRE(o->mu > 0 && o->mu0 > 0 && o->beta > 0 && o->beta0 > 0, TV3_E_ARG,
                 "mu, mu0, beta, beta0 must be positive");
     TV3_REQUIRE(o->tv_mode == TV3_TV_ANISO || o->tv_mode == TV3_TV_ISO, TV3_E_ARG, "bad tv_mode");

@@ -38,7 +38,7 @@ class Opts(C.Structure):
                 ("numInitIterations", C.c_uint32), ("skipMulRatio", C.c_double),
                 ("maxRelativeChange", C.c_double), ("tv_mode", C.c_int32),
                 ("solver", C.c_int32), ("warm_start", C.c_int32), ("graph_iters", C.c_int32),
-                ("verbose", C.c_int32)]
+                ("verbose", C.c_int32), ("tiles_per_wg", C.c_int32)]
 
 
 class Info(C.Structure):
