@@ -12,8 +12,8 @@
   every problem bitwise equal to the same problem solved alone in a batch
   (same launch shapes); dense problems against the FP32 oracle with the
   fused solver's envelope criterion (rounding-level agreement at the start,
-  then within a small multiple of the oracle's own drift under a 1e-7
-  perturbation of b) and the error against u_true.
+  then within a small multiple of the oracle's own drift under 1e-7
+  perturbations of b, three seeds) and the error against u_true.
 """
 import numpy as np
 import pytest
@@ -149,12 +149,18 @@ def test_batch_with_shared_dense_operator(ctx):
             continue
         oop = orc.dense_op(p["Ad"])
         Uo, io, ho, _ = orc.solve(oop, p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
-        xi = np.random.default_rng(0).choice([-1.0, 1.0], size=p["b"].shape)
-        bp = (p["b"].astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
-        Up, ip, hpp, _ = orc.solve(oop, p["dims"], bp, oo, Ut=p["u"], dtype=np.float32)
-        n = min(_first_divergence(hb[k], ho), _first_divergence(hpp, ho))
+        # the oracle's own sensitivity: three rounding-level perturbations of b
+        runs = []
+        for seed in (0, 1, 2):
+            xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=p["b"].shape)
+            bp = (p["b"].astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+            runs.append(orc.solve(oop, p["dims"], bp, oo, Ut=p["u"], dtype=np.float32))
+        ip = runs[0][1]
+        n = min([_first_divergence(hb[k], ho)] + [_first_divergence(r[2], ho) for r in runs])
         dev = np.abs(hb[k]["f"][:n] - ho["f"][:n]) / np.abs(ho["f"][:n])
-        env = np.maximum.accumulate(np.abs(hpp["f"][:n] - ho["f"][:n]) / np.abs(ho["f"][:n]))
+        env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:n] - ho["f"][:n])
+                                            / np.abs(ho["f"][:n])) for r in runs], axis=0)
+        spread = max(abs(r[1].rel_error - io.rel_error) for r in runs)
         print(f"dense problem {k}: iters {ib[k]['total_iters']} oracle {io.total_iters}; "
               f"decisions agree for {n}; err {ib[k]['rel_error']:.4e} oracle {io.rel_error:.4e} "
               f"perturbed {ip.rel_error:.4e}; f dev max {dev.max():.1e} envelope {env.max():.1e}")
@@ -165,8 +171,7 @@ def test_batch_with_shared_dense_operator(ctx):
         # (CSR) products of the same matrix as much as through the matrix-core
         # ones, hence 300 here against 30 for the sparse problems
         assert (dev <= 300 * env + 1e-6).all()
-        assert abs(ib[k]["rel_error"] - io.rel_error) <= max(3 * abs(ip.rel_error - io.rel_error),
-                                                             1e-3)
+        assert abs(ib[k]["rel_error"] - io.rel_error) <= max(3 * spread, 1e-3)
 
 
 def test_batch_dense_operator_errors(ctx):
