@@ -70,8 +70,8 @@ typedef struct {
     int32_t maxcnt, maxit;
     double c, gamma, gam, rate_gam;
     int32_t isreal, nonneg;
-    /* A^T-skip heuristic (3d/tval3_gpu_3d.cpp:398-468), plain and fused
-     * solvers (not batched solves): skipMulRatio = 0 turns it off.  A skipped A^T product leaves g as
+    /* A^T-skip heuristic (3d/tval3_gpu_3d.cpp:398-468), plain, fused and
+     * batched solves (a decision per problem): skipMulRatio = 0 turns it off.  A skipped A^T product leaves g as
      * A^T(last Au) - Atb (the reference subtracts Atb again from the stale g). */
     uint32_t numInitIterations;
     double skipMulRatio, maxRelativeChange;
@@ -235,7 +235,7 @@ tv3_status tv3_solve_f64(tv3_ctx *ctx, tv3_op *op, const tv3_opts *opts, const i
  * of that problem up to FP32 summation order).  Ut_dev (array or NULL,
  * entries may be NULL) gives infos[k].rel_error; hists (array of nprob or
  * NULL) receives each problem's history.  opts->solver is ignored; the
- * A^T-skip heuristic is not supported (skipMulRatio must be 0).  binfo may be
+ * A^T-skip heuristic decides per problem.  binfo may be
  * NULL; when it is not, lanes_N and lanes_T > 0 on entry fix the sparse
  * products' launch shapes and splits_N, splits_T > 0 the dense products' K
  * splits instead of timing them (out: the shapes used, the first group's). */

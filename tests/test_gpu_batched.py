@@ -10,6 +10,9 @@ of launches per iteration.
   within a small multiple of the oracle's own sensitivity to 1e-7
   perturbations of b (three seeds), and the final error.
 - Backtracking, fallbacks and the outer loop inside a batch; error paths.
+- The A^T-skip heuristic decides per problem: each problem against the
+  oracle running the same rule, batch against solo bitwise, and a rule that
+  never skips against the run without it, bitwise.
 """
 import numpy as np
 import pytest
@@ -108,9 +111,12 @@ def test_batch_graph_size_and_lanes(ctx, batch):
 
 
 def first_divergence(h1, h2):
+    """first iteration whose decisions differ: backtracking count, outer
+    index, A^T product taken or skipped"""
     n = min(len(h1["cnt"]), len(h2["cnt"]))
     for k in range(n):
-        if h1["cnt"][k] != h2["cnt"][k] or h1["outer"][k] != h2["outer"][k]:
+        if (h1["cnt"][k] != h2["cnt"][k] or h1["outer"][k] != h2["outer"][k]
+                or h1["mulT"][k] != h2["mulT"][k]):
             return k
     return n
 
@@ -183,10 +189,82 @@ def test_batch_errors(ctx, batch):
         solve(ctx, [batch[0], p16], opts())
     with pytest.raises(TV3Error):        # dims do not match the operator
         api.tval3_gpu_3d_batched(ctx, [batch[0]["A"]], [batch[0]["bt"]], [(16, 16, 15)], opts())
-    with pytest.raises(TV3Error):        # skip heuristic
-        solve(ctx, batch[:1], opts(skipMulRatio=0.5))
+    with pytest.raises(TV3Error):        # negative skip ratio
+        solve(ctx, batch[:1], opts(skipMulRatio=-0.5))
     with pytest.raises(ValueError):
         api.tval3_gpu_3d_batched(ctx, [batch[0]["A"]], [batch[0]["bt"][:-1]], [(16, 16, 16)])
     dense = api.DenseOperator(ctx, np.ones((4, 512)))
     with pytest.raises(TV3Error):        # dense operators are not batched this way
         api.tval3_gpu_3d_batched(ctx, [dense], [torch.zeros(4, device="cuda")], [(8, 8, 8)])
+
+
+def _skip_kw(skip):
+    ratio, ninit, maxrel = skip
+    return dict(skipMulRatio=ratio, numInitIterations=ninit, maxRelativeChange=maxrel)
+
+
+@pytest.mark.parametrize("skip", [(2.0, 3, 1e30), (1.0, 0, 0.05)])
+def test_batch_skip_heuristic_tracks_oracle(ctx, batch, skip):
+    """A^T-skip heuristic (3d/tval3_gpu_3d.cpp:398-468; a skipped product
+    leaves g as it was), one decision per problem: each problem of the batch
+    against the FP32 oracle running the same rule, with the envelope
+    criterion of test_batch_tracks_oracle (decisions, the product/skip
+    choice included, agree for at least 10 iterations), the same number of
+    A^T products whenever both runs take the same iterations, products
+    actually skipped, and the final error; and each problem of the batch
+    equals its solo solve bit for bit (no cross-talk between decisions)."""
+    kw = _skip_kw(skip)
+    o = opts(**kw)
+    Ub, ib, _, hb = solve(ctx, batch, o)
+    oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=60, tv_iso=1)
+    for key, v in kw.items():
+        setattr(oo, key, v)
+    for k, p in enumerate(batch):
+        Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
+        runs = []
+        for seed in (0, 1, 2):
+            xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=p["b"].shape)
+            bp = (p["b"].astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+            runs.append(orc.solve(p["oop"], p["dims"], bp, oo, Ut=p["u"], dtype=np.float32))
+        spread = max(abs(r[1].rel_error - io.rel_error) for r in runs)
+        kk = min([first_divergence(hb[k], ho)] + [first_divergence(r[2], ho) for r in runs])
+        dev = np.abs(hb[k]["f"][:kk] - ho["f"][:kk]) / np.abs(ho["f"][:kk])
+        env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:kk] - ho["f"][:kk])
+                                            / np.abs(ho["f"][:kk])) for r in runs], axis=0)
+        same = (ib[k]["total_iters"] == io.total_iters and ib[k]["fallbacks"] == io.fallbacks
+                and np.array_equal(hb[k]["outer"], ho["outer"]))
+        print(f"skip {skip} {p['dims']}: iters batch {ib[k]['total_iters']} oracle "
+              f"{io.total_iters}; A^T products batch {ib[k]['n_matvec_T'] - 1} oracle "
+              f"{io.n_matvec_T}; decisions agree for {kk}; err batch {ib[k]['rel_error']:.4e} "
+              f"oracle {io.rel_error:.4e}; max f dev {dev.max():.1e} envelope {env.max():.1e}")
+        assert io.n_matvec_T < io.total_iters             # the rule skipped products
+        assert ib[k]["n_matvec_T"] - 1 < ib[k]["total_iters"]
+        assert kk >= 10
+        assert (dev[:4] < 1e-5).all()
+        assert (dev <= 30 * env + 1e-6).all()
+        if same:
+            # the batched count holds A^T b besides the oracle's products
+            assert ib[k]["n_matvec_T"] - 1 == io.n_matvec_T
+        assert abs(ib[k]["rel_error"] - io.rel_error) <= max(3 * spread, 1e-3)
+        Us, is_, _, hs = solve(ctx, [p], o)
+        assert np.array_equal(Ub[k].cpu().numpy(), Us[0].cpu().numpy()), k
+        assert np.array_equal(hb[k]["mulT"], hs[0]["mulT"])
+        assert ib[k]["n_matvec_T"] == is_[0]["n_matvec_T"]
+
+
+def test_batch_skip_rule_that_never_skips_changes_nothing(ctx, batch):
+    """maxRelativeChange = 0: every get_g multiplies, so the batch with the
+    heuristic on equals the batch without it bit for bit, product counts
+    included; with fallbacks (Armijo constant 0, gamma 1) the forced product
+    of the fallback step goes through the same gate."""
+    for extra in ({}, dict(maxit=40, gamma=1.0, c=0.0, gam=0.0, tol_inn=1e-12)):
+        o0 = opts(**extra)
+        o1 = opts(**extra, **_skip_kw((0.5, 2, 0.0)))
+        U0, i0, _, h0 = solve(ctx, batch, o0)
+        U1, i1, _, h1 = solve(ctx, batch, o1)
+        for k in range(len(batch)):
+            assert np.array_equal(U0[k].cpu().numpy(), U1[k].cpu().numpy()), (extra, k)
+            assert np.array_equal(h0[k]["f"], h1[k]["f"])
+            assert i0[k]["total_iters"] == i1[k]["total_iters"]
+            assert i0[k]["n_matvec_T"] == i1[k]["n_matvec_T"]
+            assert (h1[k]["mulT"] == 1).all()

@@ -133,11 +133,13 @@ __device__ void seg_sums(const double *part, int nb, int lo, int hi, double *out
     __syncthreads();   // sh is reused by the next call
 }
 
-enum Pred { P_ACTIVE = 0, P_TAU = 1, P_ALL = 2, P_OUTER = 3 };
+enum Pred { P_ACTIVE = 0, P_TAU = 1, P_ALL = 2, P_OUTER = 3, P_MULT = 4 };
 
+// P_MULT: active, and get_g's A^T-skip decision multiplies
 __device__ __forceinline__ bool pred_on(const FCtl &c, int pred) {
     switch (pred) {
     case P_ACTIVE: return c.active != 0;
+    case P_MULT: return c.active != 0 && c.mul_T != 0;
     case P_TAU: return c.need_tau != 0;
     case P_OUTER: return c.outer_flag != 0;
     default: return true;
@@ -428,6 +430,52 @@ __global__ __launch_bounds__(256) void k_b_res(const BDims *__restrict__ D,
     block_sum_store<3>(acc, part, gridDim.x);
 }
 
+// A^T-skip heuristic (:398-468), per problem: |Au_t - last Au|^2, |Au_t|^2
+__global__ __launch_bounds__(256) void k_b_skip_stats(const BDims *__restrict__ D,
+                                                      const int32_t *__restrict__ rst, int P,
+                                                      const FCtl *__restrict__ C,
+                                                      const float *__restrict__ AuT,
+                                                      const float *__restrict__ last,
+                                                      double *part) {
+    const int k = find_prob(rst, P, blockIdx.x);
+    if (!C[k].active) return;
+    const BDims dm = D[k];
+    double acc[2] = {0, 0};
+    const int64_t r = (int64_t)blockIdx.x * GB + threadIdx.x;
+    if (r - dm.r0 < dm.m) {
+        const float a = AuT[r], dl = a - last[r];
+        acc[0] = (double)dl * dl;
+        acc[1] = (double)a * a;
+    }
+    block_sum_store<2>(acc, part, gridDim.x);
+}
+
+// after the gated A^T product: a problem that multiplied keeps Au_t as its
+// last Au; one that skipped keeps its gradient (g_t = g_c), as get_g does
+// (voxel groups first, then ray groups)
+__global__ __launch_bounds__(256) void k_b_skip_after(const BDims *__restrict__ D,
+                                                      const int32_t *__restrict__ vst,
+                                                      const int32_t *__restrict__ rst, int P,
+                                                      int nbV, const FCtl *__restrict__ C,
+                                                      const float *__restrict__ g0,
+                                                      const float *__restrict__ g1,
+                                                      float *__restrict__ gT,
+                                                      const float *__restrict__ AuT,
+                                                      float *__restrict__ last) {
+    const bool vox = (int)blockIdx.x < nbV;
+    const int w = vox ? (int)blockIdx.x : (int)blockIdx.x - nbV;
+    const int k = find_prob(vox ? vst : rst, P, w);
+    const FCtl &c = C[k];
+    if (!c.active) return;
+    const BDims dm = D[k];
+    const int64_t i = (int64_t)w * GB + threadIdx.x;
+    if (vox) {
+        if (!c.mul_T && i - dm.v0 < dm.n) gT[i] = (c.cur ? g1 : g0)[i];
+    } else {
+        if (c.mul_T && i - dm.r0 < dm.m) last[i] = AuT[i];
+    }
+}
+
 // |A d|^2 (first-iteration step length)
 __global__ __launch_bounds__(256) void k_b_sqnorm(const BDims *__restrict__ D,
                                                   const int32_t *__restrict__ rst, int P,
@@ -540,6 +588,17 @@ __global__ __launch_bounds__(256) void k_b_iter_end(FCtl *C, const int32_t *__re
     }
 }
 
+// the skip decision of each active problem; forced (a product) on the
+// fallback step (mode 1)
+__global__ __launch_bounds__(256) void k_b_skip_decide(FCtl *C, const int32_t *__restrict__ rst,
+                                                       int nbR, const double *pK) {
+    FCtl *c = C + blockIdx.x;
+    if (!c->active) return;
+    double a[2];
+    seg_sums<2>(pK, nbR, rst[blockIdx.x], rst[blockIdx.x + 1], a);
+    if (threadIdx.x == 0) skip_decide(c, a[0], a[1], c->mode);
+}
+
 __global__ __launch_bounds__(256) void k_b_tau(FCtl *C, const int32_t *__restrict__ vst,
                                                const int32_t *__restrict__ rst, int nbV, int nbR,
                                                const double *pD, const double *pA) {
@@ -646,9 +705,10 @@ struct Batch {
     DevBuf<int32_t> vst, rst;
     DevBuf<float> pool;
     float *U[2], *T, *g[2], *gT, *d, *sx, *sy, *sz, *ox, *oy, *oz, *dts, *Atb, *rcd;
-    float *Au[2], *AuT, *b, *delta;
+    float *Au[2], *AuT, *b, *delta, *last;
+    bool skipping = false;           // A^T-skip heuristic on (skipMulRatio != 0)
     DevBuf<double> parts, hist;
-    double *pD, *pN, *pW, *pDd, *pAd, *p4, *p5, *pOD, *pRO;
+    double *pD, *pN, *pW, *pDd, *pAd, *p4, *p5, *pOD, *pRO, *pK;
     int64_t hist_stride = 1;
     DevBuf<FCtl> ctl;
     PinnedBuf<FCtl> hctl;
@@ -713,7 +773,16 @@ struct Batch {
         prod_N(T, AuT, P_ACTIVE);
         hipLaunchKernelGGL(k_b_res, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, (int)P_ACTIVE, AuT, b,
                            delta, Au[0], Au[1], pN);
-        prod_T(AuT, gT, Atb, P_ACTIVE);
+        if (skipping) {                // get_g's A^T-skip decision (:398-468)
+            hipLaunchKernelGGL(k_b_skip_stats, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, AuT, last,
+                               pK);
+            hipLaunchKernelGGL(k_b_skip_decide, bP, t, 0, s, ctl.p, rst.p, nbR, pK);
+            prod_T(AuT, gT, Atb, P_MULT);
+            hipLaunchKernelGGL(k_b_skip_after, dim3(nbV + nbR), t, 0, s, ddims.p, vst.p, rst.p, P,
+                               nbV, ctl.p, g[0], g[1], gT, AuT, last);
+        } else {
+            prod_T(AuT, gT, Atb, P_ACTIVE);
+        }
         hipLaunchKernelGGL(k_b_search, bP, t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, pD, pN);
         hipLaunchKernelGGL(k_b_apply, dim3(nbV + nbR), t, 0, s, ddims.p, vst.p, rst.p, P, nbV,
                            ctl.p, U[0], U[1], T, g[0], g[1], gT, Au[0], Au[1], AuT);
@@ -970,7 +1039,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         Ut_dev[k] = Ut_in ? Ut_in[orig[k]] : nullptr;
     }
     // vectors
-    const int nv = 2 + 1 + 2 + 1 + 1 + 3 + 3 + 1 + 1 + 1, mv = 2 + 1 + 1 + 1;
+    const int nv = 2 + 1 + 2 + 1 + 1 + 3 + 3 + 1 + 1 + 1, mv = 2 + 1 + 1 + 1 + 1;
     B.pool.alloc((size_t)nv * N + (size_t)mv * M);
     TV3_HIP(hipMemsetAsync(B.pool.p, 0, B.pool.bytes(), s));
     {
@@ -978,12 +1047,13 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         float **nvp[] = {&B.U[0], &B.U[1], &B.T, &B.g[0], &B.g[1], &B.gT, &B.d, &B.sx, &B.sy,
                          &B.sz, &B.ox, &B.oy, &B.oz, &B.dts, &B.Atb, &B.rcd};
         for (float **q : nvp) { *q = p; p += N; }
-        float **mvp[] = {&B.Au[0], &B.Au[1], &B.AuT, &B.b, &B.delta};
+        float **mvp[] = {&B.Au[0], &B.Au[1], &B.AuT, &B.b, &B.delta, &B.last};
         for (float **q : mvp) { *q = p; p += M; }
     }
     {
         const size_t nbV = B.nbV, nbR = B.nbR;
-        B.parts.alloc(3 * nbV + 3 * nbR + 7 * nbV + nbV + nbR + nbV + nbR + nbV + nbV);
+        B.parts.alloc(3 * nbV + 3 * nbR + 7 * nbV + nbV + nbR + nbV + nbR + nbV + nbV +
+                      2 * nbR);
         double *q = B.parts.p;
         B.pD = q; q += 3 * nbV;
         B.pN = q; q += 3 * nbR;
@@ -994,6 +1064,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         B.p5 = q; q += nbR;
         B.pOD = q; q += nbV;
         B.pRO = q; q += nbV;
+        B.pK = q; q += 2 * nbR;
     }
     B.hist_stride = (int64_t)std::max(o.maxit, 1) * HF;
     B.hist.alloc((size_t)P * B.hist_stride);
@@ -1062,7 +1133,10 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         c.gam_opt = o.gam; c.gam = o.gam; c.rate_cnt = o.rate_cnt; c.mu_max = o.mu;
         c.beta_max = o.beta; c.maxit = o.maxit; c.nonneg = o.nonneg;
         c.iso = o.tv_mode == TV3_TV_ISO; c.hist_cap = o.maxit; c.active = 1; c.alpha = 1;
+        c.skip_ratio = o.skipMulRatio; c.max_rel = o.maxRelativeChange;
+        c.n_init = (int32_t)std::min<uint32_t>(o.numInitIterations, INT32_MAX);
     }
+    B.skipping = o.skipMulRatio != 0.0;
     B.write_ctl();
 
     // initialisation (:1128-1153), every problem at once
@@ -1081,6 +1155,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
     hipLaunchKernelGGL(k_b_res, dim3(B.nbR), dim3(256), 0, s, B.ddims.p, B.rst.p, P, B.ctl.p,
                        (int)P_ALL, B.Au[0], B.b, B.delta, B.Au[0], B.Au[1], B.pN);
     B.prod_T(B.Au[0], B.g[0], B.Atb, P_ALL);
+    if (B.skipping)                  // the initial get_g multiplies (forced): last Au
+        TV3_HIP(hipMemcpyAsync(B.last, B.Au[0], sizeof(float) * M, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_b_update_w, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P, B.ctl.p,
                        1, B.U[0], B.U[1], B.sx, B.sy, B.sz, B.g[0], B.g[1], B.dts, B.d, B.pW);
     hipLaunchKernelGGL(k_b_init, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p, B.rst.p, B.nbV,
@@ -1187,7 +1263,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         inf.backtracks = c.backtracks;
         inf.fallbacks = c.fallbacks;
         inf.n_matvec_N = 1 + (int64_t)c.total + c.inner_loops + 2 * (int64_t)c.fallbacks;
-        inf.n_matvec_T = 2 + (int64_t)c.total + c.fallbacks;
+        // with skipping: A^T b, the initial product and the products get_g took
+        inf.n_matvec_T = B.skipping ? 2 + (int64_t)c.n_T : 2 + (int64_t)c.total + c.fallbacks;
         inf.status = c.nonfinite ? TV3_E_NONFINITE : TV3_OK;
         tot += c.total;
         if (Ut_dev[k]) {

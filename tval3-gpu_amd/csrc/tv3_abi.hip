@@ -615,8 +615,6 @@ tv3_status tv3_solve_batched(tv3_ctx *ctx, int32_t nprob, tv3_op *const *ops,
     if (!ctx) return TV3_E_ARG;
     return guard(ctx, [&] {
         validate_opts(opts);
-        TV3_REQUIRE(opts->skipMulRatio == 0.0, TV3_E_UNSUPPORTED,
-                    "skipMulRatio != 0 is not supported by batched solves");
         TV3_REQUIRE(nprob > 0 && ops && dims && b_dev && U_dev && infos, TV3_E_ARG,
                     "NULL argument or empty batch");
         for (int32_t k = 0; k < nprob; ++k) {
