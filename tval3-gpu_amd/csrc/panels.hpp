@@ -67,6 +67,14 @@ struct EpiStoreIf {
     __device__ __forceinline__ void finish() const {}
 };
 
+// Extra workgroups an epilogue asks of a panel launch (Epi::kExtra, default
+// 0): blocks 0 .. kExtra - 1 run epi.extra() instead of a panel.
+template <typename E, typename = void> struct ExtraBlocks { static constexpr int value = 0; };
+template <typename E>
+struct ExtraBlocks<E, std::void_t<decltype(E::kExtra)>> {
+    static constexpr int value = E::kExtra;
+};
+
 // One workgroup per panel (largest panels first, so the launch ends with
 // small ones).  Block descriptors: p0 / len = padded start / padded length.
 template <int NB, typename V, bool CONTIG, typename Epi>
@@ -83,9 +91,16 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
     constexpr int NG = NB / 4;               // entry groups per block
     extern __shared__ float xs[];            // the panel's source values
     __shared__ float gs[NG];                 // group sums
+    constexpr int NX = ExtraBlocks<Epi>::value;
+    if constexpr (NX > 0) {
+        if ((int)blockIdx.x < NX) {
+            epi.extra();
+            return;
+        }
+    }
     if (epi.skip()) return;
     const int t = threadIdx.x;
-    const int pan = order[blockIdx.x];
+    const int pan = order[blockIdx.x - NX];
     const int s0 = src_off[pan], ns = src_off[pan + 1] - s0;
     const int bb = pan_blk[pan], b1 = pan_blk[pan + 1];
     BlkDesc d = desc[bb];
@@ -209,7 +224,7 @@ __global__ __launch_bounds__(256) void k_seg_sum(int64_t rows, const int32_t *__
 
 template <typename V, bool CONTIG, typename Epi>
 void launch_panel_v(const Panels &P, const float *x, const Epi &epi, hipStream_t s) {
-    const unsigned g = (unsigned)P.npan;
+    const unsigned g = (unsigned)P.npan + ExtraBlocks<Epi>::value;
     const size_t lds = sizeof(float) * (size_t)(P.max_src > 0 ? P.max_src : 1);
     if (g == 0) return;
 #define TV3_PCASE(NBV)                                                                        \

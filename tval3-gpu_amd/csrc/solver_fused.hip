@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *
                                                   const float *__restrict__ dts,
                                                   float *__restrict__ d, const FCtl *C,
                                                   double *part) {
-    if (!C->active) return;
+    if (!C->active || C->need_fallback) return;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
     const bool iso = C->iso;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, int ntiles, int tp
     __shared__ float su[NBOX], svx[NBOX], svy[NBOX], svz[NBOX];
     // control scalars, then the first tile's data loads, then the graph's
     // stop flag: the loads are in flight while the flag arrives
-    const int active = C->active;
+    const int active = C->active && !C->need_fallback;
     const float beta = (float)C->beta, ib = 1.f / beta, mdb = (float)C->muDbeta;
     const bool iso = C->iso;
     const int t = threadIdx.x;
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_apply_alpha(int64_t n, int64_t m,
                                                      float *__restrict__ gn,
                                                      const float *__restrict__ Auc,
                                                      float *__restrict__ Aun, const FCtl *C) {
-    if (!C->active || C->alpha == 1.0) return;
+    if (!C->active || C->need_fallback || C->alpha == 1.0) return;
     const float a = (float)C->alpha;
     const int64_t mx = n > m ? n : m, G = (int64_t)gridDim.x * 256;
 #pragma unroll 4
@@ -781,10 +781,84 @@ __global__ __launch_bounds__(1024) void k_line_search_parts(FCtl *C, const doubl
     if (threadIdx.x == 0) line_search(C, s, s + 3, mode);
 }
 
+// The sums of reduce_wg (1024 threads) with one 256-thread workgroup:
+// thread t plays threads t + 256 q, q = 0..3, of the larger workgroup with
+// the same loads and the same order of additions (virtual wave 4 q + wave),
+// so both give the same bits.  One array set (K sums) at a time, so few
+// registers are live: the kernel that hosts this keeps its occupancy.
+template <int K>
+__device__ __forceinline__ void reduce_256_part(const double *p, int nb, int nbm, int q,
+                                                double (*sh)[16]) {
+    const int t = threadIdx.x, tt = t + 256 * q;
+    const int lane = t & 63, wave = t >> 6;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = 0.0;
+    for (int j0 = 0; j0 < nbm; j0 += 4096) {
+        double x[4][K];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * 1024 + tt;
+#pragma unroll
+            for (int k = 0; k < K; ++k) x[u][k] = j < nb ? p[(int64_t)k * nb + j] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += (x[0][k] + x[1][k]) + (x[2][k] + x[3][k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double w = wave_sum(v[k]);
+        if (lane == 0) sh[k][4 * q + wave] = w;
+    }
+}
+
+template <int KA, int KB>
+__device__ __forceinline__ void reduce_wg_256(const double *pa, int nbA, const double *pb,
+                                              int nbB, double *out) {
+    constexpr int K = KA + KB;
+    __shared__ double sh[K][16];
+    const int t = threadIdx.x, lane = t & 63;
+    const int nbm = nbA > nbB ? nbA : nbB;
+#pragma unroll 1
+    for (int h = 0; h < 8; ++h) {
+        const int q = h >> 1;
+        if (h & 1) reduce_256_part<KB>(pb, nbB, nbm, q, sh + KA);
+        else reduce_256_part<KA>(pa, nbA, nbm, q, sh);
+    }
+    __syncthreads();
+    if (t < 64) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = wave_sum(lane < 16 ? sh[k][lane] : 0.0);
+    }
+}
+
+// The gradient epilogue of the A^T panel product with the line search of the
+// iteration in one extra workgroup (block 0) of the same launch: the search
+// reads only the partials of descend and of the A product, complete when the
+// launch starts, and leaves active as it is (the panel workgroups read it);
+// the kernels after it take need_fallback as inactive.
+struct EpiGradLS : EpiGrad {
+    static constexpr int kExtra = 1;
+    FCtl *Cw;
+    const double *pa, *pb;
+    int nbA, nbB, mode;
+    __device__ void extra() const {
+        if (!Cw->active) return;
+        double s[6];
+        reduce_wg_256<3, 3>(pa, nbA, pb, nbB, s);
+        if (threadIdx.x == 0) line_search(Cw, s, s + 3, mode, true);
+    }
+};
+
 // end of an inner iteration from update_W's block partials directly
 __global__ __launch_bounds__(1024) void k_iter_end_parts(FCtl *C, const double *p, int nb,
                                                          double *hist) {
     if (!C->active) return;
+    if (C->need_fallback) {          // the line search failed (left active set)
+        __syncthreads();             // every thread has read the flags
+        if (threadIdx.x == 0) C->active = 0;
+        return;
+    }
     double s[7];
     reduce_wg<7, 0>(p, nb, nullptr, 0, s);
     if (threadIdx.x == 0) iter_end(C, s, hist);
@@ -942,18 +1016,26 @@ struct Fused {
                                oy, oz, U[n_], ctl.p, pD);
         launch_forward<float>(*F, U[n_], EpiRes{Au[n_], b, delta, Au[c], ctl.p, pN, {0, 0, 0}}, s,
                               &ctl.p->active);
+        // A^T Au_n - Atb; with the panel form of A^T the line search runs in
+        // an extra workgroup of that launch
+        auto gradient = [&](int gated) {
+            const EpiGrad e{g[n_], Atb, ctl.p, gated};
+            if (F->panels_T) launch_panel(F->PT, Au[n_], EpiGradLS{e, ctl.p, pD, pN, nbD, nbN, mode}, s);
+            else launch_transposed<float>(*F, Au[n_], e, s);
+        };
         if (skipping) {                // get_g's A^T-skip decision (:398-468)
             hipLaunchKernelGGL(k_skip_stats, dim3(nbM), dim3(256), 0, s, m, Au[n_], lastAu, ctl.p,
                                pK);
             hipLaunchKernelGGL(k_skip_decide, dim3(1), dim3(1024), 0, s, ctl.p, pK, nbM, mode);
-            launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p, 1}, s);
+            gradient(1);
             hipLaunchKernelGGL(k_skip_after, dim3(nbM), dim3(256), 0, s, n, m, g[c], g[n_], Au[n_],
                                lastAu, ctl.p);
         } else {
-            launch_transposed<float>(*F, Au[n_], EpiGrad{g[n_], Atb, ctl.p, 0}, s);
+            gradient(0);
         }
-        hipLaunchKernelGGL(k_line_search_parts, dim3(1), dim3(1024), 0, s, ctl.p, pD, nbD, pN, nbN,
-                           mode);
+        if (!F->panels_T)
+            hipLaunchKernelGGL(k_line_search_parts, dim3(1), dim3(1024), 0, s, ctl.p, pD, nbD, pN,
+                               nbN, mode);
         hipLaunchKernelGGL(k_apply_alpha, dim3(512), dim3(256), 0, s, n, m, U[c], U[n_], g[c],
                            g[n_], Au[c], Au[n_], ctl.p);
         update_w(U[n_], U[c], g[n_]);

@@ -79,8 +79,11 @@ __device__ __forceinline__ double fval(const FCtl *C, double l1, double l2, doub
 // a = {|V_t|^2, <V_t,sigma>, <V_c,V_t>}, b = {|r_t|^2, <delta,r_t>, <r_c,r_t>}:
 // U, Au, DU are affine in alpha, so lam2/lam3 are quadratic and lam4/lam5
 // linear in alpha, and the backtracking sweeps of update_g (:706-818) are
-// evaluated on these scalars.
-__device__ inline void line_search(FCtl *C, const double *a, const double *b, int mode) {
+// evaluated on these scalars.  keep_active: a failed search (fallback) sets
+// need_fallback but leaves active to the iteration's end (kernels that run
+// beside the search read active).
+__device__ inline void line_search(FCtl *C, const double *a, const double *b, int mode,
+                                   bool keep_active = false) {
     const double l2t = a[0], l4t = a[1], x01 = a[2], l3t = b[0], l5t = b[1], r01 = b[2];
     const double l2c = C->lam2, l4c = C->lam4, l3c = C->lam3, l5c = C->lam5;
     auto at = [&](double al, double *o) {
@@ -102,7 +105,7 @@ __device__ inline void line_search(FCtl *C, const double *a, const double *b, in
             if (cnt == 5) {
                 C->need_fallback = 1;
                 C->need_tau = 1;
-                C->active = 0;
+                if (!keep_active) C->active = 0;
                 return;
             }
             alpha *= C->gamma;
