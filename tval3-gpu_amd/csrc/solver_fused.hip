@@ -4,7 +4,9 @@
 // Algorithm: 3d/tval3_gpu_3d.cpp:1088-1345 (same formulas, same order of
 // decisions).  What changes is how the work is laid out:
 //
-//  * one steady inner iteration is seven kernels:
+//  * one steady inner iteration is seven steps, one launch each, except
+//    that A x takes two with the panel form of A and the search runs inside
+//    the A^T launch when A^T has its panel form:
 //      descend   U_t = max(U - tau d, 0) and, from U and U_t on the fly,
 //                DU, W = shrink(DU - sigma/beta), V0 = DU - W, V1 = DU_t - W
 //                with |V1|^2, <V1,sigma>, <V0,V1>                  (:671-701)
@@ -14,7 +16,8 @@
 //      search    the nonmonotone Armijo test (:839-890) on f(alpha): U, Au,
 //                g, DU are affine in alpha (:717-737), so lam2/lam3 are
 //                quadratic and lam4/lam5 linear in alpha and the up to five
-//                backtracking sweeps of update_g become scalar work
+//                backtracking sweeps of update_g become scalar work; it
+//                reads only the partials of descend and A x
 //      apply     U, g, Au = prev + alpha (trial - prev), skipped if alpha = 1
 //      update_W  W = shrink(DU - sigma/beta), V = DU - W, g2 = D^T V,
 //                d = g2 + mu/beta g + DtsAtd, and the Barzilai-Borwein sums
