@@ -119,6 +119,42 @@ def test_fused_backtracking_and_fallback(ctx):
     assert np.allclose(h["f"][:4], hp["f"][:4], rtol=1e-5)
 
 
+@pytest.mark.parametrize("panels,skip", [(1, None), (3, None), (3, (2.0, 3, 1e30))])
+def test_fused_fallback_with_each_search_placement(ctx, panels, skip):
+    """The cnt == 5 fallback (3d/tval3_gpu_3d.cpp:876-883) with the line
+    search in a launch of its own (A^T by row blocks, panels = 1) and inside
+    the A^T panel launch (panels = 3), where a failed search leaves active
+    set for the panel workgroups and the kernels after it take need_fallback
+    as inactive; with and without the A^T-skip rule (whose fallback product
+    is forced).  Against the plain solver on the same options: every
+    iteration accepts or falls back, the fallback count matches the
+    recorded cnt == 6 iterations, and the decisions agree for the first
+    iterations."""
+    dims = (16, 16, 16)
+    u, A, A32, oop, b = setup(ctx, dims, 24, 64, seed=9)
+    b = wl.add_noise(b.astype(np.float64), 0.05, seed=14).astype(np.float32)
+    info = A.layout_force(dims, -1024, -1024, 16384, panels)
+    assert info["panels"] == panels
+    o = dict(maxit=40, gamma=1.0, c=0.0, gam=0.0, tol_inn=1e-12, mu=2.0 ** 9, mu0=2.0 ** 9)
+    if skip:
+        o.update(_skip_kw(skip))
+    U, inf_, h = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**o, solver="fused"))
+    Up, ip, hp = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**o))
+    k = first_divergence(h, hp)
+    print(f"panels {panels} skip {skip}: fallbacks fused {inf_['fallbacks']} plain "
+          f"{ip['fallbacks']}; decisions agree for {k}; A^T products fused "
+          f"{inf_['n_matvec_T']} plain {ip['n_matvec_T']}")
+    assert inf_["total_iters"] == 40
+    assert set(h["cnt"].tolist()) <= {0, 6}
+    assert inf_["fallbacks"] == int((h["cnt"] == 6).sum()) > 0
+    assert k >= 5
+    assert np.allclose(h["f"][:4], hp["f"][:4], rtol=1e-5)
+    assert np.isfinite(U).all()
+    if skip:
+        print("A^T products skipped:", int((h["mulT"] == 0).sum()))
+        assert (h["mulT"][h["cnt"] == 6] == 1).all()     # the fallback step multiplies
+
+
 def test_fused_fp16_weights_track_oracle(ctx):
     """FP16 ray weights, FP32 accumulation: the fused solve with FP16 weights
     against the FP32 oracle solving the same FP16-rounded matrix, with the
