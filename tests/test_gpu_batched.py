@@ -58,16 +58,26 @@ def batch(ctx):
     return [make(ctx, *pp) for pp in PROBS]
 
 
+# every voxel count a multiple of 512: A^T can take its panel form
+PROBS8 = [((16, 16, 16), 24, 64, 21), ((16, 8, 24), 16, 48, 24), ((8, 8, 8), 16, 32, 27),
+          ((24, 16, 8), 20, 56, 30)]
+
+
+@pytest.fixture(scope="module")
+def batch8(ctx):
+    return [make(ctx, *pp) for pp in PROBS8]
+
+
 def opts(**kw):
     base = dict(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, tv_mode="iso", maxit=60)
     base.update(kw)
     return api.default_opts(**base)
 
 
-def solve(ctx, probs, o, lanes=(8, 8), history=True):
+def solve(ctx, probs, o, lanes=(8, 8), history=True, panels=-1):
     return api.tval3_gpu_3d_batched(ctx, [p["A"] for p in probs], [p["bt"] for p in probs],
                                     [p["dims"] for p in probs], o, Uts=[p["ut"] for p in probs],
-                                    history=history, lanes=lanes)
+                                    history=history, lanes=lanes, panels=panels)
 
 
 @pytest.mark.parametrize("lanes", [(8, 8), (-2048, -1024)])
@@ -121,9 +131,13 @@ def first_divergence(h1, h2):
     return n
 
 
-def test_batch_tracks_oracle(ctx, batch):
+@pytest.mark.parametrize("which,panels", [("batch", -1), ("batch8", 2048)])
+def test_batch_tracks_oracle(ctx, request, which, panels):
+    """(batch8 with A^T in panel form)"""
+    batch = request.getfixturevalue(which)
     o = opts()
-    Ub, ib, _, hb = solve(ctx, batch, o)
+    Ub, ib, bi, hb = solve(ctx, batch, o, panels=panels)
+    assert bi["panels_T"] == max(panels, 0)
     oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=60, tv_iso=1)
     for k, p in enumerate(batch):
         Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
@@ -191,6 +205,10 @@ def test_batch_errors(ctx, batch):
         api.tval3_gpu_3d_batched(ctx, [batch[0]["A"]], [batch[0]["bt"]], [(16, 16, 15)], opts())
     with pytest.raises(TV3Error):        # negative skip ratio
         solve(ctx, batch[:1], opts(skipMulRatio=-0.5))
+    with pytest.raises(TV3Error):        # panels of 512 voxels would straddle two problems
+        solve(ctx, batch, opts(), panels=2048)
+    with pytest.raises(TV3Error):        # no such row-block size
+        solve(ctx, batch[:1], opts(), panels=512)
     with pytest.raises(ValueError):
         api.tval3_gpu_3d_batched(ctx, [batch[0]["A"]], [batch[0]["bt"][:-1]], [(16, 16, 16)])
     dense = api.DenseOperator(ctx, np.ones((4, 512)))
@@ -268,3 +286,33 @@ def test_batch_skip_rule_that_never_skips_changes_nothing(ctx, batch):
             assert i0[k]["total_iters"] == i1[k]["total_iters"]
             assert i0[k]["n_matvec_T"] == i1[k]["n_matvec_T"]
             assert (h1[k]["mulT"] == 1).all()
+
+
+@pytest.mark.parametrize("blk", [1024, 2048])
+def test_batch_panels_equal_solo_bitwise(ctx, batch8, blk):
+    """A^T of the sparse problems in panel form (tiles of 512 voxels, each
+    tile's rays staged in LDS): each problem of the batch equals its solo
+    solve bit for bit -- the problems stop at different iterations, and the
+    panels of a stopped problem are skipped whole while the others run --
+    and agrees with the row-kernel batch to rounding level at the start."""
+    o = opts(maxit=120, tol_inn=5e-3, mu0=2.0 ** 6, beta0=4.0)   # continuation steps
+    Ub, ib, bi, hb = solve(ctx, batch8, o, panels=blk)
+    print("iterations", [i["total_iters"] for i in ib])
+    assert bi["panels_T"] == blk
+    assert len({i["total_iters"] for i in ib}) > 1       # problems stop apart
+    for k, p in enumerate(batch8):
+        Us, is_, bs, hs = solve(ctx, [p], o, panels=blk)
+        assert bs["panels_T"] == blk
+        assert ib[k]["total_iters"] == is_[0]["total_iters"] > 0
+        assert np.array_equal(hb[k]["f"], hs[0]["f"])
+        assert np.array_equal(Ub[k].cpu().numpy(), Us[0].cpu().numpy()), k
+    # the row kernels sum in another order: rounding-level agreement at the
+    # start (accuracy over the whole run: test_batch_tracks_oracle)
+    Ur, ir, br, hr = solve(ctx, batch8, o, panels=-1)
+    assert br["panels_T"] == 0
+    for k in range(len(batch8)):
+        f, fr = hb[k]["f"][:4], hr[k]["f"][:4]
+        assert (np.abs(f - fr) <= 1e-5 * np.abs(fr)).all()
+    # timed choice (0): one of the two sizes or the row kernels, reported
+    _, _, bt, _ = solve(ctx, batch8, o, panels=0, history=False)
+    assert bt["panels_T"] in (0, 1024, 2048)

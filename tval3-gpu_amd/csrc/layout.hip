@@ -467,6 +467,18 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
     }
 }
 
+bool build_transposed_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
+                             std::vector<Panels> &out) {
+    if (dtype_size(AT.vtype) != 2 && AT.vtype != TV3_F32) return false;
+    const HostCSR h = download_csr(ctx, AT);
+    PanelPlan pl;
+    if (!plan_panels(h, 512, 16384, pl)) return false;
+    out.clear();
+    out.resize(blks.size());
+    for (size_t i = 0; i < blks.size(); ++i) build_panels_T(ctx, h, pl, blks[i], out[i]);
+    return true;
+}
+
 void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L) {
     TV3_REQUIRE(op->kind == OpKind::CSR, TV3_E_UNSUPPORTED,
                 "the fused solver needs a sparse operator (materialize a dynamic operator first)");

@@ -104,6 +104,11 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
     const int s0 = src_off[pan], ns = src_off[pan + 1] - s0;
     const int bb = pan_blk[pan], b1 = pan_blk[pan + 1];
     BlkDesc d = desc[bb];
+    if constexpr (HasSkipRows<Epi>::value) {
+        // a panel's rows lie in one row range of the epilogue (tiles never
+        // straddle one): its first row decides, uniformly
+        if (bb < b1 && epi.skip_rows(d.r0)) return;
+    }
     // stage the sources: batches of 8 per thread, all loads in flight
     for (int k0 = 0; k0 < ns; k0 += 8 * 256) {
         float xv[8];

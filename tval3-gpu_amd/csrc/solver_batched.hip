@@ -36,7 +36,7 @@
 #include <cstring>
 #include <vector>
 
-#include "spmv_stream.hpp"
+#include "panels.hpp"
 #include "tval3_step.hpp"
 #include "vec.hpp"
 
@@ -701,6 +701,8 @@ struct Batch {
     tv3_dtype vt = TV3_F32;
     SparseRows A, AT;                // block-diagonal CSR of A and of A^T
     int lanes_N = 0, lanes_T = 0;
+    Panels PT;                       // A^T in panel form (when panels_T != 0)
+    int panels_T = 0;                // its row-block size, 0: A^T by the row kernels
     DevBuf<BDims> ddims;
     DevBuf<int32_t> vst, rst;
     DevBuf<float> pool;
@@ -761,7 +763,10 @@ struct Batch {
         for (const DGroup &g : groups) group_N(g, x, y, g.sN);
     }
     void prod_T(const float *x, float *y, const float *sub, int pred) {
-        if (AT.rows) product(AT, x, y, sub, vst.p, pred);
+        if (AT.rows) {
+            if (panels_T) launch_panel(PT, x, EpiBatch{y, sub, vst.p, P, ctl.p, pred}, s);
+            else product(AT, x, y, sub, vst.p, pred);
+        }
         for (const DGroup &g : groups) group_T(g, x, y, sub, g.sT);
     }
 
@@ -876,6 +881,42 @@ struct Batch {
                     if (t < bT) { bT = t; g.sT = S; }
                 }
             }
+        }
+    }
+    // A^T in panel form: want = 1024 / 2048 (that row-block size), -1
+    // (never) or 0 (the faster of both sizes when faster than the tuned row
+    // kernel).  Tiles of 512 voxels must not straddle two problems.
+    void choose_panels(int want, const std::vector<int64_t> &v0, int ncsr) {
+        panels_T = 0;
+        if (!AT.rows || want < 0) return;
+        bool aligned = true;
+        for (int k = 0; k < ncsr; ++k) aligned = aligned && v0[k] % 512 == 0;
+        TV3_REQUIRE(want == 0 || want == 1024 || want == 2048, TV3_E_ARG,
+                    "panels_T must be 1024, 2048, -1 or 0");
+        if (!aligned) {
+            TV3_REQUIRE(want == 0, TV3_E_UNSUPPORTED,
+                        "panel form of A^T needs every sparse problem's voxel count to be a "
+                        "multiple of 512");
+            return;
+        }
+        const std::vector<int> blks = want ? std::vector<int>{want} : std::vector<int>{1024, 2048};
+        std::vector<Panels> cand;
+        if (!build_transposed_panels(ctx, AT, blks, cand)) {
+            TV3_REQUIRE(want == 0, TV3_E_UNSUPPORTED,
+                        "a tile of this batch crosses too many rays for the panel form");
+            return;
+        }
+        if (want) {
+            PT = std::move(cand[0]);
+            panels_T = want;
+            return;
+        }
+        float best = time_ms([&] { product(AT, AuT, gT, nullptr, vst.p, P_ALL); });
+        for (size_t i = 0; i < cand.size(); ++i) {
+            const float t = time_ms([&] {
+                launch_panel(cand[i], AuT, EpiBatch{gT, nullptr, vst.p, P, ctl.p, P_ALL}, s);
+            });
+            if (t < best) { best = t; PT = std::move(cand[i]); panels_T = blks[i]; }
         }
     }
     static constexpr int kMaxSplits = 32;
@@ -1086,6 +1127,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
     } else {
         B.tune();
     }
+    B.choose_panels(binfo ? binfo->panels_T : 0, v0, ncsr);
     if (binfo && binfo->splits_N > 0 && binfo->splits_T > 0) {
         for (auto &g : B.groups) {
             TV3_REQUIRE(gemm_rhs_shape_ok(g.op->m, g.op->n, binfo->splits_N) &&
@@ -1298,6 +1340,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
             binfo->splits_N = B.groups[0].sN;
             binfo->splits_T = B.groups[0].sT;
         }
+        binfo->panels_T = B.panels_T;
         binfo->graph_iters = B.graph_iters;
         binfo->graph_launches = graph_launches;
         binfo->secs = ms / 1000.0;

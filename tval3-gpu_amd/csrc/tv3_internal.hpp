@@ -311,6 +311,12 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]);
 // the layout's numbering only: voxel and ray maps, A and its CSC renumbered
 // (no panel forms, no tuned launch shapes)
 void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L);
+// panel forms of a prebuilt A^T (rows = voxels in tiles of 512 rows, sources =
+// rays), one per row-block size in blks; false when a panel would need more
+// distinct rays than its local indices address or the weights are not F32,
+// F16 or BF16
+bool build_transposed_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
+                             std::vector<Panels> &out);
 // y = A x or A^T x with the layout's kernels, natural order in and out
 void layout_matvec(tv3_op *op, const int32_t dims[3], bool trans, const float *x, float *y,
                    hipStream_t s);
