@@ -248,15 +248,16 @@ typedef struct tv3_batch_info {
     double loop_secs;           /* GPU time of the iterations                           */
     double host_secs;           /* host wall time of the call, set-up included          */
     int64_t total_iters;        /* inner iterations summed over the problems            */
-    int64_t nnz, voxels, rays;  /* batch totals (voxels, rays padded to 256 per problem) */
+    int64_t nnz, voxels, rays;  /* batch totals (per problem, voxels padded to 512 and
+                                   rays to 256)                                          */
     int32_t dense_groups;       /* shared dense operators (matrix-core products)        */
     int32_t splits_N, splits_T; /* K splits of the dense products (in/out, like lanes)  */
     int32_t panels_T;           /* A^T of the sparse problems in panel form (tiles of
-                                   512 voxels, each tile's rays staged in LDS), possible
-                                   when every sparse problem's voxel count is a multiple
-                                   of 512; in: 1024 or 2048 uses it with row blocks of
-                                   that many entries, -1 never, 0 timed against the row
-                                   kernels; out: the block size used, 0 when not used */
+                                   512 voxels, each tile's rays staged in LDS; every
+                                   problem's voxel range is padded to a multiple of 512);
+                                   in: 1024 or 2048 uses it with row blocks of that many
+                                   entries, -1 never, 0 timed against the row kernels;
+                                   out: the block size used, 0 when not used */
     int64_t dense_values;       /* entries of the shared dense operators                */
 } tv3_batch_info;
 

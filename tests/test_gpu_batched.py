@@ -58,7 +58,7 @@ def batch(ctx):
     return [make(ctx, *pp) for pp in PROBS]
 
 
-# every voxel count a multiple of 512: A^T can take its panel form
+# every dimension a multiple of 8: update_W on 8^3 tiles for every problem
 PROBS8 = [((16, 16, 16), 24, 64, 21), ((16, 8, 24), 16, 48, 24), ((8, 8, 8), 16, 32, 27),
           ((24, 16, 8), 20, 56, 30)]
 
@@ -80,15 +80,18 @@ def solve(ctx, probs, o, lanes=(8, 8), history=True, panels=-1):
                                     history=history, lanes=lanes, panels=panels)
 
 
-@pytest.mark.parametrize("lanes", [(8, 8), (-2048, -1024)])
-def test_batch_equals_solo_bitwise(ctx, batch, lanes):
-    """row-group products (8 lanes per row) and row-block products (blocks of
-    2048 / 1024 entries, which must start at every problem's first row)"""
+@pytest.mark.parametrize("lanes,panels", [((8, 8), -1), ((-2048, -1024), -1),
+                                          ((-2048, -1024), 2048)])
+def test_batch_equals_solo_bitwise(ctx, batch, lanes, panels):
+    """row-group products (8 lanes per row), row-block products (blocks of
+    2048 / 1024 entries, which must start at every problem's first row) and
+    A^T in panel form; the batch mixes the B = 8, 4 and 1 numberings, so
+    update_W runs on tiles for some problems and per voxel for the others"""
     o = opts()
-    Ub, ib, bi, hb = solve(ctx, batch, o, lanes=lanes)
+    Ub, ib, bi, hb = solve(ctx, batch, o, lanes=lanes, panels=panels)
     print("batch info", bi)
     for k, p in enumerate(batch):
-        Us, is_, _, hs = solve(ctx, [p], o, lanes=lanes)
+        Us, is_, _, hs = solve(ctx, [p], o, lanes=lanes, panels=panels)
         assert ib[k]["total_iters"] == is_[0]["total_iters"] > 0
         assert ib[k]["outer_iters"] == is_[0]["outer_iters"]
         assert (hb[k]["cnt"] == hs[0]["cnt"]).all()
@@ -205,8 +208,6 @@ def test_batch_errors(ctx, batch):
         api.tval3_gpu_3d_batched(ctx, [batch[0]["A"]], [batch[0]["bt"]], [(16, 16, 15)], opts())
     with pytest.raises(TV3Error):        # negative skip ratio
         solve(ctx, batch[:1], opts(skipMulRatio=-0.5))
-    with pytest.raises(TV3Error):        # panels of 512 voxels would straddle two problems
-        solve(ctx, batch, opts(), panels=2048)
     with pytest.raises(TV3Error):        # no such row-block size
         solve(ctx, batch[:1], opts(), panels=512)
     with pytest.raises(ValueError):

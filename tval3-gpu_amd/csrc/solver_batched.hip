@@ -9,7 +9,8 @@
 //
 //  * Problems are concatenated, each in the fused solver's numbering (voxels
 //    in 8^3 tiles of 4^3 bricks where its dims allow, rays in a locality
-//    order); each problem's voxel and ray ranges start at a multiple of 256.
+//    order); each problem's voxel range starts at a multiple of 512, its ray
+//    range at a multiple of 256.
 //    The system matrix is the block-diagonal concatenation of the problems'
 //    renumbered CSR matrices (global voxel indices), its CSC likewise.
 //  * A workgroup works inside one problem (256-element groups never cross a
@@ -37,6 +38,7 @@
 #include <vector>
 
 #include "panels.hpp"
+#include "tiles.hpp"
 #include "tval3_step.hpp"
 #include "vec.hpp"
 
@@ -221,9 +223,13 @@ __global__ __launch_bounds__(256) void k_b_descend(const BDims *__restrict__ D,
 
 // update_W + D^T + gradient + BB sums (:965-1029, :896-919, :1017-1067).
 // same = 1: the new iterate is the current copy (set-up), else the other one.
+// tiled = 1: problems of the B = 8 numbering are left to k_b_update_w_tile.
+// Workgroup w covers voxel group gfirst + w; partial sums at that group's
+// slot of nbV.
 __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
                                                     const int32_t *__restrict__ vst, int P,
                                                     const FCtl *__restrict__ C, int same,
+                                                    int tiled, int gfirst, int nbV,
                                                     const float *__restrict__ U0,
                                                     const float *__restrict__ U1,
                                                     const float *__restrict__ sx,
@@ -233,16 +239,18 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
                                                     const float *__restrict__ g1,
                                                     const float *__restrict__ dts,
                                                     float *__restrict__ d, double *part) {
-    const int k = find_prob(vst, P, blockIdx.x);
+    const int grp = gfirst + (int)blockIdx.x;
+    const int k = find_prob(vst, P, grp);
     const FCtl &c = C[k];
     if (!c.active) return;
     const BDims dm = D[k];
+    if (tiled && dm.B == 8) return;
     const int nw = same ? c.cur : c.cur ^ 1;
     const float *U = nw ? U1 : U0, *Uc = c.cur ? U1 : U0, *g = nw ? g1 : g0;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const float beta = (float)c.beta, ib = 1.f / beta, mdb = (float)c.muDbeta;
     const bool iso = c.iso;
-    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
+    const int64_t i = (int64_t)grp * GB + threadIdx.x;
     Vox v;
     if (voxel_at(dm, i, v)) {
         const int y = v.y, x = v.x, z = v.z;
@@ -292,7 +300,97 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
         acc[6] += (double)dn * dn;
         d[i] = dn;
     }
-    block_sum_store<7>(acc, part, gridDim.x);
+    block_sum_store_at<7>(acc, part, nbV, grp);
+}
+
+// k_b_update_w on one 8x8x8 tile per workgroup for the problems of the B = 8
+// numbering (whose voxel ranges start at multiples of 512): U staged in LDS
+// over the tile and a one-voxel shell, V = DU - W computed once for every
+// voxel of the tile and of its three lower faces (where D^T reads it), D^T V
+// from LDS.  Per voxel the arithmetic is k_b_update_w's, so d is the same;
+// the tile's sums go to the first of its two 256-voxel group slots and the
+// second gets zeros, so the per-problem sums read them like k_b_update_w's.
+__global__ __launch_bounds__(512) void k_b_update_w_tile(const BDims *__restrict__ D,
+                                                         const int32_t *__restrict__ vst, int P,
+                                                         const FCtl *__restrict__ C, int same,
+                                                         const float *__restrict__ U0,
+                                                         const float *__restrict__ U1,
+                                                         const float *__restrict__ sx,
+                                                         const float *__restrict__ sy,
+                                                         const float *__restrict__ sz,
+                                                         const float *__restrict__ g0,
+                                                         const float *__restrict__ g1,
+                                                         const float *__restrict__ dts,
+                                                         float *__restrict__ d, double *part,
+                                                         int nbV) {
+    constexpr int NBOX = kTileBox * kTileBox * kTileBox;
+    __shared__ float su[NBOX], svx[NBOX], svy[NBOX], svz[NBOX];
+    const int T = blockIdx.x;                       // voxels 512 T .. 512 T + 511
+    const int k = find_prob(vst, P, 2 * T);
+    const FCtl &c = C[k];
+    if (!c.active) return;
+    const BDims dm = D[k];
+    if (dm.B != 8) return;
+    const int nw = same ? c.cur : c.cur ^ 1;
+    const float *U = nw ? U1 : U0, *Uc = c.cur ? U1 : U0, *g = nw ? g1 : g0;
+    const float beta = (float)c.beta, ib = 1.f / beta, mdb = (float)c.muDbeta;
+    const bool iso = c.iso;
+    const int t = threadIdx.x;
+    const float *Uk = U + dm.v0;                     // the problem's volume
+    const int lt = T - (int)(dm.v0 >> 9);            // tile of the problem
+    const int nby = dm.nby, nbx = dm.nbx, nbz = dm.r >> 3;
+    const int ty = lt % nby, rest = lt / nby, tx = rest % nbx, tz = rest / nbx;
+    // box entries t and t + 512, sigma at the tile voxel t and (t < 192) at
+    // the lower-face voxel 512 + t; indices clamped instead of guarded
+    const int k1 = t + 512 < NBOX ? t + 512 : NBOX - 1;
+    int a0, b0, c0;
+    tile_pos(t, a0, b0, c0);
+    const int q1 = t < 192 ? t : 191;
+    const int f = q1 >> 6, j = q1 & 63, fu = j & 7, fv = j >> 3;
+    const int a1 = f == 0 ? -1 : fu, b1 = f == 0 ? fu : (f == 1 ? -1 : fv),
+              c1 = f == 2 ? -1 : fv;
+    const float ub0 = Uk[tile_gidx(nby, nbx, nbz, ty, tx, tz, box_off(t))];
+    const float ub1 = Uk[tile_gidx(nby, nbx, nbz, ty, tx, tz, box_off(k1))];
+    const int64_t i = ((int64_t)T << 9) + t;
+    const int64_t i1 = dm.v0 + tile_gidx(nby, nbx, nbz, ty, tx, tz, tile_off(a1, b1, c1));
+    const float s0[3] = {sx[i], sy[i], sz[i]}, s1[3] = {sx[i1], sy[i1], sz[i1]};
+    const float gv = g[i], dv = dts[i], ucv = Uc[i], dold = d[i];
+    su[t] = ub0;
+    if (t + 512 < NBOX) su[t + 512] = ub1;
+    __syncthreads();
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    auto vat = [&](int a, int b, int cc, const float *s, bool tile_voxel) {
+        const int sl = box_slot(a, b, cc);
+        const float u0 = su[sl], uyp = su[sl + 1], uxp = su[sl + kTileBox],
+                    uzp = su[sl + kTileBox * kTileBox];
+        float wx, wy, wz;
+        const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s[0], s[1], s[2], ib, iso,
+                                 wx, wy, wz);
+        const float vx = (uxp - u0) - wx, vy = (uyp - u0) - wy, vz = (uzp - u0) - wz;
+        svx[sl] = vx;
+        svy[sl] = vy;
+        svz[sl] = vz;
+        if (tile_voxel) {
+            acc[0] += l1;
+            acc[1] += (double)vx * vx + (double)vy * vy + (double)vz * vz;
+            acc[2] += (double)vx * s[0] + (double)vy * s[1] + (double)vz * s[2];
+        }
+    };
+    vat(a0, b0, c0, s0, true);
+    if (t < 192) vat(a1, b1, c1, s1, false);
+    __syncthreads();
+    const int sl = box_slot(a0, b0, c0);
+    const float g2 = svy[sl - 1] - svy[sl] + svx[sl - kTileBox] - svx[sl] +
+                     svz[sl - kTileBox * kTileBox] - svz[sl];              // D^T V, :501
+    const float dn = mdb * gv + g2 + dv;                                    // :915-918
+    const float du = su[sl] - ucv;
+    acc[3] += (double)du * du;
+    acc[4] += (double)du * (double)(dn - dold);
+    acc[5] += (double)ucv * ucv;
+    acc[6] += (double)dn * dn;
+    d[i] = dn;
+    block_sum_store_at<7, 8>(acc, part, nbV, 2 * T);
+    if (t < 7) part[(int64_t)t * nbV + 2 * T + 1] = 0.0;
 }
 
 // |D d|^2 for the first-iteration step length (:587-596)
@@ -683,6 +781,9 @@ __global__ void k_scatter_unscale(const float *__restrict__ in, const int32_t *_
 }
 
 inline int64_t pad256(int64_t v) { return (v + GB - 1) / GB * GB; }
+// voxel ranges: multiples of 512, so 8^3 tiles and 512-row panels never
+// straddle two problems (a problem solved alone gets the same padding)
+inline int64_t pad512(int64_t v) { return (v + 511) / 512 * 512; }
 
 inline unsigned grid_for(int64_t n) {
     int64_t b = (n + 255) / 256;
@@ -701,6 +802,9 @@ struct Batch {
     tv3_dtype vt = TV3_F32;
     SparseRows A, AT;                // block-diagonal CSR of A and of A^T
     int lanes_N = 0, lanes_T = 0;
+    bool tiles = false;              // update_W of B = 8 problems on 8^3 tiles
+    bool untiled = true;             // problems left to the per-voxel update_W
+    int gfirst = 0;                  // first voxel group of those problems
     Panels PT;                       // A^T in panel form (when panels_T != 0)
     int panels_T = 0;                // its row-block size, 0: A^T by the row kernels
     DevBuf<BDims> ddims;
@@ -791,10 +895,43 @@ struct Batch {
         hipLaunchKernelGGL(k_b_search, bP, t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, pD, pN);
         hipLaunchKernelGGL(k_b_apply, dim3(nbV + nbR), t, 0, s, ddims.p, vst.p, rst.p, P, nbV,
                            ctl.p, U[0], U[1], T, g[0], g[1], gT, Au[0], Au[1], AuT);
-        hipLaunchKernelGGL(k_b_update_w, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, 0, U[0], U[1],
-                           sx, sy, sz, g[0], g[1], dts, d, pW);
+        update_w(0);
         hipLaunchKernelGGL(k_b_iter_end, bP, t, 0, s, ctl.p, vst.p, nbV, pW, hist.p,
                            hist_stride);
+    }
+
+    // update_W of every active problem (same = 1: of the current copy)
+    void update_w(int same) {
+        if (tiles)
+            hipLaunchKernelGGL(k_b_update_w_tile, dim3(nbV / 2), dim3(512), 0, s, ddims.p, vst.p,
+                               P, ctl.p, same, U[0], U[1], sx, sy, sz, g[0], g[1], dts, d, pW,
+                               nbV);
+        if (untiled)
+            hipLaunchKernelGGL(k_b_update_w, dim3(nbV - gfirst), dim3(256), 0, s, ddims.p,
+                               vst.p, P, ctl.p, same, tiles ? 1 : 0, gfirst, nbV, U[0], U[1],
+                               sx, sy, sz, g[0], g[1], dts, d, pW);
+    }
+
+    // the tile kernel takes the B = 8 problems when all of them start at a
+    // multiple of 512 voxels (tiles then never straddle two problems)
+    void choose_tiles() {
+        bool any8 = false, aligned = true, other = false;
+        for (const BDims &bd : dims) {
+            if (bd.B == 8) {
+                any8 = true;
+                aligned = aligned && bd.v0 % 512 == 0 && bd.n % 512 == 0;
+            } else {
+                other = true;
+            }
+        }
+        tiles = any8 && aligned;
+        untiled = !tiles || other;
+        gfirst = 0;
+        if (tiles && other) {
+            gfirst = nbV;
+            for (const BDims &bd : dims)
+                if (bd.B != 8) gfirst = std::min(gfirst, (int)(bd.v0 / GB));
+        }
     }
 
     // first-iteration step lengths of the problems that need one
@@ -1015,7 +1152,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         bd.v0 = N; bd.n = n; bd.r0 = M; bd.m = op->m;
         v0[k] = N; r0[k] = M;
         vst[k] = (int32_t)(N / GB); rst[k] = (int32_t)(M / GB);
-        N += pad256(n);
+        N += pad512(n);
         M += pad256(std::max<int64_t>(op->m, 1));
         if (k + 1 == ncsr) { Ncsr = N; Mcsr = M; }
     }
@@ -1052,6 +1189,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         for (int k = 0; k < ncsr; ++k) { lay[k]->A = SparseRows(); lay[k]->AT = SparseRows(); }
     }
     TV3_HIP(hipMemcpyAsync(B.ddims.p, B.dims.data(), sizeof(BDims) * P, hipMemcpyHostToDevice, s));
+    B.choose_tiles();
     // dense groups
     size_t gscratch = 0;
     for (int k = ncsr; k < P;) {
@@ -1065,7 +1203,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         dense_rowmajor(op, s);
         Batch::DGroup g;
         g.op = op; g.k0 = k; g.G = e - k;
-        g.v0 = v0[k]; g.r0 = r0[k]; g.ldn = pad256(op->n); g.ldm = pad256(op->m);
+        g.v0 = v0[k]; g.r0 = r0[k]; g.ldn = pad512(op->n); g.ldm = pad256(op->m);
         B.groups.push_back(g);
         gscratch = std::max({gscratch, gemm_rhs_scratch(op->m, Batch::kMaxSplits),
                              gemm_rhs_scratch(op->n, Batch::kMaxSplits)});
@@ -1199,8 +1337,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
     B.prod_T(B.Au[0], B.g[0], B.Atb, P_ALL);
     if (B.skipping)                  // the initial get_g multiplies (forced): last Au
         TV3_HIP(hipMemcpyAsync(B.last, B.Au[0], sizeof(float) * M, hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_b_update_w, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P, B.ctl.p,
-                       1, B.U[0], B.U[1], B.sx, B.sy, B.sz, B.g[0], B.g[1], B.dts, B.d, B.pW);
+    B.update_w(1);
     hipLaunchKernelGGL(k_b_init, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p, B.rst.p, B.nbV,
                        B.nbR, B.pN, B.pW);
     TV3_HIP(hipGetLastError());

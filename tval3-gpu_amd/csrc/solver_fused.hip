@@ -42,6 +42,7 @@
 #include <cstring>
 
 #include "panels.hpp"
+#include "tiles.hpp"
 #include "tval3_step.hpp"
 #include "vec.hpp"
 
@@ -266,43 +267,10 @@ __global__ __launch_bounds__(256) void k_update_w(Lay L, Groups G, const float *
 // shell ([-1, 8]^3), V = DU - W is computed once for every voxel of the tile
 // and of its three lower faces (where D^T reads it), and D^T V reads V from
 // LDS.  Per voxel the arithmetic is k_update_w's, so d is identical.
-constexpr int kTileBox = 10;   // shell edge: voxels -1 .. 8
-
-__device__ __forceinline__ int box_slot(int a, int b, int c) {
-    return ((c + 1) * kTileBox + (b + 1)) * kTileBox + (a + 1);
-}
-
-// Tiles of B = 8: tile voxel q (internal order) -> offsets (a, b, c) =
-// (y, x, z) in the tile.
-__device__ __forceinline__ void tile_pos(int q, int &a, int &b, int &c) {
-    const int sub = q >> 6, w = q & 63;
-    a = ((sub & 1) << 2) | (w & 3);
-    b = (((sub >> 1) & 1) << 2) | ((w >> 2) & 3);
-    c = ((sub >> 2) << 2) | (w >> 4);
-}
-
-// A voxel at offsets (a, b, c) in [-1, 8]^3 from a tile's corner lies in the
-// tile shifted by (a >> 3, b >> 3, c >> 3) in {-1, 0, 1}^3 (periodic wrap of
-// the tile coordinates: every dimension is a multiple of 8) at the in-tile
-// position of (a & 7, b & 7, c & 7); both parts are fixed per thread.
-struct TileOff {
-    int da, db, dc, off;
-};
-
-__device__ __forceinline__ TileOff tile_off(int a, int b, int c) {
-    const int ai = a & 7, bi = b & 7, ci = c & 7;
-    const int sub = ((ci >> 2) << 2) | ((bi >> 2) << 1) | (ai >> 2);
-    return TileOff{a >> 3, b >> 3, c >> 3, (sub << 6) + ((ci & 3) << 4) + ((bi & 3) << 2) + (ai & 3)};
-}
-
 // internal index of that voxel for the tile at tile coordinates (ty, tx, tz)
 __device__ __forceinline__ int64_t tile_gidx(const Lay &L, int ty, int tx, int tz,
                                              const TileOff &w) {
-    int y = ty + w.da, x = tx + w.db, z = tz + w.dc;
-    y = y < 0 ? y + L.nby : (y >= L.nby ? y - L.nby : y);
-    x = x < 0 ? x + L.nbx : (x >= L.nbx ? x - L.nbx : x);
-    z = z < 0 ? z + L.nbz : (z >= L.nbz ? z - L.nbz : z);
-    return ((int64_t)((z * L.nbx + x) * L.nby + y) << 9) + w.off;
+    return tile_gidx(L.nby, L.nbx, L.nbz, ty, tx, tz, w);
 }
 
 __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, int ntiles, int tpw,
@@ -339,9 +307,6 @@ __global__ __launch_bounds__(512) void k_update_w_tile(Lay L, int ntiles, int tp
     const int f = q1 >> 6, j = q1 & 63, fu = j & 7, fv = j >> 3;
     const int a1 = f == 0 ? -1 : fu, b1 = f == 0 ? fu : (f == 1 ? -1 : fv),
               c1 = f == 2 ? -1 : fv;
-    auto box_off = [](int k) {
-        return tile_off(k % kTileBox - 1, (k / kTileBox) % kTileBox - 1, k / 100 - 1);
-    };
     const TileOff w0 = box_off(t), w1 = box_off(k1), wf = tile_off(a1, b1, c1);
     struct Loads { float ub[2], s0[3], s1[3], gv, dv, ucv, dold; };
     auto load = [&](int tile, Loads &r) {

@@ -33,6 +33,14 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[K], double *partia
     }
 }
 
+// The same sums stored at partials[k * nblk + slot] (a workgroup that stands
+// for several partial slots).
+template <int K, int NW = 4>
+__device__ __forceinline__ void block_sum_store_at(double (&acc)[K], double *partials, int nblk,
+                                                   int slot) {
+    block_sum_store<K, NW>(acc, partials + slot - (int64_t)blockIdx.x, nblk);
+}
+
 template <typename T> void axpy3(int64_t n, T *z, double a, const T *x, const T *y, hipStream_t s);
 template <typename T>
 void grad(int64_t n, T *d, double a, const T *g, const T *g2, const T *dts, hipStream_t s);
