@@ -259,6 +259,10 @@ typedef struct tv3_batch_info {
                                    entries, -1 never, 0 timed against the row kernels;
                                    out: the block size used, 0 when not used */
     int64_t dense_values;       /* entries of the shared dense operators                */
+    int32_t panels_A;           /* A of the sparse problems in two-pass panel form
+                                   (segment sums per tile, then per ray); in and out
+                                   like panels_T                                        */
+    int32_t reserved;
 } tv3_batch_info;
 
 tv3_status tv3_solve_batched(tv3_ctx *ctx, int32_t nprob, tv3_op *const *ops,

@@ -74,24 +74,27 @@ def opts(**kw):
     return api.default_opts(**base)
 
 
-def solve(ctx, probs, o, lanes=(8, 8), history=True, panels=-1):
+def solve(ctx, probs, o, lanes=(8, 8), history=True, panels=-1, panels_A=-1):
     return api.tval3_gpu_3d_batched(ctx, [p["A"] for p in probs], [p["bt"] for p in probs],
                                     [p["dims"] for p in probs], o, Uts=[p["ut"] for p in probs],
-                                    history=history, lanes=lanes, panels=panels)
+                                    history=history, lanes=lanes, panels=panels,
+                                    panels_A=panels_A)
 
 
-@pytest.mark.parametrize("lanes,panels", [((8, 8), -1), ((-2048, -1024), -1),
-                                          ((-2048, -1024), 2048)])
-def test_batch_equals_solo_bitwise(ctx, batch, lanes, panels):
+@pytest.mark.parametrize("lanes,panels,panels_A", [((8, 8), -1, -1), ((-2048, -1024), -1, -1),
+                                                   ((-2048, -1024), 2048, -1),
+                                                   ((-2048, -1024), 2048, 1024)])
+def test_batch_equals_solo_bitwise(ctx, batch, lanes, panels, panels_A):
     """row-group products (8 lanes per row), row-block products (blocks of
     2048 / 1024 entries, which must start at every problem's first row) and
-    A^T in panel form; the batch mixes the B = 8, 4 and 1 numberings, so
-    update_W runs on tiles for some problems and per voxel for the others"""
+    A^T and A in panel form; the batch mixes the B = 8, 4 and 1 numberings,
+    so update_W runs on tiles for some problems and per voxel for the others"""
     o = opts()
-    Ub, ib, bi, hb = solve(ctx, batch, o, lanes=lanes, panels=panels)
+    Ub, ib, bi, hb = solve(ctx, batch, o, lanes=lanes, panels=panels, panels_A=panels_A)
+    assert (bi["panels_T"], bi["panels_A"]) == (max(panels, 0), max(panels_A, 0))
     print("batch info", bi)
     for k, p in enumerate(batch):
-        Us, is_, _, hs = solve(ctx, [p], o, lanes=lanes, panels=panels)
+        Us, is_, _, hs = solve(ctx, [p], o, lanes=lanes, panels=panels, panels_A=panels_A)
         assert ib[k]["total_iters"] == is_[0]["total_iters"] > 0
         assert ib[k]["outer_iters"] == is_[0]["outer_iters"]
         assert (hb[k]["cnt"] == hs[0]["cnt"]).all()
@@ -136,11 +139,11 @@ def first_divergence(h1, h2):
 
 @pytest.mark.parametrize("which,panels", [("batch", -1), ("batch8", 2048)])
 def test_batch_tracks_oracle(ctx, request, which, panels):
-    """(batch8 with A^T in panel form)"""
+    """(batch8 with A^T and A in panel form)"""
     batch = request.getfixturevalue(which)
     o = opts()
-    Ub, ib, bi, hb = solve(ctx, batch, o, panels=panels)
-    assert bi["panels_T"] == max(panels, 0)
+    Ub, ib, bi, hb = solve(ctx, batch, o, panels=panels, panels_A=panels)
+    assert bi["panels_T"] == bi["panels_A"] == max(panels, 0)
     oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=60, tv_iso=1)
     for k, p in enumerate(batch):
         Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
@@ -292,28 +295,29 @@ def test_batch_skip_rule_that_never_skips_changes_nothing(ctx, batch):
 @pytest.mark.parametrize("blk", [1024, 2048])
 def test_batch_panels_equal_solo_bitwise(ctx, batch8, blk):
     """A^T of the sparse problems in panel form (tiles of 512 voxels, each
-    tile's rays staged in LDS): each problem of the batch equals its solo
+    tile's rays staged in LDS) and A in two-pass panel form (segment sums
+    per tile, then per ray): each problem of the batch equals its solo
     solve bit for bit -- the problems stop at different iterations, and the
     panels of a stopped problem are skipped whole while the others run --
     and agrees with the row-kernel batch to rounding level at the start."""
     o = opts(maxit=120, tol_inn=5e-3, mu0=2.0 ** 6, beta0=4.0)   # continuation steps
-    Ub, ib, bi, hb = solve(ctx, batch8, o, panels=blk)
+    Ub, ib, bi, hb = solve(ctx, batch8, o, panels=blk, panels_A=blk)
     print("iterations", [i["total_iters"] for i in ib])
-    assert bi["panels_T"] == blk
+    assert bi["panels_T"] == bi["panels_A"] == blk
     assert len({i["total_iters"] for i in ib}) > 1       # problems stop apart
     for k, p in enumerate(batch8):
-        Us, is_, bs, hs = solve(ctx, [p], o, panels=blk)
-        assert bs["panels_T"] == blk
+        Us, is_, bs, hs = solve(ctx, [p], o, panels=blk, panels_A=blk)
+        assert bs["panels_T"] == bs["panels_A"] == blk
         assert ib[k]["total_iters"] == is_[0]["total_iters"] > 0
         assert np.array_equal(hb[k]["f"], hs[0]["f"])
         assert np.array_equal(Ub[k].cpu().numpy(), Us[0].cpu().numpy()), k
     # the row kernels sum in another order: rounding-level agreement at the
     # start (accuracy over the whole run: test_batch_tracks_oracle)
-    Ur, ir, br, hr = solve(ctx, batch8, o, panels=-1)
-    assert br["panels_T"] == 0
+    Ur, ir, br, hr = solve(ctx, batch8, o, panels=-1, panels_A=-1)
+    assert br["panels_T"] == br["panels_A"] == 0
     for k in range(len(batch8)):
         f, fr = hb[k]["f"][:4], hr[k]["f"][:4]
         assert (np.abs(f - fr) <= 1e-5 * np.abs(fr)).all()
     # timed choice (0): one of the two sizes or the row kernels, reported
-    _, _, bt, _ = solve(ctx, batch8, o, panels=0, history=False)
-    assert bt["panels_T"] in (0, 1024, 2048)
+    _, _, bt, _ = solve(ctx, batch8, o, panels=0, panels_A=0, history=False)
+    assert bt["panels_T"] in (0, 1024, 2048) and bt["panels_A"] in (0, 1024, 2048)

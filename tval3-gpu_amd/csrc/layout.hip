@@ -479,6 +479,27 @@ bool build_transposed_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vect
     return true;
 }
 
+bool build_forward_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
+                          const std::vector<int64_t> &row_starts, std::vector<Panels> &P,
+                          std::vector<SegSum> &R, std::vector<int32_t> &seg_starts) {
+    if (dtype_size(AT.vtype) != 2 && AT.vtype != TV3_F32) return false;
+    const HostCSR h = download_csr(ctx, AT);
+    PanelPlan pl;
+    if (!plan_panels(h, 512, 16384, pl)) return false;
+    // panels never straddle a 512-row unit, so each range's segments are
+    // those of the panels from its first row on
+    seg_starts.assign(row_starts.size() + 1, (int32_t)pl.seg_ray.size());
+    size_t p = 0;
+    for (size_t k = 0; k < row_starts.size(); ++k) {
+        while (p + 1 < pl.pan_row.size() && pl.pan_row[p] < row_starts[k]) ++p;
+        seg_starts[k] = pl.pan_seg[p];
+    }
+    P.clear(); R.clear();
+    P.resize(blks.size()); R.resize(blks.size());
+    for (size_t i = 0; i < blks.size(); ++i) build_panels_A(ctx, h, pl, blks[i], P[i], R[i]);
+    return true;
+}
+
 void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L) {
     TV3_REQUIRE(op->kind == OpKind::CSR, TV3_E_UNSUPPORTED,
                 "the fused solver needs a sparse operator (materialize a dynamic operator first)");

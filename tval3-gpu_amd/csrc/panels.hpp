@@ -186,6 +186,10 @@ __global__ __launch_bounds__(256) void k_seg_sum(int64_t rows, const int32_t *__
         const int64_t lb = w * NLB + j;
         const int64_t row0 = (lb * 256 + threadIdx.x) / L;
         has[j] = lb < nlogical && row0 < rows;
+        if constexpr (HasSkipRows<Epi>::value) {
+            // a logical block's rays lie in one row range of the epilogue
+            if (has[j] && epi.skip_rows((int)(lb * (256 / L)))) has[j] = false;
+        }
         row[j] = has[j] ? row0 : rows - 1;
         beg[j] = ptr[row[j]];
         end[j] = has[j] ? ptr[row[j] + 1] : beg[j];

@@ -172,6 +172,27 @@ struct EpiBatch {
     __device__ __forceinline__ void finish() const {}
 };
 
+// Pass 1 of A x in panel form over the sparse problems: segment partial sums;
+// a panel whose first segment belongs to a problem whose predicate is off is
+// skipped (a panel holds the segments of one tile, so of one problem).
+struct EpiSegBatch {
+    float *part;
+    const int32_t *seg_starts;   // first segment of each sparse problem (+ total)
+    int P;                       // sparse problems
+    const FCtl *C;
+    int pred;
+    struct Pre {};
+    __device__ bool skip() const { return false; }
+    __device__ __forceinline__ bool skip_rows(int row) const {
+        return !pred_on(C[find_prob(seg_starts, P, row)], pred);
+    }
+    __device__ __forceinline__ Pre load(int) const { return {}; }
+    __device__ __forceinline__ void row(int r, float s, bool active, const Pre &) const {
+        if (active) part[r] = s;
+    }
+    __device__ __forceinline__ void finish() const {}
+};
+
 // ------------------------------------------------------------------ stencils
 // descend: U_t = max(U_c - tau d, 0) (:690-696); V0/V1 statistics.
 __global__ __launch_bounds__(256) void k_b_descend(const BDims *__restrict__ D,
@@ -807,6 +828,11 @@ struct Batch {
     int gfirst = 0;                  // first voxel group of those problems
     Panels PT;                       // A^T in panel form (when panels_T != 0)
     int panels_T = 0;                // its row-block size, 0: A^T by the row kernels
+    Panels PA;                       // A in two-pass panel form (when panels_A != 0)
+    SegSum RS;
+    DevBuf<int32_t> segst;           // first segment of each sparse problem (+ total)
+    int panels_A = 0;
+    int ncsr = 0;                    // sparse problems (the first ncsr)
     DevBuf<BDims> ddims;
     DevBuf<int32_t> vst, rst;
     DevBuf<float> pool;
@@ -863,7 +889,14 @@ struct Batch {
                      g.ldn, sub ? sub + g.v0 + c * g.ldn : nullptr, splits, gpart.p, s);
     }
     void prod_N(const float *x, float *y, int pred) {
-        if (A.rows) product(A, x, y, nullptr, rst.p, pred);
+        if (A.rows) {
+            if (panels_A) {
+                launch_panel(PA, x, EpiSegBatch{RS.part.p, segst.p, ncsr, ctl.p, pred}, s);
+                launch_seg_sum(RS, EpiBatch{y, nullptr, rst.p, P, ctl.p, pred}, s);
+            } else {
+                product(A, x, y, nullptr, rst.p, pred);
+            }
+        }
         for (const DGroup &g : groups) group_N(g, x, y, g.sN);
     }
     void prod_T(const float *x, float *y, const float *sub, int pred) {
@@ -1054,6 +1087,45 @@ struct Batch {
                 launch_panel(cand[i], AuT, EpiBatch{gT, nullptr, vst.p, P, ctl.p, P_ALL}, s);
             });
             if (t < best) { best = t; PT = std::move(cand[i]); panels_T = blks[i]; }
+        }
+    }
+    // A in two-pass panel form (segment sums per tile, then per ray): want as
+    // for choose_panels
+    void choose_panels_A(int want, const std::vector<int64_t> &v0) {
+        panels_A = 0;
+        if (!A.rows || want < 0) return;
+        TV3_REQUIRE(want == 0 || want == 1024 || want == 2048, TV3_E_ARG,
+                    "panels_A must be 1024, 2048, -1 or 0");
+        const std::vector<int> blks = want ? std::vector<int>{want} : std::vector<int>{1024, 2048};
+        std::vector<Panels> cp;
+        std::vector<SegSum> cr;
+        std::vector<int32_t> st;
+        const std::vector<int64_t> starts(v0.begin(), v0.begin() + ncsr);
+        if (!build_forward_panels(ctx, AT, blks, starts, cp, cr, st)) {
+            TV3_REQUIRE(want == 0, TV3_E_UNSUPPORTED,
+                        "a tile of this batch crosses too many rays for the panel form");
+            return;
+        }
+        segst.alloc(st.size());
+        TV3_HIP(hipMemcpyAsync(segst.p, st.data(), sizeof(int32_t) * st.size(),
+                               hipMemcpyHostToDevice, s));
+        for (SegSum &r : cr) r.rows = A.rows;      // the sparse problems' rays only
+        auto run = [&](const Panels &pp, const SegSum &rr) {
+            launch_panel(pp, U[0], EpiSegBatch{rr.part.p, segst.p, ncsr, ctl.p, P_ALL}, s);
+            launch_seg_sum(rr, EpiBatch{AuT, nullptr, rst.p, P, ctl.p, P_ALL}, s);
+        };
+        int pick = want ? 0 : -1;
+        if (!want) {
+            float best = time_ms([&] { product(A, U[0], AuT, nullptr, rst.p, P_ALL); });
+            for (size_t i = 0; i < cp.size(); ++i) {
+                const float t = time_ms([&] { run(cp[i], cr[i]); });
+                if (t < best) { best = t; pick = (int)i; }
+            }
+        }
+        if (pick >= 0) {
+            PA = std::move(cp[pick]);
+            RS = std::move(cr[pick]);
+            panels_A = blks[pick];
         }
     }
     static constexpr int kMaxSplits = 32;
@@ -1266,6 +1338,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         B.tune();
     }
     B.choose_panels(binfo ? binfo->panels_T : 0, v0, ncsr);
+    B.ncsr = ncsr;
+    B.choose_panels_A(binfo ? binfo->panels_A : 0, v0);
     if (binfo && binfo->splits_N > 0 && binfo->splits_T > 0) {
         for (auto &g : B.groups) {
             TV3_REQUIRE(gemm_rhs_shape_ok(g.op->m, g.op->n, binfo->splits_N) &&
@@ -1478,6 +1552,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
             binfo->splits_T = B.groups[0].sT;
         }
         binfo->panels_T = B.panels_T;
+        binfo->panels_A = B.panels_A;
         binfo->graph_iters = B.graph_iters;
         binfo->graph_launches = graph_launches;
         binfo->secs = ms / 1000.0;

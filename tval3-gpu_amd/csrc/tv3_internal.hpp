@@ -317,6 +317,13 @@ void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L);
 // F16 or BF16
 bool build_transposed_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
                              std::vector<Panels> &out);
+// the two-pass panel form of A from a prebuilt A^T (rows of A^T = voxels in
+// units of 512), one per row-block size; seg_starts[k] = first segment of
+// the voxel range starting at row_starts[k] (row_starts ascending, multiples
+// of 512; seg_starts gets one more entry: the segment count)
+bool build_forward_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
+                          const std::vector<int64_t> &row_starts, std::vector<Panels> &P,
+                          std::vector<SegSum> &R, std::vector<int32_t> &seg_starts);
 // y = A x or A^T x with the layout's kernels, natural order in and out
 void layout_matvec(tv3_op *op, const int32_t dims[3], bool trans, const float *x, float *y,
                    hipStream_t s);

@@ -385,7 +385,7 @@ def tval3_gpu_3d(ctx, A, b, dims, opts=None, Ut=None, U0=None, history=True, his
 
 
 def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
-                         hist_cap=4096, lanes=None, splits=None, panels=None):
+                         hist_cap=4096, lanes=None, splits=None, panels=None, panels_A=None):
     """Solve many problems together (tv3_solve_batched): ops a list of sparse
     operators (one weight type), bs their b (float32 torch tensors on the
     device), dims a list of (dim_y, dim_x, dim_z).  Every kernel of an
@@ -395,7 +395,8 @@ def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
     products' launch shapes, splits = (splits_N, splits_T) the dense
     products' K splits (default: timed); panels = 1024 or 2048 puts A^T of the
     sparse problems in panel form with row blocks of that size, -1 never
-    (default: timed where possible).  Returns (list of U tensors, list of
+    (default: timed where possible); panels_A likewise for the two-pass
+    panel form of A.  Returns (list of U tensors, list of
     info dicts, batch info dict, list of history dicts or None)."""
     import torch
     o = opts if opts is not None else default_opts()
@@ -434,6 +435,8 @@ def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
         bi.splits_N, bi.splits_T = splits
     if panels is not None:
         bi.panels_T = panels
+    if panels_A is not None:
+        bi.panels_A = panels_A
     torch.cuda.synchronize()
     ctx.check(L.load().tv3_solve_batched(ctx.h, P, opsa, d, C.byref(o), ba, ua, uta, infos,
                                          hp, C.byref(bi)))
