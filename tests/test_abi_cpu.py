@@ -54,10 +54,12 @@ def test_struct_layouts_match_header():
 #include <stddef.h>
 #include "tval3_mi355x.h"
 int main(void){
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tv3_opts), sizeof(tv3_info),
-         sizeof(tv3_history), sizeof(tv3_geometry), offsetof(tv3_opts, solver),
-         offsetof(tv3_geometry, origin), sizeof(tv3_layout_info),
-         offsetof(tv3_layout_info, max_seg));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tv3_opts),
+         sizeof(tv3_info), sizeof(tv3_history), sizeof(tv3_geometry),
+         offsetof(tv3_opts, solver), offsetof(tv3_geometry, origin), sizeof(tv3_layout_info),
+         offsetof(tv3_layout_info, max_seg), sizeof(tv3_batch_info),
+         offsetof(tv3_batch_info, panels_T), offsetof(tv3_batch_info, panels_A),
+         offsetof(tv3_opts, tiles_per_wg));
   return 0;
 }'''
     tmp = os.path.join(ROOT, "tests", "_layout_probe")
@@ -72,7 +74,8 @@ int main(void){
                 os.remove(p)
     exp = [C.sizeof(L.Opts), C.sizeof(L.Info), C.sizeof(L.History), C.sizeof(L.Geometry),
            L.Opts.solver.offset, L.Geometry.origin.offset, C.sizeof(L.LayoutInfo),
-           L.LayoutInfo.max_seg.offset]
+           L.LayoutInfo.max_seg.offset, C.sizeof(L.BatchInfo), L.BatchInfo.panels_T.offset,
+           L.BatchInfo.panels_A.offset, L.Opts.tiles_per_wg.offset]
     assert got == exp
 
 
