@@ -123,7 +123,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the solver has no "
-              "multi-GPU path in this build", file=sys.stderr)
+              "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     import torch

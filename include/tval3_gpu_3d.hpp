@@ -173,7 +173,7 @@ inline tv3_ctx *context(int device) {
 
 inline int device_of(int mainGPU, int numGPUs, const int *gpuArray) {
     if (numGPUs > 1 || (gpuArray && numGPUs == 1 && gpuArray[0] != (mainGPU < 0 ? 0 : mainGPU)))
-        throw tval3_gpu_exception("tval3: this build runs on one GPU (numGPUs <= 1)");
+        throw tval3_gpu_exception("tval3: this library runs on one GPU (numGPUs <= 1)");
     return mainGPU < 0 ? 0 : mainGPU;
 }
 

@@ -40,7 +40,7 @@ typedef enum {
     TV3_E_HIP = 5,        /* HIP runtime error (reference: HANDLE_ERROR / cuda_exception) */
     TV3_E_NONFINITE = 6,  /* NaN/Inf in b, A or the iterates                            */
     TV3_E_INTERNAL = 7,
-    TV3_E_UNSUPPORTED = 8 /* option the reference has but this build does not support    */
+    TV3_E_UNSUPPORTED = 8 /* option the reference has but this library does not support  */
 } tv3_status;
 
 typedef enum { TV3_N = 0, TV3_T = 1 } tv3_trans;   /* CUBLAS_OP_N / CUBLAS_OP_T */

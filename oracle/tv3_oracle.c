@@ -1,5 +1,5 @@
 /*
- * tv3_oracle.c -- CPU restatement of the TVAL3 3-D hot path (checker only).
+ * tv3_oracle.c -- CPU restatement of the TVAL3 3-D hot path (for tests only).
  *
  * TEST INFRASTRUCTURE ONLY: linked or called only by tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py.

@@ -1,8 +1,8 @@
 /*
  * tv3_oracle.h -- CPU restatement of the TVAL3 3-D hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * implementation: it may be linked or called only from tests/, from
+ * TEST INFRASTRUCTURE ONLY.  The HIP implementation's results are checked
+ * against this library: it may be linked or called only from tests/, from
  * __graft_entry__.smoke() and from the cpu_baseline leg of bench.py.  The
  * product path never calls it.
  *

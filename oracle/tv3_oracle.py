@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/libtv3oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product path never imports
+cpu_baseline leg of bench.py, to check results against.  The product path never imports
 this module.
 
 The C sources it binds restate the reference algorithm; each C function cites
