@@ -11,10 +11,11 @@ namespace tv3 {
 
 // Sum K per-thread doubles over a workgroup of NW wavefronts (4: 256
 // threads, 8: 512) -- wavefront sums (wave_sum), then the wavefront totals
-// in a fixed order; thread 0 stores partials[k*nblk + blockIdx.x].  Every
-// thread of the workgroup calls it.
+// in a fixed order; thread 0 stores partials[k*nblk + slot].  Every thread of
+// the workgroup calls it.
 template <int K, int NW = 4>
-__device__ __forceinline__ void block_sum_store(double (&acc)[K], double *partials, int nblk) {
+__device__ __forceinline__ void block_sum_store_at(double (&acc)[K], double *partials, int nblk,
+                                                   int slot) {
     __shared__ double sh[K][NW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -28,17 +29,15 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[K], double *partia
         for (int k = 0; k < K; ++k) {
             double v = (sh[k][0] + sh[k][1]) + (sh[k][2] + sh[k][3]);
             if (NW == 8) v += (sh[k][4] + sh[k][5]) + (sh[k][6] + sh[k][7]);
-            partials[(int64_t)k * nblk + blockIdx.x] = v;
+            partials[(int64_t)k * nblk + slot] = v;
         }
     }
 }
 
-// The same sums stored at partials[k * nblk + slot] (a workgroup that stands
-// for several partial slots).
+// The same at the workgroup's own slot (blockIdx.x).
 template <int K, int NW = 4>
-__device__ __forceinline__ void block_sum_store_at(double (&acc)[K], double *partials, int nblk,
-                                                   int slot) {
-    block_sum_store<K, NW>(acc, partials + slot - (int64_t)blockIdx.x, nblk);
+__device__ __forceinline__ void block_sum_store(double (&acc)[K], double *partials, int nblk) {
+    block_sum_store_at<K, NW>(acc, partials, nblk, (int)blockIdx.x);
 }
 
 template <typename T> void axpy3(int64_t n, T *z, double a, const T *x, const T *y, hipStream_t s);
