@@ -251,13 +251,15 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
                                                     const int32_t *__restrict__ vst, int P,
                                                     const FCtl *__restrict__ C, int same,
                                                     int tiled, int gfirst, int nbV,
-                                                    const float *__restrict__ U0,
-                                                    const float *__restrict__ U1,
+                                                    float *__restrict__ U0,
+                                                    float *__restrict__ U1,
+                                                    const float *__restrict__ T,
                                                     const float *__restrict__ sx,
                                                     const float *__restrict__ sy,
                                                     const float *__restrict__ sz,
-                                                    const float *__restrict__ g0,
-                                                    const float *__restrict__ g1,
+                                                    float *__restrict__ g0,
+                                                    float *__restrict__ g1,
+                                                    const float *__restrict__ gT,
                                                     const float *__restrict__ dts,
                                                     float *__restrict__ d, double *part) {
     const int grp = gfirst + (int)blockIdx.x;
@@ -266,8 +268,18 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
     if (!c.active) return;
     const BDims dm = D[k];
     if (tiled && dm.B == 8) return;
-    const int nw = same ? c.cur : c.cur ^ 1;
-    const float *U = nw ? U1 : U0, *Uc = c.cur ? U1 : U0, *g = nw ? g1 : g0;
+    const float *Uc = c.cur ? U1 : U0;
+    // the new iterate: the current copy (same) or the accepted step, read
+    // from the trial T (blended with U_c when alpha != 1) and stored here
+    const float *Ur = same ? Uc : T;
+    const bool blend = !same && c.alpha != 1.0;
+    const float al = (float)c.alpha;
+    auto U = [&](int64_t j) {
+        const float t = Ur[j];
+        if (!blend) return t;
+        const float u = Uc[j];
+        return u + al * (t - u);
+    };
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const float beta = (float)c.beta, ib = 1.f / beta, mdb = (float)c.muDbeta;
     const bool iso = c.iso;
@@ -277,8 +289,8 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
         const int y = v.y, x = v.x, z = v.z;
         const int yp = nxt(y, v.p), ym = prv(y, v.p), xp = nxt(x, v.q), xm = prv(x, v.q),
                   zp = nxt(z, v.r), zm = prv(z, v.r);
-        const float u0 = U[i];
-        const float uyp = U[v.at(yp, x, z)], uxp = U[v.at(y, xp, z)], uzp = U[v.at(y, x, zp)];
+        const float u0 = U(i);
+        const float uyp = U(v.at(yp, x, z)), uxp = U(v.at(y, xp, z)), uzp = U(v.at(y, x, zp));
         float wx, wy, wz;
         const float s_x = sx[i], s_y = sy[i], s_z = sz[i];
         const float l1 = shrink3(uxp - u0, uyp - u0, uzp - u0, s_x, s_y, s_z, ib, iso, wx,
@@ -287,30 +299,42 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
         float vyb, vxb, vzb;
         {
             const int64_t j = v.at(ym, x, z);
-            const float uj = U[j];
-            const float dyj = u0 - uj, dxj = U[v.at(ym, xp, z)] - uj, dzj = U[v.at(ym, x, zp)] - uj;
+            const float uj = U(j);
+            const float dyj = u0 - uj, dxj = U(v.at(ym, xp, z)) - uj, dzj = U(v.at(ym, x, zp)) - uj;
             float a, b, cc;
             shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, cc);
             vyb = dyj - b;
         }
         {
             const int64_t j = v.at(y, xm, z);
-            const float uj = U[j];
-            const float dxj = u0 - uj, dyj = U[v.at(yp, xm, z)] - uj, dzj = U[v.at(y, xm, zp)] - uj;
+            const float uj = U(j);
+            const float dxj = u0 - uj, dyj = U(v.at(yp, xm, z)) - uj, dzj = U(v.at(y, xm, zp)) - uj;
             float a, b, cc;
             shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, cc);
             vxb = dxj - a;
         }
         {
             const int64_t j = v.at(y, x, zm);
-            const float uj = U[j];
-            const float dzj = u0 - uj, dxj = U[v.at(y, xp, zm)] - uj, dyj = U[v.at(yp, x, zm)] - uj;
+            const float uj = U(j);
+            const float dzj = u0 - uj, dxj = U(v.at(y, xp, zm)) - uj, dyj = U(v.at(yp, x, zm)) - uj;
             float a, b, cc;
             shrink3(dxj, dyj, dzj, sx[j], sy[j], sz[j], ib, iso, a, b, cc);
             vzb = dzj - cc;
         }
         const float g2 = vyb - vy + vxb - vx + vzb - vz;            // D^T V, :501
-        const float dn = mdb * g[i] + g2 + dts[i];                   // :915-918
+        float gv;
+        if (same) {
+            gv = (c.cur ? g1 : g0)[i];
+        } else {
+            gv = gT[i];
+            if (blend) {
+                const float gc = (c.cur ? g1 : g0)[i];
+                gv = gc + al * (gv - gc);
+            }
+            (c.cur ? U0 : U1)[i] = u0;
+            (c.cur ? g0 : g1)[i] = gv;
+        }
+        const float dn = mdb * gv + g2 + dts[i];                     // :915-918
         const float uc = Uc[i], du = u0 - uc;
         acc[0] += l1;
         acc[1] += (double)vx * vx + (double)vy * vy + (double)vz * vz;
@@ -334,13 +358,15 @@ __global__ __launch_bounds__(256) void k_b_update_w(const BDims *__restrict__ D,
 __global__ __launch_bounds__(512) void k_b_update_w_tile(const BDims *__restrict__ D,
                                                          const int32_t *__restrict__ vst, int P,
                                                          const FCtl *__restrict__ C, int same,
-                                                         const float *__restrict__ U0,
-                                                         const float *__restrict__ U1,
+                                                         float *__restrict__ U0,
+                                                         float *__restrict__ U1,
+                                                         const float *__restrict__ Ut,
                                                          const float *__restrict__ sx,
                                                          const float *__restrict__ sy,
                                                          const float *__restrict__ sz,
-                                                         const float *__restrict__ g0,
-                                                         const float *__restrict__ g1,
+                                                         float *__restrict__ g0,
+                                                         float *__restrict__ g1,
+                                                         const float *__restrict__ gT,
                                                          const float *__restrict__ dts,
                                                          float *__restrict__ d, double *part,
                                                          int nbV) {
@@ -352,12 +378,16 @@ __global__ __launch_bounds__(512) void k_b_update_w_tile(const BDims *__restrict
     if (!c.active) return;
     const BDims dm = D[k];
     if (dm.B != 8) return;
-    const int nw = same ? c.cur : c.cur ^ 1;
-    const float *U = nw ? U1 : U0, *Uc = c.cur ? U1 : U0, *g = nw ? g1 : g0;
+    // the new iterate: as in k_b_update_w (same: the current copy, else the
+    // trial blended at alpha, stored in the other copy)
+    const float *Uc = c.cur ? U1 : U0;
+    const float *Ur = same ? Uc : Ut;
+    const bool blend = !same && c.alpha != 1.0;
+    const float al = (float)c.alpha;
     const float beta = (float)c.beta, ib = 1.f / beta, mdb = (float)c.muDbeta;
     const bool iso = c.iso;
     const int t = threadIdx.x;
-    const float *Uk = U + dm.v0;                     // the problem's volume
+    const float *Uk = Ur + dm.v0;                    // the problem's volume
     const int lt = T - (int)(dm.v0 >> 9);            // tile of the problem
     const int nby = dm.nby, nbx = dm.nbx, nbz = dm.r >> 3;
     const int ty = lt % nby, rest = lt / nby, tx = rest % nbx, tz = rest / nbx;
@@ -370,12 +400,21 @@ __global__ __launch_bounds__(512) void k_b_update_w_tile(const BDims *__restrict
     const int f = q1 >> 6, j = q1 & 63, fu = j & 7, fv = j >> 3;
     const int a1 = f == 0 ? -1 : fu, b1 = f == 0 ? fu : (f == 1 ? -1 : fv),
               c1 = f == 2 ? -1 : fv;
-    const float ub0 = Uk[tile_gidx(nby, nbx, nbz, ty, tx, tz, box_off(t))];
-    const float ub1 = Uk[tile_gidx(nby, nbx, nbz, ty, tx, tz, box_off(k1))];
+    const int64_t j0 = tile_gidx(nby, nbx, nbz, ty, tx, tz, box_off(t));
+    const int64_t j1 = tile_gidx(nby, nbx, nbz, ty, tx, tz, box_off(k1));
+    float ub0 = Uk[j0], ub1 = Uk[j1];
     const int64_t i = ((int64_t)T << 9) + t;
     const int64_t i1 = dm.v0 + tile_gidx(nby, nbx, nbz, ty, tx, tz, tile_off(a1, b1, c1));
     const float s0[3] = {sx[i], sy[i], sz[i]}, s1[3] = {sx[i1], sy[i1], sz[i1]};
-    const float gv = g[i], dv = dts[i], ucv = Uc[i], dold = d[i];
+    const float dv = dts[i], ucv = Uc[i], dold = d[i];
+    float gv = same ? (c.cur ? g1 : g0)[i] : gT[i];
+    if (blend) {
+        const float c0u = Uc[dm.v0 + j0], c1u = Uc[dm.v0 + j1];
+        ub0 = c0u + al * (ub0 - c0u);
+        ub1 = c1u + al * (ub1 - c1u);
+        const float gc = (c.cur ? g1 : g0)[i];
+        gv = gc + al * (gv - gc);
+    }
     su[t] = ub0;
     if (t + 512 < NBOX) su[t + 512] = ub1;
     __syncthreads();
@@ -410,6 +449,10 @@ __global__ __launch_bounds__(512) void k_b_update_w_tile(const BDims *__restrict
     acc[5] += (double)ucv * ucv;
     acc[6] += (double)dn * dn;
     d[i] = dn;
+    if (!same) {
+        (c.cur ? U0 : U1)[i] = su[sl];
+        (c.cur ? g0 : g1)[i] = gv;
+    }
     block_sum_store_at<7, 8>(acc, part, nbV, 2 * T);
     if (t < 7) part[(int64_t)t * nbV + 2 * T + 1] = 0.0;
 }
@@ -633,51 +676,28 @@ __global__ __launch_bounds__(256) void k_b_mlp_delta(const BDims *__restrict__ D
     block_sum_store<1>(acc, part, gridDim.x);
 }
 
-// accepted copies: X_new = X_c + alpha (X_t - X_c), X_t itself when alpha = 1
-// (voxel groups first, then ray groups)
+// accepted Au: Au_new = Au_c + alpha (Au_t - Au_c), Au_t itself when alpha = 1
+// (ray groups; U and g are accepted inside update_W)
 __global__ __launch_bounds__(256) void k_b_apply(const BDims *__restrict__ D,
-                                                 const int32_t *__restrict__ vst,
-                                                 const int32_t *__restrict__ rst, int P, int nbV,
+                                                 const int32_t *__restrict__ rst, int P,
                                                  const FCtl *__restrict__ C,
-                                                 float *__restrict__ U0, float *__restrict__ U1,
-                                                 const float *__restrict__ T,
-                                                 float *__restrict__ g0, float *__restrict__ g1,
-                                                 const float *__restrict__ gT,
                                                  float *__restrict__ Au0, float *__restrict__ Au1,
                                                  const float *__restrict__ AuT) {
-    const bool vox = (int)blockIdx.x < nbV;
-    const int w = vox ? (int)blockIdx.x : (int)blockIdx.x - nbV;
-    const int k = find_prob(vox ? vst : rst, P, w);
+    const int w = (int)blockIdx.x;
+    const int k = find_prob(rst, P, w);
     const FCtl &c = C[k];
     if (!c.active) return;
     const BDims dm = D[k];
-    const float a = (float)c.alpha;
-    const bool one = c.alpha == 1.0;
     const int64_t i = (int64_t)w * GB + threadIdx.x;
-    if (vox) {
-        if (i - dm.v0 >= dm.n) return;
-        const float *Uc = c.cur ? U1 : U0, *gc = c.cur ? g1 : g0;
-        float *Un = c.cur ? U0 : U1, *gn = c.cur ? g0 : g1;
-        const float t = T[i], gt = gT[i];
-        if (one) {
-            Un[i] = t;
-            gn[i] = gt;
-        } else {
-            const float u = Uc[i], g = gc[i];
-            Un[i] = u + a * (t - u);
-            gn[i] = g + a * (gt - g);
-        }
+    if (i - dm.r0 >= dm.m) return;
+    const float *Ac = c.cur ? Au1 : Au0;
+    float *An = c.cur ? Au0 : Au1;
+    const float t = AuT[i];
+    if (c.alpha == 1.0) {
+        An[i] = t;
     } else {
-        if (i - dm.r0 >= dm.m) return;
-        const float *Ac = c.cur ? Au1 : Au0;
-        float *An = c.cur ? Au0 : Au1;
-        const float t = AuT[i];
-        if (one) {
-            An[i] = t;
-        } else {
-            const float v = Ac[i];
-            An[i] = v + a * (t - v);
-        }
+        const float a = (float)c.alpha, v = Ac[i];
+        An[i] = v + a * (t - v);
     }
 }
 
@@ -926,8 +946,7 @@ struct Batch {
             prod_T(AuT, gT, Atb, P_ACTIVE);
         }
         hipLaunchKernelGGL(k_b_search, bP, t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, pD, pN);
-        hipLaunchKernelGGL(k_b_apply, dim3(nbV + nbR), t, 0, s, ddims.p, vst.p, rst.p, P, nbV,
-                           ctl.p, U[0], U[1], T, g[0], g[1], gT, Au[0], Au[1], AuT);
+        hipLaunchKernelGGL(k_b_apply, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, Au[0], Au[1], AuT);
         update_w(0);
         hipLaunchKernelGGL(k_b_iter_end, bP, t, 0, s, ctl.p, vst.p, nbV, pW, hist.p,
                            hist_stride);
@@ -937,12 +956,12 @@ struct Batch {
     void update_w(int same) {
         if (tiles)
             hipLaunchKernelGGL(k_b_update_w_tile, dim3(nbV / 2), dim3(512), 0, s, ddims.p, vst.p,
-                               P, ctl.p, same, U[0], U[1], sx, sy, sz, g[0], g[1], dts, d, pW,
-                               nbV);
+                               P, ctl.p, same, U[0], U[1], T, sx, sy, sz, g[0], g[1], gT, dts, d,
+                               pW, nbV);
         if (untiled)
             hipLaunchKernelGGL(k_b_update_w, dim3(nbV - gfirst), dim3(256), 0, s, ddims.p,
-                               vst.p, P, ctl.p, same, tiles ? 1 : 0, gfirst, nbV, U[0], U[1],
-                               sx, sy, sz, g[0], g[1], dts, d, pW);
+                               vst.p, P, ctl.p, same, tiles ? 1 : 0, gfirst, nbV, U[0], U[1], T,
+                               sx, sy, sz, g[0], g[1], gT, dts, d, pW);
     }
 
     // the tile kernel takes the B = 8 problems when all of them start at a
