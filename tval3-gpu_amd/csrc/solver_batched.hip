@@ -22,15 +22,18 @@
 //  * Per-problem reductions: every workgroup stores its partial sums; one
 //    workgroup per problem sums its problem's slice in a fixed order and runs
 //    the scalar step of that problem.
-//  * Products: the fused solver's CSR kernels over the block-diagonal matrix
-//    -- row groups of L lanes per row, or row blocks of NB entries staged in
-//    LDS (shape timed per direction at set-up) -- whose row groups and blocks
-//    skip the rows of inactive problems; problems sharing a dense BF16
+//  * Products: the fused solver's kernels over the block-diagonal matrix --
+//    row groups of L lanes per row, row blocks of NB entries staged in LDS,
+//    or the panel forms (A^T per 512-voxel tile with the tile's rays in LDS;
+//    A as segment sums per tile, then per ray), chosen by timing at set-up
+//    -- whose row groups, blocks and panels skip inactive problems; update_W
+//    runs on 8^3 tiles for the B = 8 problems; problems sharing a dense BF16
 //    operator run as one matrix-core product with up to 64 right-hand sides.
 //
 // One inner iteration: descend (trial U_t, V statistics), A U_t, residual
-// statistics, A^T A U_t - Atb, line search, apply alpha (accepted copies),
-// update_W (d, BB sums), end of iteration -- 8 launches for any batch.
+// statistics, A^T A U_t - Atb, line search, apply alpha to Au, update_W
+// (accepts U and g at alpha; d, BB sums), end of iteration -- a fixed number
+// of launches for any batch.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
