@@ -321,3 +321,36 @@ def test_batch_panels_equal_solo_bitwise(ctx, batch8, blk):
     # timed choice (0): one of the two sizes or the row kernels, reported
     _, _, bt, _ = solve(ctx, batch8, o, panels=0, panels_A=0, history=False)
     assert bt["panels_T"] in (0, 1024, 2048) and bt["panels_A"] in (0, 1024, 2048)
+
+
+def test_batch_backtracking_blends(ctx, batch):
+    """Backtracking steps (1 <= cnt <= 5, alpha < 1): update_W reads the
+    trial blended with the current iterate at alpha and stores the accepted
+    U and g (apply keeps only Au).  A larger Armijo constant makes the search
+    backtrack; each problem of the batch equals its solo solve bit for bit
+    and tracks the FP32 oracle (same decisions, the oracle's envelope)."""
+    o = opts(c=0.3, maxit=40)
+    Ub, ib, _, hb = solve(ctx, batch, o, lanes=(-2048, -1024))
+    cnts = np.concatenate([h["cnt"] for h in hb])
+    print("cnt histogram", np.bincount(cnts, minlength=7).tolist())
+    assert ((cnts >= 1) & (cnts <= 5)).any()
+    oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=40, tv_iso=1, c=0.3)
+    for k, p in enumerate(batch):
+        Us, is_, _, hs = solve(ctx, [p], o, lanes=(-2048, -1024))
+        assert np.array_equal(Ub[k].cpu().numpy(), Us[0].cpu().numpy()), k
+        assert np.array_equal(hb[k]["f"], hs[0]["f"])
+        Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
+        runs = []
+        for seed in (0, 1, 2):
+            xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=p["b"].shape)
+            bp = (p["b"].astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+            runs.append(orc.solve(p["oop"], p["dims"], bp, oo, Ut=p["u"], dtype=np.float32))
+        kk = min([first_divergence(hb[k], ho)] + [first_divergence(r[2], ho) for r in runs])
+        dev = np.abs(hb[k]["f"][:kk] - ho["f"][:kk]) / np.abs(ho["f"][:kk])
+        env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:kk] - ho["f"][:kk])
+                                            / np.abs(ho["f"][:kk])) for r in runs], axis=0)
+        print(f"{p['dims']}: decisions agree for {kk}; backtracks {ib[k]['backtracks']}; "
+              f"max f dev {dev.max():.1e} envelope {env.max():.1e}")
+        assert kk >= 10
+        assert (dev[:4] < 1e-5).all()
+        assert (dev <= 30 * env + 1e-6).all()
