@@ -348,3 +348,38 @@ def test_fused_tiles_per_workgroup(ctx):
         assert k >= 20
         assert (dev[:10] <= 1e-9).all()
         assert abs(ik["rel_error"] - i1["rel_error"]) <= 1e-3
+
+
+@pytest.mark.parametrize("panels", [1, 3])
+def test_fused_backtracking_steps_track_oracle(ctx, panels):
+    """Backtracking steps (1 <= cnt <= 5, alpha < 1; a larger Armijo constant
+    makes the search backtrack) with the line search in its own launch
+    (panels = 1) and inside the A^T panel launch (panels = 3): the blend at
+    alpha (k_apply_alpha) and the search's alpha against the FP32 oracle --
+    the same decisions, the same alpha, the oracle's envelope for f."""
+    dims = (16, 16, 16)
+    u, A, A32, oop, b = setup(ctx, dims, 24, 64, seed=23)
+    info = A.layout_force(dims, -1024, -1024, 16384, panels)
+    assert info["panels"] == panels
+    o = opts(maxit=40, c=0.3)
+    Uf, inf_, hf = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**o, solver="fused"), Ut=u)
+    oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=40, tv_iso=1, c=0.3)
+    Uo, io, ho, _ = orc.solve(oop, dims, b, oo, Ut=u, dtype=np.float32)
+    runs = []
+    for s_ in (0, 1, 2):
+        xi = np.random.default_rng(s_).choice([-1.0, 1.0], size=b.shape)
+        bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+        runs.append(orc.solve(oop, dims, bp, oo, Ut=u, dtype=np.float32))
+    k = min([first_divergence(hf, ho)] + [first_divergence(r[2], ho) for r in runs])
+    dev_f = np.abs(hf["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k])
+    env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k]))
+                  for r in runs], axis=0)
+    back = (hf["cnt"][:k] >= 1) & (hf["cnt"][:k] <= 5)
+    print(f"panels {panels}: cnt histogram {np.bincount(hf['cnt'], minlength=7).tolist()}; "
+          f"decisions agree for {k}; max f dev {dev_f.max():.1e} envelope {env.max():.1e}")
+    assert back.any()
+    assert k >= 10
+    assert (dev_f[:4] < 1e-5).all()
+    assert (dev_f <= 30 * env + 1e-6).all()
+    live = ho["cnt"][:k] < 6
+    assert np.allclose(hf["alpha"][:k][live], ho["alpha"][:k][live], rtol=1e-5)
