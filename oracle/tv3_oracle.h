@@ -62,6 +62,16 @@ typedef struct {
     int *cnt;            /* backtracking steps taken (6 = fallback) */
     int *outer;          /* outer index of the iteration */
     int *mulT;           /* 1: the iteration's get_g multiplied by A^T; NULL to omit */
+    /* line search of min_u (3d/tval3_gpu_3d.cpp:839-890), each NULL to omit:
+     * f of the first trial (alpha = 1), f of the accepted trial (before
+     * update_W), the reference value C the Armijo test compared against,
+     * c_armij = |d|^2 tau c beta, |d|^2 of the direction, and gam as used by
+     * the C/Q update of the iteration (:1176-1177) */
+    double *f_trial, *f_acc, *C_in, *c_armij, *dd, *gam;
+    /* state at the end of the solve (scaled problem, REAL values widened),
+     * each NULL to omit: sigma (3n: x, y, z components), delta (m), the
+     * DtsAtd term of the gradient (n), the last gradient d (n) */
+    double *sigma_out, *delta_out, *dts_out, *d_out;
 } orc_history;
 
 void orc_opts_default(orc_opts *o);

@@ -50,13 +50,18 @@ class OrcInfo(C.Structure):
 
 _HIST_D = ["f", "lam1", "lam2", "lam3", "lam4", "lam5", "tau", "alpha", "relchg", "mu",
            "beta", "C"]
+# line-search record of each iteration (orc_history in tv3_oracle.h)
+_HIST_LS = ["f_trial", "f_acc", "C_in", "c_armij", "dd", "gam"]
+# state at the end of the solve
+_HIST_STATE = ["sigma_out", "delta_out", "dts_out", "d_out"]
 
 
 class OrcHistory(C.Structure):
     _fields_ = [("cap", C.c_int), ("count", C.c_int)] + \
                [(k, C.POINTER(C.c_double)) for k in _HIST_D] + \
                [("cnt", C.POINTER(C.c_int)), ("outer", C.POINTER(C.c_int)),
-                ("mulT", C.POINTER(C.c_int))]
+                ("mulT", C.POINTER(C.c_int))] + \
+               [(k, C.POINTER(C.c_double)) for k in _HIST_LS + _HIST_STATE]
 
 
 _P = C.c_void_p
@@ -276,8 +281,11 @@ def scaleb(b, dtype=np.float64):
     return b, scl
 
 
-def solve(op, dims, b, opts=None, Ut=None, dtype=np.float64, hist_cap=4096):
-    """Run the oracle TVAL3 loop.  Returns (U, info, history dict)."""
+def solve(op, dims, b, opts=None, Ut=None, dtype=np.float64, hist_cap=4096, state=False):
+    """Run the oracle TVAL3 loop.  Returns (U, info, history dict, status).
+    The history holds the per-iteration record (including the line search's
+    f_trial, f_acc, C_in, c_armij, dd, gam); with state=True also the final
+    sigma (3 x n), delta (m), DtsAtd (n) and d (n) of the scaled problem."""
     p, q, r = dims
     o = opts if opts is not None else default_opts()
     b = np.ascontiguousarray(b, dtype)
@@ -287,12 +295,19 @@ def solve(op, dims, b, opts=None, Ut=None, dtype=np.float64, hist_cap=4096):
     h = OrcHistory()
     h.cap = hist_cap
     h.count = 0
-    harr = {k: np.zeros(hist_cap, np.float64) for k in _HIST_D}
+    harr = {k: np.zeros(hist_cap, np.float64) for k in _HIST_D + _HIST_LS}
     harr["cnt"] = np.zeros(hist_cap, np.int32)
     harr["outer"] = np.zeros(hist_cap, np.int32)
     harr["mulT"] = np.zeros(hist_cap, np.int32)
-    for k in _HIST_D:
+    for k in _HIST_D + _HIST_LS:
         setattr(h, k, harr[k].ctypes.data_as(C.POINTER(C.c_double)))
+    sarr = {}
+    if state:
+        n, m = p * q * r, int(op.s.m)
+        sarr = {"sigma_out": np.zeros(3 * n), "delta_out": np.zeros(m),
+                "dts_out": np.zeros(n), "d_out": np.zeros(n)}
+        for k, v in sarr.items():
+            setattr(h, k, v.ctypes.data_as(C.POINTER(C.c_double)))
     h.cnt = harr["cnt"].ctypes.data_as(C.POINTER(C.c_int))
     h.outer = harr["outer"].ctypes.data_as(C.POINTER(C.c_int))
     h.mulT = harr["mulT"].ctypes.data_as(C.POINTER(C.c_int))
@@ -300,4 +315,10 @@ def solve(op, dims, b, opts=None, Ut=None, dtype=np.float64, hist_cap=4096):
                                                    _ptr(U), _ptr(ut), C.byref(info),
                                                    C.byref(h))
     hist = {k: v[:h.count].copy() for k, v in harr.items()}
+    if state:
+        n = p * q * r
+        hist["sigma"] = sarr["sigma_out"].reshape(3, n)
+        hist["delta"] = sarr["delta_out"]
+        hist["DtsAtd"] = sarr["dts_out"]
+        hist["d"] = sarr["d_out"]
     return U, info, hist, st

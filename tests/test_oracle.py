@@ -438,3 +438,215 @@ def test_solve_fp32_tracks_fp64_config0(orc):
     rel = np.linalg.norm(U32 - U64) / np.linalg.norm(U64)
     assert i64.rel_error < 0.2
     assert rel < 5e-3, rel
+
+
+# ------------------------------------------------ whole-solve pins (orc_solve)
+# Independent numpy restatements (periodic differences by np.roll, shrinkage
+# as soft thresholding) used as the checker of the oracle's whole solve.
+def _np_D(u3):
+    """(Ux, Uy, Uz) of u3[z, x, y]: periodic forward differences"""
+    return (np.roll(u3, -1, 1) - u3, np.roll(u3, -1, 2) - u3, np.roll(u3, -1, 0) - u3)
+
+
+def _np_Dt(X, Y, Z):
+    return (np.roll(X, 1, 1) - X) + (np.roll(Y, 1, 2) - Y) + (np.roll(Z, 1, 0) - Z)
+
+
+def _np_shrink(X, Y, Z, t, iso):
+    if iso:
+        nr = np.sqrt(X * X + Y * Y + Z * Z)
+        f = np.where(nr > t, (nr - t) / np.where(nr > 0, nr, 1.0), 0.0)
+        return X * f, Y * f, Z * f
+    s = lambda a: np.sign(a) * np.maximum(np.abs(a) - t, 0.0)
+    return s(X), s(Y), s(Z)
+
+
+def _huber_tv_grad(u3, b3, beta, mu, iso):
+    """gradient of F(u) = sum_i min_w |w| + beta/2 |D_i u - w|^2 + mu/2 |u - b|^2,
+    the objective of TVAL3's first inner loop for A = I (sigma = delta = 0)"""
+    X, Y, Z = _np_D(u3)
+    Wx, Wy, Wz = _np_shrink(X, Y, Z, 1.0 / beta, iso)
+    return beta * _np_Dt(X - Wx, Y - Wy, Z - Wz) + mu * (u3 - b3)
+
+
+def _fista_nonneg(b3, beta, mu, iso, iters=20000):
+    """projected accelerated gradient (Beck-Teboulle 2009) on F over u >= 0;
+    |D^T D| <= 12 bounds the Lipschitz constant"""
+    L = 12.0 * beta + mu
+    x = b3.copy()
+    y = x.copy()
+    t = 1.0
+    for _ in range(iters):
+        xn = np.maximum(y - _huber_tv_grad(y, b3, beta, mu, iso) / L, 0.0)
+        tn = (1.0 + np.sqrt(1.0 + 4.0 * t * t)) / 2.0
+        y = xn + (t - 1.0) / tn * (xn - x)
+        x, t = xn, tn
+    return x
+
+
+def _kkt_residual(u3, b3, beta, mu, iso):
+    """natural residual |u - P_{u>=0}(u - grad F/L)| L / (mu |b|)"""
+    L = 12.0 * beta + mu
+    G = _huber_tv_grad(u3, b3, beta, mu, iso)
+    return float(np.linalg.norm(u3 - np.maximum(u3 - G / L, 0.0)) * L / (mu * np.linalg.norm(b3)))
+
+
+def _identity_op(orc, n):
+    ip = np.arange(n + 1, dtype=np.int32)
+    return orc.csr_op(n, n, ip, np.arange(n, dtype=np.int32), np.ones(n))
+
+
+@pytest.mark.parametrize("N,iso,offset", [(8, 1, 0.2), (8, 0, 0.2), (10, 1, -0.3), (10, 0, -0.3)])
+def test_solve_tv_denoising_optimality(orc, N, iso, offset):
+    """A = I, one outer iteration (mu0 = mu, beta0 = beta, so sigma = delta = 0):
+    the inner loop of 3d/tval3_gpu_3d.cpp:1161-1202 minimises
+    F(u) = sum_i |D_i u|_beta + mu/2 |u - b|^2 over u >= 0, the Huber-smoothed
+    TV (iso or aniso) denoising problem.  Its solution satisfies the projected
+    optimality condition and equals an independent accelerated projected
+    gradient solution of F.  b is chosen in the range scaleb leaves
+    unchanged; offset -0.3 makes the nonnegativity constraint active."""
+    dims = (N, N, N)
+    u = wl.phantom_3d(dims=dims, seed=3, contrast=0.5)
+    u = (u - u.min()) / (u.max() - u.min()) * 0.8 + offset
+    b = u + 0.05 * np.random.default_rng(5).standard_normal(u.size)
+    assert orc.scaleb(b)[1] == 1.0
+    mu, beta = 16.0, 32.0
+    o = orc.default_opts(mu=mu, mu0=mu, beta=beta, beta0=beta, maxcnt=1, tol_inn=1e-10,
+                         maxit=3000, tv_iso=iso)
+    U, info, hist, st = orc.solve(_identity_op(orc, N ** 3), dims, b, o)
+    assert st == 0
+    u3, b3 = U.reshape(N, N, N), b.reshape(N, N, N)
+    ref = _fista_nonneg(b3, beta, mu, iso)
+    assert _kkt_residual(ref, b3, beta, mu, iso) < 1e-12
+    kkt = _kkt_residual(u3, b3, beta, mu, iso)
+    err = np.linalg.norm(u3 - ref) / np.linalg.norm(ref)
+    print(f"N={N} iso={iso}: {info.total_iters} iterations, KKT {kkt:.1e}, |U - U_fista| {err:.1e}, "
+          f"zeros {(U == 0).mean():.3f}")
+    assert kkt < 1e-7 and err < 1e-7
+    if offset < 0:
+        assert (U == 0).any() and ((u3 == 0) == (ref == 0)).all()
+
+
+def _cs_problem(N=32, frac=0.35, seed=31):
+    u = wl.phantom_2d(N, 6, seed=seed)
+    m = int(frac * N * N)
+    A = wl.dense_gaussian(m, N * N, seed=seed + 1)
+    return u, A, A @ u
+
+
+_CS_OPTS = dict(mu=2.0 ** 12, mu0=2.0 ** 8, beta=2.0 ** 7, beta0=2.0 ** 4, tol=1e-7,
+                tol_inn=1e-4, maxit=3000, maxcnt=30)
+
+
+@pytest.mark.parametrize("iso", [0, 1])
+def test_solve_underdetermined_recovery_with_continuation(orc, iso):
+    """m = 0.35 n Gaussian measurements of a piecewise-constant image, no
+    noise: the null space of A is large (the least-squares solution is off by
+    ~0.8), TV minimisation subject to Au = b picks the image.  The run uses
+    the continuation scheme (mu0 < mu, beta0 < beta, 3d/tval3_gpu_3d.cpp:
+    1222-1249) and the multiplier updates, so it needs the outer loop right."""
+    u, A, b = _cs_problem()
+    N = 32
+    U, info, hist, st = orc.solve(orc.dense_op(A), (N, N, 1), b,
+                                  orc.default_opts(**_CS_OPTS, tv_iso=iso), Ut=u)
+    lsq = np.linalg.lstsq(A, b, rcond=None)[0]
+    e_lsq = np.linalg.norm(lsq - u) / np.linalg.norm(u)
+    res = np.linalg.norm(A @ U - b) / np.linalg.norm(b)
+    print(f"iso={iso}: rel error {info.rel_error:.2e} (least squares {e_lsq:.2f}), residual "
+          f"{res:.1e}, {info.total_iters} iterations, {info.outer_iters} outer")
+    assert st == 0 and e_lsq > 0.5
+    assert info.rel_error < 3e-3 and res < 1e-4
+    assert hist["beta"][0] == 2.0 ** 4 and hist["beta"][-1] == 2.0 ** 7
+
+
+@pytest.mark.parametrize("iso", [0, 1])
+def test_solve_multiplier_state_identity(orc, iso):
+    """update_mlp (3d/tval3_gpu_3d.cpp:1034-1083) sets sigma -= beta V and
+    delta -= mu (Au - b); the outer step then sets DtsAtd = beta0/beta d
+    (:1240-1249).  With d = D^T V + mu/beta A^T(Au - b) + DtsAtd this keeps
+    DtsAtd = -(D^T sigma + A^T delta)/beta at every outer step, so the final
+    state of a run with continuation must satisfy it (sign, ratio and
+    multiplier updates all enter)."""
+    u, A, b = _cs_problem()
+    N = 32
+    o = orc.default_opts(**dict(_CS_OPTS, maxcnt=8), tv_iso=iso)
+    U, info, h, st = orc.solve(orc.dense_op(A), (N, N, 1), b, o, state=True)
+    assert st == 0 and info.outer_iters >= 6
+    beta = h["beta"][-1]
+    sig = h["sigma"].reshape(3, 1, N, N)
+    want = -(_np_Dt(sig[0], sig[1], sig[2]).reshape(-1) + A.T @ h["delta"]) / beta
+    err = np.linalg.norm(want - h["DtsAtd"]) / np.linalg.norm(h["DtsAtd"])
+    print(f"iso={iso}: |DtsAtd + (D^T sigma + A^T delta)/beta| rel {err:.1e}")
+    assert np.linalg.norm(h["DtsAtd"]) > 0 and err < 1e-9
+
+
+@pytest.mark.parametrize("iso", [0, 1])
+def test_solve_history_f_closed_form(orc, iso):
+    """update_W (3d/tval3_gpu_3d.cpp:965-1029) and update_mlp (:1034-1083)
+    keep f incrementally (subtract the old terms, add the new); every recorded
+    f must equal lam1 + beta/2 lam2 + mu/2 lam3 - lam4 - lam5 of the same
+    row."""
+    u, A, b = _cs_problem()
+    U, info, h, st = orc.solve(orc.dense_op(A), (32, 32, 1), b,
+                               orc.default_opts(**dict(_CS_OPTS, maxcnt=6), tv_iso=iso))
+    fc = h["lam1"] + h["beta"] / 2 * h["lam2"] + h["mu"] / 2 * h["lam3"] - h["lam4"] - h["lam5"]
+    assert len(h["f"]) > 50 and len(set(h["beta"].tolist())) > 1
+    assert np.max(np.abs(fc - h["f"]) / np.abs(h["f"])) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["cs", "noisy3d", "fallback"])
+def test_solve_nonmonotone_acceptance_and_reference_recursion(orc, case):
+    """Default gam (0.9995): C is the Zhang-Hager weighted average
+    Q_k = gam Q_{k-1} + 1, C_k = (gam Q_{k-1} C_{k-1} + f_k)/Q_k
+    (3d/tval3_gpu_3d.cpp:1176-1177), reset to Q = 1, C = f at each outer step
+    (:1253), with gam shrunk by rate_gam on each cnt == 5 fallback (:877).
+    Recomputed from the recorded f, cnt and outer index; and every accepted
+    step satisfies the Armijo test of min_u (:848, :875):
+    |f(alpha)| <= |C - alpha c| with alpha = gamma^cnt and
+    c = |d|^2 tau c beta, while a backtracked step failed it at alpha = 1.
+    Case "fallback" takes gamma = 1, so every failed test ends in the
+    fallback and gam shrinks (and is reset by the next outer step)."""
+    if case in ("cs", "fallback"):
+        u, A, b = _cs_problem(seed=41)
+        op, dims = orc.dense_op(A), (32, 32, 1)
+        o = orc.default_opts(**dict(_CS_OPTS, maxcnt=8))
+        if case == "fallback":
+            o.gamma = 1.0
+    else:
+        dims = (12, 12, 12)
+        u = wl.phantom_3d(dims=dims, seed=4, contrast=0.5)
+        A = wl.dense_gaussian(700, u.size, seed=6)
+        b = wl.add_noise(A @ u, 0.02, seed=7)
+        op = orc.dense_op(A)
+        o = orc.default_opts(maxit=200, tol_inn=1e-4, tv_iso=1, beta0=4.0, mu0=64.0)
+    U, info, h, st = orc.solve(op, dims, b, o)
+    assert st == 0
+    cnt, outer = h["cnt"], h["outer"]
+    if case == "fallback":                   # fallbacks, then a later outer loop
+        fb = np.flatnonzero(cnt == 6)
+        assert len(fb) >= 2 and outer[-1] > outer[fb[0]]
+    else:
+        assert (cnt[cnt < 6] > 0).any()      # the test exercises backtracking
+    Q = C = gam = None
+    for k in range(len(cnt)):
+        if k == 0 or outer[k] != outer[k - 1]:
+            Q, C, gam = 1.0, h["C_in"][k], o.gam
+        else:
+            assert h["C_in"][k] == C
+        if cnt[k] == 6:
+            gam *= o.rate_gam
+        assert h["gam"][k] == gam
+        Qn = gam * Q + 1
+        C = (gam * Q * C + h["f"][k]) / Qn
+        Q = Qn
+        assert abs(h["C"][k] - C) <= 1e-13 * abs(C)
+        c = h["c_armij"][k]
+        if cnt[k] < 6:                       # a fallback row records the new tau
+            assert c == h["dd"][k] * h["tau"][k] * o.c * h["beta"][k]
+            al = o.gamma ** cnt[k]
+            assert h["alpha"][k] == pytest.approx(al, rel=1e-15)
+            assert abs(h["f_acc"][k]) <= abs(h["C_in"][k] - al * c)
+            if cnt[k] == 0:
+                assert h["f_trial"][k] == h["f_acc"][k]
+            else:
+                assert abs(h["f_trial"][k]) > abs(h["C_in"][k] - c)
