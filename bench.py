@@ -102,6 +102,75 @@ def timed_solve(ctx, op, b_dev, dims, opts, ut_dev):
     return time.perf_counter() - t0, U, info
 
 
+def rel_l2(a, b, mask=None):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_leg(A32, b, dims, runs_all=5, iters_all=20, runs_one=3, iters_one=10):
+    """The CPU port (oracle/, C + OpenMP) of the same TVAL3 loop on the same
+    CSR on the host cores, loop-only time as the reference's loop timer
+    (3d/tval3_gpu_3d.cpp:1159,1254-1257): one warm-up, then the median of
+    `runs` runs of a fixed iteration count, FP32 and FP64, all threads and
+    one thread.  Its FP64 solve to the default stopping rule is the FP64
+    reference the GPU results are compared with."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import tv3_oracle as orc
+    rowptr, col, val = A32.download_csr()
+    oop = orc.csr_op(A32.m, A32.n, rowptr, col, val)
+    fixed = dict(mu=2.0 ** 9, mu0=2.0 ** 9, tv_iso=1, tol=0.0, tol_inn=0.0, maxcnt=1 << 30)
+    threads = orc.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+
+    def rate(dtype, nthr, runs, iters):
+        orc.set_threads(nthr)
+        o = orc.default_opts(**fixed, maxit=iters)
+        orc.solve(oop, dims, b, o, dtype=dtype)                 # warm-up
+        r = []
+        for _ in range(runs):
+            _, info, _, _ = orc.solve(oop, dims, b, o, dtype=dtype)
+            r.append(iters / info.loop_secs)
+        return float(np.median(r))
+
+    t0 = time.perf_counter()
+    res = {"f32_all": rate(np.float32, threads, runs_all, iters_all),
+           "f64_all": rate(np.float64, threads, runs_all, iters_all),
+           "f32_one": rate(np.float32, 1, runs_one, iters_one),
+           "f64_one": rate(np.float64, 1, runs_one, iters_one)}
+    tcpu = time.perf_counter() - t0
+    orc.set_threads(threads)
+    conv = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tv_iso=1)
+    t0 = time.perf_counter()
+    U64, i64, _, _ = orc.solve(oop, dims, b.astype(np.float64), conv)
+    tref = time.perf_counter() - t0
+    log(f"cpu: FP32 {res['f32_all']:.2f} it/s ({threads} threads), FP64 {res['f64_all']:.2f}; "
+        f"one thread FP32 {res['f32_one']:.2f}, FP64 {res['f64_one']:.2f} ({tcpu:.0f}s); "
+        f"FP64 reference solve {i64.total_iters} inner / {i64.outer_iters} outer ({tref:.0f}s)")
+    cpu = {"value": res["f32_all"], "unit": "inner iter/s", "cores": threads, "kind": "port",
+           "sample": f"oracle TVAL3 loop (C, OpenMP) on the same {dims[0]}^3 Siddon CSR "
+                     f"({A32.nnz} nnz, FP32 weights), loop-only, 1 warm-up then median of "
+                     f"{runs_all} runs x {iters_all} inner iterations ({threads} threads) and "
+                     f"of {runs_one} x {iters_one} (1 thread)",
+           "f64": res["f64_all"], "f32_1core": res["f32_one"], "f64_1core": res["f64_one"],
+           "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+           "cpu_model": cpu_model()}
+    ref = {"inner": i64.total_iters, "outer": i64.outer_iters, "secs": tref}
+    return cpu, U64, ref
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,7 +180,6 @@ def main():
     ap.add_argument("--emitters", type=int, default=256)
     ap.add_argument("--receivers", type=int, default=1024)
     ap.add_argument("--path", default="fused-f16", choices=["plain-f32", "fused-f32", "fused-f16"])
-    ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -136,6 +204,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     ctx = api.Context(0)
+    copy_bw, read_bw = ctx.bandwidth_probe(1 << 30, 10)
+    log(f"HBM probe (1 GiB): copy {copy_bw:.0f} GB/s, read {read_bw:.0f} GB/s")
     N = args.size
     dims = (N, N, N)
     u, geom, dyn, A32, b = build_workload(ctx, N, args.emitters, args.receivers)
@@ -167,7 +237,7 @@ def main():
                              ut_dev)
     assert info["total_iters"] == args.steps, info
     value = args.steps / t
-    log(f"timed: {args.steps} iters in {t:.3f}s -> {value:.1f} iter/s "
+    log(f"timed: {args.steps} iters in {t:.4f}s -> {value:.1f} iter/s "
         f"(loop-only {info['total_iters'] / info['loop_secs']:.1f}/s) rel_err={info['rel_error']:.4e}")
 
     # dominant kernels: the A and A^T products the solver runs, timed alone
@@ -194,36 +264,24 @@ def main():
     traffic = measured_traffic(dom_key, N, args.path)
 
     extra = {}
+    runs = {}
     if not args.no_extra:
-        # equal rel-err: FP16 fused against the reference-shaped FP32 loop, both
-        # run to the default stopping rule (tol 1e-6, tol_inn 1e-3, maxcnt 10,
-        # maxit 1025)
+        # equal rel-err: the fast path against the reference-shaped FP32 loop,
+        # both to the default stopping rule (tol 1e-6, tol_inn 1e-3, maxcnt 10,
+        # maxit 1025), on b and three rounding-level (1e-7) perturbations of it
         conv = dict(mu=2.0 ** 9, mu0=2.0 ** 9, tv_mode="iso")
-        tf, _, inf16 = timed_solve(ctx, op, b_dev, dims,
-                                   api.default_opts(**dict(conv, solver=solver)), ut_dev)
-        tp, _, inp = timed_solve(ctx, A32, b_dev, dims,
-                                 api.default_opts(**dict(conv, solver="plain")), ut_dev)
-        extra["equal_error"] = {
-            "stopping": "tol 1e-6, tol_inn 1e-3, maxcnt 10, maxit 1025",
-            "fast": {"path": args.path, "rel_err": inf16["rel_error"],
-                     "inner": inf16["total_iters"], "outer": inf16["outer_iters"], "secs": tf,
-                     "inner_per_s": inf16["total_iters"] / tf,
-                     "outer_per_s": inf16["outer_iters"] / tf},
-            "plain_f32": {"rel_err": inp["rel_error"], "inner": inp["total_iters"],
-                          "outer": inp["outer_iters"], "secs": tp,
-                          "inner_per_s": inp["total_iters"] / tp,
-                          "outer_per_s": inp["outer_iters"] / tp},
-        }
-        ee = extra["equal_error"]
-        ee["err_diff"] = ee["fast"]["rel_err"] - ee["plain_f32"]["rel_err"]
-        ee["speedup_outer"] = ee["fast"]["outer_per_s"] / ee["plain_f32"]["outer_per_s"]
-        ee["pass"] = bool(ee["err_diff"] <= 1e-3 and ee["speedup_outer"] >= 1.8)
-        log(f"equal error: fast err {ee['fast']['rel_err']:.4e} ({ee['fast']['inner']} inner, "
-            f"{ee['fast']['outer']} outer, {tf:.3f}s); plain FP32 err {ee['plain_f32']['rel_err']:.4e} "
-            f"({ee['plain_f32']['inner']} inner, {ee['plain_f32']['outer']} outer, {tp:.3f}s); "
-            f"outer-iter speedup {ee['speedup_outer']:.2f}")
-        # the reference-shaped FP32 loop and the dynamic Siddon operator,
-        # fixed iteration counts
+        for seed in (None, 0, 1, 2):
+            if seed is None:
+                bd = b_dev
+            else:
+                xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=b.shape)
+                bd = torch.from_numpy((b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)).cuda()
+            tf, Uf, inf16 = timed_solve(ctx, op, bd, dims,
+                                        api.default_opts(**dict(conv, solver=solver)), ut_dev)
+            tp, Up, inp = timed_solve(ctx, A32, bd, dims,
+                                      api.default_opts(**dict(conv, solver="plain")), ut_dev)
+            runs[seed] = {"fast": (tf, Uf.cpu().numpy(), inf16), "plain": (tp, Up.cpu().numpy(), inp)}
+        # the reference-shaped FP32 loop and the dynamic Siddon operator
         kp = min(args.steps, 100)
         tp, _, ip = timed_solve(ctx, A32, b_dev, dims,
                                 api.default_opts(**dict(fixed, solver="plain", maxit=kp)), ut_dev)
@@ -232,6 +290,9 @@ def main():
         td, _, idn = timed_solve(ctx, dyn, b_dev, dims,
                                  api.default_opts(**dict(fixed, solver="plain", maxit=kd)), ut_dev)
         extra["dynamic_siddon_f32_iter_per_s"] = kd / td
+        tdc, Udyn, idc = timed_solve(ctx, dyn, b_dev, dims,
+                                     api.default_opts(**dict(conv, solver="plain")), ut_dev)
+        runs["dynamic"] = (tdc, Udyn.cpu().numpy(), idc)
         tdn = time_spmv(dyn, False, 10)
         tdt = time_spmv(dyn, True, 10)
         extra["dynamic_A_us"] = tdn * 1e6
@@ -241,24 +302,54 @@ def main():
         log(f"plain-f32 {extra['plain_f32_iter_per_s']:.1f} it/s; dynamic {extra['dynamic_siddon_f32_iter_per_s']:.1f} it/s "
             f"(A {tdn * 1e6:.0f} us, A^T {tdt * 1e6:.0f} us)")
 
-    cpu = None
+    cpu, U64, ref64 = None, None, None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import tv3_oracle as orc
-        rowptr, col, val = A32.download_csr()
-        oop = orc.csr_op(m, n, rowptr, col, val)
-        oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tv_iso=1, tol=0.0, tol_inn=0.0,
-                              maxcnt=1 << 30, maxit=args.cpu_iters)
-        t0 = time.perf_counter()
-        orc.solve(oop, dims, b, oo, dtype=np.float32)
-        tc = time.perf_counter() - t0
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": args.cpu_iters / tc, "unit": "inner iter/s", "cores": cores,
-               "kind": "port",
-               "sample": f"oracle FP32 TVAL3 (C, OpenMP) on the same {N}^3 Siddon CSR "
-                         f"({nnz} nnz, FP32 weights), {args.cpu_iters} inner iterations "
-                         f"including its setup (scaleb, A^T b), {tc:.1f}s"}
-        log(f"cpu: {cpu['value']:.3f} iter/s on {cores} threads")
+        cpu, U64, ref64 = cpu_baseline_leg(A32, b, dims)
+
+    if runs:
+        # voxels some ray crosses (non-zero column of A); the rest are fixed
+        # by the TV term alone
+        _, col, _ = A32.download_csr()
+        covered = np.bincount(col, minlength=n) > 0
+        ut = u.astype(np.float64)
+
+        def errs(Ux):
+            e = {"vs_utrue": rel_l2(Ux, ut), "vs_utrue_covered": rel_l2(Ux, ut, covered)}
+            if U64 is not None:
+                e["vs_fp64"] = rel_l2(Ux, U64)
+                e["vs_fp64_covered"] = rel_l2(Ux, U64, covered)
+            return e
+
+        per = {"fast": [], "plain_f32": []}
+        for seed in (None, 0, 1, 2):
+            for key, name in (("fast", "fast"), ("plain", "plain_f32")):
+                tt, Ux, inf_ = runs[seed][key]
+                per[name].append(dict(errs(Ux), inner=inf_["total_iters"], outer=inf_["outer_iters"],
+                                      secs=tt, outer_per_s=inf_["outer_iters"] / tt,
+                                      inner_per_s=inf_["total_iters"] / tt, seed=seed))
+        med = lambda rows, k: float(np.median([r[k] for r in rows]))
+        ee = {"stopping": "tol 1e-6, tol_inn 1e-3, maxcnt 10, maxit 1025",
+              "perturbations": "b and b(1 + 1e-7 xi), xi = +-1 seeds 0, 1, 2; medians over the four",
+              "fast_path": args.path, "runs": per}
+        for name in ("fast", "plain_f32"):
+            ee[name] = {k: med(per[name], k) for k in per[name][0] if k != "seed"}
+        ee["err_diff"] = ee["fast"]["vs_utrue"] - ee["plain_f32"]["vs_utrue"]
+        ee["err_diff_covered"] = ee["fast"]["vs_utrue_covered"] - ee["plain_f32"]["vs_utrue_covered"]
+        ee["speedup_outer"] = ee["fast"]["outer_per_s"] / ee["plain_f32"]["outer_per_s"]
+        ee["pass"] = bool(ee["err_diff"] <= 1e-3 and ee["speedup_outer"] >= 1.8)
+        extra["equal_error"] = ee
+        tdc, Udyn, idc = runs["dynamic"]
+        extra["dynamic_f32_errors"] = dict(errs(Udyn), inner=idc["total_iters"],
+                                           outer=idc["outer_iters"])
+        if ref64 is not None:
+            extra["fp64_reference"] = dict(ref64, rel_err_vs_utrue=rel_l2(U64, ut),
+                                           rel_err_vs_utrue_covered=rel_l2(U64, ut, covered),
+                                           covered_voxels=int(covered.sum()))
+        log(f"equal error (medians): fast err {ee['fast']['vs_utrue']:.4e} (covered "
+            f"{ee['fast']['vs_utrue_covered']:.4e}, vs FP64 {ee['fast'].get('vs_fp64', float('nan')):.2e}); "
+            f"plain FP32 err {ee['plain_f32']['vs_utrue']:.4e} (covered {ee['plain_f32']['vs_utrue_covered']:.4e}, "
+            f"vs FP64 {ee['plain_f32'].get('vs_fp64', float('nan')):.2e}); outer-iter speedup "
+            f"{ee['speedup_outer']:.2f}; pass {ee['pass']}")
 
     out = {
         "metric": "TVAL3 iter/s + A,A^T HBM GB/s at 128^3 vox; fp16 vs fp32 at equal rel-err",
@@ -279,7 +370,9 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": dom_b / dom_t / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": dom_b / dom_t / PEAK_HBM, "traffic": traffic,
-                     "kernel": dom_name},
+                     "kernel": dom_name, "achievable_read_GBps": read_bw,
+                     "achievable_copy_GBps": copy_bw,
+                     "frac_of_read_probe": dom_b / dom_t / 1e9 / read_bw},
         "cpu_baseline": cpu,
         "A_GBps": gA / 1e9,
         "AT_GBps": gAT / 1e9,
@@ -289,6 +382,7 @@ def main():
         "loop_iter_per_s": info["total_iters"] / info["loop_secs"],
         "lanes": list(lanes),
         "layout": layout,
+        "hbm_probe_GBps": {"copy": copy_bw, "read": read_bw, "bytes": 1 << 30},
     }
     out.update(extra)
     print(json.dumps(out))

@@ -251,6 +251,26 @@ inline tv3_weight_mode mode_of(const geometry_host &g) {
     return g.pos_emitters && g.pos_receivers ? TV3_WEIGHT_SIDDON : TV3_WEIGHT_BRESENHAM;
 }
 
+// the geometry's traversal for A x, A_mat (CSC as given, CSR transposed on
+// the device) for A^T x
+template <class T>
+op_ptr make_hybrid(tv3_ctx *ctx, const geometry_host &geo, const sparse_mat_host<T> &A) {
+    const tv3_geometry g = to_geometry(geo);
+    tv3_op *op = nullptr;
+    const bool csc = A.format == sparse_mat_csc;
+    check(tv3_op_hybrid_create(ctx, &g, mode_of(geo), A.nnz, csc ? nullptr : A.ptr(),
+                               csc ? nullptr : A.ind(), csc ? nullptr : A.val(), dtype_of<T>(),
+                               csc ? A.ptr() : nullptr, csc ? A.ind() : nullptr,
+                               csc ? A.val() : nullptr, &op),
+          ctx);
+    op_ptr r(op);
+    int64_t m = 0, n = 0;
+    check(tv3_op_info(op, &m, &n, nullptr), ctx);
+    if (m != A.rows || n != A.cols)
+        throw tval3_gpu_exception("tval3: A_mat does not have the geometry's shape");
+    return r;
+}
+
 template <class T>
 tval3_info<T> solve(tv3_ctx *ctx, tv3_op *op, mat_host<T> &U, const mat_host<T> &b,
                     const tval3_options<T> &opts, const mat_host<T> &Ut) {
@@ -322,12 +342,12 @@ const tval3_info<T> tval3_gpu_3d(mat_host<T> &U, const geometry_host &A, const m
     return tval3_detail::solve(ctx, op, U, b, opts, Ut);
 }
 
-// hybrid: A x by the geometry, A^T x by a precomputed matrix of the same
-// geometry (3d/tval3_gpu_3d.cpp:1603-1668).  The CSC is rebuilt on the device
-// from the geometry (same matrix); A_mat is checked for shape only.
+// hybrid: A x by the geometry, A^T x by A_mat, the caller's precomputed
+// matrix (sparse_geom_mm, 3d/mat_vec_mul.h:104-120,732-756; entry
+// 3d/tval3_gpu_3d.cpp:1603-1668)
 template <class T>
 const tval3_info<T> tval3_gpu_3d(mat_host<T> &U, const geometry_host &A_geo,
-                                 sparse_mat_host<T> &A_mat, const mat_host<T> &b,
+                                 const sparse_mat_host<T> &A_mat, const mat_host<T> &b,
                                  const tval3_options<T> &opts, const mat_host<T> &Ut,
                                  bool pagelocked = true, int mainGPU = 0, int numGPUs = 1,
                                  int *gpuArray = nullptr) {
@@ -335,13 +355,8 @@ const tval3_info<T> tval3_gpu_3d(mat_host<T> &U, const geometry_host &A_geo,
     tv3_ctx *ctx = tval3_detail::context(tval3_detail::device_of(mainGPU, numGPUs, gpuArray));
     if (A_mat.rows != b.len || A_mat.cols != U.len)
         throw tval3_gpu_exception("tval3: dimensions of A, b and U do not match");
-    const tv3_geometry g = tval3_detail::to_geometry(A_geo);
-    tv3_op *op = nullptr;
-    // the device-built CSC keeps F32 weights; FP64 solves read them in FP64
-    tval3_detail::check(
-        tv3_op_hybrid_create(ctx, &g, tval3_detail::mode_of(A_geo), TV3_F32, &op), ctx);
-    tval3_detail::op_ptr hold(op);
-    return tval3_detail::solve(ctx, op, U, b, opts, Ut);
+    auto op = tval3_detail::make_hybrid(ctx, A_geo, A_mat);
+    return tval3_detail::solve(ctx, op.get(), U, b, opts, Ut);
 }
 
 // ---- operator plug-in: mat_vec_mul(trans, A, x, y, stream, mulChain) ------
@@ -360,6 +375,10 @@ public:
         tval3_detail::check(tv3_op_dynamic_create(ctx_, &t, tval3_detail::mode_of(g), &op), ctx_);
         op_.reset(op);
     }
+    // sparse_geom_mm: A x by the geometry, A^T x by A_mat
+    template <class T>
+    tval3_operator(const geometry_host &g, const sparse_mat_host<T> &A_mat, int device = 0)
+        : ctx_(tval3_detail::context(device)), op_(tval3_detail::make_hybrid(ctx_, g, A_mat)) {}
     tv3_op *get() const { return op_.get(); }
     tv3_ctx *context() const { return ctx_; }
 };

@@ -104,6 +104,9 @@ typedef struct {
     int32_t *outer;
     int32_t *mulT;    /* 1: the iteration's get_g took the A^T product, 0: the
                        * A^T-skip heuristic skipped it; NULL to omit */
+    double *f_trial;  /* f of the iteration's first trial (alpha = 1) that the
+                       * Armijo test of min_u saw (3d/tval3_gpu_3d.cpp:848); NULL
+                       * to omit */
 } tv3_history;
 
 /* Transducer geometry of a dynamic operator (geometry_host / geometry_device,
@@ -157,11 +160,17 @@ tv3_status tv3_op_dense_create(tv3_ctx *ctx, int64_t m, int64_t n, const void *A
 tv3_status tv3_op_dynamic_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weight_mode mode,
                                  tv3_op **out);
 
-/* Hybrid operator: replaces sparse_geom_mm (3d/mat_vec_mul.h:74-756): A x by
- * the dynamic traversal, A^T x by the CSC of a precomputed matrix of the same
- * geometry (built on the device from the traversal). */
+/* Hybrid operator: replaces sparse_geom_mm(hA_geo, hA_mat, ...)
+ * (3d/mat_vec_mul.h:104-120, products :732-756; entry 3d/tval3_gpu_3d.cpp:
+ * 1603-1668): A x by the dynamic traversal of geom, A^T x by the caller's
+ * precomputed matrix (num_paths x voxels, host arrays, values in val_type).
+ * The CSC (csc_*) is used for A^T as given; a CSR alone is transposed on the
+ * device; with neither (all pointers NULL) the matrix of the geometry itself
+ * is built on the device. */
 tv3_status tv3_op_hybrid_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weight_mode mode,
-                                tv3_dtype val_type, tv3_op **out);
+                                int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                                const void *val, tv3_dtype val_type, const int32_t *csc_colptr,
+                                const int32_t *csc_row, const void *csc_val, tv3_op **out);
 
 /* Precompute the sparse matrix of a dynamic operator on the device (CSR by a
  * count pass, a scan and a fill pass; CSC by a stable transpose).  Values are
@@ -354,6 +363,15 @@ tv3_status tv3_op_layout_matvec(tv3_op *op, const int32_t dims[3], tv3_trans tra
  * products. */
 tv3_status tv3_op_layout_time(tv3_op *op, const int32_t dims[3], tv3_trans trans, int32_t reps,
                               double *us);
+
+/* Achievable HBM bandwidth on the context's device (no reference
+ * counterpart; the measurement the roofline fractions are also quoted
+ * against): a streaming copy of `bytes` (GB/s counts bytes read plus bytes
+ * written) and a streaming read of `bytes`, each the median of `reps`
+ * launches timed with HIP events on the context stream.  Either output may
+ * be NULL. */
+tv3_status tv3_bandwidth_probe(tv3_ctx *ctx, int64_t bytes, int32_t reps, double *copy_GBps,
+                               double *read_GBps);
 
 /* ---- tuning ------------------------------------------------------------- */
 /* Time the sparse kernels of op over their launch shapes and keep the

@@ -53,6 +53,8 @@ typedef struct {
     int backtracks, fallbacks;
     int status;          /* 0 ok, 6 non-finite */
     int n_matvec_T;      /* A^T products of the loop (A^T b excluded) */
+    double loop_secs;    /* wall time of the outer loop (setup excluded), as the
+                          * reference's loop timer (3d/tval3_gpu_3d.cpp:1159,1254-1257) */
 } orc_info;
 
 /* per inner iteration record (capacity = cap entries, count written) */
@@ -75,6 +77,11 @@ typedef struct {
 } orc_history;
 
 void orc_opts_default(orc_opts *o);
+
+/* OpenMP threads of the parallel loops (n > 0 sets, n <= 0 queries); returns
+ * the count now in effect.  Results do not depend on it: every reduction
+ * adds 64 fixed chunks in order. */
+int orc_set_threads(int n);
 
 /* ------------------------------------------------------------------ */
 /* operators                                                           */

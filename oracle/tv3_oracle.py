@@ -45,7 +45,7 @@ class OrcOpts(C.Structure):
 class OrcInfo(C.Structure):
     _fields_ = [("total_iters", C.c_int), ("outer_iters", C.c_int), ("rel_chg", C.c_double),
                 ("rel_error", C.c_double), ("backtracks", C.c_int), ("fallbacks", C.c_int),
-                ("status", C.c_int), ("n_matvec_T", C.c_int)]
+                ("status", C.c_int), ("n_matvec_T", C.c_int), ("loop_secs", C.c_double)]
 
 
 _HIST_D = ["f", "lam1", "lam2", "lam3", "lam4", "lam5", "tau", "alpha", "relchg", "mu",
@@ -87,6 +87,8 @@ DENSE, CSR, BRESENHAM, SIDDON = 0, 1, 2, 3
 
 def _declare(L):
     L.orc_opts_default.argtypes = [C.POINTER(OrcOpts)]
+    L.orc_set_threads.restype = C.c_int
+    L.orc_set_threads.argtypes = [C.c_int]
     L.orc_build_paths.restype = C.c_int64
     L.orc_build_paths.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P]
     L.orc_bresenham_ray.restype = C.c_int
@@ -118,6 +120,11 @@ def _ptr(a):
 
 def _sfx(dtype):
     return "_f64" if np.dtype(dtype) == np.float64 else "_f32"
+
+
+def set_threads(n=0):
+    """OpenMP threads of the oracle (n > 0 sets); returns the count in effect"""
+    return lib().orc_set_threads(int(n))
 
 
 def default_opts(**kw):

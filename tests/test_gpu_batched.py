@@ -354,3 +354,20 @@ def test_batch_backtracking_blends(ctx, batch):
         assert kk >= 10
         assert (dev[:4] < 1e-5).all()
         assert (dev <= 30 * env + 1e-6).all()
+
+
+def test_batch_continuation_tracks_oracle(ctx, batch8):
+    """beta0 < beta: after each outer step the first trial of the new inner
+    loop uses W of the last update_W, built from the old sigma at the old
+    beta, in every problem of the batch (same criterion as the fused
+    solver)."""
+    from test_gpu_fused import _envelope_runs, check_continuation_tracks
+    kw = dict(mu=2.0 ** 9, mu0=2.0 ** 6, beta=32.0, beta0=4.0, tol_inn=5e-3, maxit=100)
+    o = opts(**kw)
+    Ub, ib, bi, hb = solve(ctx, batch8, o, panels=2048, panels_A=2048)
+    oo = orc.default_opts(**kw, tv_iso=1)
+    for k, p in enumerate(batch8):
+        Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
+        runs = _envelope_runs(p["oop"], p["dims"], p["b"], oo, p["u"])
+        kk = check_continuation_tracks(hb[k], ho, runs, f"batch {p['dims']}")
+        assert kk >= 15

@@ -383,3 +383,79 @@ def test_fused_backtracking_steps_track_oracle(ctx, panels):
     assert (dev_f <= 30 * env + 1e-6).all()
     live = ho["cnt"][:k] < 6
     assert np.allclose(hf["alpha"][:k][live], ho["alpha"][:k][live], rtol=1e-5)
+
+
+def _envelope_runs(oop, dims, b, oo, u, seeds=(0, 1, 2)):
+    """the oracle's own sensitivity: rounding-level (1e-7) perturbations of b"""
+    runs = []
+    for seed in seeds:
+        xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=b.shape)
+        bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+        runs.append(orc.solve(oop, dims, bp, oo, Ut=u, dtype=np.float32))
+    return runs
+
+
+def check_continuation_tracks(h, ho, runs, tag):
+    """With beta0 < beta the first trial of every inner loop after an outer
+    step sees W of the last update_W: built from the old sigma at the old beta
+    (3d/tval3_gpu_3d.cpp:484-492 reads the stored W; update_mlp :1034-1083
+    leaves it).  f of every first trial (alpha = 1, the value the Armijo test
+    sees) and of every iteration stay inside a small multiple of the oracle's
+    rounding envelope, and the first iteration of every inner loop takes the
+    oracle's decision and alpha."""
+    kk = min([first_divergence(h, ho)] + [first_divergence(r[2], ho) for r in runs])
+    env = {}
+    for key in ("f", "f_trial"):
+        env[key] = np.max([np.maximum.accumulate(np.abs(r[2][key][:kk] - ho[key][:kk])
+                                                 / np.abs(ho[key][:kk])) for r in runs], axis=0)
+        dev = np.abs(h[key][:kk] - ho[key][:kk]) / np.abs(ho[key][:kk])
+        print(f"{tag} {key}: decisions agree for {kk}; max dev {dev.max():.1e} envelope "
+              f"{env[key].max():.1e}")
+        assert (dev <= 30 * env[key] + 1e-6).all(), key
+    starts = [k for k in range(kk) if k == 0 or ho["outer"][k] != ho["outer"][k - 1]]
+    print(f"{tag}: inner loops starting at {starts}, betas {[ho['beta'][k] for k in starts]}")
+    assert len(starts) >= 3 and len({ho["beta"][k] for k in starts}) >= 3
+    for k in starts:
+        assert h["cnt"][k] == ho["cnt"][k]
+        if ho["cnt"][k] < 6:
+            assert h["alpha"][k] == pytest.approx(ho["alpha"][k], rel=1e-6)
+    return kk
+
+
+@pytest.mark.parametrize("iso", [1, 0])
+def test_fused_continuation_tracks_oracle(ctx, iso):
+    dims = (16, 16, 16)
+    u, A, A32, oop, b = setup(ctx, dims, 24, 64, seed=3)
+    kw = dict(mu=2.0 ** 9, mu0=2.0 ** 6, beta=32.0, beta0=4.0, tol_inn=5e-3, maxit=100)
+    U, inf_, h = api.tval3_gpu_3d(ctx, A, b, dims,
+                                  api.default_opts(**kw, tv_mode="iso" if iso else "aniso",
+                                                   solver="fused"), Ut=u)
+    oo = orc.default_opts(**kw, tv_iso=iso)
+    Uo, io, ho, _ = orc.solve(oop, dims, b, oo, Ut=u, dtype=np.float32)
+    runs = _envelope_runs(oop, dims, b, oo, u)
+    kk = check_continuation_tracks(h, ho, runs, f"fused iso={iso}")
+    assert kk >= 20
+
+
+@pytest.mark.parametrize("solver", ["plain", "fused"])
+@pytest.mark.parametrize("fallbacks", [False, True])
+def test_product_counts_exact(ctx, solver, fallbacks):
+    """A products: the initial A U, one per completed iteration and per
+    rejected trial, one A d per first-iteration step length (every inner
+    loop and every fallback, 3d/tval3_gpu_3d.cpp:583-625); A^T products: A^T b,
+    the initial get_g, one per completed iteration and per rejected trial.
+    The loop ends at maxcnt outer iterations, so every inner loop started is
+    an outer iteration."""
+    dims = (8, 8, 8)
+    u, A, A32, oop, b = setup(ctx, dims, 16, 32, seed=9)
+    o = dict(mu=2.0 ** 9, mu0=2.0 ** 6, beta0=8.0, maxcnt=3, tol=0.0, tol_inn=1e-2, maxit=200)
+    if fallbacks:
+        b = wl.add_noise(b.astype(np.float64), 0.05, seed=14).astype(np.float32)
+        o.update(gamma=1.0, c=0.0, gam=0.0, tol_inn=1e-12, maxit=40, maxcnt=1)
+    U, info, h = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**o, solver=solver))
+    T, F, O = info["total_iters"], info["fallbacks"], info["outer_iters"]
+    print(f"{solver}: {T} iterations, {F} fallbacks, {O} outer; A {info['n_matvec_N']}, "
+          f"A^T {info['n_matvec_T']}")
+    assert O == o["maxcnt"] and (F > 0 or not fallbacks)
+    assert info["n_matvec_N"] == 1 + T + F + (O + F)
+    assert info["n_matvec_T"] == 2 + T + F

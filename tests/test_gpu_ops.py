@@ -259,3 +259,33 @@ def test_D_Dt_shrink_match_oracle(ctx):
         for a, b in zip(W[:3], Wo[:3]):
             assert np.abs(H(a) - b).max() <= 2e-7 * max(1, np.abs(b).max())
         assert np.isclose(W[3], Wo[3], rtol=1e-12)
+
+
+@pytest.mark.parametrize("given", ["csr", "csc"])
+def test_hybrid_transpose_follows_callers_matrix(ctx, given):
+    """sparse_geom_mm(hA_geo, hA_mat) runs A x by the geometry and A^T x on
+    hA_mat (3d/mat_vec_mul.h:104-120,732-756).  Here the geometry walks
+    Bresenham rays while hA_mat holds other weights of the same shape
+    (Siddon lengths, scaled and rounded to integers): A x must be the
+    Bresenham product and A^T r the product with hA_mat, both bit for bit
+    (integer weights and inputs make every sum exact)."""
+    N = 12
+    g = _geom(16, 24, N)
+    sid = orc.siddon_op(g)
+    rp, col, val = orc.siddon_csr(sid)
+    val = np.maximum(np.round(val * 40.0), 1.0)          # integers 1 .. ~7
+    m, n = len(rp) - 1, N ** 3
+    kw = {"csr": (rp, col, val)} if given == "csr" else \
+        {"csc": orc.csr_transpose(m, n, rp, col, val)}
+    hyb = api.HybridOperator(ctx, g, "bresenham", "f32", **kw)
+    dyn = api.DynamicOperator(ctx, g, "bresenham")
+    assert (hyb.m, hyb.n) == (m, n)
+    rng = np.random.default_rng(19)
+    x = rng.integers(0, 5, size=n).astype(np.float32)
+    r = rng.integers(-4, 5, size=m).astype(np.float32)
+    assert (H(hyb.mat_vec_mul(T(x))) == H(dyn.mat_vec_mul(T(x)))).all()
+    want = orc.matvec(orc.csr_op(m, n, rp, col, val), 1, r.astype(np.float64))
+    got = H(hyb.mat_vec_mul(T(r), trans=True))
+    assert (got == want).all()
+    bres = orc.matvec(orc.bresenham_op(g), 1, r.astype(np.float64))
+    assert not np.allclose(want, bres)                    # the two matrices differ

@@ -593,6 +593,7 @@ static void apply_code(tv3_ctx *ctx, SparseRows &S, int code) {
 void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz,
                   int panels) {
     FusedLayout &F = fused_layout(op, dims);
+    F.session.reset();               // its graphs hold the old launch shapes
     tv3_ctx *ctx = op->ctx;
     const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0;
     if (pa || pt) {
@@ -632,6 +633,72 @@ __global__ void k_scrub(const float4 *__restrict__ p, int64_t n, float *sink) {
         a += v.x + v.y + v.z + v.w;
     }
     if (a == -1.f) *sink = a;   // the buffer is zero: never taken, keeps the loads
+}
+
+// Bandwidth probes: a streaming copy (read n, write n float4) and a streaming
+// read, four independent 16-byte accesses per thread per pass.
+__global__ __launch_bounds__(256) void k_probe_copy(const float4 *__restrict__ src,
+                                                    float4 *__restrict__ dst, int64_t n) {
+    const int64_t G = (int64_t)gridDim.x * 256;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * G < n; i += 4 * G) {
+        const float4 a = src[i], b = src[i + G], c = src[i + 2 * G], d = src[i + 3 * G];
+        dst[i] = a; dst[i + G] = b; dst[i + 2 * G] = c; dst[i + 3 * G] = d;
+    }
+    for (; i < n; i += G) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_probe_read(const float4 *__restrict__ src, int64_t n,
+                                                    float *sink) {
+    const int64_t G = (int64_t)gridDim.x * 256;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float s = 0.f;
+    for (; i + 3 * G < n; i += 4 * G) {
+        const float4 a = src[i], b = src[i + G], c = src[i + 2 * G], d = src[i + 3 * G];
+        s += (a.x + b.x) + (c.x + d.x);
+    }
+    for (; i < n; i += G) s += src[i].x;
+    if (s == -1.f) *sink = s;   // the buffer is zero: never taken, keeps the loads
+}
+
+void bandwidth_probe(tv3_ctx *ctx, int64_t bytes, int reps, double *copy_gbps,
+                     double *read_gbps) {
+    hipStream_t s = ctx->stream;
+    const int64_t n = bytes / 16;
+    TV3_REQUIRE(n >= 1024, TV3_E_ARG, "probe size below 16 KiB");
+    DevBuf<float4> a(n), b(n);
+    DevBuf<float> sink(1);
+    TV3_HIP(hipMemsetAsync(a.p, 0, a.bytes(), s));
+    TV3_HIP(hipMemsetAsync(b.p, 0, b.bytes(), s));
+    const unsigned grid = (unsigned)std::max(1, ctx->num_cus) * 8;
+    hipEvent_t e0, e1;
+    TV3_HIP(hipEventCreate(&e0));
+    TV3_HIP(hipEventCreate(&e1));
+    auto median_ms = [&](auto launch) {
+        std::vector<float> t;
+        launch();
+        for (int r = 0; r < std::max(1, reps); ++r) {
+            TV3_HIP(hipEventRecord(e0, s));
+            launch();
+            TV3_HIP(hipEventRecord(e1, s));
+            TV3_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            TV3_HIP(hipEventElapsedTime(&ms, e0, e1));
+            t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        return (double)t[t.size() / 2];
+    };
+    const double tc = median_ms([&] {
+        hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(256), 0, s, a.p, b.p, n);
+    });
+    const double tr = median_ms([&] {
+        hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(256), 0, s, a.p, n, sink.p);
+    });
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (copy_gbps) *copy_gbps = 2.0 * 16.0 * (double)n / (tc * 1e-3) / 1e9;
+    if (read_gbps) *read_gbps = 16.0 * (double)n / (tr * 1e-3) / 1e9;
 }
 
 double layout_time(tv3_op *op, const int32_t dims[3], bool trans, int reps) {

@@ -217,8 +217,11 @@ __global__ __launch_bounds__(256) void k_b_descend(const BDims *__restrict__ D,
     const BDims dm = D[k];
     const float *Uc = c.cur ? U1 : U0;
     double acc[3] = {0, 0, 0};
-    const float tau = (float)c.tau, beta = (float)c.beta, ib = 1.f / beta;
+    const float tau = (float)c.tau, beta = (float)c.beta;
     const bool nn = c.nonneg, iso = c.iso, old = c.sigw_old;
+    // W of the last update_W: old sigma at the old beta after an outer step
+    // (3d/tval3_gpu_3d.cpp:484-492 reads the stored W)
+    const float ib = 1.f / (old ? (float)c.beta_w : beta);
     const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
     Vox v;
     if (voxel_at(dm, i, v)) {
@@ -1537,7 +1540,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         inf.loop_secs = msl / 1000.0;
         inf.backtracks = c.backtracks;
         inf.fallbacks = c.fallbacks;
-        inf.n_matvec_N = 1 + (int64_t)c.total + c.inner_loops + 2 * (int64_t)c.fallbacks;
+        inf.n_matvec_N = 1 + (int64_t)c.total + c.fallbacks + c.n_first;
         // with skipping: A^T b, the initial product and the products get_g took
         inf.n_matvec_T = B.skipping ? 2 + (int64_t)c.n_T : 2 + (int64_t)c.total + c.fallbacks;
         inf.status = c.nonfinite ? TV3_E_NONFINITE : TV3_OK;
@@ -1560,6 +1563,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
                 h.relchg[j] = q[8]; h.mu[j] = q[9]; h.beta[j] = q[10]; h.C[j] = q[11];
                 h.cnt[j] = (int32_t)q[12]; h.outer[j] = (int32_t)q[13];
                 if (h.mulT) h.mulT[j] = (int32_t)q[14];
+                if (h.f_trial) h.f_trial[j] = q[15];
             }
             h.count = cnt;
         }

@@ -146,6 +146,9 @@ __global__ __launch_bounds__(256) void k_descend(Lay L, Groups G, const float *_
     const float tau = (float)C->tau, beta = (float)C->beta, ib = 1.f / beta;
     const bool nn = C->nonneg, iso = C->iso;
     const bool old = C->sigw_old;
+    // after an outer step W is still the one update_W built from the old
+    // sigma at the old beta (3d/tval3_gpu_3d.cpp:484-492 reads the stored W)
+    const float ibw = old ? 1.f / (float)C->beta_w : ib;
     int g0, g1;
     group_range(G.count, g0, g1);
     // with B = 8 every voxel of a group is valid and the grid has one group
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(256) void k_descend(Lay L, Groups G, const float *_
         };
         const float t0 = step(u0, d0), ty = step(uy, dy), tx = step(ux, dx), tz = step(uz, dz);
         float wx, wy, wz;
-        shrink3(ux - u0, uy - u0, uz - u0, w_x, w_y, w_z, ib, iso, wx, wy, wz);
+        shrink3(ux - u0, uy - u0, uz - u0, w_x, w_y, w_z, ibw, iso, wx, wy, wz);
         const float v0x = (ux - u0) - wx, v0y = (uy - u0) - wy, v0z = (uz - u0) - wz;
         const float v1x = (tx - t0) - wx, v1y = (ty - t0) - wy, v1z = (tz - t0) - wz;
         acc[0] += (double)v1x * v1x + (double)v1y * v1y + (double)v1z * v1z;
@@ -398,8 +401,10 @@ __global__ __launch_bounds__(512) void k_descend_tile(Lay L, int ntiles, int tpw
     constexpr int NSHELL = NB - 512;                  // 217 upper-shell voxels
     __shared__ float su[NB], sd[NB];
     const int active = C->active;
-    const float tau = (float)C->tau, beta = (float)C->beta, ib = 1.f / beta;
+    const float tau = (float)C->tau, beta = (float)C->beta;
     const bool nn = C->nonneg, iso = C->iso, old = C->sigw_old;
+    // W of the last update_W: old sigma at the old beta after an outer step
+    const float ibw = 1.f / (old ? (float)C->beta_w : beta);
     const int t = threadIdx.x;
     const int tile0 = xcd_remap(blockIdx.x, gridDim.x) * tpw;
     const int tile1 = min(ntiles, tile0 + tpw);
@@ -452,7 +457,7 @@ __global__ __launch_bounds__(512) void k_descend_tile(Lay L, int ntiles, int tpw
         const float t0 = step(u0, sd[sl]), ty = step(uy, sd[sl + 1]),
                     tx = step(ux, sd[sl + kDBox]), tz = step(uz, sd[sl + kDBox * kDBox]);
         float wx, wy, wz;
-        shrink3(ux - u0, uy - u0, uz - u0, cur.w[0], cur.w[1], cur.w[2], ib, iso, wx, wy, wz);
+        shrink3(ux - u0, uy - u0, uz - u0, cur.w[0], cur.w[1], cur.w[2], ibw, iso, wx, wy, wz);
         const float v0x = (ux - u0) - wx, v0y = (uy - u0) - wy, v0z = (uz - u0) - wz;
         const float v1x = (tx - t0) - wx, v1y = (ty - t0) - wy, v1z = (tz - t0) - wz;
         acc[0] += (double)v1x * v1x + (double)v1y * v1y + (double)v1z * v1z;
@@ -877,6 +882,86 @@ __global__ void k_set_dd(FCtl *C, const double *p, int nb) {
     if (threadIdx.x == 0) C->dd = a[0];
 }
 
+// ------------------------------------------------------------------ scaleb
+// scaleb_gpu (3d/tval3_gpu_3d.cpp:225-266) on the device: the entries of
+// smallest and largest magnitude, the first such entry on ties as in the
+// reference's forward scan, and whether b holds NaN or Inf.  Block partials,
+// then one workgroup; (|b|, index) pairs form a total order, so the result
+// does not depend on the grid.
+struct BRange {
+    float amin, amax;
+    int64_t imin, imax;
+    int32_t bad;
+};
+
+__device__ __forceinline__ void brange_merge(BRange &a, const BRange &b) {
+    if (b.amin < a.amin || (b.amin == a.amin && b.imin < a.imin)) { a.amin = b.amin; a.imin = b.imin; }
+    if (b.amax > a.amax || (b.amax == a.amax && b.imax < a.imax)) { a.amax = b.amax; a.imax = b.imax; }
+    a.bad |= b.bad;
+}
+
+__device__ __forceinline__ BRange brange_block(BRange v) {
+    __shared__ BRange sh[256];
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) brange_merge(sh[t], sh[t + off]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+constexpr int kScalebBlocks = 512;
+
+__global__ __launch_bounds__(256) void k_scaleb_parts(int64_t m, const float *__restrict__ b,
+                                                      BRange *part) {
+    BRange v{INFINITY, -1.f, INT64_MAX, INT64_MAX, 0};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * 256) {
+        const float x = b[i], a = fabsf(x);
+        if (!isfinite(x)) { v.bad = 1; continue; }
+        if (a < v.amin) { v.amin = a; v.imin = i; }     // i grows: first index kept
+        if (a > v.amax) { v.amax = a; v.imax = i; }
+    }
+    const BRange r = brange_block(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// range of the selected entries scaled to 0.5 (below 0.5) or 1.5 (above 1.5)
+__global__ __launch_bounds__(256) void k_scaleb_final(const float *__restrict__ b,
+                                                      const BRange *part, int nb, FCtl *C) {
+    BRange v{INFINITY, -1.f, INT64_MAX, INT64_MAX, 0};
+    for (int j = threadIdx.x; j < nb; j += 256) brange_merge(v, part[j]);
+    const BRange r = brange_block(v);
+    if (threadIdx.x == 0) {
+        C->b_bad = r.bad;
+        float s = 1.f;
+        if (!r.bad) {
+            const float dif = fabsf(b[r.imax] - b[r.imin]);
+            if (dif < 0.5f) s = 0.5f / dif;
+            else if (dif > 1.5f) s = 1.5f / dif;
+        }
+        C->scl = s;
+    }
+}
+
+// y = scl x (scaling b, :1122-1124; a warm-start U) and y = x / scl (the
+// result, :1275-1277) with scl from the control block
+__global__ __launch_bounds__(256) void k_scale_ctl(int64_t n, float *__restrict__ y,
+                                                   const float *__restrict__ x, const FCtl *C,
+                                                   int divide) {
+    const float s = (float)C->scl;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        y[i] = divide ? x[i] / s : s * x[i];
+}
+
+static unsigned ew_grid(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 8192));
+}
+
 // ------------------------------------------------------------------ host side
 // workgroups of the launch that runs the A x epilogue (block partial sums)
 int product_blocks(const FusedLayout &F) {
@@ -887,12 +972,23 @@ int product_blocks(const FusedLayout &F) {
     return (int)((S.rows * L + 255) / 256);
 }
 
+// The solver's state for one layout, kept in FusedLayout::session between
+// solves: buffers, control block, partials and the two instantiated graphs
+// are made by the first solve and reused by later ones with the same
+// context, A^T-skip setting, graph size and tile setting.
 template <int B>
-struct Fused {
-    tv3_ctx *ctx;
-    FusedLayout *F;
-    const tv3_opts *o;
-    hipStream_t s;
+struct Fused : FusedSession {
+    tv3_ctx *ctx = nullptr;
+    FusedLayout *F = nullptr;
+    const tv3_opts *o = nullptr;
+    hipStream_t s = nullptr;
+    // what the captured graphs depend on
+    bool k_skip = false;
+    int k_graph_iters = 0, k_tiles = 0;
+    size_t hist_cap = 0;             // history rows the buffer holds
+    bool captured = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    DevBuf<BRange> brange;
     Lay L;
     int64_t n, m;
     int nbS, nbN, nbNd, nbM, nbW, nbD;
@@ -961,7 +1057,28 @@ struct Fused {
         pK = q; q += 2 * (size_t)nbM;
         ctl.alloc(1);
         hctl.alloc(1);
-        hist.alloc((size_t)(o->maxit > 0 ? o->maxit : 1) * HF);
+        brange.alloc(kScalebBlocks);
+        for (auto &e : ev) TV3_HIP(hipEventCreate(&e));
+    }
+
+    bool matches(const tv3_ctx *c, bool skip, int gi, int tiles) const {
+        return c == ctx && c->stream == s && skip == k_skip && gi == k_graph_iters &&
+               tiles == k_tiles;
+    }
+
+    // history rows for maxit iterations; a new buffer invalidates the graphs
+    void ensure_hist(int maxit) {
+        const size_t rows = (size_t)std::max(maxit, 1);
+        if (rows <= hist_cap) return;
+        drop_graphs();
+        hist.alloc(rows * HF);
+        hist_cap = rows;
+    }
+
+    void drop_graphs() {
+        for (auto &gx : graph)
+            if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+        captured = false;
     }
 
     void read_ctl() {
@@ -1035,49 +1152,51 @@ struct Fused {
         TV3_HIP(hipGraphDestroy(gr));
     }
 
-    ~Fused() {
-        for (auto &gx : graph)
-            if (gx) (void)hipGraphExecDestroy(gx);
+    ~Fused() override {
+        if (s) (void)hipStreamSynchronize(s);
+        drop_graphs();
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
+
+// The session of this layout for these options: the kept one when it
+// matches, else a new one (buffers allocated, graphs captured on first use).
+template <int B>
+Fused<B> &fused_session(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o) {
+    const bool skip = o.skipMulRatio != 0.0;
+    const int gi = o.graph_iters > 0 ? o.graph_iters : 8;
+    auto *cur = dynamic_cast<Fused<B> *>(FL.session.get());
+    if (cur && cur->matches(ctx, skip, gi, o.tiles_per_wg)) {
+        cur->o = &o;
+        return *cur;
+    }
+    FL.session.reset();
+    std::unique_ptr<Fused<B>> S(new Fused<B>());
+    S->ctx = ctx; S->F = &FL; S->o = &o; S->s = ctx->stream;
+    S->L.p = FL.dims[0]; S->L.q = FL.dims[1]; S->L.r = FL.dims[2];
+    S->L.nby = FL.nby; S->L.nbx = FL.nbx; S->L.nbz = S->L.r / 8;
+    S->L.pq = (int64_t)S->L.p * S->L.q;
+    S->n = (int64_t)S->L.p * S->L.q * S->L.r;
+    S->m = FL.A.rows;
+    S->graph_iters = gi;
+    S->skipping = skip;
+    S->k_skip = skip; S->k_graph_iters = gi; S->k_tiles = o.tiles_per_wg;
+    S->alloc();
+    Fused<B> &r = *S;
+    FL.session = std::move(S);
+    return r;
+}
 
 template <int B>
 void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_dev, float *U_out,
                const float *Ut_dev, tv3_info *info, tv3_history *hist) {
-    Fused<B> S;
-    S.ctx = ctx; S.F = &FL; S.o = &o; S.s = ctx->stream;
+    Fused<B> &S = fused_session<B>(ctx, FL, o);
     hipStream_t s = S.s;
-    S.L.p = FL.dims[0]; S.L.q = FL.dims[1]; S.L.r = FL.dims[2];
-    S.L.nby = FL.nby; S.L.nbx = FL.nbx; S.L.nbz = S.L.r / 8;
-    S.L.pq = (int64_t)S.L.p * S.L.q;
-    S.n = (int64_t)S.L.p * S.L.q * S.L.r;
-    S.m = FL.A.rows;
     const int64_t n = S.n, m = S.m;
-    S.graph_iters = o.graph_iters > 0 ? o.graph_iters : 8;
-    S.alloc();
-
-    hipEvent_t ev0, ev1, evl0, evl1;
-    TV3_HIP(hipEventCreate(&ev0)); TV3_HIP(hipEventCreate(&ev1));
-    TV3_HIP(hipEventCreate(&evl0)); TV3_HIP(hipEventCreate(&evl1));
+    S.ensure_hist(o.maxit);
+    hipEvent_t ev0 = S.ev[0], ev1 = S.ev[1], evl0 = S.ev[2], evl1 = S.ev[3];
     TV3_HIP(hipEventRecord(ev0, s));
-
-    // scaleb (:225-266) on a host copy; b into the internal ray order
-    std::vector<float> bh(m);
-    TV3_HIP(hipMemcpyAsync(bh.data(), b_dev, sizeof(float) * m, hipMemcpyDeviceToHost, s));
-    TV3_HIP(hipStreamSynchronize(s));
-    float bmin = bh[0], bmax = bmin, amin = std::fabs(bmin), amax = amin;
-    for (float v : bh) {
-        TV3_REQUIRE(std::isfinite(v), TV3_E_NONFINITE, "b holds NaN or Inf");
-        const float a = std::fabs(v);
-        if (a < amin) { bmin = v; amin = a; }
-        if (a > amax) { bmax = v; amax = a; }
-    }
-    const float dif = std::fabs(bmax - bmin);
-    float scl = 1.f;
-    if (dif < 0.5f) scl = 0.5f / dif;
-    else if (dif > 1.5f) scl = 1.5f / dif;
-    permute_gather(b_dev, FL.r_int2nat.p, S.tmp, m, s);
-    scale<float>(m, S.b, scl, S.tmp, s);
 
     FCtl &H = *S.hctl.p;
     std::memset(&H, 0, sizeof(FCtl));
@@ -1085,17 +1204,30 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     H.c = o.c; H.gamma = o.gamma; H.rate_gam = o.rate_gam; H.tol_inn = o.tol_inn;
     H.gam_opt = o.gam; H.gam = o.gam; H.rate_cnt = o.rate_cnt; H.mu_max = o.mu; H.beta_max = o.beta;
     H.maxit = o.maxit; H.nonneg = o.nonneg; H.iso = o.tv_mode == TV3_TV_ISO;
-    H.hist_cap = o.maxit; H.active = 1; H.alpha = 1;
-    S.skipping = o.skipMulRatio != 0.0;
+    H.hist_cap = o.maxit; H.active = 1; H.alpha = 1; H.scl = 1;
     H.skip_ratio = o.skipMulRatio; H.max_rel = o.maxRelativeChange;
     H.n_init = (int32_t)std::min<uint32_t>(o.numInitIterations, INT32_MAX);
     S.write_ctl();
+
+    // multipliers and the DtsAtd term start at zero (:1128-1139): sigma, the
+    // old sigma and DtsAtd are consecutive in the pool
+    TV3_HIP(hipMemsetAsync(S.sx, 0, (size_t)(S.Atb - S.sx) * sizeof(float), s));
+    TV3_HIP(hipMemsetAsync(S.delta, 0, sizeof(float) * m, s));
+
+    // scaleb (:225-266) on the device; b into the internal ray order
+    hipLaunchKernelGGL(k_scaleb_parts, dim3(kScalebBlocks), dim3(256), 0, s, m, b_dev,
+                       S.brange.p);
+    hipLaunchKernelGGL(k_scaleb_final, dim3(1), dim3(256), 0, s, b_dev, S.brange.p,
+                       kScalebBlocks, S.ctl.p);
+    permute_gather(b_dev, FL.r_int2nat.p, S.tmp, m, s);
+    hipLaunchKernelGGL(k_scale_ctl, dim3(ew_grid(m)), dim3(256), 0, s, m, S.b, S.tmp, S.ctl.p, 0);
 
     // initialisation (:1128-1153)
     launch_transposed<float>(FL, S.b, EpiStore<float>{S.Atb}, s);
     if (o.warm_start) {
         permute_gather(U_out, FL.v_int2nat.p, S.tmp, n, s);
-        scale<float>(n, S.U[0], scl, S.tmp, s);
+        hipLaunchKernelGGL(k_scale_ctl, dim3(ew_grid(n)), dim3(256), 0, s, n, S.U[0], S.tmp,
+                           S.ctl.p, 0);
     } else {
         TV3_HIP(hipMemcpyAsync(S.U[0], S.Atb, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     }
@@ -1109,13 +1241,17 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pN, S.nbN, S.pW, S.nbW);
     TV3_HIP(hipGetLastError());
 
-    S.capture(0);
-    S.capture(1);
+    if (!S.captured) {
+        S.capture(0);
+        S.capture(1);
+        S.captured = true;
+    }
     TV3_HIP(hipEventRecord(evl0, s));
 
     // U[c] holds the accepted iterate; every completed iteration flips c
     int c = 0, outer = 0;
     S.read_ctl();
+    TV3_REQUIRE(!H.b_bad, TV3_E_NONFINITE, "b holds NaN or Inf");
     while (outer < o.maxcnt && H.total < o.maxit) {
         int prev = H.total;
         S.tau_first(0);              // first iteration of the inner loop
@@ -1132,7 +1268,14 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
                 continue;
             }
             if (H.inner_done || H.nonfinite || H.total >= o.maxit) break;
-            TV3_HIP(hipGraphLaunch(S.graph[c], s));
+            // a whole graph, or the last iterations maxit leaves launched
+            // one by one (a graph would run its tail as no-op launches)
+            const int left = o.maxit - H.total;
+            if (left >= S.graph_iters) {
+                TV3_HIP(hipGraphLaunch(S.graph[c], s));
+            } else {
+                for (int k = 0; k < left; ++k) S.iteration((c + k) & 1, 0);
+            }
             S.read_ctl();
         }
         if (H.nonfinite) break;
@@ -1141,6 +1284,9 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
         hipLaunchKernelGGL(k_relout_scalar, dim3(1), dim3(1024), 0, s, S.ctl.p, S.pRO, S.nbM);
         S.read_ctl();
         if (H.relout < o.tol) break;
+        // the loop ends here: the multiplier and penalty updates would not be
+        // read (the result, counts and rel_chg are final), so they are skipped
+        if (outer >= o.maxcnt || H.total >= o.maxit) break;
         hipLaunchKernelGGL(k_mlp_sigma<B>, dim3(S.nbS), dim3(256), 0, s, S.L, S.G, S.U[c], S.sx,
                            S.sy, S.sz, S.ox, S.oy, S.oz, S.ctl.p, S.pS4);
         hipLaunchKernelGGL(k_mlp_delta, dim3(S.nbM), dim3(256), 0, s, m, S.Au[c], S.b, S.delta,
@@ -1155,7 +1301,8 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     TV3_HIP(hipEventRecord(evl1, s));
 
     // U (natural order) = U_int / scl (:1275-1277)
-    divide<float>(n, S.tmp, scl, S.U[c], s);
+    hipLaunchKernelGGL(k_scale_ctl, dim3(ew_grid(n)), dim3(256), 0, s, n, S.tmp, S.U[c], S.ctl.p,
+                       1);
     permute_scatter(S.tmp, FL.v_int2nat.p, U_out, n, s);
     TV3_HIP(hipEventRecord(ev1, s));
     S.read_ctl();
@@ -1169,9 +1316,10 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
     info->loop_secs = msl / 1000.0;
     info->backtracks = H.backtracks;
     info->fallbacks = H.fallbacks;
-    // per completed iteration 1 A + 1 A^T; per inner loop one A d; per
-    // fallback the rejected trial (A, A^T) and one A d; setup A^T b, A U, A^T
-    info->n_matvec_N = 1 + (int64_t)H.total + H.inner_loops + 2 * (int64_t)H.fallbacks;
+    // setup A U; per completed iteration and per rejected trial (fallback)
+    // one A; one A d per first-iteration step length (inner loops started
+    // and fallbacks, counted where tau is formed)
+    info->n_matvec_N = 1 + (int64_t)H.total + H.fallbacks + H.n_first;
     // with skipping: A^T b, the initial product and the products get_g took
     info->n_matvec_T = S.skipping ? 2 + (int64_t)H.n_T : 2 + (int64_t)H.total + H.fallbacks;
     info->status = H.nonfinite ? TV3_E_NONFINITE : TV3_OK;
@@ -1192,11 +1340,10 @@ void run_fused(tv3_ctx *ctx, FusedLayout &FL, const tv3_opts &o, const float *b_
             hist->relchg[k] = h[8]; hist->mu[k] = h[9]; hist->beta[k] = h[10]; hist->C[k] = h[11];
             hist->cnt[k] = (int32_t)h[12]; hist->outer[k] = (int32_t)h[13];
             if (hist->mulT) hist->mulT[k] = (int32_t)h[14];
+            if (hist->f_trial) hist->f_trial[k] = h[15];
         }
         hist->count = cnt;
     }
-    (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1);
-    (void)hipEventDestroy(evl0); (void)hipEventDestroy(evl1);
 }
 
 }  // namespace

@@ -230,6 +230,7 @@ void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
             S.copy(S.Uzp, S.Uz, n);
             S.descend(U, tau, false);
             double alpha = 1.0;
+            const double f_trial = S.f;
             axpy3<T>(n, S.dU, -1.0, S.Up, U, s);
             double c_armij = dot<T>(ctx, n, S.d, S.d, s) * tau * o.c * S.beta;
             int cnt = 0;
@@ -271,6 +272,7 @@ void solve_plain(tv3_ctx *ctx, tv3_op *op, const tv3_opts &o, const int32_t dims
                 hist->mu[k] = S.mu; hist->beta[k] = S.beta; hist->C[k] = C;
                 hist->cnt[k] = cnt; hist->outer[k] = outer;
                 if (hist->mulT) hist->mulT[k] = S.last_mul;
+                if (hist->f_trial) hist->f_trial[k] = f_trial;
             }
             if (!std::isfinite(S.f)) { nonfinite = true; break; }
             if (RelChg < o.tol_inn && !fst) break;

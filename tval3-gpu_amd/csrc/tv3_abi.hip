@@ -250,6 +250,19 @@ const char *tv3_last_error(const tv3_ctx *ctx) {
 
 void *tv3_ctx_stream(tv3_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
+// host rows (CSR of A, or its CSC as the CSR of A^T) to the device, with the
+// default launch shape and 2048-entry row blocks
+static void upload_rows(tv3_ctx *ctx, SparseRows &R, int64_t rows, int64_t cols, int64_t nnz,
+                        const int32_t *ptr, const int32_t *ind, const void *val, tv3_dtype vt) {
+    hipStream_t s = ctx->stream;
+    R.rows = rows; R.cols = cols; R.nnz = nnz; R.vtype = vt;
+    upload(R.ptr, ptr, rows + 1, s);
+    upload(R.ind, ind, nnz, s);
+    upload_bytes(R.val, val, nnz * dtype_size(vt), s);
+    R.lanes = default_lanes(rows, nnz);
+    build_row_blocks(ctx, R, 2048);
+}
+
 tv3_status tv3_op_csr_create(tv3_ctx *ctx, int64_t m, int64_t n, int64_t nnz,
                              const int32_t *rowptr, const int32_t *col, const void *val,
                              tv3_dtype vt, const int32_t *cptr, const int32_t *crow,
@@ -275,26 +288,10 @@ tv3_status tv3_op_csr_create(tv3_ctx *ctx, int64_t m, int64_t n, int64_t nnz,
         op->device = ctx->device;
         op->kind = OpKind::CSR;
         op->m = m; op->n = n;
-        hipStream_t s = ctx->stream;
-        SparseRows &A = op->A;
-        A.rows = m; A.cols = n; A.nnz = nnz; A.vtype = vt;
-        upload(A.ptr, rowptr, m + 1, s);
-        upload(A.ind, col, nnz, s);
-        upload_bytes(A.val, val, nnz * dtype_size(vt), s);
-        A.lanes = default_lanes(m, nnz);
-        build_row_blocks(ctx, A, 2048);
-        if (has_csc) {
-            SparseRows &T = op->AT;
-            T.rows = n; T.cols = m; T.nnz = nnz; T.vtype = vt;
-            upload(T.ptr, cptr, n + 1, s);
-            upload(T.ind, crow, nnz, s);
-            upload_bytes(T.val, cval, nnz * dtype_size(vt), s);
-            T.lanes = default_lanes(n, nnz);
-            build_row_blocks(ctx, T, 2048);
-        } else {
-            build_transpose(ctx, A, op->AT);
-        }
-        TV3_HIP(hipStreamSynchronize(s));
+        upload_rows(ctx, op->A, m, n, nnz, rowptr, col, val, vt);
+        if (has_csc) upload_rows(ctx, op->AT, n, m, nnz, cptr, crow, cval, vt);
+        else build_transpose(ctx, op->A, op->AT);
+        TV3_HIP(hipStreamSynchronize(ctx->stream));
         *out = op.release();
     });
 }
@@ -417,20 +414,40 @@ tv3_status tv3_op_dynamic_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_wei
 }
 
 tv3_status tv3_op_hybrid_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_weight_mode mode,
-                                tv3_dtype vt, tv3_op **out) {
+                                int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                                const void *val, tv3_dtype vt, const int32_t *cptr,
+                                const int32_t *crow, const void *cval, tv3_op **out) {
     if (!ctx) return TV3_E_ARG;
     return guard(ctx, [&] {
         TV3_REQUIRE(out, TV3_E_ARG, "out is NULL");
         *out = nullptr;
+        TV3_REQUIRE(dtype_size(vt) > 0, TV3_E_ARG, "bad value type");
         set_device(ctx);
         std::unique_ptr<tv3_op> op(new tv3_op());
         op->ctx = ctx;
         op->device = ctx->device;
         op->kind = OpKind::HYBRID;
         fill_geometry(ctx, geom, mode, op.get());
-        SparseRows A;
-        dynamic_materialize(op.get(), vt, A);
-        build_transpose(ctx, A, op->AT);
+        const int64_t m = op->m, n = op->n;
+        SparseRows A;                        // freed after the stream has drained
+        if (cptr) {                          // the caller's CSC is A^T's rows
+            TV3_REQUIRE(nnz >= 0 && nnz <= INT32_MAX, TV3_E_SHAPE, "bad sparse sizes");
+            TV3_REQUIRE(nnz == 0 || (crow && cval), TV3_E_ARG, "partial CSC arrays");
+            check_csr_host(n, m, nnz, cptr, crow, "CSC");
+            TV3_REQUIRE(finite_host(cval, vt, nnz), TV3_E_NONFINITE, "A holds NaN or Inf");
+            upload_rows(ctx, op->AT, n, m, nnz, cptr, crow, cval, vt);
+        } else if (rowptr) {                 // the caller's CSR, transposed on the device
+            TV3_REQUIRE(nnz >= 0 && nnz <= INT32_MAX, TV3_E_SHAPE, "bad sparse sizes");
+            TV3_REQUIRE(nnz == 0 || (col && val), TV3_E_ARG, "partial CSR arrays");
+            check_csr_host(m, n, nnz, rowptr, col, "CSR");
+            TV3_REQUIRE(finite_host(val, vt, nnz), TV3_E_NONFINITE, "A holds NaN or Inf");
+            upload_rows(ctx, A, m, n, nnz, rowptr, col, val, vt);
+            build_transpose(ctx, A, op->AT);
+        } else {                             // no matrix given: the geometry's own
+            dynamic_materialize(op.get(), vt, A);
+            build_transpose(ctx, A, op->AT);
+        }
+        TV3_HIP(hipStreamSynchronize(ctx->stream));
         *out = op.release();
     });
 }
@@ -860,6 +877,16 @@ tv3_status tv3_op_layout_matvec(tv3_op *op, const int32_t dims[3], tv3_trans tra
         check_dims(op, dims);
         set_device(op->ctx);
         layout_matvec(op, dims, trans == TV3_T, x, y, (hipStream_t)stream);
+    });
+}
+
+tv3_status tv3_bandwidth_probe(tv3_ctx *ctx, int64_t bytes, int32_t reps, double *copy_GBps,
+                               double *read_GBps) {
+    if (!ctx) return TV3_E_ARG;
+    return guard(ctx, [&] {
+        TV3_REQUIRE(bytes > 0 && reps > 0, TV3_E_ARG, "bytes and reps must be positive");
+        set_device(ctx);
+        bandwidth_probe(ctx, bytes, reps, copy_GBps, read_GBps);
     });
 }
 

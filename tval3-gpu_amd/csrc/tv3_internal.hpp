@@ -213,6 +213,12 @@ struct SegSum {
 // brick_index (B = 8 when every dimension is a multiple of 8, else 4 when a
 // multiple of 4, else 1), rays in a locality order; A and its CSC in that
 // numbering, and their panel forms where those were tuned in.
+// What a fused solve keeps for the next solve with the same layout: state
+// buffers, control block, partial sums and its instantiated graphs.
+struct FusedSession {
+    virtual ~FusedSession() = default;
+};
+
 struct FusedLayout {
     int32_t dims[3] = {0, 0, 0};
     int B = 1, nby = 0, nbx = 0;
@@ -227,6 +233,9 @@ struct FusedLayout {
     int max_seg = 0;
     DevBuf<int32_t> v_nat2int, v_int2nat;   // voxel maps (n)
     DevBuf<int32_t> r_int2nat;              // ray map (m): internal ray -> natural ray
+    // the solver's buffers and graphs for this layout (dropped with it, and
+    // whenever its launch shapes change)
+    std::unique_ptr<FusedSession> session;
 };
 
 }  // namespace tv3
@@ -330,6 +339,9 @@ void layout_matvec(tv3_op *op, const int32_t dims[3], bool trans, const float *x
 // override the layout's launch shapes (tv3_op_layout_force)
 void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz,
                   int panels);
+// HBM bandwidth probes (GB/s): streaming copy (bytes read + bytes written)
+// and streaming read of a buffer of the given size, median of reps launches
+void bandwidth_probe(tv3_ctx *ctx, int64_t bytes, int reps, double *copy_gbps, double *read_gbps);
 // average duration (us) of one layout product kernel launch
 double layout_time(tv3_op *op, const int32_t dims[3], bool trans, int reps);
 void permute_gather(const float *in, const int32_t *map, float *out, int64_t n, hipStream_t s);

@@ -20,8 +20,13 @@ struct FCtl {
     // state
     double f, lam1, lam2, lam3, lam4, lam5, C, Q, gam, tau, alpha, dd, nrmup2, relchg;
     double ratio, mdb_old, relout;
+    double beta_w;        // beta of the stored W while sigw_old is set (the old beta)
+    double f_trial;       // f of the iteration's first trial (alpha = 1)
+    double scl;           // scaleb's factor of b (:225-266), found on the device
+    int32_t b_bad;        // b holds NaN or Inf
     int32_t total, fst, active, inner_done, need_fallback, nonfinite, sigw_old, cnt;
     int32_t backtracks, fallbacks, inner_loops;
+    int32_t n_first;      // first-iteration step lengths taken (one A d product each)
     // batched solver only
     int32_t cur;          // copy (0/1) of U, g, Au holding the accepted iterate
     int32_t mode;         // 1: the next iteration accepts its trial (the fallback step)
@@ -36,7 +41,7 @@ struct FCtl {
     int32_t n_T;          // A^T products taken by get_g inside iterations
 };
 
-constexpr int HF = 15;   // history fields per iteration (the last: A^T product taken)
+constexpr int HF = 16;   // history fields per iteration (14: A^T product taken, 15: f_trial)
 
 // W = shrink(DU - sigma/beta) of one voxel (3d/tval3_gpu_3d_kernels.cu:1249-1300
 // and the isotropic variant); returns the lam1 contribution.  ib = 1/beta:
@@ -95,6 +100,7 @@ __device__ inline void line_search(FCtl *C, const double *a, const double *b, in
     };
     double L[4] = {l2t, l3t, l4t, l5t};
     double f = fval(C, C->lam1, L[0], L[1], L[2], L[3]);
+    if (mode != 1) C->f_trial = f;   // a fallback keeps the rejected trial's
     double alpha = 1.0;
     int cnt = 0;
     const double c_arm = C->dd * C->tau * C->c * C->beta;
@@ -141,6 +147,7 @@ __device__ inline void iter_end(FCtl *C, const double *a, double *hist) {
         h[5] = C->lam5; h[6] = C->tau; h[7] = C->alpha; h[8] = C->relchg; h[9] = C->mu;
         h[10] = C->beta; h[11] = C->C; h[12] = C->cnt; h[13] = C->inner_loops - 1;
         h[14] = C->skip_ratio != 0.0 ? C->mul_T : 1;
+        h[15] = C->f_trial;
     }
     C->total = k + 1;
     C->tau = fabs(ss / sy);                                          // BB step, :663
@@ -163,6 +170,7 @@ __device__ inline void tau_first_step(FCtl *C, double dDd, double Add, int fallb
         C->mode = 1;
     }
     C->need_tau = 0;
+    C->n_first += 1;
     C->tau = fabs(C->dd / (dDd + C->muDbeta * Add));
 }
 
@@ -175,7 +183,7 @@ __device__ inline void init_step(FCtl *C, const double *a, const double *w) {
     C->Q = 1; C->C = C->f; C->gam = C->gam_opt;
     C->total = 0; C->fst = 1; C->active = 1; C->inner_done = 0; C->need_fallback = 0;
     C->nonfinite = 0; C->sigw_old = 0; C->cnt = 0; C->alpha = 1; C->backtracks = 0;
-    C->fallbacks = 0; C->inner_loops = 1; C->mode = 0; C->need_tau = 1;
+    C->fallbacks = 0; C->inner_loops = 1; C->mode = 0; C->need_tau = 1; C->n_first = 0;
 }
 
 // A^T-skip decision of get_g (:398-468) from |Au - last Au|^2 and |Au|^2:
@@ -216,6 +224,7 @@ __device__ inline void outer_step(FCtl *C, double l4, double l5) {
     C->ratio = beta0 / C->beta;
     C->f = fval(C, C->lam1, C->lam2, C->lam3, C->lam4, C->lam5);
     C->gam = C->gam_opt; C->Q = 1; C->C = C->f;
+    C->beta_w = beta0;    // W of the last update_W: old sigma, old beta (:984-992)
     C->sigw_old = 1; C->fst = 1; C->inner_done = 0; C->active = 1; C->need_fallback = 0;
     C->inner_loops += 1;
     C->need_tau = 1;

@@ -60,7 +60,7 @@ class History(C.Structure):
     _fields_ = [("cap", C.c_int32), ("count", C.c_int32)] + \
                [(k, C.POINTER(C.c_double)) for k in HIST_D] + \
                [("cnt", C.POINTER(C.c_int32)), ("outer", C.POINTER(C.c_int32)),
-                ("mulT", C.POINTER(C.c_int32))]
+                ("mulT", C.POINTER(C.c_int32)), ("f_trial", C.POINTER(C.c_double))]
 
 
 class Geometry(C.Structure):
@@ -104,8 +104,8 @@ SIGNATURES = [
     ("tv3_op_dense_create", C.c_int, [_P, C.c_int64, C.c_int64, _P, C.c_int64, C.c_int,
                                       C.POINTER(_P)]),
     ("tv3_op_dynamic_create", C.c_int, [_P, C.POINTER(Geometry), C.c_int, C.POINTER(_P)]),
-    ("tv3_op_hybrid_create", C.c_int, [_P, C.POINTER(Geometry), C.c_int, C.c_int,
-                                       C.POINTER(_P)]),
+    ("tv3_op_hybrid_create", C.c_int, [_P, C.POINTER(Geometry), C.c_int, C.c_int64, _P, _P, _P,
+                                       C.c_int, _P, _P, _P, C.POINTER(_P)]),
     ("tv3_op_materialize", C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     ("tv3_op_destroy", None, [_P]),
     ("tv3_op_info", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -137,6 +137,8 @@ SIGNATURES = [
                                       C.POINTER(LayoutInfo)]),
     ("tv3_op_layout_matvec", C.c_int, [_P, _P, C.c_int, _P, _P, _P]),
     ("tv3_op_layout_time", C.c_int, [_P, _P, C.c_int, C.c_int32, C.POINTER(C.c_double)]),
+    ("tv3_bandwidth_probe", C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
     ("tv3_op_autotune", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("tv3_op_set_lanes", C.c_int, [_P, C.c_int32, C.c_int32]),
 ]

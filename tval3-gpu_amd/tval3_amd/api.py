@@ -67,6 +67,14 @@ class Context:
     def check(self, st):
         L.check(st, self.h)
 
+    def bandwidth_probe(self, nbytes=1 << 30, reps=10):
+        """(copy GB/s, read GB/s) of streaming kernels over nbytes on this
+        context's device (tv3_bandwidth_probe)"""
+        cp, rd = C.c_double(), C.c_double()
+        self.check(L.load().tv3_bandwidth_probe(self.h, int(nbytes), int(reps), C.byref(cp),
+                                                C.byref(rd)))
+        return cp.value, rd.value
+
     # building blocks -------------------------------------------------------
     def D(self, U, dims):
         import torch
@@ -312,16 +320,34 @@ class DynamicOperator(Operator):
 
 
 class HybridOperator(Operator):
-    """sparse_geom_mm: A x by the traversal, A^T x by a device-built CSC
-    (3d/mat_vec_mul.h:74-756)."""
+    """sparse_geom_mm(hA_geo, hA_mat): A x by the traversal of the geometry,
+    A^T x by the caller's matrix (3d/mat_vec_mul.h:104-120, products
+    :732-756).  csr = (rowptr, col, val) and/or csc = (colptr, row, val) of
+    A (num_paths x voxels); the CSC is used as given, a CSR alone is
+    transposed on the device; with neither, the geometry's own matrix is
+    built on the device."""
     kind = "hybrid"
 
-    def __init__(self, ctx, geom, mode="siddon", dtype="f32"):
+    def __init__(self, ctx, geom, mode="siddon", dtype="f32", csr=None, csc=None):
         g, keep = make_geometry(geom)
         h = C.c_void_p()
         md = L.SIDDON if mode == "siddon" else L.BRESENHAM
-        ctx.check(L.load().tv3_op_hybrid_create(ctx.h, C.byref(g), md, L.DTYPES[dtype],
-                                                C.byref(h)))
+        rp = ci = cvv = cp = cr = cv = None
+        nnz = 0
+        if csr is not None:
+            rp = np.ascontiguousarray(csr[0], np.int32)
+            ci = np.ascontiguousarray(csr[1], np.int32)
+            cvv = _to_dtype_bytes(csr[2], dtype)
+            nnz = int(rp[-1])
+        if csc is not None:
+            cp = np.ascontiguousarray(csc[0], np.int32)
+            cr = np.ascontiguousarray(csc[1], np.int32)
+            cv = _to_dtype_bytes(csc[2], dtype)
+            nnz = int(cp[-1])
+        keep += [a for a in (rp, ci, cvv, cp, cr, cv) if a is not None]
+        ctx.check(L.load().tv3_op_hybrid_create(ctx.h, C.byref(g), md, nnz, _vp(rp), _vp(ci),
+                                                _vp(cvv), L.DTYPES[dtype], _vp(cp), _vp(cr),
+                                                _vp(cv), C.byref(h)))
         super().__init__(ctx, h, keep)
         self.mode = mode
 
@@ -339,6 +365,8 @@ def _history(cap):
     h.cnt = arrs["cnt"].ctypes.data_as(C.POINTER(C.c_int32))
     h.outer = arrs["outer"].ctypes.data_as(C.POINTER(C.c_int32))
     h.mulT = arrs["mulT"].ctypes.data_as(C.POINTER(C.c_int32))
+    arrs["f_trial"] = np.zeros(cap, np.float64)
+    h.f_trial = arrs["f_trial"].ctypes.data_as(C.POINTER(C.c_double))
     return h, arrs
 
 
