@@ -200,6 +200,18 @@ tv3_status tv3_matvec(tv3_op *op, tv3_trans trans, const float *x_dev, float *y_
                       void *stream);
 tv3_status tv3_matvec_f64(tv3_op *op, tv3_trans trans, const double *x_dev, double *y_dev,
                           void *stream);
+/* tv3_matvec with no device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The only scratch a
+ * product needs, the split-K partial sums of a dense operator's A x, lives in
+ * a static device array of 16 MiB per GPU (set up by tv3_ctx_create) instead
+ * of the context's workspace; a dense A x that needs more (rows times column
+ * slabs, min(128, ceil(cols/64)) slabs, 4 bytes each) returns
+ * TV3_E_UNSUPPORTED.  Every other product uses no scratch.  Results equal
+ * tv3_matvec's bit for bit.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (two streams, two host threads) race. */
+tv3_status tv3_matvec_noalloc(tv3_op *op, tv3_trans trans, const float *x_dev, float *y_dev,
+                              void *stream);
 /* Several products at once, Y[:, j] = op X[:, j] (or op^T X[:, j]) for
  * j < nrhs; X and Y column-major device arrays (column j at X + j ldx).  The
  * reference multiplies a dense matrix by several vectors as one GEMM for its

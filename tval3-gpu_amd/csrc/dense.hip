@@ -66,8 +66,11 @@ __global__ __launch_bounds__(256) void k_gemv_t(int64_t m, int64_t n, int64_t ld
     if (lane == 0) y[j] = (T)s;
 }
 
+// ws: the split-K partials of A x go to this workspace of ws_bytes (the
+// allocation-free path) instead of the context's, which grows on demand
 template <typename V, typename T, typename AccT>
-static void gemv(tv3_op *op, bool trans, const T *x, T *y, hipStream_t s) {
+static void gemv(tv3_op *op, bool trans, const T *x, T *y, hipStream_t s, void *ws = nullptr,
+                 size_t ws_bytes = 0) {
     const V *A = reinterpret_cast<const V *>(op->dense.p);
     int64_t m = op->m, n = op->n, lda = op->lda;
     if (!trans) {
@@ -77,8 +80,16 @@ static void gemv(tv3_op *op, bool trans, const T *x, T *y, hipStream_t s) {
         int64_t cols_per = (n + ks - 1) / ks;
         ks = (int)((n + cols_per - 1) / cols_per);
         size_t need = sizeof(AccT) * (size_t)ks * (size_t)m;
-        if (op->ctx->scratch.bytes() < need) op->ctx->scratch.alloc(need);
-        AccT *part = reinterpret_cast<AccT *>(op->ctx->scratch.p);
+        AccT *part;
+        if (ws) {
+            TV3_REQUIRE(need <= ws_bytes, TV3_E_UNSUPPORTED,
+                        "dense A x needs " + std::to_string(need) + " bytes of partial sums; the "
+                        "allocation-free path holds " + std::to_string(ws_bytes));
+            part = reinterpret_cast<AccT *>(ws);
+        } else {
+            if (op->ctx->scratch.bytes() < need) op->ctx->scratch.alloc(need);
+            part = reinterpret_cast<AccT *>(op->ctx->scratch.p);
+        }
         hipLaunchKernelGGL((k_gemv_n_part<V, T, AccT>), dim3((unsigned)((m + 255) / 256), ks), dim3(256),
                            0, s, m, n, lda, A, x, part, cols_per);
         hipLaunchKernelGGL((k_gemv_n_sum<T, AccT>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m,
@@ -103,6 +114,16 @@ void dense_gemv(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,
         acc64 ? gemv<__hip_bfloat16, float, double>(op, trans, x, y, s)
               : gemv<__hip_bfloat16, float, float>(op, trans, x, y, s);
         break;
+    default: throw tv3_error(TV3_E_FORMAT, "FP32 dense products need F32, F16 or BF16 values");
+    }
+}
+
+void dense_gemv_ws(tv3_op *op, const float *x, float *y, hipStream_t s, void *ws,
+                   size_t ws_bytes) {
+    switch (op->dtype) {
+    case TV3_F32: gemv<float, float, float>(op, false, x, y, s, ws, ws_bytes); break;
+    case TV3_F16: gemv<__half, float, float>(op, false, x, y, s, ws, ws_bytes); break;
+    case TV3_BF16: gemv<__hip_bfloat16, float, float>(op, false, x, y, s, ws, ws_bytes); break;
     default: throw tv3_error(TV3_E_FORMAT, "FP32 dense products need F32, F16 or BF16 values");
     }
 }

@@ -291,6 +291,13 @@ void dense_gemv(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,
                 bool acc64 = false);
 void dense_gemv(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
                 bool acc64 = true);
+// y = A x as dense_gemv (FP32 accumulation) with the split-K partials in the
+// caller's workspace ws of ws_bytes; a larger need is rejected
+void dense_gemv_ws(tv3_op *op, const float *x, float *y, hipStream_t s, void *ws,
+                   size_t ws_bytes);
+// the allocation-free path (noalloc.hip): per-device address of its scratch
+// array, looked up once per device with that device current
+void noalloc_setup(int device);
 
 // dense multi-right-hand-side products on the matrix cores (dense_mfma.hip):
 // Y = M X (- sub), M row-major BF16 R x K, X/Y column-major, G <= 64

@@ -139,6 +139,7 @@ SIGNATURES = [
     ("tv3_op_layout_time", C.c_int, [_P, _P, C.c_int, C.c_int32, C.POINTER(C.c_double)]),
     ("tv3_bandwidth_probe", C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]),
+    ("tv3_matvec_noalloc", C.c_int, [_P, C.c_int, _P, _P, _P]),
     ("tv3_op_autotune", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("tv3_op_set_lanes", C.c_int, [_P, C.c_int32, C.c_int32]),
 ]

@@ -142,6 +142,19 @@ class Operator:
         self.ctx.check(f(self.h, 1 if trans else 0, _vp(x), _vp(y), _torch_stream()))
         return y
 
+    def mat_vec_mul_noalloc(self, x, y, trans=False):
+        """mat_vec_mul through tv3_matvec_noalloc: no device allocation (a
+        dense A x keeps its partial sums in the library's fixed per-GPU
+        scratch array); float32 tensors, y preallocated."""
+        import torch
+        assert x.dtype == torch.float32 and y.dtype == torch.float32
+        assert x.is_contiguous() and y.is_contiguous()
+        assert x.numel() == (self.m if trans else self.n)
+        assert y.numel() == (self.n if trans else self.m)
+        self.ctx.check(L.load().tv3_matvec_noalloc(self.h, 1 if trans else 0, _vp(x), _vp(y),
+                                                   _torch_stream()))
+        return y
+
     def mat_mat_mul(self, X, Y=None, trans=False):
         """Y[:, j] = A X[:, j] (or A^T X[:, j]) for the columns of X, a float32
         (length, nrhs) tensor with columns contiguous (X.t() contiguous, e.g.
