@@ -217,7 +217,8 @@ tv3_status tv3_matvec_noalloc(tv3_op *op, tv3_trans trans, const float *x_dev, f
  * reference multiplies a dense matrix by several vectors as one GEMM for its
  * multi-layer case (2d/mat_vec_mul.h:91-99).  A dense BF16 operator whose
  * sizes are multiples of 256 runs them on the matrix cores, 64 columns per
- * product (X split into two BF16 halves, FP32 accumulation); any other
+ * product (each FP32 value of X split into three BF16 terms, hi + mid + lo,
+ * FP32 accumulation); any other
  * operator applies tv3_matvec per column.  Asynchronous on stream. */
 tv3_status tv3_matvec_multi(tv3_op *op, tv3_trans trans, int32_t nrhs, const float *X_dev,
                             int64_t ldx, float *Y_dev, int64_t ldy, void *stream);

@@ -1,0 +1,147 @@
+"""Parity at the stated size of config 1 and with BF16 weights.
+
+- Config 1 (64^3 voxels, 128 emitters x 512 receivers = 65,536 rays, Siddon
+  CSR with FP32 weights, isotropic TV, mu = 2^9) to the default stopping
+  rule: the reference-shaped FP32 loop reproduces the FP32 oracle's iterates
+  (same decisions and counts, final U bit for bit, f to FP64 summation order),
+  and the fused
+  schedule stays inside the oracle's own rounding envelope.
+- BF16 weights with FP32 accumulation (the weight type of config 3): fused
+  and batched solves against the FP32 oracle solving the same BF16-rounded
+  matrix, with the envelope criterion.
+Every case prints its error against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import tv3_oracle as orc
+from tval3_amd import api, workload as wl
+
+from test_gpu_fused import first_divergence, _envelope_runs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available()
+    return api.Context(0)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def problem(ctx, dims, E, R, seed, dtype="f32"):
+    u = wl.phantom_3d(dims=dims, seed=seed)
+    g = wl.transducer_geometry(E, R, dims=dims, seed=seed + 1)
+    dyn = api.DynamicOperator(ctx, g, "siddon")
+    A32 = dyn.materialize("f32")
+    A = A32 if dtype == "f32" else dyn.materialize(dtype)
+    rp, c, v = A.download_csr()
+    oop = orc.csr_op(A.m, A.n, rp, c, v)
+    rp32, c32, v32 = A32.download_csr()
+    oop32 = orc.csr_op(A32.m, A32.n, rp32, c32, v32)
+    b = wl.add_noise(orc.matvec(oop32, 0, u), 0.005, seed=seed + 2).astype(np.float32)
+    return u, A, oop, b
+
+
+CFG1 = dict(mu=2.0 ** 9, mu0=2.0 ** 9)
+
+
+@pytest.fixture(scope="module")
+def cfg1(ctx):
+    return problem(ctx, (64, 64, 64), 128, 512, seed=1)
+
+
+def check_envelope(h, ho, runs, io, info, U, Uo, u, tag, rtol_alpha=1e-5):
+    """rounding level at the start, then within 30x the oracle's envelope
+    under 1e-7 perturbations of b; the same alpha on every non-fallback
+    iteration; U within 10x the perturbed runs' distance to the oracle's U,
+    and the final error within the error spread of those runs or within
+    |U - U_oracle| / |u_true| (the triangle inequality bounds the change of
+    the error by the change of U)"""
+    k = min([first_divergence(h, ho)] + [first_divergence(r[2], ho) for r in runs])
+    dev = np.abs(h["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k])
+    env = np.max([np.maximum.accumulate(np.abs(r[2]["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k]))
+                  for r in runs], axis=0)
+    spread = max(abs(r[1].rel_error - io.rel_error) for r in runs)
+    print(f"{tag}: iters {info['total_iters']} oracle {io.total_iters}; decisions agree for {k}; "
+          f"err {info['rel_error']:.4e} oracle {io.rel_error:.4e} (spread {spread:.1e}); "
+          f"max f dev {dev.max():.1e} envelope {env.max():.1e}; rel(U, U_oracle) {rel(U, Uo):.2e}")
+    assert k >= 10
+    assert (dev[:4] < 1e-5).all()
+    assert (dev <= 30 * env + 1e-6).all()
+    live = ho["cnt"][:k] < 6
+    assert np.allclose(h["alpha"][:k][live], ho["alpha"][:k][live], rtol=rtol_alpha)
+    assert rel(U, Uo) <= max(10 * max(rel(r[0], Uo) for r in runs), 1e-4)
+    bound = rel(U, Uo) * np.linalg.norm(Uo) / np.linalg.norm(u)
+    assert abs(info["rel_error"] - io.rel_error) <= max(3 * spread, 1e-3, 1.01 * bound)
+
+
+def test_cfg1_plain_fp32_reproduces_fp32_oracle(ctx, cfg1):
+    u, A, oop, b = cfg1
+    dims = (64, 64, 64)
+    assert A.m == 65536
+    o = api.default_opts(**CFG1, tv_mode="iso")
+    U, info, h = api.tval3_gpu_3d(ctx, A, b, dims, o, Ut=u)
+    oo = orc.default_opts(**CFG1, tv_iso=1)
+    Uo, io, ho, _ = orc.solve(oop, dims, b, oo, Ut=u, dtype=np.float32)
+    U64, i64, _, _ = orc.solve(oop, dims, b.astype(np.float64), oo, Ut=u)
+    print(f"cfg1 plain: {info['total_iters']} inner / {info['outer_iters']} outer (oracle "
+          f"{io.total_iters} / {io.outer_iters}); rel(U, oracle FP32) {rel(U, Uo):.2e}, "
+          f"rel(U, oracle FP64) {rel(U, U64):.2e}; err {info['rel_error']:.4e}, FP64 "
+          f"{i64.rel_error:.4e}; U bitwise equal: {bool((U == Uo).all())}")
+    assert info["total_iters"] == io.total_iters and info["outer_iters"] == io.outer_iters
+    assert (h["cnt"] == ho["cnt"]).all() and (h["outer"] == ho["outer"]).all()
+    # iterates bit for bit; f is formed from FP64 sums in another order
+    assert np.array_equal(U, Uo)
+    assert np.allclose(h["f"], ho["f"], rtol=1e-12, atol=0)
+
+
+def test_cfg1_fused_tracks_oracle(ctx, cfg1):
+    u, A, oop, b = cfg1
+    dims = (64, 64, 64)
+    o = api.default_opts(**CFG1, tv_mode="iso", solver="fused", maxit=80)
+    U, info, h = api.tval3_gpu_3d(ctx, A, b, dims, o, Ut=u)
+    oo = orc.default_opts(**CFG1, tv_iso=1, maxit=80)
+    Uo, io, ho, _ = orc.solve(oop, dims, b, oo, Ut=u, dtype=np.float32)
+    runs = _envelope_runs(oop, dims, b, oo, u)
+    U64, i64, _, _ = orc.solve(oop, dims, b.astype(np.float64), oo, Ut=u)
+    print(f"cfg1 fused: rel(U, oracle FP64) {rel(U, U64):.2e}")
+    check_envelope(h, ho, runs, io, info, U, Uo, u, "cfg1 fused")
+
+
+def test_fused_bf16_weights_track_oracle(ctx):
+    dims = (24, 24, 24)
+    u, A16, oop16, b = problem(ctx, dims, 64, 128, seed=3, dtype="bf16")
+    rp, c, v = A16.download_csr()
+    bf = (np.asarray(v, np.float32).view(np.uint32) & 0xFFFF0000).view(np.float32)
+    assert np.array_equal(v, bf.astype(np.float64))          # values are BF16 numbers
+    o = api.default_opts(**CFG1, tv_mode="iso", solver="fused", maxit=80)
+    U, info, h = api.tval3_gpu_3d(ctx, A16, b, dims, o, Ut=u)
+    oo = orc.default_opts(**CFG1, tv_iso=1, maxit=80)
+    Uo, io, ho, _ = orc.solve(oop16, dims, b, oo, Ut=u, dtype=np.float32)
+    runs = _envelope_runs(oop16, dims, b, oo, u)
+    U64, _, _, _ = orc.solve(oop16, dims, b.astype(np.float64), oo, Ut=u)
+    print(f"bf16 fused: rel(U, oracle FP64) {rel(U, U64):.2e}")
+    check_envelope(h, ho, runs, io, info, U, Uo, u, "bf16 fused")
+
+
+def test_batch_bf16_weights_track_oracle(ctx):
+    import torch
+    probs = [((16, 16, 16), 24, 64, 41), ((16, 8, 24), 16, 48, 44), ((24, 16, 8), 20, 56, 47)]
+    P = [problem(ctx, *pp, dtype="bf16") for pp in probs]
+    o = api.default_opts(**CFG1, tv_mode="iso", maxit=60)
+    Ub, ib, bi, hb = api.tval3_gpu_3d_batched(
+        ctx, [p[1] for p in P], [torch.from_numpy(p[3]).cuda() for p in P],
+        [pp[0] for pp in probs], o,
+        Uts=[torch.from_numpy(p[0].astype(np.float32)).cuda() for p in P], history=True,
+        lanes=(-2048, -1024), panels=2048, panels_A=2048)
+    oo = orc.default_opts(**CFG1, tv_iso=1, maxit=60)
+    for k, (pp, (u, A, oop, b)) in enumerate(zip(probs, P)):
+        dims = pp[0]
+        Uo, io, ho, _ = orc.solve(oop, dims, b, oo, Ut=u, dtype=np.float32)
+        runs = _envelope_runs(oop, dims, b, oo, u)
+        check_envelope(hb[k], ho, runs, io, ib[k], Ub[k].cpu().numpy(), Uo, u,
+                       f"bf16 batch {dims}")
