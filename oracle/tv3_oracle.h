@@ -149,6 +149,11 @@ double orc_shrink##SFX(int64_t n, int iso, const REAL *Ux, const REAL *Uy,      
                        const REAL *Uz, const REAL *sx, const REAL *sy, const REAL *sz,   \
                        double beta, REAL *Wx, REAL *Wy, REAL *Wz);                       \
 double orc_scaleb##SFX(int64_t m, REAL *b);                                              \
+int orc_siddon_walk##SFX(const REAL p0[3], const REAL p1[3], const REAL org[3],           \
+                         const REAL h[3], int rv_x, int rv_y, int rv_z, int32_t *vox,     \
+                         REAL *len, int cap);                                            \
+int64_t orc_siddon_walk_csr##SFX(const orc_op *op, int32_t *rowptr, int32_t *col,         \
+                                 REAL *val);                                             \
 int orc_solve##SFX(const orc_op *op, const orc_opts *o, int p, int q, int r,              \
                    const REAL *b, REAL *U, const REAL *Ut, orc_info *info,               \
                    orc_history *hist);

@@ -108,6 +108,11 @@ def _declare(L):
             [C.c_double] + [_P] * 3
         getattr(L, "orc_scaleb" + s).restype = C.c_double
         getattr(L, "orc_scaleb" + s).argtypes = [C.c_int64, _P]
+        getattr(L, "orc_siddon_walk" + s).restype = C.c_int
+        getattr(L, "orc_siddon_walk" + s).argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int,
+                                                      _P, _P, C.c_int]
+        getattr(L, "orc_siddon_walk_csr" + s).restype = C.c_int64
+        getattr(L, "orc_siddon_walk_csr" + s).argtypes = [C.POINTER(OrcOp), _P, _P, _P]
         getattr(L, "orc_solve" + s).restype = C.c_int
         getattr(L, "orc_solve" + s).argtypes = [C.POINTER(OrcOp), C.POINTER(OrcOpts),
                                                 C.c_int, C.c_int, C.c_int, _P, _P, _P,
@@ -235,6 +240,31 @@ def siddon_ray(p0, p1, origin, voxel, rv, min_len=0.0):
     c = lib().orc_siddon_ray(_ptr(a), _ptr(b), _ptr(o), _ptr(h), *rv, _ptr(vox), _ptr(ln), cap,
                              min_len)
     return vox[:c], ln[:c]
+
+
+def siddon_walk(p0, p1, origin, voxel, rv, dtype=np.float64):
+    """the tracked-plane traversal of the device products in dtype (float32
+    or float64): (voxel indices, lengths)"""
+    cap = 2 * sum(rv) + 8
+    vox = np.zeros(cap, np.int32)
+    ln = np.zeros(cap, dtype)
+    a, b, o, h = (np.ascontiguousarray(v, dtype) for v in (p0, p1, origin, voxel))
+    c = getattr(lib(), "orc_siddon_walk" + _sfx(dtype))(_ptr(a), _ptr(b), _ptr(o), _ptr(h), *rv,
+                                                         _ptr(vox), _ptr(ln), cap)
+    return vox[:c], ln[:c]
+
+
+def siddon_walk_csr(op, dtype=np.float64):
+    """CSR of a Siddon geometry by the tracked-plane traversal in dtype
+    (positions, origin and voxel size rounded to dtype first)"""
+    P = op.s.num_paths
+    f = getattr(lib(), "orc_siddon_walk_csr" + _sfx(dtype))
+    rowptr = np.zeros(P + 1, np.int32)
+    nnz = f(C.byref(op.s), _ptr(rowptr), None, None)
+    col = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1), dtype)
+    f(C.byref(op.s), _ptr(rowptr), _ptr(col), _ptr(val))
+    return rowptr, col[:nnz], val[:nnz]
 
 
 def csr_transpose(m, n, rowptr, col, val):

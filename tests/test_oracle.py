@@ -650,3 +650,58 @@ def test_solve_nonmonotone_acceptance_and_reference_recursion(orc, case):
                 assert h["f_trial"][k] == h["f_acc"][k]
             else:
                 assert abs(h["f_trial"][k]) > abs(h["C_in"][k] - c)
+
+
+# ------------------------------------------ tracked-plane Siddon traversal
+def _geoms():
+    yield wl.transducer_geometry(16, 24, N=12, seed=2)
+    yield wl.transducer_geometry(64, 128, N=64, seed=2)
+    yield wl.transducer_geometry(24, 40, dims=(20, 12, 8), seed=5)     # anisotropic voxels
+    g = wl.transducer_geometry(8, 8, N=10, seed=3)                     # axis-parallel rays
+    g["r_pos"] = g["e_pos"].copy()
+    g["r_pos"][:, 0] *= -1.0
+    g["e_pos"][:4, 1] = 0.3
+    g["r_pos"][:4, 1] = 0.3
+    yield g
+
+
+def test_siddon_walk_fp64_equals_textbook_merge(orc):
+    """orc_siddon_walk (tracked planes, the traversal of the device products)
+    in FP64 against the textbook Siddon merge of sorted plane crossings
+    (orc_siddon_ray): the same voxels in the same order and the same lengths
+    to FP64 rounding, on every ray, including anisotropic voxels and rays
+    parallel to two axes."""
+    for g in _geoms():
+        op = orc.siddon_op(g)
+        ra, ca, va = orc.siddon_csr(op)
+        rb, cb, vb = orc.siddon_walk_csr(op, np.float64)
+        assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+        assert np.abs(va - vb).max() <= 1e-14 * 2.2
+
+
+def test_siddon_walk_fp32_tracks_fp64(orc):
+    """The FP32 traversal (what the device computes) against the FP64 one:
+    each ray's length sum (its chord through the volume) to FP32 accuracy; the
+    voxels and lengths equal up to the FP32 rounding of the plane parameters
+    t = (org + k h - p0)/d: per ray tol = 8 eps32 (|org| + N h + |p0|) max|1/d| L,
+    which grows on grazing rays (small |d| along an axis); voxels whose
+    length exceeds tol appear in both, with lengths within 2 tol."""
+    eps = 2.0 ** -24
+    for g in _geoms():
+        op = orc.siddon_op(g)
+        r64, c64, v64 = orc.siddon_walk_csr(op, np.float64)
+        r32, c32, v32 = orc.siddon_walk_csr(op, np.float32)
+        pe, pr = orc.build_paths(g["E"], g["R"], g.get("mps", 1))
+        ext = np.abs(np.asarray(g["origin"])) + np.asarray(g["rv"]) * np.asarray(g["voxel"])
+        for k in range(len(r64) - 1):
+            p0, p1 = g["e_pos"][pe[k]], g["r_pos"][pr[k]]
+            d = p1 - p0
+            inv = np.max(1.0 / np.maximum(np.abs(d), 1e-30))
+            tol = 8 * eps * float(np.max(ext + np.abs(p0))) * inv * np.linalg.norm(d)
+            a, b = slice(r64[k], r64[k + 1]), slice(r32[k], r32[k + 1])
+            m64 = dict(zip(c64[a].tolist(), v64[a].tolist()))
+            m32 = dict(zip(c32[b].tolist(), v32[b].astype(np.float64).tolist()))
+            for vox in set(m64) | set(m32):
+                x, y = m64.get(vox, 0.0), m32.get(vox, 0.0)
+                assert abs(x - y) <= 2 * tol + 1e-6 * x, (k, vox, x, y, tol)
+            assert abs(v64[a].sum() - v32[b].sum()) <= 2 * tol * max(1, len(m64)) + 1e-6

@@ -10,14 +10,15 @@
 //    :2167-2170, and does not check T).
 //  * SIDDON -- exact intersection lengths of the emitter->receiver segment
 //    with the voxel grid (Siddon, Med. Phys. 12(2), 1985), traversed in
-//    increasing ray parameter; each segment's voxel is the one containing the
-//    segment midpoint.
+//    increasing ray parameter by tracking the next plane of every axis
+//    (siddon.hpp).
 //
 // One thread walks one ray.  N gathers x along the ray (x stays resident in
 // L2 / MALL) and writes y[ray] once; T scatters x[ray]*w into y with FP32
 // hardware atomics (no replicated sub-volumes).  The same walk, with FP
 // contraction disabled, fills the CSR in dynamic_materialize so a
 // precomputed matrix holds exactly the weights the dynamic product uses.
+#include "siddon.hpp"
 #include "tv3_internal.hpp"
 
 #include <cmath>
@@ -61,74 +62,17 @@ __device__ __forceinline__ void bresenham_walk(int x0, int y0, int z0, int x1, i
     }
 }
 
-// Siddon traversal of p0->p1 through [org, org + N*h]; emit(voxel, length).
+// Siddon traversal of p0->p1 through [org, org + N*h] (siddon.hpp);
+// emit(linear voxel index, length) in traversal order.
 template <typename R, typename F>
 __device__ __forceinline__ void siddon_walk(const R p0[3], const R p1[3], const R org[3],
                                             const R h[3], const int N[3], int rvy_stride, F emit) {
-#pragma clang fp contract(off)
-    const R INF = (R)1e30;
-    R d[3], inv[3], amin = 0, amax = 1;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) d[a] = p1[a] - p0[a];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        R lo = org[a], hi = org[a] + (R)N[a] * h[a];
-        if (d[a] == 0) {
-            inv[a] = 0;
-            if (p0[a] <= lo || p0[a] >= hi) return;
-        } else {
-            inv[a] = (R)1 / d[a];
-            R t0 = (lo - p0[a]) * inv[a], t1 = (hi - p0[a]) * inv[a];
-            if (t0 > t1) { R t = t0; t0 = t1; t1 = t; }
-            amin = t0 > amin ? t0 : amin;
-            amax = t1 < amax ? t1 : amax;
-        }
-    }
-    if (!(amin < amax)) return;
-    const R L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    int k[3], st[3];
-    R an[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (d[a] == 0) { an[a] = INF; k[a] = 0; st[a] = 0; continue; }
-        st[a] = d[a] > 0 ? 1 : -1;
-        R f = (p0[a] + amin * d[a] - org[a]) / h[a];
-        int kk = d[a] > 0 ? (int)floor(f) + 1 : (int)ceil(f) - 1;
-        kk = clampi(kk, 0, N[a]);
-        R t = (org[a] + (R)kk * h[a] - p0[a]) * inv[a];
-        if (t <= amin) {                 // rounding put the plane at/behind the entry
-            kk += st[a];
-            t = (kk < 0 || kk > N[a]) ? INF : (org[a] + (R)kk * h[a] - p0[a]) * inv[a];
-        }
-        k[a] = kk;
-        an[a] = t;
-    }
-    R acur = amin;
-    const int maxsteps = N[0] + N[1] + N[2] + 4;
-    for (int step = 0; step < maxsteps; ++step) {
-        R anext = an[0] < an[1] ? an[0] : an[1];
-        anext = an[2] < anext ? an[2] : anext;
-        anext = amax < anext ? amax : anext;
-        R seg = (anext - acur) * L;
-        if (seg > 0) {
-            R mid = (acur + anext) * (R)0.5;
-            int ic[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                ic[a] = clampi((int)floor((p0[a] + mid * d[a] - org[a]) / h[a]), 0, N[a] - 1);
-            emit((int64_t)ic[2] * N[0] * rvy_stride + (int64_t)ic[0] * rvy_stride + ic[1], seg);
-        }
-        if (anext >= amax) break;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (an[a] == anext) {
-                k[a] += st[a];
-                an[a] = (k[a] < 0 || k[a] > N[a]) ? INF
-                                                  : (org[a] + (R)k[a] * h[a] - p0[a]) * inv[a];
-            }
-        }
-        acur = anext;
-    }
+    SidRay<R> s;
+    sid_setup(s, p0, p1, org, h, N);
+    if (!s.hit) return;
+    sid_walk(s, s.amin, s.amax, [&](int cx, int cy, int cz, R w) {
+        emit((int64_t)cz * N[0] * rvy_stride + (int64_t)cx * rvy_stride + cy, w);
+    });
 }
 
 struct GeoView {
@@ -158,6 +102,8 @@ __device__ __forceinline__ void ray_ends(const GeoView &g, int64_t k, R p0[3], R
     for (int a = 0; a < 3; ++a) { p0[a] = (R)g.ep[3 * e + a]; p1[a] = (R)g.rp[3 * r + a]; }
 }
 
+// The Siddon operator is the FP32 traversal: FP64 products accumulate its
+// weights in FP64 (the same matrix in both precisions).
 template <typename T, typename AccT>
 __global__ __launch_bounds__(256) void k_dyn_N(GeoView g, int siddon, const T *__restrict__ x,
                                                T *__restrict__ y) {
@@ -165,12 +111,11 @@ __global__ __launch_bounds__(256) void k_dyn_N(GeoView g, int siddon, const T *_
     if (k >= g.P) return;
     AccT acc = 0;
     if (siddon) {
-        T p0[3], p1[3], o[3] = {(T)g.org[0], (T)g.org[1], (T)g.org[2]},
-                        h[3] = {(T)g.h[0], (T)g.h[1], (T)g.h[2]};
+        float p0[3], p1[3];
         int N[3] = {g.rvx, g.rvy, g.rvz};
-        ray_ends<T>(g, k, p0, p1);
-        siddon_walk<T>(p0, p1, o, h, N, g.rvy,
-                       [&](int64_t v, T w) { acc += (AccT)w * (AccT)x[v]; });
+        ray_ends<float>(g, k, p0, p1);
+        siddon_walk<float>(p0, p1, g.org, g.h, N, g.rvy,
+                           [&](int64_t v, float w) { acc += (AccT)w * (AccT)x[v]; });
     } else {
         int e = g.path[2 * k], r = g.path[2 * k + 1];
         T w = (T)g.scale;
@@ -189,11 +134,11 @@ __global__ __launch_bounds__(256) void k_dyn_T(GeoView g, int siddon, const T *_
     const T xr = x[k];
     if (xr == 0) return;
     if (siddon) {
-        T p0[3], p1[3], o[3] = {(T)g.org[0], (T)g.org[1], (T)g.org[2]},
-                        h[3] = {(T)g.h[0], (T)g.h[1], (T)g.h[2]};
+        float p0[3], p1[3];
         int N[3] = {g.rvx, g.rvy, g.rvz};
-        ray_ends<T>(g, k, p0, p1);
-        siddon_walk<T>(p0, p1, o, h, N, g.rvy, [&](int64_t v, T w) { atomicAdd(&y[v], w * xr); });
+        ray_ends<float>(g, k, p0, p1);
+        siddon_walk<float>(p0, p1, g.org, g.h, N, g.rvy,
+                           [&](int64_t v, float w) { atomicAdd(&y[v], (T)w * xr); });
     } else {
         int e = g.path[2 * k], r = g.path[2 * k + 1];
         const T v0 = xr * (T)g.scale;
@@ -217,6 +162,8 @@ static void dyn_apply(tv3_op *op, bool trans, const T *x, T *y, hipStream_t s) {
 }
 
 void dynamic_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s, bool acc64) {
+    // a Siddon operator with tile products: deterministic, no atomics
+    if (op->kind == OpKind::DYNAMIC && dyn_tile_apply(op, trans, x, y, s)) return;
     acc64 ? dyn_apply<float, double>(op, trans, x, y, s) : dyn_apply<float, float>(op, trans, x, y, s);
 }
 void dynamic_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s, bool) {

@@ -502,12 +502,17 @@ bool build_forward_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<
 
 void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L) {
     TV3_REQUIRE(op->kind == OpKind::CSR, TV3_E_UNSUPPORTED,
-                "the fused solver needs a sparse operator (materialize a dynamic operator first)");
-    tv3_ctx *ctx = op->ctx;
+                "the fused solver needs a sparse operator or a matrix-free Siddon operator with "
+                "every dimension a multiple of 4");
+    renumber_rows(op->ctx, op->A, op->ray_order, dims, L);
+}
+
+void renumber_rows(tv3_ctx *ctx, const SparseRows &A, const std::vector<int32_t> &ray_order,
+                   const int32_t dims[3], FusedLayout *L) {
     hipStream_t s = ctx->stream;
     for (int a = 0; a < 3; ++a) L->dims[a] = dims[a];
     const int p = dims[0], q = dims[1], r = dims[2];
-    const int64_t n = (int64_t)p * q * r, m = op->m;
+    const int64_t n = (int64_t)p * q * r, m = A.rows;
     L->B = (p % 8 == 0 && q % 8 == 0 && r % 8 == 0) ? 8
            : (p % 4 == 0 && q % 4 == 0 && r % 4 == 0) ? 4 : 1;
     L->nby = L->B == 8 ? p / 8 : p / 4;
@@ -516,7 +521,7 @@ void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L) {
     L->v_int2nat.alloc(n);
     hipLaunchKernelGGL(k_vox_maps, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, q, r, L->B,
                        L->nby, L->nbx, L->v_nat2int.p, L->v_int2nat.p);
-    std::vector<int32_t> order = op->ray_order;
+    std::vector<int32_t> order = ray_order;
     if ((int64_t)order.size() != m) {
         order.resize(m);
         std::iota(order.begin(), order.end(), 0);
@@ -524,7 +529,6 @@ void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L) {
     L->r_int2nat.alloc(m);
     TV3_HIP(hipMemcpyAsync(L->r_int2nat.p, order.data(), sizeof(int32_t) * m,
                            hipMemcpyHostToDevice, s));
-    const SparseRows &A = op->A;
     SparseRows &N = L->A;
     N.rows = m; N.cols = n; N.nnz = A.nnz; N.vtype = A.vtype;
     N.ptr.alloc(m + 1);
@@ -551,6 +555,9 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
     if (op->fused && op->fused->dims[0] == dims[0] && op->fused->dims[1] == dims[1] &&
         op->fused->dims[2] == dims[2])
         return *op->fused;
+    TV3_REQUIRE(op->kind != OpKind::DYNAMIC || !op->fused, TV3_E_SHAPE,
+                "a matrix-free operator's layout is the volume of its geometry: dims must be "
+                "(rv_y, rv_x, rv_z)");
     tv3_ctx *ctx = op->ctx;
     hipStream_t s = ctx->stream;
     std::unique_ptr<FusedLayout> L(new FusedLayout());
@@ -593,6 +600,8 @@ static void apply_code(tv3_ctx *ctx, SparseRows &S, int code) {
 void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz,
                   int panels) {
     FusedLayout &F = fused_layout(op, dims);
+    TV3_REQUIRE(!F.dyn, TV3_E_UNSUPPORTED, "a matrix-free operator's tile products have no "
+                "launch shapes to force");
     F.session.reset();               // its graphs hold the old launch shapes
     tv3_ctx *ctx = op->ctx;
     const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0;

@@ -20,6 +20,7 @@
 #include <type_traits>
 
 #include "lanes.hpp"
+#include "siddon.hpp"
 #include "spmv_stream.hpp"
 
 namespace tv3 {
@@ -269,6 +270,145 @@ void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s
     }
 }
 
+// ------------------------------------------------------------------------
+// Matrix-free Siddon products in boxes (DynTiles, built by dyntile.hip).
+constexpr int kDynEmax = 11264;   // entries of one box: its A^T products in LDS
+constexpr int kDynSmax = 4096;    // segments of one box: their residuals in LDS
+
+struct DynView {
+    const DynPan *pan;
+    const int32_t *order, *seg_ray, *seg_eoff;
+    const float *rays;
+    const uint32_t *pptr;
+    const uint16_t *pos;
+    float org[3], h[3];
+    int N[3];
+};
+
+inline DynView dyn_view(const DynTiles &D) {
+    DynView v;
+    v.pan = D.pan.p; v.order = D.order.p; v.seg_ray = D.seg_ray.p; v.seg_eoff = D.seg_eoff.p;
+    v.rays = D.rays.p; v.pptr = D.pptr.p; v.pos = D.pos.p;
+    for (int a = 0; a < 3; ++a) { v.org[a] = D.org[a]; v.h[a] = D.h[a]; v.N[a] = D.N[a]; }
+    return v;
+}
+
+// offset of voxel (a, b, c) = (y, x, z) offsets in a box, in the internal
+// order of brick_index: bricks of 4^3 in (y, x, z) bit order, y fastest in a
+// brick (a 4^3 box holds one brick)
+__device__ __forceinline__ int dyn_local(int a, int b, int c) {
+    return ((((c >> 2) << 2) | ((b >> 2) << 1) | (a >> 2)) << 6) | ((c & 3) << 4) |
+           ((b & 3) << 2) | (a & 3);
+}
+
+// The traversal of one segment (the part of an internal ray inside box P):
+// emit(offset of the voxel in the box, length) in traversal order -- the
+// entries the walk of the whole ray emits inside the box (siddon.hpp).
+template <typename F>
+__device__ __forceinline__ void dyn_seg_walk(const DynView &g, const DynPan &P, int ray, F emit) {
+    float p0[3], p1[3];
+    const float *rp = g.rays + 6 * (int64_t)ray;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { p0[a] = rp[a]; p1[a] = rp[3 + a]; }
+    SidRay<float> s;
+    sid_setup(s, p0, p1, g.org, g.h, g.N);
+    const int b0[3] = {P.bx, P.by, P.bz}, b1[3] = {P.bx + P.bs, P.by + P.bs, P.bz + P.bs};
+    float ts, te;
+    if (!s.hit || !sid_block_range(s, b0, b1, ts, te)) return;
+    const int m = P.bs - 1;
+    sid_walk(s, ts, te, [&](int cx, int cy, int cz, float w) {
+        // the walk stays in the box; clamped all the same, so that no index
+        // leaves the values staged for it
+        const int a = min(max(cy - P.by, 0), m), b = min(max(cx - P.bx, 0), m),
+                  c = min(max(cz - P.bz, 0), m);
+        emit(dyn_local(a, b, c), w);
+    });
+}
+
+// A x, first pass: per box its voxels staged in LDS, each thread walks
+// segments and stores their partial sums (walk order) in part[segment].
+// `active` (may be NULL) predicates the launch like an epilogue's skip().
+template <typename T>
+__global__ __launch_bounds__(256) void k_dyn_tile_N(DynView g, const T *__restrict__ x,
+                                                    T *__restrict__ part,
+                                                    const int32_t *active) {
+    if (active && !*active) return;
+    __shared__ T xs[512];
+    const DynPan P = g.pan[g.order[blockIdx.x]];
+    const int t = threadIdx.x, nv = P.bs * P.bs * P.bs;
+    for (int i = t; i < nv; i += 256) xs[i] = x[P.v0 + i];
+    __syncthreads();
+    for (int sg = P.s0 + t; sg < P.s1; sg += 256) {
+        T acc = 0;
+        dyn_seg_walk(g, P, g.seg_ray[sg], [&](int li, float w) { acc += (T)w * xs[li]; });
+        part[sg] = acc;
+    }
+}
+
+// A^T r: per box the residuals of its rays staged in LDS, each thread walks
+// segments and writes w r_ray at the entry's position of the box's entry
+// array (ray-major); then each voxel row sums its entries in ray order
+// through the stored positions (padding points at the zero slot ne) and the
+// epilogue takes the sum.  Blocks 0 .. kExtra - 1 run epi.extra().
+template <typename T, typename Epi>
+__global__ __launch_bounds__(256) void k_dyn_tile_T(DynView g, const T *__restrict__ x, Epi epi) {
+    extern __shared__ __align__(16) unsigned char dyn_smem[];
+    constexpr int NX = ExtraBlocks<Epi>::value;
+    if constexpr (NX > 0) {
+        if ((int)blockIdx.x < NX) {
+            epi.extra();
+            return;
+        }
+    }
+    if (epi.skip()) return;
+    const DynPan P = g.pan[g.order[blockIdx.x - NX]];
+    T *pe = reinterpret_cast<T *>(dyn_smem);
+    T *xr = pe + kDynEmax + 1;
+    const int t = threadIdx.x, ns = P.s1 - P.s0;
+    for (int j = t; j < ns; j += 256) xr[j] = x[g.seg_ray[P.s0 + j]];
+    __syncthreads();
+    for (int j = t; j < ns; j += 256) {
+        const int sg = P.s0 + j;
+        const int e1 = j + 1 < ns ? g.seg_eoff[sg + 1] : P.ne;
+        const T xv = xr[j];
+        int e = g.seg_eoff[sg];
+        dyn_seg_walk(g, P, g.seg_ray[sg], [&](int, float w) {
+            if (e < e1) pe[e] = (T)w * xv;
+            ++e;
+        });
+    }
+    if (t == 0) pe[P.ne] = 0;
+    __syncthreads();
+    const int nv = P.bs * P.bs * P.bs;
+    for (int r = t; r < nv; r += 256) {
+        const int row = P.v0 + r;
+        const auto pre = epi.load(row);
+        const uint32_t q0 = g.pptr[row], q1 = g.pptr[row + 1];
+        T acc = 0;
+        for (uint32_t q = q0; q < q1; q += 4) {
+            const uint2 u = *reinterpret_cast<const uint2 *>(g.pos + q);
+            acc += (pe[u.x & 0xffffu] + pe[u.x >> 16]) + (pe[u.y & 0xffffu] + pe[u.y >> 16]);
+        }
+        epi.row(row, acc, true, pre);
+    }
+    epi.finish();
+}
+
+template <typename T>
+void launch_dyn_N(const DynTiles &D, const T *x, T *part, const int32_t *active, hipStream_t s) {
+    if (D.npan == 0) return;
+    hipLaunchKernelGGL((k_dyn_tile_N<T>), dim3((unsigned)D.npan), dim3(256), 0, s, dyn_view(D),
+                       x, part, active);
+}
+
+template <typename T, typename Epi>
+void launch_dyn_T(const DynTiles &D, const T *x, const Epi &epi, hipStream_t s) {
+    const unsigned g = (unsigned)D.npan + ExtraBlocks<Epi>::value;
+    if (g == 0) return;
+    const size_t lds = sizeof(T) * (size_t)(kDynEmax + 1 + kDynSmax);
+    hipLaunchKernelGGL((k_dyn_tile_T<T, Epi>), dim3(g), dim3(256), lds, s, dyn_view(D), x, epi);
+}
+
 constexpr int kSegSumNLB = 2;   // logical blocks (256 / L rays each) per pass-2 workgroup
 
 // Logical blocks and workgroups of pass 2 of A x.  The workgroup count is
@@ -305,7 +445,10 @@ void launch_seg_sum(const SegSum &R, const Epi &epi, hipStream_t s) {
 template <typename Acc, typename Epi>
 void launch_forward(const FusedLayout &F, const float *x, const Epi &epi, hipStream_t s,
                     const int32_t *active = nullptr) {
-    if (F.panels_A) {
+    if (F.dyn) {
+        launch_dyn_N<float>(*F.dyn, x, F.RS.part.p, active, s);
+        launch_seg_sum(F.RS, epi, s);
+    } else if (F.panels_A) {
         launch_panel(F.PA, x, EpiStoreIf{F.RS.part.p, active}, s);
         launch_seg_sum(F.RS, epi, s);
     } else {
@@ -316,7 +459,8 @@ void launch_forward(const FusedLayout &F, const float *x, const Epi &epi, hipStr
 // A^T r of a fused layout (rows = voxels) with epilogue epi.
 template <typename Acc, typename Epi>
 void launch_transposed(const FusedLayout &F, const float *x, const Epi &epi, hipStream_t s) {
-    if (F.panels_T) launch_panel(F.PT, x, epi, s);
+    if (F.dyn) launch_dyn_T<float>(*F.dyn, x, epi, s);
+    else if (F.panels_T) launch_panel(F.PT, x, epi, s);
     else launch_product<Acc, Epi>(F.AT, x, epi, s);
 }
 

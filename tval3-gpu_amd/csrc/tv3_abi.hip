@@ -410,6 +410,7 @@ tv3_status tv3_op_dynamic_create(tv3_ctx *ctx, const tv3_geometry *geom, tv3_wei
         op->device = ctx->device;
         op->kind = OpKind::DYNAMIC;
         fill_geometry(ctx, geom, mode, op.get());
+        op->fused = build_dyn_layout(op.get());   // Siddon, dims multiples of 4: tile products
         *out = op.release();
     });
 }

@@ -209,6 +209,39 @@ struct SegSum {
     DevBuf<float> part;        // nseg partial sums (written by the first pass)
 };
 
+// Matrix-free Siddon products in tiles (dyntile.hip): the volume is cut into
+// boxes (8^3 tiles, or their 4^3 bricks where a tile holds too many
+// entries); a box's segments are the rays crossing it (ascending internal
+// ray order).  A x walks each segment inside its box against the box's
+// voxels staged in LDS, then sums each ray's segments; A^T r walks each
+// segment into an LDS array of the box's entries (ray-major) and sums every
+// voxel's entries through stored positions in ray order -- no atomics, the
+// same bits on every run.  Only the pattern's positions are stored (16 bits
+// per entry); every weight is recomputed by the traversal of siddon.hpp.
+struct DynPan {
+    int32_t v0;          // first internal voxel of the box (its voxels are contiguous)
+    int32_t bx, by, bz;  // box corner in voxel coordinates (x, y, z)
+    int32_t bs;          // box edge: 8 (a tile) or 4 (a brick)
+    int32_t s0, s1;      // its segments
+    int32_t ne;          // entries of its segments
+};
+
+struct DynTiles {
+    int64_t npan = 0, nseg = 0, nnz = 0;
+    int32_t emax = 0, smax = 0;      // most entries / segments of one box
+    float org[3] = {0, 0, 0}, h[3] = {1, 1, 1};
+    int32_t N[3] = {0, 0, 0};        // voxels along x, y, z
+    DevBuf<DynPan> pan;
+    DevBuf<int32_t> order;           // boxes, decreasing work
+    DevBuf<int32_t> seg_ray;         // internal ray of each segment
+    DevBuf<int32_t> seg_eoff;        // first entry of each segment in its box
+    DevBuf<float> rays;              // 6 per internal ray: emitter, receiver (x, y, z)
+    DevBuf<uint32_t> pptr;           // n + 1: padded starts of the voxel rows
+    DevBuf<uint16_t> pos;            // per voxel row its entries' positions in the box,
+                                     // ray order, padded to 4 with the position ne
+    DevBuf<float> xin, yout;         // natural <-> internal order staging (max(n, m))
+};
+
 // The fused solver's internal layout of a sparse operator: voxels numbered by
 // brick_index (B = 8 when every dimension is a multiple of 8, else 4 when a
 // multiple of 4, else 1), rays in a locality order; A and its CSC in that
@@ -236,6 +269,9 @@ struct FusedLayout {
     // the solver's buffers and graphs for this layout (dropped with it, and
     // whenever its launch shapes change)
     std::unique_ptr<FusedSession> session;
+    // a matrix-free Siddon operator: its tile products (RS is their second
+    // pass); A and AT then keep only their sizes
+    std::unique_ptr<DynTiles> dyn;
 };
 
 }  // namespace tv3
@@ -327,6 +363,16 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]);
 // the layout's numbering only: voxel and ray maps, A and its CSC renumbered
 // (no panel forms, no tuned launch shapes)
 void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L);
+// the same for the rows A (m rays, natural order) of any operator
+void renumber_rows(tv3_ctx *ctx, const SparseRows &A, const std::vector<int32_t> &ray_order,
+                   const int32_t dims[3], FusedLayout *L);
+// tile products of a matrix-free Siddon operator (dyntile.hip): the layout
+// for dims = (rv_y, rv_x, rv_z) with every dimension a multiple of 4, built
+// at creation; null when the geometry does not allow it
+std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op);
+// y = A x / A^T x of a matrix-free operator by its tile products, natural
+// order in and out; false when it has none (FP32 vectors)
+bool dyn_tile_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s);
 // panel forms of a prebuilt A^T (rows = voxels in tiles of 512 rows, sources =
 // rays), one per row-block size in blks; false when a panel would need more
 // distinct rays than its local indices address or the weights are not F32,
