@@ -38,7 +38,8 @@ def build_workload(ctx, N, E, R):
     t0 = time.time()
     u = wl.phantom_3d(N, 6).astype(np.float32)
     geom = wl.transducer_geometry(E, R, N=N)
-    dyn = api.DynamicOperator(ctx, geom, "siddon")
+    dyn = api.DynamicOperator(ctx, geom, "siddon")   # builds its tile layout
+    log(f"matrix-free operator: {time.time() - t0:.1f}s")
     A32 = dyn.materialize("f32")
     ut = torch.from_numpy(u).cuda()
     Au = A32.mat_vec_mul(ut)
@@ -210,6 +211,7 @@ def main():
     dims = (N, N, N)
     u, geom, dyn, A32, b = build_workload(ctx, N, args.emitters, args.receivers)
     m, n, nnz = A32.m, A32.n, A32.nnz
+    dlay = dyn.prepare(dims)           # tile layout of the matrix-free operator (built at creation)
     b_dev = torch.from_numpy(b).cuda()
     ut_dev = torch.from_numpy(u).cuda()
 
@@ -286,21 +288,33 @@ def main():
         tp, _, ip = timed_solve(ctx, A32, b_dev, dims,
                                 api.default_opts(**dict(fixed, solver="plain", maxit=kp)), ut_dev)
         extra["plain_f32_iter_per_s"] = kp / tp
-        kd = min(args.steps, 20)
-        td, _, idn = timed_solve(ctx, dyn, b_dev, dims,
-                                 api.default_opts(**dict(fixed, solver="plain", maxit=kd)), ut_dev)
-        extra["dynamic_siddon_f32_iter_per_s"] = kd / td
+        # the matrix-free Siddon operator (weights recomputed in every product,
+        # tile products without atomics): fused and plain solves
+        kd = min(args.steps, 100)
+        dfixed = dict(fixed, solver="fused", maxit=kd)
+        timed_solve(ctx, dyn, b_dev, dims, api.default_opts(**dict(dfixed, maxit=5)), ut_dev)
+        td, _, idn = timed_solve(ctx, dyn, b_dev, dims, api.default_opts(**dfixed), ut_dev)
+        extra["dynamic_fused_f32_iter_per_s"] = kd / td
+        extra["dynamic_fused_f32_loop_iter_per_s"] = idn["total_iters"] / idn["loop_secs"]
+        kp2 = min(args.steps, 20)
+        tdp, _, _ = timed_solve(ctx, dyn, b_dev, dims,
+                                api.default_opts(**dict(fixed, solver="plain", maxit=kp2)), ut_dev)
+        extra["dynamic_plain_f32_iter_per_s"] = kp2 / tdp
         tdc, Udyn, idc = timed_solve(ctx, dyn, b_dev, dims,
-                                     api.default_opts(**dict(conv, solver="plain")), ut_dev)
+                                     api.default_opts(**dict(conv, solver="fused")), ut_dev)
         runs["dynamic"] = (tdc, Udyn.cpu().numpy(), idc)
-        tdn = time_spmv(dyn, False, 10)
-        tdt = time_spmv(dyn, True, 10)
+        tdn = dyn.layout_time(dims, False, 20)
+        tdt = dyn.layout_time(dims, True, 20)
         extra["dynamic_A_us"] = tdn * 1e6
         extra["dynamic_AT_us"] = tdt * 1e6
         extra["dynamic_csr_equiv_GBps"] = [bytes_A(nnz, m, n, 4) / tdn / 1e9,
                                            bytes_AT(nnz, m, n, 4) / tdt / 1e9]
-        log(f"plain-f32 {extra['plain_f32_iter_per_s']:.1f} it/s; dynamic {extra['dynamic_siddon_f32_iter_per_s']:.1f} it/s "
-            f"(A {tdn * 1e6:.0f} us, A^T {tdt * 1e6:.0f} us)")
+        extra["dynamic_layout"] = dlay
+        log(f"plain-f32 {extra['plain_f32_iter_per_s']:.1f} it/s; matrix-free fused "
+            f"{extra['dynamic_fused_f32_iter_per_s']:.1f} it/s (loop "
+            f"{extra['dynamic_fused_f32_loop_iter_per_s']:.1f}), plain "
+            f"{extra['dynamic_plain_f32_iter_per_s']:.1f} it/s (A {tdn * 1e6:.0f} us, "
+            f"A^T {tdt * 1e6:.0f} us)")
 
     cpu, U64, ref64 = None, None, None
     if not args.no_cpu:
@@ -339,8 +353,8 @@ def main():
         ee["pass"] = bool(ee["err_diff"] <= 1e-3 and ee["speedup_outer"] >= 1.8)
         extra["equal_error"] = ee
         tdc, Udyn, idc = runs["dynamic"]
-        extra["dynamic_f32_errors"] = dict(errs(Udyn), inner=idc["total_iters"],
-                                           outer=idc["outer_iters"])
+        extra["dynamic_fused_f32_errors"] = dict(errs(Udyn), inner=idc["total_iters"],
+                                                 outer=idc["outer_iters"], secs=tdc)
         if ref64 is not None:
             extra["fp64_reference"] = dict(ref64, rel_err_vs_utrue=rel_l2(U64, ut),
                                            rel_err_vs_utrue_covered=rel_l2(U64, ut, covered),

@@ -45,7 +45,7 @@ def rel(a, b):
 
 
 GEOMS = {"tiles": dict(E=24, R=40, dims=(16, 16, 16)),       # few rays: whole 8^3 tiles
-         "bricks": dict(E=64, R=128, dims=(16, 16, 16)),     # dense: tiles split into bricks
+         "bricks": dict(E=96, R=256, dims=(16, 16, 16)),     # dense: tiles split into bricks
          "B4": dict(E=24, R=64, dims=(20, 12, 8))}           # bricks only (dims multiples of 4)
 
 

@@ -235,7 +235,10 @@ def test_hybrid_operator(ctx):
     rng = np.random.default_rng(17)
     x = rng.uniform(size=dyn.n).astype(np.float32)
     r = rng.normal(size=dyn.m).astype(np.float32)
-    assert (H(hyb.mat_vec_mul(T(x))) == H(dyn.mat_vec_mul(T(x)))).all()
+    # the same traversal; the hybrid sums each ray in one pass, the
+    # matrix-free operator's tile products by segments (another FP32 order)
+    y1, y2 = H(hyb.mat_vec_mul(T(x))), H(dyn.mat_vec_mul(T(x)))
+    assert np.abs(y1 - y2).max() <= 1e-6 * np.abs(y2).max()
     z1 = H(hyb.mat_vec_mul(T(r), trans=True))
     z2 = H(dyn.mat_vec_mul(T(r), trans=True))
     assert np.abs(z1 - z2).max() < 1e-5 * np.abs(z2).max()

@@ -38,11 +38,133 @@ void upload_vec(tv3_ctx *ctx, DevBuf<T> &d, const std::vector<T> &h) {
 
 }  // namespace
 
+// traversal constants of every internal ray (see dyn_ray in panels.hpp)
+__global__ void k_ray_pre(int64_t m, const float *__restrict__ ends, DynView g,
+                          float *__restrict__ rec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    float p0[3], p1[3];
+    for (int a = 0; a < 3; ++a) { p0[a] = ends[6 * i + a]; p1[a] = ends[6 * i + 3 + a]; }
+    SidRay<float> s;
+    sid_setup(s, p0, p1, g.org, g.h, g.N);
+    float *o = rec + 16 * i;
+    for (int a = 0; a < 3; ++a) { o[a] = s.p0[a]; o[3 + a] = s.d[a]; o[6 + a] = s.inv[a]; }
+    o[9] = s.amin; o[10] = s.amax; o[11] = s.L;
+    int pk = s.hit ? 64 : 0, cf = 0;
+    for (int a = 0; a < 3; ++a) {
+        pk |= (s.st[a] > 0 ? 1 : (s.st[a] < 0 ? 2 : 0)) << (2 * a);
+        cf |= s.cfix[a] << (10 * a);
+    }
+    o[12] = __int_as_float(pk); o[13] = __int_as_float(cf); o[14] = 0.f; o[15] = 0.f;
+}
+
+namespace {
+
+// Segments of the boxes box_of (per internal brick) along the traversal CSR:
+// numbered box by box, rays ascending inside a box; fills X (host side) and
+// returns each entry's segment and the segment lengths.
+struct SegPlan {
+    std::vector<int32_t> seg_box_ptr, seg_ray, seg_len, seg_of_entry;
+};
+
+SegPlan plan_segments(const std::vector<int32_t> &ptr, const std::vector<int32_t> &ind,
+                      const std::vector<int32_t> &box_of, int64_t npan, int64_t m) {
+    SegPlan S;
+    S.seg_box_ptr.assign(npan + 1, 0);
+    for (int64_t r = 0; r < m; ++r) {
+        int32_t last = -1;
+        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+            const int32_t b = box_of[ind[k] >> 6];
+            if (b != last) ++S.seg_box_ptr[b + 1];
+            last = b;
+        }
+    }
+    for (int64_t b = 0; b < npan; ++b) S.seg_box_ptr[b + 1] += S.seg_box_ptr[b];
+    const int64_t nseg = S.seg_box_ptr[npan];
+    const int64_t nnz = ptr[m];
+    S.seg_ray.assign(nseg > 0 ? nseg : 1, 0);
+    S.seg_len.assign(nseg > 0 ? nseg : 1, 0);
+    S.seg_of_entry.assign(nnz > 0 ? nnz : 1, 0);
+    std::vector<int32_t> fill(S.seg_box_ptr.begin(), S.seg_box_ptr.end() - 1);
+    for (int64_t r = 0; r < m; ++r) {
+        int32_t last = -1, cur = -1;
+        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+            const int32_t b = box_of[ind[k] >> 6];
+            if (b != last) {
+                TV3_REQUIRE(fill[b] == S.seg_box_ptr[b] || S.seg_ray[fill[b] - 1] != (int32_t)r,
+                            TV3_E_INTERNAL, "a ray enters a box twice");
+                cur = fill[b]++;
+                S.seg_ray[cur] = (int32_t)r;
+            }
+            ++S.seg_len[cur];
+            S.seg_of_entry[k] = cur;
+            last = b;
+        }
+    }
+    return S;
+}
+
+// boxes: tile u (B = 8) whole or as its 8 bricks, or brick u (B = 4)
+void add_boxes(std::vector<DynPan> &pan, std::vector<int32_t> &box_of, int B, int64_t u, int nby,
+               int nbx, bool split) {
+    const int ty = (int)(u % nby), tx = (int)((u / nby) % nbx),
+              tz = (int)(u / ((int64_t)nby * nbx));
+    if (B == 8 && !split) {
+        pan.push_back(DynPan{(int32_t)(u << 9), 8 * tx, 8 * ty, 8 * tz, 8, 0, 0, 0});
+        for (int k = 0; k < 8; ++k) box_of[(u << 3) + k] = (int32_t)pan.size() - 1;
+    } else if (B == 8) {
+        for (int sb = 0; sb < 8; ++sb) {   // sub-brick bits: y (1), x (2), z (4)
+            pan.push_back(DynPan{(int32_t)((u << 9) + (sb << 6)), 8 * tx + 4 * ((sb >> 1) & 1),
+                                 8 * ty + 4 * (sb & 1), 8 * tz + 4 * ((sb >> 2) & 1), 4, 0, 0, 0});
+            box_of[(u << 3) + sb] = (int32_t)pan.size() - 1;
+        }
+    } else {
+        pan.push_back(DynPan{(int32_t)(u << 6), 4 * tx, 4 * ty, 4 * tz, 4, 0, 0, 0});
+        box_of[u] = (int32_t)pan.size() - 1;
+    }
+}
+
+// upload one direction's boxes (segment ranges, entry offsets, work order)
+void finish_boxes(tv3_ctx *ctx, std::vector<DynPan> &pan, const SegPlan &S, bool offsets,
+                  DynBoxes &X, std::vector<int32_t> *eoff_out) {
+    const int64_t npan = (int64_t)pan.size();
+    std::vector<int32_t> eoff(S.seg_ray.size(), 0);
+    X.emax = 0;
+    X.smax = 0;
+    for (int64_t b = 0; b < npan; ++b) {
+        int32_t e = 0;
+        for (int32_t sg = S.seg_box_ptr[b]; sg < S.seg_box_ptr[b + 1]; ++sg) {
+            eoff[sg] = e;
+            e += S.seg_len[sg];
+        }
+        pan[b].s0 = S.seg_box_ptr[b];
+        pan[b].s1 = S.seg_box_ptr[b + 1];
+        pan[b].ne = e;
+        X.emax = std::max(X.emax, e);
+        X.smax = std::max(X.smax, S.seg_box_ptr[b + 1] - S.seg_box_ptr[b]);
+    }
+    std::vector<int32_t> bord(npan);
+    std::iota(bord.begin(), bord.end(), 0);
+    std::stable_sort(bord.begin(), bord.end(), [&](int32_t a, int32_t b) {
+        return pan[a].ne + 16 * (pan[a].s1 - pan[a].s0) > pan[b].ne + 16 * (pan[b].s1 - pan[b].s0);
+    });
+    X.npan = npan;
+    X.nseg = S.seg_box_ptr[npan];
+    upload_vec(ctx, X.pan, pan);
+    upload_vec(ctx, X.order, bord);
+    upload_vec(ctx, X.seg_ray, S.seg_ray);
+    if (offsets) upload_vec(ctx, X.seg_eoff, eoff);
+    if (eoff_out) *eoff_out = std::move(eoff);
+}
+
+}  // namespace
+
 std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
     const DynGeom &G = op->geo;
     if (G.mode != TV3_WEIGHT_SIDDON) return nullptr;
     const int32_t dims[3] = {G.rv_y, G.rv_x, G.rv_z};
     if (dims[0] % 4 || dims[1] % 4 || dims[2] % 4) return nullptr;
+    if (G.rv_x > 1024 || G.rv_y > 1024 || G.rv_z > 1024) return nullptr;   // 10-bit indices
     tv3_ctx *ctx = op->ctx;
     std::unique_ptr<FusedLayout> L(new FusedLayout());
     const std::vector<int32_t> ray_nat = geometry_ray_order(G);
@@ -61,11 +183,20 @@ std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
         order.resize(m);
         std::iota(order.begin(), order.end(), 0);
     }
+    std::unique_ptr<DynTiles> D(new DynTiles());
+    D->nnz = nnz;
+    for (int a = 0; a < 3; ++a) { D->org[a] = G.origin[a]; D->h[a] = G.voxel[a]; }
+    D->N[0] = G.rv_x; D->N[1] = G.rv_y; D->N[2] = G.rv_z;
+    // A^T boxes keep their entries and rays in LDS: the larger allocations of
+    // devices that allow 160 KiB per workgroup hold whole tiles
+    const size_t lds = std::min<size_t>(ctx->lds_per_block, size_t(160) << 10);
+    D->smax_cap = 4096;
+    D->emax_cap = lds >= (size_t(150) << 10) ? 32768 : 11264;
+    D->threads_T = 1024;
     const int B = L->B, nby = L->nby, nbx = L->nbx;   // tiles (B = 8) or bricks (B = 4)
     const int64_t nunit = n / (B == 8 ? 512 : 64);
-    // boxes: one per tile, or its 8 bricks where the tile is too full
-    std::vector<int32_t> unit_ent(nunit, 0), unit_seg(nunit, 0);
     const int ushift = B == 8 ? 9 : 6;
+    std::vector<int32_t> unit_ent(nunit, 0), unit_seg(nunit, 0);
     for (int64_t r = 0; r < m; ++r) {
         int32_t last = -1;
         for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
@@ -75,78 +206,23 @@ std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
             last = u;
         }
     }
-    std::vector<DynPan> pan;
-    std::vector<int32_t> box_of_brick((size_t)(n >> 6), -1);
+    // forward boxes: whole units; backward: split where the LDS limits ask
+    std::vector<DynPan> panF, panT;
+    std::vector<int32_t> boxF((size_t)(n >> 6), -1), boxT((size_t)(n >> 6), -1);
     for (int64_t u = 0; u < nunit; ++u) {
-        int ty = 0, tx = 0, tz = 0;
-        ty = (int)(u % nby);
-        tx = (int)((u / nby) % nbx);
-        tz = (int)(u / ((int64_t)nby * nbx));
-        if (B == 8 && unit_ent[u] <= kDynEmax && unit_seg[u] <= kDynSmax) {
-            pan.push_back(DynPan{(int32_t)(u << 9), 8 * tx, 8 * ty, 8 * tz, 8, 0, 0, 0});
-            for (int k = 0; k < 8; ++k) box_of_brick[(u << 3) + k] = (int32_t)pan.size() - 1;
-        } else if (B == 8) {
-            for (int sb = 0; sb < 8; ++sb) {   // sub-brick bits: y (1), x (2), z (4)
-                pan.push_back(DynPan{(int32_t)((u << 9) + (sb << 6)), 8 * tx + 4 * ((sb >> 1) & 1),
-                                     8 * ty + 4 * (sb & 1), 8 * tz + 4 * ((sb >> 2) & 1), 4, 0, 0,
-                                     0});
-                box_of_brick[(u << 3) + sb] = (int32_t)pan.size() - 1;
-            }
-        } else {
-            pan.push_back(DynPan{(int32_t)(u << 6), 4 * tx, 4 * ty, 4 * tz, 4, 0, 0, 0});
-            box_of_brick[u] = (int32_t)pan.size() - 1;
-        }
+        add_boxes(panF, boxF, B, u, nby, nbx, false);
+        add_boxes(panT, boxT, B, u, nby, nbx,
+                  unit_ent[u] > D->emax_cap || unit_seg[u] > D->smax_cap);
     }
-    const int64_t npan = (int64_t)pan.size();
-    // segments: runs of one box along each ray (a box is entered once)
-    std::vector<int32_t> nseg_box(npan + 1, 0);
-    for (int64_t r = 0; r < m; ++r) {
-        int32_t last = -1;
-        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
-            const int32_t b = box_of_brick[ind[k] >> 6];
-            if (b != last) ++nseg_box[b + 1];
-            last = b;
-        }
-    }
-    for (int64_t b = 0; b < npan; ++b) nseg_box[b + 1] += nseg_box[b];
-    const int64_t nseg = nseg_box[npan];
-    std::vector<int32_t> seg_ray(nseg > 0 ? nseg : 1), seg_len(nseg > 0 ? nseg : 1, 0),
-        fill(nseg_box.begin(), nseg_box.end() - 1), seg_of_entry(nnz > 0 ? nnz : 1);
-    std::vector<int32_t> rseg_cnt(m + 1, 0);
-    for (int64_t r = 0; r < m; ++r) {
-        int32_t last = -1, cur = -1;
-        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
-            const int32_t b = box_of_brick[ind[k] >> 6];
-            if (b != last) {
-                TV3_REQUIRE(fill[b] == nseg_box[b] || seg_ray[fill[b] - 1] != (int32_t)r,
-                            TV3_E_INTERNAL, "a ray enters a box twice");
-                cur = fill[b]++;
-                seg_ray[cur] = (int32_t)r;
-                ++rseg_cnt[r + 1];
-            }
-            ++seg_len[cur];
-            seg_of_entry[k] = cur;
-            last = b;
-        }
-    }
-    // entry offsets inside each box; positions of every voxel's entries
-    std::vector<int32_t> seg_eoff(nseg > 0 ? nseg : 1, 0);
-    int32_t emax = 0, smax = 0;
-    for (int64_t b = 0; b < npan; ++b) {
-        int32_t e = 0;
-        for (int32_t sg = nseg_box[b]; sg < nseg_box[b + 1]; ++sg) {
-            seg_eoff[sg] = e;
-            e += seg_len[sg];
-        }
-        pan[b].s0 = nseg_box[b];
-        pan[b].s1 = nseg_box[b + 1];
-        pan[b].ne = e;
-        emax = std::max(emax, e);
-        smax = std::max(smax, nseg_box[b + 1] - nseg_box[b]);
-    }
-    TV3_REQUIRE(emax <= kDynEmax && smax <= kDynSmax, TV3_E_UNSUPPORTED,
+    const SegPlan SF = plan_segments(ptr, ind, boxF, (int64_t)panF.size(), m);
+    const SegPlan ST = plan_segments(ptr, ind, boxT, (int64_t)panT.size(), m);
+    finish_boxes(ctx, panF, SF, false, D->fwd, nullptr);
+    std::vector<int32_t> eoffT;
+    finish_boxes(ctx, panT, ST, true, D->bwd, &eoffT);
+    TV3_REQUIRE(D->bwd.emax <= D->emax_cap && D->bwd.smax <= D->smax_cap, TV3_E_UNSUPPORTED,
                 "a 4^3 brick of the volume holds more ray entries than the tile products keep in "
                 "LDS");
+    // positions of every voxel's entries in its backward box, ray order
     std::vector<uint32_t> vcnt(n + 1, 0);
     for (int64_t k = 0; k < nnz; ++k) ++vcnt[ind[k] + 1];
     std::vector<uint32_t> pptr(n + 1, 0);
@@ -154,72 +230,70 @@ std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
     TV3_REQUIRE(pptr[n] < (1u << 31), TV3_E_UNSUPPORTED, "too many padded entries");
     std::vector<uint16_t> pos(pptr[n] > 0 ? pptr[n] : 4, 0);
     std::vector<uint32_t> vfill(pptr.begin(), pptr.end() - 1);
-    // the j-th entry of segment sg sits at seg_eoff[sg] + j of its box; rays
+    // the j-th entry of segment sg sits at eoff[sg] + j of its box; rays
     // ascending, so every voxel's list is in ray order
     for (int64_t r = 0; r < m; ++r) {
         int32_t j = 0, last = -1;
         for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
-            const int32_t sg = seg_of_entry[k];
+            const int32_t sg = ST.seg_of_entry[k];
             j = sg == last ? j + 1 : 0;
             last = sg;
-            pos[vfill[ind[k]]++] = (uint16_t)(seg_eoff[sg] + j);
+            pos[vfill[ind[k]]++] = (uint16_t)(eoffT[sg] + j);
         }
     }
-    for (int64_t b = 0; b < npan; ++b) {              // padding: the box's zero slot ne
-        const int nv = pan[b].bs * pan[b].bs * pan[b].bs;
-        for (int32_t v = pan[b].v0; v < pan[b].v0 + nv; ++v)
-            for (uint32_t q = vfill[v]; q < pptr[v + 1]; ++q) pos[q] = (uint16_t)pan[b].ne;
+    for (const DynPan &P : panT) {                    // padding: the box's zero slot ne
+        const int nv = P.bs * P.bs * P.bs;
+        for (int32_t v = P.v0; v < P.v0 + nv; ++v)
+            for (uint32_t q = vfill[v]; q < pptr[v + 1]; ++q) pos[q] = (uint16_t)P.ne;
     }
-    // the second pass of A x: each ray's segments, ascending
-    for (int64_t r = 0; r < m; ++r) rseg_cnt[r + 1] += rseg_cnt[r];
-    std::vector<int32_t> rseg(nseg > 0 ? nseg : 1), rfill(rseg_cnt.begin(), rseg_cnt.end() - 1);
-    for (int64_t sg = 0; sg < nseg; ++sg) rseg[rfill[seg_ray[sg]]++] = (int32_t)sg;
-    // ray end points in internal order
-    std::vector<float> rays(6 * (m > 0 ? m : 1), 0.f);
+    upload_vec(ctx, D->pptr, pptr);
+    upload_vec(ctx, D->pos, pos);
+    // the second pass of A x: each ray's forward segments, ascending
+    const int64_t nsegF = D->fwd.nseg;
+    std::vector<int32_t> rptr(m + 1, 0), rseg(nsegF > 0 ? nsegF : 1);
+    for (int64_t sg = 0; sg < nsegF; ++sg) ++rptr[SF.seg_ray[sg] + 1];
+    for (int64_t r = 0; r < m; ++r) rptr[r + 1] += rptr[r];
+    std::vector<int32_t> rfill(rptr.begin(), rptr.end() - 1);
+    for (int64_t sg = 0; sg < nsegF; ++sg) rseg[rfill[SF.seg_ray[sg]]++] = (int32_t)sg;
+    SegSum &R = L->RS;
+    R.rows = m;
+    R.nseg = nsegF;
+    const double per_ray = m > 0 ? (double)nsegF / (double)m : 0.0;
+    R.lanes = per_ray > 24 ? 16 : per_ray > 12 ? 8 : 4;
+    upload_vec(ctx, R.ptr, rptr);
+    upload_vec(ctx, R.seg, rseg);
+    R.part.alloc(nsegF > 0 ? nsegF : 1);
+    // ray end points in internal order -> traversal records on the device
+    std::vector<float> ends(6 * (m > 0 ? m : 1), 0.f);
     for (int64_t i = 0; i < m; ++i) {
         const int32_t r = order[i], e = G.path_h[2 * r], c = G.path_h[2 * r + 1];
         for (int a = 0; a < 3; ++a) {
-            rays[6 * i + a] = G.epos_h[3 * e + a];
-            rays[6 * i + 3 + a] = G.rpos_h[3 * c + a];
+            ends[6 * i + a] = G.epos_h[3 * e + a];
+            ends[6 * i + 3 + a] = G.rpos_h[3 * c + a];
         }
     }
-    // boxes in decreasing work
-    std::vector<int32_t> bord(npan);
-    std::iota(bord.begin(), bord.end(), 0);
-    std::stable_sort(bord.begin(), bord.end(), [&](int32_t a, int32_t b) {
-        return pan[a].ne + 16 * (pan[a].s1 - pan[a].s0) > pan[b].ne + 16 * (pan[b].s1 - pan[b].s0);
-    });
-    std::unique_ptr<DynTiles> D(new DynTiles());
-    D->npan = npan; D->nseg = nseg; D->nnz = nnz; D->emax = emax; D->smax = smax;
-    for (int a = 0; a < 3; ++a) { D->org[a] = G.origin[a]; D->h[a] = G.voxel[a]; }
-    D->N[0] = G.rv_x; D->N[1] = G.rv_y; D->N[2] = G.rv_z;
-    upload_vec(ctx, D->pan, pan);
-    upload_vec(ctx, D->order, bord);
-    upload_vec(ctx, D->seg_ray, seg_ray);
-    upload_vec(ctx, D->seg_eoff, seg_eoff);
-    upload_vec(ctx, D->rays, rays);
-    upload_vec(ctx, D->pptr, pptr);
-    upload_vec(ctx, D->pos, pos);
+    {
+        DevBuf<float> dends;
+        upload_vec(ctx, dends, ends);
+        D->ray.alloc(16 * (m > 0 ? m : 1));
+        if (m > 0)
+            hipLaunchKernelGGL(k_ray_pre, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+                               ctx->stream, m, dends.p, dyn_view(*D, D->fwd), D->ray.p);
+        TV3_HIP(hipGetLastError());
+        TV3_HIP(hipStreamSynchronize(ctx->stream));
+    }
     const int64_t big = n > m ? n : m;
     D->xin.alloc(big > 0 ? big : 1);
     D->yout.alloc(big > 0 ? big : 1);
-    SegSum &R = L->RS;
-    R.rows = m;
-    R.nseg = nseg;
-    const double per_ray = m > 0 ? (double)nseg / (double)m : 0.0;
-    R.lanes = per_ray > 24 ? 16 : per_ray > 12 ? 8 : 4;
-    upload_vec(ctx, R.ptr, rseg_cnt);
-    upload_vec(ctx, R.seg, rseg);
-    R.part.alloc(nseg > 0 ? nseg : 1);
     TV3_HIP(hipStreamSynchronize(ctx->stream));
     // the matrices themselves are not kept: sizes only
     for (SparseRows *S : {&L->A, &L->AT}) {
         S->ptr.release(); S->ind.release(); S->val.release(); S->desc.release();
         S->nblk = 0; S->blk_nnz = 0;
     }
-    L->npan = npan;
-    L->nseg = nseg;
-    L->max_seg = smax;
+    L->npan = D->bwd.npan;
+    L->nseg = D->bwd.nseg;
+    L->max_seg = D->bwd.smax;
     L->dyn = std::move(D);
     return L;
 }

@@ -272,25 +272,48 @@ void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s
 
 // ------------------------------------------------------------------------
 // Matrix-free Siddon products in boxes (DynTiles, built by dyntile.hip).
-constexpr int kDynEmax = 11264;   // entries of one box: its A^T products in LDS
-constexpr int kDynSmax = 4096;    // segments of one box: their residuals in LDS
-
 struct DynView {
     const DynPan *pan;
     const int32_t *order, *seg_ray, *seg_eoff;
-    const float *rays;
+    const float *ray;
     const uint32_t *pptr;
     const uint16_t *pos;
-    float org[3], h[3];
+    float org[3], h[3], ih[3];
     int N[3];
+    int emax_cap;
 };
 
-inline DynView dyn_view(const DynTiles &D) {
+inline DynView dyn_view(const DynTiles &D, const DynBoxes &X) {
     DynView v;
-    v.pan = D.pan.p; v.order = D.order.p; v.seg_ray = D.seg_ray.p; v.seg_eoff = D.seg_eoff.p;
-    v.rays = D.rays.p; v.pptr = D.pptr.p; v.pos = D.pos.p;
-    for (int a = 0; a < 3; ++a) { v.org[a] = D.org[a]; v.h[a] = D.h[a]; v.N[a] = D.N[a]; }
+    v.pan = X.pan.p; v.order = X.order.p; v.seg_ray = X.seg_ray.p; v.seg_eoff = X.seg_eoff.p;
+    v.ray = D.ray.p; v.pptr = D.pptr.p; v.pos = D.pos.p;
+    for (int a = 0; a < 3; ++a) {
+        v.org[a] = D.org[a]; v.h[a] = D.h[a]; v.ih[a] = 1.f / D.h[a]; v.N[a] = D.N[a];
+    }
+    v.emax_cap = D.emax_cap;
     return v;
+}
+
+// Record of one ray's traversal constants (16 words, written by sid_setup at
+// creation): p0, d, 1/d, amin, amax, L, then the direction codes (2 bits per
+// axis: 1 = +1, 2 = -1, 0 = parallel) and the hit bit (bit 6), then the
+// voxel index of each parallel axis (10 bits per axis).
+__device__ __forceinline__ void dyn_ray(const DynView &g, int ray, SidRay<float> &s) {
+    const float4 *r = reinterpret_cast<const float4 *>(g.ray + 16 * (int64_t)ray);
+    const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+    s.p0[0] = a.x; s.p0[1] = a.y; s.p0[2] = a.z;
+    s.d[0] = a.w; s.d[1] = b.x; s.d[2] = b.y;
+    s.inv[0] = b.z; s.inv[1] = b.w; s.inv[2] = c.x;
+    s.amin = c.y; s.amax = c.z; s.L = c.w;
+    const int pk = __float_as_int(d.x), cf = __float_as_int(d.y);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s.org[k] = g.org[k]; s.h[k] = g.h[k]; s.ih[k] = g.ih[k]; s.N[k] = g.N[k];
+        const int code = (pk >> (2 * k)) & 3;
+        s.st[k] = code == 1 ? 1 : (code == 2 ? -1 : 0);
+        s.cfix[k] = (cf >> (10 * k)) & 1023;
+    }
+    s.hit = ((pk >> 6) & 1) != 0;
 }
 
 // offset of voxel (a, b, c) = (y, x, z) offsets in a box, in the internal
@@ -306,22 +329,16 @@ __device__ __forceinline__ int dyn_local(int a, int b, int c) {
 // entries the walk of the whole ray emits inside the box (siddon.hpp).
 template <typename F>
 __device__ __forceinline__ void dyn_seg_walk(const DynView &g, const DynPan &P, int ray, F emit) {
-    float p0[3], p1[3];
-    const float *rp = g.rays + 6 * (int64_t)ray;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { p0[a] = rp[a]; p1[a] = rp[3 + a]; }
     SidRay<float> s;
-    sid_setup(s, p0, p1, g.org, g.h, g.N);
+    dyn_ray(g, ray, s);
     const int b0[3] = {P.bx, P.by, P.bz}, b1[3] = {P.bx + P.bs, P.by + P.bs, P.bz + P.bs};
     float ts, te;
     if (!s.hit || !sid_block_range(s, b0, b1, ts, te)) return;
-    const int m = P.bs - 1;
+    // the walk stays in the box; masked all the same, so that no offset
+    // leaves the values staged for the box
+    const int mask = P.bs * P.bs * P.bs - 1;
     sid_walk(s, ts, te, [&](int cx, int cy, int cz, float w) {
-        // the walk stays in the box; clamped all the same, so that no index
-        // leaves the values staged for it
-        const int a = min(max(cy - P.by, 0), m), b = min(max(cx - P.bx, 0), m),
-                  c = min(max(cz - P.bz, 0), m);
-        emit(dyn_local(a, b, c), w);
+        emit(dyn_local(cy - P.by, cx - P.bx, cz - P.bz) & mask, w);
     });
 }
 
@@ -346,28 +363,22 @@ __global__ __launch_bounds__(256) void k_dyn_tile_N(DynView g, const T *__restri
 }
 
 // A^T r: per box the residuals of its rays staged in LDS, each thread walks
-// segments and writes w r_ray at the entry's position of the box's entry
-// array (ray-major); then each voxel row sums its entries in ray order
-// through the stored positions (padding points at the zero slot ne) and the
-// epilogue takes the sum.  Blocks 0 .. kExtra - 1 run epi.extra().
-template <typename T, typename Epi>
-__global__ __launch_bounds__(256) void k_dyn_tile_T(DynView g, const T *__restrict__ x, Epi epi) {
+// segments and writes w r_ray at the entry's position in the box's entry
+// array (ray-major, emax_cap + 1 slots); then each voxel row sums its entries
+// in ray order through the stored positions (padding points at the zero slot
+// ne) and the epilogue takes the sum.
+template <typename T, int NT, typename Epi>
+__global__ __launch_bounds__(NT) void k_dyn_tile_T(DynView g, const T *__restrict__ x, Epi epi) {
+    static_assert(ExtraBlocks<Epi>::value == 0, "no extra workgroups in the tile product");
     extern __shared__ __align__(16) unsigned char dyn_smem[];
-    constexpr int NX = ExtraBlocks<Epi>::value;
-    if constexpr (NX > 0) {
-        if ((int)blockIdx.x < NX) {
-            epi.extra();
-            return;
-        }
-    }
     if (epi.skip()) return;
-    const DynPan P = g.pan[g.order[blockIdx.x - NX]];
+    const DynPan P = g.pan[g.order[blockIdx.x]];
     T *pe = reinterpret_cast<T *>(dyn_smem);
-    T *xr = pe + kDynEmax + 1;
+    T *xr = pe + g.emax_cap + 1;
     const int t = threadIdx.x, ns = P.s1 - P.s0;
-    for (int j = t; j < ns; j += 256) xr[j] = x[g.seg_ray[P.s0 + j]];
+    for (int j = t; j < ns; j += NT) xr[j] = x[g.seg_ray[P.s0 + j]];
     __syncthreads();
-    for (int j = t; j < ns; j += 256) {
+    for (int j = t; j < ns; j += NT) {
         const int sg = P.s0 + j;
         const int e1 = j + 1 < ns ? g.seg_eoff[sg + 1] : P.ne;
         const T xv = xr[j];
@@ -380,7 +391,7 @@ __global__ __launch_bounds__(256) void k_dyn_tile_T(DynView g, const T *__restri
     if (t == 0) pe[P.ne] = 0;
     __syncthreads();
     const int nv = P.bs * P.bs * P.bs;
-    for (int r = t; r < nv; r += 256) {
+    for (int r = t; r < nv; r += NT) {
         const int row = P.v0 + r;
         const auto pre = epi.load(row);
         const uint32_t q0 = g.pptr[row], q1 = g.pptr[row + 1];
@@ -396,17 +407,17 @@ __global__ __launch_bounds__(256) void k_dyn_tile_T(DynView g, const T *__restri
 
 template <typename T>
 void launch_dyn_N(const DynTiles &D, const T *x, T *part, const int32_t *active, hipStream_t s) {
-    if (D.npan == 0) return;
-    hipLaunchKernelGGL((k_dyn_tile_N<T>), dim3((unsigned)D.npan), dim3(256), 0, s, dyn_view(D),
-                       x, part, active);
+    if (D.fwd.npan == 0) return;
+    hipLaunchKernelGGL((k_dyn_tile_N<T>), dim3((unsigned)D.fwd.npan), dim3(256), 0, s,
+                       dyn_view(D, D.fwd), x, part, active);
 }
 
 template <typename T, typename Epi>
 void launch_dyn_T(const DynTiles &D, const T *x, const Epi &epi, hipStream_t s) {
-    const unsigned g = (unsigned)D.npan + ExtraBlocks<Epi>::value;
-    if (g == 0) return;
-    const size_t lds = sizeof(T) * (size_t)(kDynEmax + 1 + kDynSmax);
-    hipLaunchKernelGGL((k_dyn_tile_T<T, Epi>), dim3(g), dim3(256), lds, s, dyn_view(D), x, epi);
+    if (D.bwd.npan == 0) return;
+    const size_t lds = sizeof(T) * (size_t)(D.emax_cap + 1 + D.smax_cap);
+    hipLaunchKernelGGL((k_dyn_tile_T<T, 1024, Epi>), dim3((unsigned)D.bwd.npan), dim3(1024), lds,
+                       s, dyn_view(D, D.bwd), x, epi);
 }
 
 constexpr int kSegSumNLB = 2;   // logical blocks (256 / L rays each) per pass-2 workgroup

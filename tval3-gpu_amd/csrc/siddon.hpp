@@ -32,7 +32,7 @@ namespace tv3 {
 
 template <typename R>
 struct SidRay {
-    R p0[3], d[3], inv[3], org[3], h[3];
+    R p0[3], d[3], inv[3], org[3], h[3], ih[3];   // ih = 1/h: seeds the plane search only
     R amin, amax, L;
     int N[3];
     int st[3];       // +1 / -1: direction of travel along the axis; 0: parallel
@@ -62,6 +62,7 @@ __host__ __device__ inline void sid_setup(SidRay<R> &s, const R p0[3], const R p
         s.p0[a] = p0[a];
         s.org[a] = org[a];
         s.h[a] = h[a];
+        s.ih[a] = (R)1 / h[a];
         s.N[a] = N[a];
         s.d[a] = p1[a] - p0[a];
         s.cfix[a] = 0;
@@ -92,7 +93,7 @@ template <typename R>
 __host__ __device__ inline int sid_next_plane(const SidRay<R> &s, int a, R t) {
 #pragma clang fp contract(off)
     const int N = s.N[a];
-    const R f = (s.p0[a] + t * s.d[a] - s.org[a]) / s.h[a];
+    const R f = (s.p0[a] + t * s.d[a] - s.org[a]) * s.ih[a];
     if (s.st[a] > 0) {
         int k = sid_clampi<R>((int)floor(f) + 1, 1, N);
         while (k > 1 && sid_plane(s, a, k - 1) > t) --k;

@@ -1105,9 +1105,7 @@ struct Fused : FusedSession {
         // an extra workgroup of that launch
         auto gradient = [&](int gated) {
             const EpiGrad e{g[n_], Atb, ctl.p, gated};
-            const EpiGradLS ls{e, ctl.p, pD, pN, nbD, nbN, mode};
-            if (F->dyn) launch_dyn_T<float>(*F->dyn, Au[n_], ls, s);
-            else if (F->panels_T) launch_panel(F->PT, Au[n_], ls, s);
+            if (F->panels_T) launch_panel(F->PT, Au[n_], EpiGradLS{e, ctl.p, pD, pN, nbD, nbN, mode}, s);
             else launch_transposed<float>(*F, Au[n_], e, s);
         };
         if (skipping) {                // get_g's A^T-skip decision (:398-468)
@@ -1120,7 +1118,7 @@ struct Fused : FusedSession {
         } else {
             gradient(0);
         }
-        if (!F->panels_T && !F->dyn)
+        if (!F->panels_T)
             hipLaunchKernelGGL(k_line_search_parts, dim3(1), dim3(1024), 0, s, ctl.p, pD, nbD, pN,
                                nbN, mode);
         hipLaunchKernelGGL(k_apply_alpha, dim3(512), dim3(256), 0, s, n, m, U[c], U[n_], g[c],

@@ -230,6 +230,7 @@ tv3_status tv3_ctx_create(int32_t device, tv3_ctx **out) {
         hipDeviceProp_t prop;
         TV3_HIP(hipGetDeviceProperties(&prop, device));
         c->num_cus = prop.multiProcessorCount;
+        c->lds_per_block = prop.sharedMemPerBlock;
         noalloc_setup(device);           // device current: its scratch array's address
         *out = c.release();
     });

@@ -226,19 +226,29 @@ struct DynPan {
     int32_t ne;          // entries of its segments
 };
 
-struct DynTiles {
-    int64_t npan = 0, nseg = 0, nnz = 0;
+// One direction's boxes and segments.
+struct DynBoxes {
+    int64_t npan = 0, nseg = 0;
     int32_t emax = 0, smax = 0;      // most entries / segments of one box
-    float org[3] = {0, 0, 0}, h[3] = {1, 1, 1};
-    int32_t N[3] = {0, 0, 0};        // voxels along x, y, z
     DevBuf<DynPan> pan;
     DevBuf<int32_t> order;           // boxes, decreasing work
     DevBuf<int32_t> seg_ray;         // internal ray of each segment
-    DevBuf<int32_t> seg_eoff;        // first entry of each segment in its box
-    DevBuf<float> rays;              // 6 per internal ray: emitter, receiver (x, y, z)
+    DevBuf<int32_t> seg_eoff;        // first entry of each segment in its box (A^T)
+};
+
+struct DynTiles {
+    int64_t nnz = 0;
+    int32_t emax_cap = 0, smax_cap = 0;   // A^T box limits: entries / segments in LDS
+    int32_t threads_T = 1024;             // workgroup of the A^T product
+    float org[3] = {0, 0, 0}, h[3] = {1, 1, 1};
+    int32_t N[3] = {0, 0, 0};        // voxels along x, y, z
+    DynBoxes fwd;                    // A x: whole tiles (or bricks when B = 4)
+    DynBoxes bwd;                    // A^T r: tiles, split into bricks where a tile's
+                                     // entries or segments exceed the LDS limits
+    DevBuf<float> ray;               // 16 per internal ray: its traversal constants
     DevBuf<uint32_t> pptr;           // n + 1: padded starts of the voxel rows
-    DevBuf<uint16_t> pos;            // per voxel row its entries' positions in the box,
-                                     // ray order, padded to 4 with the position ne
+    DevBuf<uint16_t> pos;            // per voxel row its entries' positions in its bwd
+                                     // box, ray order, padded to 4 with the position ne
     DevBuf<float> xin, yout;         // natural <-> internal order staging (max(n, m))
 };
 
@@ -281,6 +291,7 @@ struct tv3_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int num_cus = 256;
+    size_t lds_per_block = 65536;   // largest LDS allocation of one workgroup
     // reduction scratch
     tv3::DevBuf<double> red_partials;
     tv3::PinnedBuf<double> red_host;
