@@ -124,6 +124,29 @@ void add_boxes(std::vector<DynPan> &pan, std::vector<int32_t> &box_of, int B, in
     }
 }
 
+// Inside every box, segments by decreasing length (ties: ascending ray), so
+// that the threads of a wavefront walk segments of about the same length.
+void sort_segments(SegPlan &S, int64_t npan) {
+    const int64_t nseg = S.seg_box_ptr[npan];
+    std::vector<int32_t> newid(nseg > 0 ? nseg : 1), ids;
+    for (int64_t b = 0; b < npan; ++b) {
+        const int32_t p0 = S.seg_box_ptr[b], p1 = S.seg_box_ptr[b + 1];
+        ids.resize(p1 - p0);
+        std::iota(ids.begin(), ids.end(), p0);
+        std::stable_sort(ids.begin(), ids.end(),
+                         [&](int32_t a, int32_t c) { return S.seg_len[a] > S.seg_len[c]; });
+        for (int32_t k = 0; k < p1 - p0; ++k) newid[ids[k]] = p0 + k;
+    }
+    std::vector<int32_t> ray(S.seg_ray.size()), len(S.seg_len.size());
+    for (int64_t sg = 0; sg < nseg; ++sg) {
+        ray[newid[sg]] = S.seg_ray[sg];
+        len[newid[sg]] = S.seg_len[sg];
+    }
+    S.seg_ray.swap(ray);
+    S.seg_len.swap(len);
+    for (auto &e : S.seg_of_entry) e = newid[e];
+}
+
 // upload one direction's boxes (segment ranges, entry offsets, work order)
 void finish_boxes(tv3_ctx *ctx, std::vector<DynPan> &pan, const SegPlan &S, bool offsets,
                   DynBoxes &X, std::vector<int32_t> *eoff_out) {
@@ -214,15 +237,18 @@ std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
         add_boxes(panT, boxT, B, u, nby, nbx,
                   unit_ent[u] > D->emax_cap || unit_seg[u] > D->smax_cap);
     }
-    const SegPlan SF = plan_segments(ptr, ind, boxF, (int64_t)panF.size(), m);
-    const SegPlan ST = plan_segments(ptr, ind, boxT, (int64_t)panT.size(), m);
+    SegPlan SF = plan_segments(ptr, ind, boxF, (int64_t)panF.size(), m);
+    SegPlan ST = plan_segments(ptr, ind, boxT, (int64_t)panT.size(), m);
+    sort_segments(SF, (int64_t)panF.size());
+    sort_segments(ST, (int64_t)panT.size());
     finish_boxes(ctx, panF, SF, false, D->fwd, nullptr);
     std::vector<int32_t> eoffT;
     finish_boxes(ctx, panT, ST, true, D->bwd, &eoffT);
     TV3_REQUIRE(D->bwd.emax <= D->emax_cap && D->bwd.smax <= D->smax_cap, TV3_E_UNSUPPORTED,
                 "a 4^3 brick of the volume holds more ray entries than the tile products keep in "
                 "LDS");
-    // positions of every voxel's entries in its backward box, ray order
+    // positions of every voxel's entries in its backward box; each voxel's
+    // list in ascending ray order (the fixed order of its sum)
     std::vector<uint32_t> vcnt(n + 1, 0);
     for (int64_t k = 0; k < nnz; ++k) ++vcnt[ind[k] + 1];
     std::vector<uint32_t> pptr(n + 1, 0);
@@ -230,8 +256,7 @@ std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
     TV3_REQUIRE(pptr[n] < (1u << 31), TV3_E_UNSUPPORTED, "too many padded entries");
     std::vector<uint16_t> pos(pptr[n] > 0 ? pptr[n] : 4, 0);
     std::vector<uint32_t> vfill(pptr.begin(), pptr.end() - 1);
-    // the j-th entry of segment sg sits at eoff[sg] + j of its box; rays
-    // ascending, so every voxel's list is in ray order
+    // the j-th entry of segment sg sits at eoff[sg] + j of its box
     for (int64_t r = 0; r < m; ++r) {
         int32_t j = 0, last = -1;
         for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {

@@ -316,30 +316,67 @@ __device__ __forceinline__ void dyn_ray(const DynView &g, int ray, SidRay<float>
     s.hit = ((pk >> 6) & 1) != 0;
 }
 
-// offset of voxel (a, b, c) = (y, x, z) offsets in a box, in the internal
-// order of brick_index: bricks of 4^3 in (y, x, z) bit order, y fastest in a
-// brick (a 4^3 box holds one brick)
-__device__ __forceinline__ int dyn_local(int a, int b, int c) {
-    return ((((c >> 2) << 2) | ((b >> 2) << 1) | (a >> 2)) << 6) | ((c & 3) << 4) |
-           ((b & 3) << 2) | (a & 3);
+// Part of the offset of a voxel in a box contributed by its offset v along
+// axis a (0 = x, 1 = y, 2 = z), in the internal order of brick_index:
+// bricks of 4^3 in (y, x, z) bit order, y fastest in a brick (a 4^3 box
+// holds one brick); the offset is the sum over the three axes.
+__device__ __forceinline__ int dyn_axis_off(int a, int v) {
+    return a == 0 ? (((v >> 2) << 7) | ((v & 3) << 2))
+                  : (a == 1 ? (((v >> 2) << 6) | (v & 3)) : (((v >> 2) << 8) | ((v & 3) << 4)));
 }
 
 // The traversal of one segment (the part of an internal ray inside box P):
 // emit(offset of the voxel in the box, length) in traversal order -- the
-// entries the walk of the whole ray emits inside the box (siddon.hpp).
+// entries the walk of the whole ray emits inside the box.  The loop is
+// sid_walk's (siddon.hpp: same state, same arithmetic, so the same entries)
+// with each axis's share of the box offset updated when that axis steps.
 template <typename F>
 __device__ __forceinline__ void dyn_seg_walk(const DynView &g, const DynPan &P, int ray, F emit) {
+#pragma clang fp contract(off)
     SidRay<float> s;
     dyn_ray(g, ray, s);
     const int b0[3] = {P.bx, P.by, P.bz}, b1[3] = {P.bx + P.bs, P.by + P.bs, P.bz + P.bs};
     float ts, te;
     if (!s.hit || !sid_block_range(s, b0, b1, ts, te)) return;
-    // the walk stays in the box; masked all the same, so that no offset
-    // leaves the values staged for the box
+    // the walk stays in the box; the offset is masked all the same, so that
+    // no index leaves the values staged for the box
     const int mask = P.bs * P.bs * P.bs - 1;
-    sid_walk(s, ts, te, [&](int cx, int cy, int cz, float w) {
-        emit(dyn_local(cy - P.by, cx - P.bx, cz - P.bz) & mask, w);
-    });
+    const float INF = 1e30f;
+    int kn[3], c[3], off[3];
+    float tk[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (s.st[a] == 0) {
+            kn[a] = 0;
+            c[a] = s.cfix[a];
+            tk[a] = INF;
+        } else {
+            kn[a] = sid_next_plane(s, a, ts);
+            c[a] = s.st[a] > 0 ? kn[a] - 1 : kn[a];
+            tk[a] = sid_plane(s, a, kn[a]);
+        }
+        off[a] = dyn_axis_off(a, c[a] - b0[a]);
+    }
+    float tcur = ts;
+    const int maxsteps = s.N[0] + s.N[1] + s.N[2] + 4;
+    for (int step = 0; step < maxsteps; ++step) {
+        float tnext = te;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) tnext = tk[a] < tnext ? tk[a] : tnext;
+        const float seg = (tnext - tcur) * s.L;
+        if (seg > 0) emit((off[0] + off[1] + off[2]) & mask, seg);
+        if (!(tnext < te)) break;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (tk[a] == tnext) {
+                c[a] += s.st[a];
+                kn[a] += s.st[a];
+                tk[a] = (kn[a] < 0 || kn[a] > s.N[a]) ? INF : sid_plane(s, a, kn[a]);
+                off[a] = dyn_axis_off(a, c[a] - b0[a]);
+            }
+        }
+        tcur = tnext;
+    }
 }
 
 // A x, first pass: per box its voxels staged in LDS, each thread walks
