@@ -216,16 +216,20 @@ void orc_csr_transpose(int64_t m, int64_t n, const int32_t *rowptr, const int32_
 /* ---- precision instantiations ---- */
 #define REAL double
 #define RSQRT sqrt
+#define RFMA fma
 #define SFX(name) name##_f64
 #include "tv3_oracle_t.inc"
 #undef REAL
 #undef RSQRT
+#undef RFMA
 #undef SFX
 
 #define REAL float
 #define RSQRT sqrtf
+#define RFMA fmaf
 #define SFX(name) name##_f32
 #include "tv3_oracle_t.inc"
 #undef REAL
 #undef RSQRT
+#undef RFMA
 #undef SFX

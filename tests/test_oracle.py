@@ -670,20 +670,32 @@ def test_siddon_walk_fp64_equals_textbook_merge(orc):
     in FP64 against the textbook Siddon merge of sorted plane crossings
     (orc_siddon_ray): the same voxels in the same order and the same lengths
     to FP64 rounding, on every ray, including anisotropic voxels and rays
-    parallel to two axes."""
+    parallel to two axes.  The two evaluate the plane parameters differently
+    ((org + k h - p0) / d against fma(k, h/d, (org - p0)/d)), so lengths agree
+    to the per-ray rounding bound 8 eps64 (|org| + N h + |p0|) max|1/d| L of
+    the FP32 test below."""
+    eps = 2.0 ** -53
     for g in _geoms():
         op = orc.siddon_op(g)
         ra, ca, va = orc.siddon_csr(op)
         rb, cb, vb = orc.siddon_walk_csr(op, np.float64)
         assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
-        assert np.abs(va - vb).max() <= 1e-14 * 2.2
+        pe, pr = orc.build_paths(g["E"], g["R"], g.get("mps", 1))
+        ext = np.abs(np.asarray(g["origin"])) + np.asarray(g["rv"]) * np.asarray(g["voxel"])
+        for k in range(len(ra) - 1):
+            p0, p1 = g["e_pos"][pe[k]], g["r_pos"][pr[k]]
+            d = p1 - p0
+            inv = np.max(1.0 / np.maximum(np.abs(d), 1e-300))
+            tol = 8 * eps * float(np.max(ext + np.abs(p0))) * inv * np.linalg.norm(d)
+            s = slice(ra[k], ra[k + 1])
+            assert np.all(np.abs(va[s] - vb[s]) <= tol), (k, tol)
 
 
 def test_siddon_walk_fp32_tracks_fp64(orc):
     """The FP32 traversal (what the device computes) against the FP64 one:
     each ray's length sum (its chord through the volume) to FP32 accuracy; the
     voxels and lengths equal up to the FP32 rounding of the plane parameters
-    t = (org + k h - p0)/d: per ray tol = 8 eps32 (|org| + N h + |p0|) max|1/d| L,
+    t = fma(k, h/d, (org - p0)/d): per ray tol = 8 eps32 (|org| + N h + |p0|) max|1/d| L,
     which grows on grazing rays (small |d| along an axis); voxels whose
     length exceeds tol appear in both, with lengths within 2 tol."""
     eps = 2.0 ** -24

@@ -4,14 +4,26 @@
 // recompute every weight.
 //
 // Boxes: the 8^3 tiles of the fused solver's voxel numbering (B = 8), or
-// their 4^3 bricks where a tile holds more than kDynEmax entries or
-// kDynSmax rays (and every brick when B = 4).  A box is convex, so a ray's
+// their 4^3 bricks where a tile holds more ray entries than the A^T product
+// keeps in LDS (and every brick when B = 4).  A box is convex, so a ray's
 // entries inside it are consecutive in traversal order: one segment.
-// Segments are numbered box by box, rays ascending inside a box; a box's
-// entries are its segments' entries in that order, and every voxel keeps
-// the positions of its entries there (ray order).
+// Segments are numbered box by box, by decreasing length inside a box (ties:
+// ascending ray); a box's entries are its segments' entries in that order,
+// and every voxel keeps the positions of its entries there (ray order).
 #include <algorithm>
 #include <numeric>
+
+// build-time tuning of the A^T boxes: workgroup size, and the entries one
+// box keeps in LDS where a workgroup may hold 160 KiB (15360 otherwise).
+// 512 threads and 19456 entries (76 KiB) let two workgroups share a CU, one
+// walking while the other sums: measured at 128^3 with 262144 rays, 238 us
+// against 302 us for one 1024-thread workgroup holding 32768 entries.
+#ifndef TV3_DYN_T_THREADS
+#define TV3_DYN_T_THREADS 512
+#endif
+#ifndef TV3_DYN_EMAX_BIG
+#define TV3_DYN_EMAX_BIG 19456
+#endif
 
 #include "panels.hpp"
 
@@ -38,24 +50,24 @@ void upload_vec(tv3_ctx *ctx, DevBuf<T> &d, const std::vector<T> &h) {
 
 }  // namespace
 
-// traversal constants of every internal ray (see dyn_ray in panels.hpp)
-__global__ void k_ray_pre(int64_t m, const float *__restrict__ ends, DynView g,
+struct RayGrid {
+    float org[3], h[3];
+    int N[3];
+};
+
+// traversal record of every internal ray (the layout dyn_seg_walk reads:
+// A[3], L, B[3], amin, amax, 3 unused)
+__global__ void k_ray_pre(int64_t m, const float *__restrict__ ends, RayGrid G,
                           float *__restrict__ rec) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     float p0[3], p1[3];
     for (int a = 0; a < 3; ++a) { p0[a] = ends[6 * i + a]; p1[a] = ends[6 * i + 3 + a]; }
     SidRay<float> s;
-    sid_setup(s, p0, p1, g.org, g.h, g.N);
-    float *o = rec + 16 * i;
-    for (int a = 0; a < 3; ++a) { o[a] = s.p0[a]; o[3 + a] = s.d[a]; o[6 + a] = s.inv[a]; }
-    o[9] = s.amin; o[10] = s.amax; o[11] = s.L;
-    int pk = s.hit ? 64 : 0, cf = 0;
-    for (int a = 0; a < 3; ++a) {
-        pk |= (s.st[a] > 0 ? 1 : (s.st[a] < 0 ? 2 : 0)) << (2 * a);
-        cf |= s.cfix[a] << (10 * a);
-    }
-    o[12] = __int_as_float(pk); o[13] = __int_as_float(cf); o[14] = 0.f; o[15] = 0.f;
+    sid_setup(s, p0, p1, G.org, G.h, G.N);
+    float *o = rec + 12 * i;
+    for (int a = 0; a < 3; ++a) { o[a] = s.A[a]; o[4 + a] = s.B[a]; }
+    o[3] = s.L; o[7] = s.amin; o[8] = s.amax; o[9] = 0.f; o[10] = 0.f; o[11] = 0.f;
 }
 
 namespace {
@@ -147,9 +159,35 @@ void sort_segments(SegPlan &S, int64_t npan) {
     for (auto &e : S.seg_of_entry) e = newid[e];
 }
 
-// upload one direction's boxes (segment ranges, entry offsets, work order)
-void finish_boxes(tv3_ctx *ctx, std::vector<DynPan> &pan, const SegPlan &S, bool offsets,
-                  DynBoxes &X, std::vector<int32_t> *eoff_out) {
+// box-local coordinates of each segment's first voxel (x | y << 4 | z << 8)
+std::vector<uint16_t> segment_starts(const std::vector<int32_t> &ptr,
+                                     const std::vector<int32_t> &ind, const SegPlan &S,
+                                     const std::vector<int32_t> &box_of,
+                                     const std::vector<DynPan> &pan, int64_t m) {
+    const int64_t nseg = S.seg_box_ptr.back();
+    std::vector<uint16_t> c0(nseg > 0 ? nseg : 1, 0);
+    for (int64_t r = 0; r < m; ++r) {
+        int32_t last = -1;
+        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+            const int32_t sg = S.seg_of_entry[k];
+            if (sg == last) continue;
+            last = sg;
+            const DynPan &P = pan[box_of[ind[k] >> 6]];
+            int x, y, z;
+            dyn_local_xyz(ind[k] - P.v0, x, y, z);
+            TV3_REQUIRE(x < P.bs && y < P.bs && z < P.bs, TV3_E_INTERNAL,
+                        "a segment starts outside its box");
+            c0[sg] = (uint16_t)(x | (y << 4) | (z << 8));
+        }
+    }
+    return c0;
+}
+
+// upload one direction's boxes (segment ranges, entry offsets, first voxels,
+// work order)
+void finish_boxes(tv3_ctx *ctx, std::vector<DynPan> &pan, const SegPlan &S,
+                  const std::vector<uint16_t> &c0, bool offsets, DynBoxes &X,
+                  std::vector<int32_t> *eoff_out) {
     const int64_t npan = (int64_t)pan.size();
     std::vector<int32_t> eoff(S.seg_ray.size(), 0);
     X.emax = 0;
@@ -176,6 +214,7 @@ void finish_boxes(tv3_ctx *ctx, std::vector<DynPan> &pan, const SegPlan &S, bool
     upload_vec(ctx, X.pan, pan);
     upload_vec(ctx, X.order, bord);
     upload_vec(ctx, X.seg_ray, S.seg_ray);
+    upload_vec(ctx, X.seg_c0, c0);
     if (offsets) upload_vec(ctx, X.seg_eoff, eoff);
     if (eoff_out) *eoff_out = std::move(eoff);
 }
@@ -213,38 +252,31 @@ std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
     // A^T boxes keep their entries and rays in LDS: the larger allocations of
     // devices that allow 160 KiB per workgroup hold whole tiles
     const size_t lds = std::min<size_t>(ctx->lds_per_block, size_t(160) << 10);
-    D->smax_cap = 4096;
-    D->emax_cap = lds >= (size_t(150) << 10) ? 32768 : 11264;
-    D->threads_T = 1024;
+    D->emax_cap = lds >= (size_t(150) << 10) ? TV3_DYN_EMAX_BIG : 15360;
+    D->threads_T = TV3_DYN_T_THREADS;
     const int B = L->B, nby = L->nby, nbx = L->nbx;   // tiles (B = 8) or bricks (B = 4)
     const int64_t nunit = n / (B == 8 ? 512 : 64);
     const int ushift = B == 8 ? 9 : 6;
-    std::vector<int32_t> unit_ent(nunit, 0), unit_seg(nunit, 0);
-    for (int64_t r = 0; r < m; ++r) {
-        int32_t last = -1;
-        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
-            const int32_t u = ind[k] >> ushift;
-            ++unit_ent[u];
-            if (u != last) ++unit_seg[u];
-            last = u;
-        }
-    }
+    std::vector<int32_t> unit_ent(nunit, 0);
+    for (int64_t k = 0; k < nnz; ++k) ++unit_ent[ind[k] >> ushift];
     // forward boxes: whole units; backward: split where the LDS limits ask
     std::vector<DynPan> panF, panT;
     std::vector<int32_t> boxF((size_t)(n >> 6), -1), boxT((size_t)(n >> 6), -1);
     for (int64_t u = 0; u < nunit; ++u) {
         add_boxes(panF, boxF, B, u, nby, nbx, false);
         add_boxes(panT, boxT, B, u, nby, nbx,
-                  unit_ent[u] > D->emax_cap || unit_seg[u] > D->smax_cap);
+                  unit_ent[u] > D->emax_cap);
     }
     SegPlan SF = plan_segments(ptr, ind, boxF, (int64_t)panF.size(), m);
     SegPlan ST = plan_segments(ptr, ind, boxT, (int64_t)panT.size(), m);
     sort_segments(SF, (int64_t)panF.size());
     sort_segments(ST, (int64_t)panT.size());
-    finish_boxes(ctx, panF, SF, false, D->fwd, nullptr);
+    finish_boxes(ctx, panF, SF, segment_starts(ptr, ind, SF, boxF, panF, m), false, D->fwd,
+                 nullptr);
     std::vector<int32_t> eoffT;
-    finish_boxes(ctx, panT, ST, true, D->bwd, &eoffT);
-    TV3_REQUIRE(D->bwd.emax <= D->emax_cap && D->bwd.smax <= D->smax_cap, TV3_E_UNSUPPORTED,
+    finish_boxes(ctx, panT, ST, segment_starts(ptr, ind, ST, boxT, panT, m), true, D->bwd,
+                 &eoffT);
+    TV3_REQUIRE(D->bwd.emax <= D->emax_cap, TV3_E_UNSUPPORTED,
                 "a 4^3 brick of the volume holds more ray entries than the tile products keep in "
                 "LDS");
     // positions of every voxel's entries in its backward box; each voxel's
@@ -300,10 +332,12 @@ std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
     {
         DevBuf<float> dends;
         upload_vec(ctx, dends, ends);
-        D->ray.alloc(16 * (m > 0 ? m : 1));
+        D->ray.alloc(12 * (m > 0 ? m : 1));
+        RayGrid rg;
+        for (int a = 0; a < 3; ++a) { rg.org[a] = D->org[a]; rg.h[a] = D->h[a]; rg.N[a] = D->N[a]; }
         if (m > 0)
             hipLaunchKernelGGL(k_ray_pre, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
-                               ctx->stream, m, dends.p, dyn_view(*D, D->fwd), D->ray.p);
+                               ctx->stream, m, dends.p, rg, D->ray.p);
         TV3_HIP(hipGetLastError());
         TV3_HIP(hipStreamSynchronize(ctx->stream));
     }

@@ -275,105 +275,97 @@ void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s
 struct DynView {
     const DynPan *pan;
     const int32_t *order, *seg_ray, *seg_eoff;
+    const uint16_t *seg_c0;
     const float *ray;
     const uint32_t *pptr;
     const uint16_t *pos;
-    float org[3], h[3], ih[3];
-    int N[3];
     int emax_cap;
 };
 
 inline DynView dyn_view(const DynTiles &D, const DynBoxes &X) {
     DynView v;
     v.pan = X.pan.p; v.order = X.order.p; v.seg_ray = X.seg_ray.p; v.seg_eoff = X.seg_eoff.p;
+    v.seg_c0 = X.seg_c0.p;
     v.ray = D.ray.p; v.pptr = D.pptr.p; v.pos = D.pos.p;
-    for (int a = 0; a < 3; ++a) {
-        v.org[a] = D.org[a]; v.h[a] = D.h[a]; v.ih[a] = 1.f / D.h[a]; v.N[a] = D.N[a];
-    }
     v.emax_cap = D.emax_cap;
     return v;
 }
 
-// Record of one ray's traversal constants (16 words, written by sid_setup at
-// creation): p0, d, 1/d, amin, amax, L, then the direction codes (2 bits per
-// axis: 1 = +1, 2 = -1, 0 = parallel) and the hit bit (bit 6), then the
-// voxel index of each parallel axis (10 bits per axis).
-__device__ __forceinline__ void dyn_ray(const DynView &g, int ray, SidRay<float> &s) {
-    const float4 *r = reinterpret_cast<const float4 *>(g.ray + 16 * (int64_t)ray);
-    const float4 a = r[0], b = r[1], c = r[2], d = r[3];
-    s.p0[0] = a.x; s.p0[1] = a.y; s.p0[2] = a.z;
-    s.d[0] = a.w; s.d[1] = b.x; s.d[2] = b.y;
-    s.inv[0] = b.z; s.inv[1] = b.w; s.inv[2] = c.x;
-    s.amin = c.y; s.amax = c.z; s.L = c.w;
-    const int pk = __float_as_int(d.x), cf = __float_as_int(d.y);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        s.org[k] = g.org[k]; s.h[k] = g.h[k]; s.ih[k] = g.ih[k]; s.N[k] = g.N[k];
-        const int code = (pk >> (2 * k)) & 3;
-        s.st[k] = code == 1 ? 1 : (code == 2 ? -1 : 0);
-        s.cfix[k] = (cf >> (10 * k)) & 1023;
-    }
-    s.hit = ((pk >> 6) & 1) != 0;
+// Box-local coordinates (x, y, z) of the voxel at offset i of a box in the
+// internal order of brick_index: bricks of 4^3 in (y, x, z) bit order, y
+// fastest in a brick (a 4^3 box holds one brick).
+__host__ __device__ __forceinline__ void dyn_local_xyz(int i, int &x, int &y, int &z) {
+    y = (((i >> 6) & 1) << 2) | (i & 3);
+    x = (((i >> 7) & 1) << 2) | ((i >> 2) & 3);
+    z = (((i >> 8) & 1) << 2) | ((i >> 4) & 3);
 }
 
-// Part of the offset of a voxel in a box contributed by its offset v along
-// axis a (0 = x, 1 = y, 2 = z), in the internal order of brick_index:
-// bricks of 4^3 in (y, x, z) bit order, y fastest in a brick (a 4^3 box
-// holds one brick); the offset is the sum over the three axes.
-__device__ __forceinline__ int dyn_axis_off(int a, int v) {
-    return a == 0 ? (((v >> 2) << 7) | ((v & 3) << 2))
-                  : (a == 1 ? (((v >> 2) << 6) | (v & 3)) : (((v >> 2) << 8) | ((v & 3) << 4)));
-}
-
-// The traversal of one segment (the part of an internal ray inside box P):
-// emit(offset of the voxel in the box, length) in traversal order -- the
-// entries the walk of the whole ray emits inside the box.  The loop is
-// sid_walk's (siddon.hpp: same state, same arithmetic, so the same entries)
-// with each axis's share of the box offset updated when that axis steps.
+// The traversal of one segment (the part of an internal ray inside box P,
+// starting in the voxel of box-local coordinates c0 = x | y << 4 | z << 8):
+// emit(offset of the voxel in the box, length) in traversal order, the
+// offset in linear box order y + bs x + bs^2 z.  These are the entries the
+// walk of the whole ray (sid_walk, siddon.hpp) emits inside the box:
+//   * the ray record holds sid_setup's A, B, L, amin, amax (12 words:
+//     A[3], L, B[3], amin, amax, 3 unused);
+//   * at its first voxel c the walk's state is the next plane c + 1 (c when
+//     travelling down) per axis, and its parameter the voxel's entry event
+//     max(amin, t of the entry plane per axis) -- the largest plane
+//     parameter not above it, as in sid_box;
+//   * it ends at min(amax, t of the box's exit plane per axis);
+//   * each step is sid_walk's: the nearest next plane, the length
+//     (t_next - t_cur) L emitted when positive, every axis whose next plane
+//     is crossed at t_next advanced; the plane parameters are the same fused
+//     operation on the same values.  A parallel axis has A = 0, B = +inf, so
+//     its next plane never comes; no plane beyond the box's exit is ever
+//     evaluated (the walk stops at its parameter first).
 template <typename F>
-__device__ __forceinline__ void dyn_seg_walk(const DynView &g, const DynPan &P, int ray, F emit) {
+__host__ __device__ __forceinline__ void dyn_seg_walk(const DynView &g, const DynPan &P, int ray,
+                                                      int c0, F emit) {
 #pragma clang fp contract(off)
-    SidRay<float> s;
-    dyn_ray(g, ray, s);
-    const int b0[3] = {P.bx, P.by, P.bz}, b1[3] = {P.bx + P.bs, P.by + P.bs, P.bz + P.bs};
-    float ts, te;
-    if (!s.hit || !sid_block_range(s, b0, b1, ts, te)) return;
-    // the walk stays in the box; the offset is masked all the same, so that
-    // no index leaves the values staged for the box
-    const int mask = P.bs * P.bs * P.bs - 1;
-    const float INF = 1e30f;
-    int kn[3], c[3], off[3];
-    float tk[3];
+    const float *rec = g.ray + 12 * (int64_t)ray;
+    const float4 ra = reinterpret_cast<const float4 *>(rec)[0];
+    const float4 rb = reinterpret_cast<const float4 *>(rec)[1];
+    const float amax = rec[8];
+    const float A[3] = {ra.x, ra.y, ra.z}, B[3] = {rb.x, rb.y, rb.z};
+    const int lc[3] = {c0 & 15, (c0 >> 4) & 15, (c0 >> 8) & 15};
+    const int b0[3] = {P.bx, P.by, P.bz};
+    const int bs = P.bs;
+    const int stride[3] = {bs, 1, bs * bs};
+    float kf[3], tk[3], st[3];
+    int dstep[3], off = 0;
+    float ts = rb.w, te = amax;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (s.st[a] == 0) {
-            kn[a] = 0;
-            c[a] = s.cfix[a];
-            tk[a] = INF;
-        } else {
-            kn[a] = sid_next_plane(s, a, ts);
-            c[a] = s.st[a] > 0 ? kn[a] - 1 : kn[a];
-            tk[a] = sid_plane(s, a, kn[a]);
+        const bool up = A[a] > 0.f, par = A[a] == 0.f;
+        const int c = b0[a] + lc[a];
+        st[a] = up ? 1.f : (par ? 0.f : -1.f);
+        kf[a] = (float)(up ? c + 1 : c);
+        tk[a] = sid_fma(kf[a], A[a], B[a]);
+        if (!par) {
+            const float tin = sid_fma((float)(up ? c : c + 1), A[a], B[a]);
+            const float tout = sid_fma((float)(up ? b0[a] + bs : b0[a]), A[a], B[a]);
+            ts = tin > ts ? tin : ts;
+            te = tout < te ? tout : te;
         }
-        off[a] = dyn_axis_off(a, c[a] - b0[a]);
+        off += lc[a] * stride[a];
+        dstep[a] = up ? stride[a] : (par ? 0 : -stride[a]);
     }
+    // the offset is masked all the same, so that no index leaves the box
+    const int mask = bs * bs * bs - 1;
     float tcur = ts;
-    const int maxsteps = s.N[0] + s.N[1] + s.N[2] + 4;
-    for (int step = 0; step < maxsteps; ++step) {
+    for (int step = 0; step < 3 * bs + 4; ++step) {
         float tnext = te;
 #pragma unroll
         for (int a = 0; a < 3; ++a) tnext = tk[a] < tnext ? tk[a] : tnext;
-        const float seg = (tnext - tcur) * s.L;
-        if (seg > 0) emit((off[0] + off[1] + off[2]) & mask, seg);
+        const float seg = (tnext - tcur) * ra.w;
+        if (seg > 0) emit(off & mask, seg);
         if (!(tnext < te)) break;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            if (tk[a] == tnext) {
-                c[a] += s.st[a];
-                kn[a] += s.st[a];
-                tk[a] = (kn[a] < 0 || kn[a] > s.N[a]) ? INF : sid_plane(s, a, kn[a]);
-                off[a] = dyn_axis_off(a, c[a] - b0[a]);
-            }
+            const bool m = tk[a] == tnext;
+            kf[a] += m ? st[a] : 0.f;
+            off += m ? dstep[a] : 0;
+            tk[a] = sid_fma(kf[a], A[a], B[a]);
         }
         tcur = tnext;
     }
@@ -389,21 +381,26 @@ __global__ __launch_bounds__(256) void k_dyn_tile_N(DynView g, const T *__restri
     if (active && !*active) return;
     __shared__ T xs[512];
     const DynPan P = g.pan[g.order[blockIdx.x]];
-    const int t = threadIdx.x, nv = P.bs * P.bs * P.bs;
-    for (int i = t; i < nv; i += 256) xs[i] = x[P.v0 + i];
+    const int t = threadIdx.x, bs = P.bs, nv = bs * bs * bs;
+    for (int i = t; i < nv; i += 256) {     // linear box order y + bs x + bs^2 z
+        int lx, ly, lz;
+        dyn_local_xyz(i, lx, ly, lz);
+        xs[ly + bs * (lx + bs * lz)] = x[P.v0 + i];
+    }
     __syncthreads();
     for (int sg = P.s0 + t; sg < P.s1; sg += 256) {
         T acc = 0;
-        dyn_seg_walk(g, P, g.seg_ray[sg], [&](int li, float w) { acc += (T)w * xs[li]; });
+        dyn_seg_walk(g, P, g.seg_ray[sg], g.seg_c0[sg],
+                     [&](int li, float w) { acc += (T)w * xs[li]; });
         part[sg] = acc;
     }
 }
 
-// A^T r: per box the residuals of its rays staged in LDS, each thread walks
-// segments and writes w r_ray at the entry's position in the box's entry
-// array (ray-major, emax_cap + 1 slots); then each voxel row sums its entries
-// in ray order through the stored positions (padding points at the zero slot
-// ne) and the epilogue takes the sum.
+// A^T r: per box each thread walks segments and writes w r_ray at the
+// entry's position in the box's entry array in LDS (ray-major, emax_cap + 1
+// slots); then each voxel row sums its entries in ray order through the
+// stored positions (padding points at the zero slot ne) and the epilogue
+// takes the sum.
 template <typename T, int NT, typename Epi>
 __global__ __launch_bounds__(NT) void k_dyn_tile_T(DynView g, const T *__restrict__ x, Epi epi) {
     static_assert(ExtraBlocks<Epi>::value == 0, "no extra workgroups in the tile product");
@@ -411,16 +408,13 @@ __global__ __launch_bounds__(NT) void k_dyn_tile_T(DynView g, const T *__restric
     if (epi.skip()) return;
     const DynPan P = g.pan[g.order[blockIdx.x]];
     T *pe = reinterpret_cast<T *>(dyn_smem);
-    T *xr = pe + g.emax_cap + 1;
     const int t = threadIdx.x, ns = P.s1 - P.s0;
-    for (int j = t; j < ns; j += NT) xr[j] = x[g.seg_ray[P.s0 + j]];
-    __syncthreads();
     for (int j = t; j < ns; j += NT) {
-        const int sg = P.s0 + j;
+        const int sg = P.s0 + j, ray = g.seg_ray[sg];
         const int e1 = j + 1 < ns ? g.seg_eoff[sg + 1] : P.ne;
-        const T xv = xr[j];
+        const T xv = x[ray];
         int e = g.seg_eoff[sg];
-        dyn_seg_walk(g, P, g.seg_ray[sg], [&](int, float w) {
+        dyn_seg_walk(g, P, ray, g.seg_c0[sg], [&](int, float w) {
             if (e < e1) pe[e] = (T)w * xv;
             ++e;
         });
@@ -452,9 +446,22 @@ void launch_dyn_N(const DynTiles &D, const T *x, T *part, const int32_t *active,
 template <typename T, typename Epi>
 void launch_dyn_T(const DynTiles &D, const T *x, const Epi &epi, hipStream_t s) {
     if (D.bwd.npan == 0) return;
-    const size_t lds = sizeof(T) * (size_t)(D.emax_cap + 1 + D.smax_cap);
-    hipLaunchKernelGGL((k_dyn_tile_T<T, 1024, Epi>), dim3((unsigned)D.bwd.npan), dim3(1024), lds,
-                       s, dyn_view(D, D.bwd), x, epi);
+    const size_t lds = sizeof(T) * (size_t)(D.emax_cap + 1);
+    switch (D.threads_T) {
+    case 256:
+        hipLaunchKernelGGL((k_dyn_tile_T<T, 256, Epi>), dim3((unsigned)D.bwd.npan), dim3(256), lds,
+                           s, dyn_view(D, D.bwd), x, epi);
+        break;
+    case 512:
+        hipLaunchKernelGGL((k_dyn_tile_T<T, 512, Epi>), dim3((unsigned)D.bwd.npan), dim3(512), lds,
+                           s, dyn_view(D, D.bwd), x, epi);
+        break;
+    case 1024:
+        hipLaunchKernelGGL((k_dyn_tile_T<T, 1024, Epi>), dim3((unsigned)D.bwd.npan), dim3(1024),
+                           lds, s, dyn_view(D, D.bwd), x, epi);
+        break;
+    default: throw tv3_error(TV3_E_ARG, "the A^T tile product runs 256, 512 or 1024 threads");
+    }
 }
 
 constexpr int kSegSumNLB = 2;   // logical blocks (256 / L rays each) per pass-2 workgroup

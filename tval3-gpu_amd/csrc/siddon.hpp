@@ -5,7 +5,9 @@
 // N[0] x N[1] x N[2] voxels (axes x, y, z) of size h from corner org (Siddon,
 // Med. Phys. 12(2), 1985), traversed in increasing ray parameter t:
 //   plane k (0..N) of axis a is crossed at t(a, k) = (org + k h - p0) / d,
-//   evaluated as (org + k h - p0) * (1 / d) with no contraction;
+//   evaluated as fma(k, A, B) with A = h * (1 / d), B = (org - p0) * (1 / d)
+//   (each rounded once; no other contraction): one fused operation per
+//   plane, monotone in k;
 //   the ray is clipped to [amin, amax] by planes 0 and N of every axis;
 //   the walk keeps, per axis, the next plane with t > t_cur and the voxel
 //   index it tracks, and emits each voxel it crosses with length
@@ -32,7 +34,8 @@ namespace tv3 {
 
 template <typename R>
 struct SidRay {
-    R p0[3], d[3], inv[3], org[3], h[3], ih[3];   // ih = 1/h: seeds the plane search only
+    R A[3], B[3];    // plane k of axis a at t = fma(k, A, B); a parallel axis: A = 0, B = +inf
+    R rA[3];         // 1 / A: seeds the plane search only
     R amin, amax, L;
     int N[3];
     int st[3];       // +1 / -1: direction of travel along the axis; 0: parallel
@@ -40,10 +43,14 @@ struct SidRay {
     bool hit;        // the clipped ray is not empty
 };
 
+__host__ __device__ __forceinline__ float sid_fma(float k, float a, float b) { return fmaf(k, a, b); }
+__host__ __device__ __forceinline__ double sid_fma(double k, double a, double b) {
+    return fma(k, a, b);
+}
+
 template <typename R>
 __host__ __device__ __forceinline__ R sid_plane(const SidRay<R> &s, int a, int k) {
-#pragma clang fp contract(off)
-    return (s.org[a] + (R)k * s.h[a] - s.p0[a]) * s.inv[a];
+    return sid_fma((R)k, s.A[a], s.B[a]);
 }
 
 template <typename R>
@@ -58,23 +65,25 @@ __host__ __device__ inline void sid_setup(SidRay<R> &s, const R p0[3], const R p
     s.amin = 0;
     s.amax = 1;
     s.hit = true;
+    R d[3];
     for (int a = 0; a < 3; ++a) {
-        s.p0[a] = p0[a];
-        s.org[a] = org[a];
-        s.h[a] = h[a];
-        s.ih[a] = (R)1 / h[a];
         s.N[a] = N[a];
-        s.d[a] = p1[a] - p0[a];
+        d[a] = p1[a] - p0[a];
         s.cfix[a] = 0;
-        if (s.d[a] == 0) {
+        if (d[a] == 0) {
             s.st[a] = 0;
-            s.inv[a] = 0;
+            s.A[a] = 0;
+            s.B[a] = (R)INFINITY;
+            s.rA[a] = 0;
             const R lo = org[a], hi = org[a] + (R)N[a] * h[a];
             if (!(p0[a] > lo && p0[a] < hi)) s.hit = false;
             s.cfix[a] = sid_clampi<R>((int)floor((p0[a] - org[a]) / h[a]), 0, N[a] - 1);
         } else {
-            s.st[a] = s.d[a] > 0 ? 1 : -1;
-            s.inv[a] = (R)1 / s.d[a];
+            s.st[a] = d[a] > 0 ? 1 : -1;
+            const R inv = (R)1 / d[a];
+            s.A[a] = h[a] * inv;
+            s.B[a] = (org[a] - p0[a]) * inv;
+            s.rA[a] = (R)1 / s.A[a];
             const R t0 = sid_plane(s, a, 0), t1 = sid_plane(s, a, N[a]);
             const R lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
             s.amin = lo > s.amin ? lo : s.amin;
@@ -82,7 +91,7 @@ __host__ __device__ inline void sid_setup(SidRay<R> &s, const R p0[3], const R p
         }
     }
     if (!(s.amin < s.amax)) s.hit = false;
-    s.L = sqrt(s.d[0] * s.d[0] + s.d[1] * s.d[1] + s.d[2] * s.d[2]);
+    s.L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 }
 
 // First plane along the direction of travel of axis a crossed after t: the
@@ -93,7 +102,7 @@ template <typename R>
 __host__ __device__ inline int sid_next_plane(const SidRay<R> &s, int a, R t) {
 #pragma clang fp contract(off)
     const int N = s.N[a];
-    const R f = (s.p0[a] + t * s.d[a] - s.org[a]) * s.ih[a];
+    const R f = (t - s.B[a]) * s.rA[a];
     if (s.st[a] > 0) {
         int k = sid_clampi<R>((int)floor(f) + 1, 1, N);
         while (k > 1 && sid_plane(s, a, k - 1) > t) --k;

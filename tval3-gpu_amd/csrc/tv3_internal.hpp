@@ -234,18 +234,20 @@ struct DynBoxes {
     DevBuf<int32_t> order;           // boxes, decreasing work
     DevBuf<int32_t> seg_ray;         // internal ray of each segment
     DevBuf<int32_t> seg_eoff;        // first entry of each segment in its box (A^T)
+    DevBuf<uint16_t> seg_c0;         // box-local first voxel of each segment: x | y << 4 | z << 8
 };
 
 struct DynTiles {
     int64_t nnz = 0;
-    int32_t emax_cap = 0, smax_cap = 0;   // A^T box limits: entries / segments in LDS
-    int32_t threads_T = 1024;             // workgroup of the A^T product
+    int32_t emax_cap = 0;                 // A^T box limit: entries in LDS
+    int32_t threads_T = 512;              // workgroup of the A^T product
     float org[3] = {0, 0, 0}, h[3] = {1, 1, 1};
     int32_t N[3] = {0, 0, 0};        // voxels along x, y, z
     DynBoxes fwd;                    // A x: whole tiles (or bricks when B = 4)
     DynBoxes bwd;                    // A^T r: tiles, split into bricks where a tile's
-                                     // entries or segments exceed the LDS limits
-    DevBuf<float> ray;               // 16 per internal ray: its traversal constants
+                                     // entries exceed the LDS limit
+    DevBuf<float> ray;               // 12 per internal ray: sid_setup's A[3], L, B[3],
+                                     // amin, amax (3 unused)
     DevBuf<uint32_t> pptr;           // n + 1: padded starts of the voxel rows
     DevBuf<uint16_t> pos;            // per voxel row its entries' positions in its bwd
                                      // box, ray order, padded to 4 with the position ne
