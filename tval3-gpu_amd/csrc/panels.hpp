@@ -12,9 +12,14 @@
 //          sources = the tile's 512 voxels (a contiguous range); pass 2 sums
 //          each ray's segment partials (k_seg_sum) and runs the epilogue.
 //
-// Phase 1 of a row block: each thread loads groups of 4 entries of one row
-// (16 bytes), multiplies with the staged x and sums the group in registers;
-// phase 2 sums each row's group sums (a quarter of the entries) from LDS.
+// A row block's rows are stored back to back from a 4-aligned start, each
+// padded with zero entries to a multiple of the group size G (1, 2 or 4),
+// the block's end to a multiple of 4.  Phase 1: each thread loads 4
+// consecutive entries (16 bytes; 4 / G whole groups of one row each),
+// multiplies with the staged x and stores the 4 / G group sums in LDS;
+// phase 2 sums each row's group sums (row_sum, a power-of-two lane group per
+// row) and hands the total to the epilogue.  G trades padding (entries read)
+// against phase 2's LDS reads: short rows favour small G.
 #pragma once
 
 #include <type_traits>
@@ -34,25 +39,34 @@ template <> __device__ __forceinline__ float pk_weight<__hip_bfloat16>(uint32_t 
     return __uint_as_float(u & 0xffff0000u);
 }
 
-// sum of the 4 products of the entry group at padded position q
+// The 4 stored entries at (4-aligned) position q: packed 16-bit weights and
+// indices, or 16-bit indices and FP32 weights.
+template <typename V> struct Quad { uint4 u; };
+template <> struct Quad<float> { uint2 u; float4 w; };
+
 template <typename V>
-__device__ __forceinline__ float group_sum(const uint32_t *pk, const uint16_t *, const float *,
-                                          int64_t q, const float *xs) {
-    const uint4 u = *reinterpret_cast<const uint4 *>(pk + q);
-    const float a = pk_weight<V>(u.x) * xs[u.x & 0xffffu];
-    const float b = pk_weight<V>(u.y) * xs[u.y & 0xffffu];
-    const float c = pk_weight<V>(u.z) * xs[u.z & 0xffffu];
-    const float d = pk_weight<V>(u.w) * xs[u.w & 0xffffu];
-    return (a + b) + (c + d);
+__device__ __forceinline__ Quad<V> load_quad(const uint32_t *pk, const uint16_t *, const float *,
+                                             int64_t q) {
+    return Quad<V>{*reinterpret_cast<const uint4 *>(pk + q)};
 }
 template <>
-__device__ __forceinline__ float group_sum<float>(const uint32_t *, const uint16_t *ps,
-                                                 const float *pv, int64_t q, const float *xs) {
-    const uint2 u = *reinterpret_cast<const uint2 *>(ps + q);
-    const float4 w = *reinterpret_cast<const float4 *>(pv + q);
-    const float a = w.x * xs[u.x & 0xffffu], b = w.y * xs[u.x >> 16];
-    const float c = w.z * xs[u.y & 0xffffu], d = w.w * xs[u.y >> 16];
-    return (a + b) + (c + d);
+__device__ __forceinline__ Quad<float> load_quad<float>(const uint32_t *, const uint16_t *ps,
+                                                        const float *pv, int64_t q) {
+    return Quad<float>{*reinterpret_cast<const uint2 *>(ps + q),
+                       *reinterpret_cast<const float4 *>(pv + q)};
+}
+
+// their 4 products with the staged sources
+template <typename V>
+__device__ __forceinline__ float4 quad_prod(const Quad<V> &e, const float *xs) {
+    const uint4 u = e.u;
+    return make_float4(pk_weight<V>(u.x) * xs[u.x & 0xffffu], pk_weight<V>(u.y) * xs[u.y & 0xffffu],
+                       pk_weight<V>(u.z) * xs[u.z & 0xffffu], pk_weight<V>(u.w) * xs[u.w & 0xffffu]);
+}
+template <>
+__device__ __forceinline__ float4 quad_prod<float>(const Quad<float> &e, const float *xs) {
+    return make_float4(e.w.x * xs[e.u.x & 0xffffu], e.w.y * xs[e.u.x >> 16],
+                       e.w.z * xs[e.u.y & 0xffffu], e.w.w * xs[e.u.y >> 16]);
 }
 
 // Epilogue that stores y[row] = sum unless the predicate *active is 0.
@@ -77,8 +91,16 @@ struct ExtraBlocks<E, std::void_t<decltype(E::kExtra)>> {
 };
 
 // One workgroup per panel (largest panels first, so the launch ends with
-// small ones).  Block descriptors: p0 / len = padded start / padded length.
-template <int NB, typename V, bool CONTIG, typename Epi>
+// small ones).  Block descriptors: p0 / len = start / length of the block's
+// stored entries (both multiples of 4); row r's entries start at pptr[r].
+// A panel's blocks are stored back to back, so block b + 1 starts where
+// block b ends: its entries are requested before block b's products are
+// formed (the first block's before the sources are staged).  Such a
+// request covers NB entries from the block's start whatever its length,
+// clamped to the last stored quad qlast: entries past the block belong to
+// later blocks (or repeat the last quad), their 16-bit indices stay inside
+// the LDS sources of the largest panel, and phase 2 never sums them.
+template <int NB, int G, typename V, bool CONTIG, typename Epi>
 __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order,
                                                const int32_t *__restrict__ src_off,
                                                const int32_t *__restrict__ pan_blk,
@@ -87,11 +109,12 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
                                                const int32_t *__restrict__ pptr,
                                                const uint32_t *__restrict__ pk,
                                                const uint16_t *__restrict__ ps,
-                                               const float *__restrict__ pv,
+                                               const float *__restrict__ pv, int64_t qlast,
                                                const float *__restrict__ x, Epi epi) {
-    constexpr int NG = NB / 4;               // entry groups per block
-    extern __shared__ float xs[];            // the panel's source values
-    __shared__ float gs[NG];                 // group sums
+    static_assert(G == 1 || G == 2 || G == 4, "group of 1, 2 or 4 entries");
+    constexpr int NQ = NB / 4;                   // 16-byte entry quads per block
+    extern __shared__ float xs[];                // the panel's source values
+    __shared__ __align__(16) float prod[NB / G]; // the block's group sums
     constexpr int NX = ExtraBlocks<Epi>::value;
     if constexpr (NX > 0) {
         if ((int)blockIdx.x < NX) {
@@ -110,6 +133,11 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
         // straddle one): its first row decides, uniformly
         if (bb < b1 && epi.skip_rows(d.r0)) return;
     }
+    constexpr int J = NQ / 256;
+    Quad<V> cur[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+        cur[j] = load_quad<V>(pk, ps, pv, min((int64_t)d.p0 + 4 * (j * 256 + t), qlast));
     // stage the sources: batches of 8 per thread, all loads in flight
     for (int k0 = 0; k0 < ns; k0 += 8 * 256) {
         float xv[8];
@@ -130,36 +158,56 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
     __syncthreads();
     for (int b = bb; b < b1; ++b) {
         const BlkDesc dn = desc[b + 1];      // in flight while this block runs
+        Quad<V> nxt[J];
+        if (b + 1 < b1) {
+            const int64_t pn = (int64_t)d.p0 + d.len;
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+                nxt[j] = load_quad<V>(pk, ps, pv, min(pn + 4 * (j * 256 + t), qlast));
+        }
         if (d.len <= NB) {
             const int row = block_row(d);
-            const int ga = (pptr[row] - d.p0) >> 2, ge = (pptr[row + 1] - d.p0) >> 2;
+            const int a = (pptr[row] - d.p0) / G, e = (pptr[row + 1] - d.p0) / G;
             const auto pre = epi.load(row);
             if (d.len > 0) {
-                const int glast = (d.len >> 2) - 1;
-                float v[NG / 256];
+                float4 v[J];
 #pragma unroll
-                for (int j = 0; j < NG / 256; ++j)
-                    v[j] = group_sum<V>(pk, ps, pv, (int64_t)d.p0 + 4 * min(j * 256 + t, glast),
-                                        xs);
+                for (int j = 0; j < J; ++j) v[j] = quad_prod<V>(cur[j], xs);
 #pragma unroll
-                for (int j = 0; j < NG / 256; ++j) gs[j * 256 + t] = v[j];
+                for (int j = 0; j < J; ++j) {
+                    const int q = j * 256 + t;
+                    if constexpr (G == 4) {
+                        prod[q] = (v[j].x + v[j].y) + (v[j].z + v[j].w);
+                    } else if constexpr (G == 2) {
+                        *reinterpret_cast<float2 *>(prod + 2 * q) =
+                            make_float2(v[j].x + v[j].y, v[j].z + v[j].w);
+                    } else {
+                        *reinterpret_cast<float4 *>(prod + 4 * q) = v[j];
+                    }
+                }
             }
             __syncthreads();
-            block_rows<NG>(gs, d, ga, ge, epi, row, pre);
+            block_rows<NB / G>(prod, d, a, e, epi, row, pre);
         } else {
             // one row longer than NB
             const auto pre = epi.load(d.r0);
             float s = 0;
-            for (int g = t; g < (d.len >> 2); g += 256)
-                s += group_sum<V>(pk, ps, pv, (int64_t)d.p0 + 4 * g, xs);
+            for (int g = t; g < (d.len >> 2); g += 256) {
+                const float4 v = quad_prod<V>(load_quad<V>(pk, ps, pv, (int64_t)d.p0 + 4 * g), xs);
+                s += (v.x + v.y) + (v.z + v.w);
+            }
             s = lane_sum<64>(s);
-            if ((t & 63) == 0) gs[t >> 6] = s;
+            if ((t & 63) == 0) prod[t >> 6] = s;
             __syncthreads();
-            if (t == 0) s = (gs[0] + gs[1]) + (gs[2] + gs[3]);
+            if (t == 0) s = (prod[0] + prod[1]) + (prod[2] + prod[3]);
             epi.row(d.r0, s, t == 0, pre);
         }
-        __syncthreads();             // gs is rewritten by the next row block
+        __syncthreads();             // prod is rewritten by the next row block
         d = dn;
+        if (b + 1 < b1) {
+#pragma unroll
+            for (int j = 0; j < J; ++j) cur[j] = nxt[j];
+        }
     }
     epi.finish();
 }
@@ -237,15 +285,19 @@ void launch_panel_v(const Panels &P, const float *x, const Epi &epi, hipStream_t
     const unsigned g = (unsigned)P.npan + ExtraBlocks<Epi>::value;
     const size_t lds = sizeof(float) * (size_t)(P.max_src > 0 ? P.max_src : 1);
     if (g == 0) return;
-#define TV3_PCASE(NBV)                                                                        \
-    case NBV:                                                                                 \
-        hipLaunchKernelGGL((k_panel<NBV, V, CONTIG, Epi>), dim3(g), dim3(256), lds, s,          \
+#define TV3_PCASE(NBV, GV)                                                                    \
+    case NBV * 8 + GV:                                                                        \
+        hipLaunchKernelGGL((k_panel<NBV, GV, V, CONTIG, Epi>), dim3(g), dim3(256), lds, s,      \
                            P.order.p, P.src_off.p, P.pan_blk.p, P.src.p, P.desc.p, P.pptr.p,  \
-                           P.pk.p, P.ps.p, P.pv.p, x, epi);                                   \
+                           P.pk.p, P.ps.p, P.pv.p, P.stored - 4, x, epi);                     \
         break;
-    switch (P.blk_nnz) {
-        TV3_PCASE(1024) TV3_PCASE(2048) TV3_PCASE(4096)
-    default: throw tv3_error(TV3_E_ARG, "panel block size must be 1024, 2048 or 4096");
+    switch (P.blk_nnz * 8 + P.group) {
+        TV3_PCASE(1024, 1) TV3_PCASE(2048, 1) TV3_PCASE(4096, 1)
+        TV3_PCASE(1024, 2) TV3_PCASE(2048, 2) TV3_PCASE(4096, 2)
+        TV3_PCASE(1024, 4) TV3_PCASE(2048, 4) TV3_PCASE(4096, 4)
+    default:
+        throw tv3_error(TV3_E_ARG, "panel block size must be 1024, 2048 or 4096 entries and "
+                                   "the group 1, 2 or 4");
     }
 #undef TV3_PCASE
 }
@@ -464,7 +516,10 @@ void launch_dyn_T(const DynTiles &D, const T *x, const Epi &epi, hipStream_t s) 
     }
 }
 
-constexpr int kSegSumNLB = 2;   // logical blocks (256 / L rays each) per pass-2 workgroup
+#ifndef TV3_SEGSUM_NLB
+#define TV3_SEGSUM_NLB 2
+#endif
+constexpr int kSegSumNLB = TV3_SEGSUM_NLB;   // logical blocks (256 / L rays each) per pass-2 workgroup
 
 // Logical blocks and workgroups of pass 2 of A x.  The workgroup count is
 // also the number of block partials its epilogue stores: every reader of

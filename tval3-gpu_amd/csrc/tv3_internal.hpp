@@ -187,6 +187,8 @@ struct Panels {
     int64_t rows = 0, npan = 0, nsrc = 0;
     int max_src = 0;           // largest source list of a panel (LDS floats)
     int blk_nnz = 0;
+    int group = 4;             // rows padded to groups of 1, 2 or 4 entries
+    int64_t stored = 4;        // stored entries (at least 4, a multiple of 4)
     bool contiguous = false;   // sources of panel p: x[src_off[p] .. src_off[p + 1])
     tv3_dtype vtype = TV3_F16;
     DevBuf<int32_t> order;     // npan panels, decreasing work
