@@ -470,9 +470,11 @@ static float tune_shape(tv3_ctx *ctx, ColdTimer &time_launches, SparseRows &S, i
     return best;
 }
 
-// Panel forms of A^T and A (tiles of 512 internal voxels, panels of at most
-// 16384 entries) timed over row blocks of 1024 and 2048 entries; each kept
-// when faster than the tuned plain kernel (t_A, t_T in ms).
+// Panel forms of A^T (tiles of 512 internal voxels, panels of at most 16384
+// entries) and of A (panels of one tile, or of two -- 16 x 8 x 8 voxels, at
+// most 32768 entries -- whose rays leave fewer, longer segments and so fewer
+// partial sums for the second pass) timed over row blocks of 1024 and 2048
+// entries; each kept when faster than the tuned plain kernel (t_A, t_T in ms).
 static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, float t_A,
                         float t_T) {
     hipStream_t s = ctx->stream;
@@ -495,15 +497,26 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
             time_launches([&]() { launch_panel(P, xr.p, EpiStoreIf{y.p, nullptr}, s); });
         if (t < t_T) { t_T = t; F.PT = std::move(P); F.panels_T = true; }
     }
-    for (int nb : {1024, 2048}) {
-        Panels P;
-        SegSum R;
-        build_panels_A(ctx, h, pl, nb, P, R);
-        const float t = time_launches([&]() {
-            launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
-            launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
-        });
-        if (t < t_A) { t_A = t; F.PA = std::move(P); F.RS = std::move(R); F.panels_A = true; }
+    for (int unit : {512, 1024}) {
+        PanelPlan pl2;
+        if (unit != 512 && !plan_panels(h, unit, 16384 * unit / 512, pl2)) continue;
+        const PanelPlan &pa = unit == 512 ? pl : pl2;
+        for (int nb : {1024, 2048}) {
+            Panels P;
+            SegSum R;
+            build_panels_A(ctx, h, pa, nb, P, R);
+            const float t = time_launches([&]() {
+                launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
+                launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
+            });
+            if (t < t_A) {
+                t_A = t;
+                F.PA = std::move(P);
+                F.RS = std::move(R);
+                F.panels_A = true;
+                F.unit_A = unit;
+            }
+        }
     }
 }
 
@@ -645,6 +658,7 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
     F.session.reset();               // its graphs hold the old launch shapes
     tv3_ctx *ctx = op->ctx;
     const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0;
+    const int unit_A = (panels & 4) ? 1024 : 512;
     if (pa || pt) {
         TV3_REQUIRE(pan_nnz > 0, TV3_E_ARG, "panels need pan_nnz > 0");
         for (int c : {pa ? code_N : -1024, pt ? code_T : -1024})
@@ -658,7 +672,14 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
         F.nseg = (int64_t)pl.seg_ray.size();
         F.max_seg = (int)pl.max_seg;
         if (pt) build_panels_T(ctx, h, pl, -code_T, F.PT);
-        if (pa) build_panels_A(ctx, h, pl, -code_N, F.PA, F.RS);
+        if (pa && unit_A == 512) build_panels_A(ctx, h, pl, -code_N, F.PA, F.RS);
+        if (pa && unit_A != 512) {
+            PanelPlan pl2;
+            TV3_REQUIRE(plan_panels(h, unit_A, 2 * pan_nnz, pl2), TV3_E_UNSUPPORTED,
+                        "a panel of this operator would need more rays than LDS holds");
+            build_panels_A(ctx, h, pl2, -code_N, F.PA, F.RS);
+        }
+        F.unit_A = unit_A;
     }
     F.panels_A = pa;
     F.panels_T = pt;

@@ -832,7 +832,8 @@ static void fill_layout_info(const FusedLayout &F, tv3_layout_info *info) {
     std::memset(info, 0, sizeof(*info));
     auto code = [](const SparseRows &S) { return S.blk_nnz ? -S.blk_nnz : S.lanes; };
     info->brick = F.B;
-    info->panels = (F.panels_A ? 1 : 0) | (F.panels_T ? 2 : 0);
+    info->panels = (F.panels_A ? 1 : 0) | (F.panels_T ? 2 : 0) |
+                   (F.panels_A && F.unit_A != 512 ? 4 : 0);
     info->code_N = F.panels_A ? -F.PA.blk_nnz : code(F.A);
     info->code_T = F.panels_T ? -F.PT.blk_nnz : code(F.AT);
     info->npan = F.npan;
