@@ -61,18 +61,25 @@ def test_fused_tracks_oracle(ctx, dims):
     perturbed by 1e-7 relative, drifts to ~1e-4 in f after ~25 iterations and
     ~1e-3 in the final U.  The fused path (FP32 products in another order)
     must agree to rounding level at the start and stay inside a small multiple
-    of that envelope afterwards."""
+    of that envelope afterwards.  Every envelope (trajectory, final error, U)
+    is the widest of three perturbations: one can underestimate the drift
+    tenfold, and the tuned layout, hence the rounding order, can differ
+    between runs."""
     u, A, A32, oop, b = setup(ctx, dims, 24, 64)
     o = opts(maxit=60)
     Uf, inf_, hf = api.tval3_gpu_3d(ctx, A, b, dims, api.default_opts(**o, solver="fused"), Ut=u)
     oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=60, tv_iso=1)
     Uo, io, ho, _ = orc.solve(oop, dims, b, oo, Ut=u, dtype=np.float32)
-    xi = np.random.default_rng(0).choice([-1.0, 1.0], size=b.shape)
-    bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
-    Up, ip, hpp, _ = orc.solve(oop, dims, bp, oo, Ut=u, dtype=np.float32)
-    k = min(first_divergence(hf, ho), first_divergence(hpp, ho))
+    pert = []
+    for seed in (0, 1, 2):
+        xi = np.random.default_rng(seed).choice([-1.0, 1.0], size=b.shape)
+        bp = (b.astype(np.float64) * (1 + 1e-7 * xi)).astype(np.float32)
+        pert.append(orc.solve(oop, dims, bp, oo, Ut=u, dtype=np.float32))
+    Up, ip, hpp, _ = pert[0]
+    k = min([first_divergence(hf, ho)] + [first_divergence(p[2], ho) for p in pert])
     dev_f = np.abs(hf["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k])
-    env = np.maximum.accumulate(np.abs(hpp["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k]))
+    env = np.max([np.maximum.accumulate(np.abs(p[2]["f"][:k] - ho["f"][:k]) / np.abs(ho["f"][:k]))
+                  for p in pert], axis=0)
     print(f"{dims}: iters fused {inf_['total_iters']} oracle {io.total_iters}; decisions agree "
           f"for {k}; err fused {inf_['rel_error']:.4e} oracle {io.rel_error:.4e} perturbed "
           f"{ip.rel_error:.4e}; rel(U fused, oracle) {rel(Uf, Uo):.2e}, rel(U perturbed, "
@@ -83,8 +90,13 @@ def test_fused_tracks_oracle(ctx, dims):
     # alpha of a cnt == 6 (fallback) iteration is a dead local in the reference
     live = ho["cnt"][:k] < 6
     assert np.allclose(hf["alpha"][:k][live], ho["alpha"][:k][live], rtol=1e-5)
-    assert abs(inf_["rel_error"] - io.rel_error) <= max(3 * abs(ip.rel_error - io.rel_error), 1e-3)
-    assert rel(Uf, Uo) <= max(10 * rel(Up, Uo), 1e-4)
+    # U within a small multiple of the oracle's own spread; the final error
+    # then differs by at most |U - U_oracle| / |u_true| (triangle inequality)
+    env_err = max(abs(p[1].rel_error - io.rel_error) for p in pert)
+    env_U = max(rel(p[0], Uo) for p in pert)
+    assert rel(Uf, Uo) <= max(10 * env_U, 1e-4)
+    bound = rel(Uf, Uo) * np.linalg.norm(Uo) / np.linalg.norm(u)
+    assert abs(inf_["rel_error"] - io.rel_error) <= max(3 * env_err, 1e-3, 1.01 * bound)
 
 
 def test_fused_graph_size_invariance_and_outer_loop(ctx):

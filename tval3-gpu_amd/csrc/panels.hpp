@@ -97,9 +97,10 @@ struct ExtraBlocks<E, std::void_t<decltype(E::kExtra)>> {
 // block b ends: its entries are requested before block b's products are
 // formed (the first block's before the sources are staged).  Such a
 // request covers NB entries from the block's start whatever its length,
-// clamped to the last stored quad qlast: entries past the block belong to
-// later blocks (or repeat the last quad), their 16-bit indices stay inside
-// the LDS sources of the largest panel, and phase 2 never sums them.
+// clamped to the panel's last stored quad (so no entries past the panel are
+// fetched from memory; qlast, the last stored quad overall, bounds a panel
+// without entries): entries past the block belong to the panel's later
+// blocks or repeat its last quad, and phase 2 never sums them.
 template <int NB, int G, typename V, bool CONTIG, typename Epi>
 __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order,
                                                const int32_t *__restrict__ src_off,
@@ -134,10 +135,11 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
         if (bb < b1 && epi.skip_rows(d.r0)) return;
     }
     constexpr int J = NQ / 256;
+    const int64_t qhi = min(qlast, max((int64_t)desc[b1].p0 - 4, (int64_t)d.p0));
     Quad<V> cur[J];
 #pragma unroll
     for (int j = 0; j < J; ++j)
-        cur[j] = load_quad<V>(pk, ps, pv, min((int64_t)d.p0 + 4 * (j * 256 + t), qlast));
+        cur[j] = load_quad<V>(pk, ps, pv, min((int64_t)d.p0 + 4 * (j * 256 + t), qhi));
     // stage the sources: batches of 8 per thread, all loads in flight
     for (int k0 = 0; k0 < ns; k0 += 8 * 256) {
         float xv[8];
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
             const int64_t pn = (int64_t)d.p0 + d.len;
 #pragma unroll
             for (int j = 0; j < J; ++j)
-                nxt[j] = load_quad<V>(pk, ps, pv, min(pn + 4 * (j * 256 + t), qlast));
+                nxt[j] = load_quad<V>(pk, ps, pv, min(pn + 4 * (j * 256 + t), qhi));
         }
         if (d.len <= NB) {
             const int row = block_row(d);

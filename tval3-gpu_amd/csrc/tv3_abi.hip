@@ -863,8 +863,11 @@ tv3_status tv3_op_layout_force(tv3_op *op, const int32_t dims[3], int32_t code_N
         return L == 0 || (L >= 2 && L <= 64 && (L & (L - 1)) == 0) || L == -1024 || L == -2048 ||
                L == -4096;
     };
-    if (!ok(code_N) || !ok(code_T) || pan_nnz < 0 || panels < 0 || panels > 3) return TV3_E_ARG;
     return guard(op->ctx, [&] {
+        TV3_REQUIRE(ok(code_N) && ok(code_T), TV3_E_ARG,
+                    "launch codes are 0, a power of two in [2, 64], -1024, -2048 or -4096");
+        TV3_REQUIRE(pan_nnz >= 0 && panels >= 0 && panels <= 7, TV3_E_ARG,
+                    "pan_nnz must be >= 0 and panels in [0, 7]");
         check_dims(op, dims);
         set_device(op->ctx);
         layout_force(op, dims, code_N, code_T, pan_nnz, panels);
