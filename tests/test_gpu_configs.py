@@ -79,6 +79,18 @@ def check_envelope(h, ho, runs, io, info, U, Uo, u, tag, rtol_alpha=1e-5):
     assert abs(info["rel_error"] - io.rel_error) <= max(3 * spread, 1e-3, 1.01 * bound)
 
 
+def test_cfg1_device_csr_equals_oracle_fp32_walk(ctx, cfg1):
+    """At config 1's size the device CSR build (the FP32 traversal) is the
+    oracle's FP32 walk bit for bit: row pointers, columns and weights."""
+    u, A, oop, b = cfg1
+    g = wl.transducer_geometry(128, 512, dims=(64, 64, 64), seed=2)    # cfg1's geometry
+    rp, c, v = A.download_csr()
+    orp, oc, ov = orc.siddon_walk_csr(orc.siddon_op(g), np.float32)
+    print(f"cfg1 CSR: {A.m} rays, {len(c)} entries")
+    assert np.array_equal(rp, orp) and np.array_equal(c, oc)
+    assert np.array_equal(v.astype(np.float32), ov)
+
+
 def test_cfg1_plain_fp32_reproduces_fp32_oracle(ctx, cfg1):
     u, A, oop, b = cfg1
     dims = (64, 64, 64)

@@ -13,8 +13,11 @@
 //    increasing ray parameter by tracking the next plane of every axis
 //    (siddon.hpp).
 //
-// One thread walks one ray.  N gathers x along the ray (x stays resident in
-// L2 / MALL) and writes y[ray] once; T scatters x[ray]*w into y with FP32
+// FP32 products of a Siddon operator whose dimensions are multiples of 4
+// run as tile products (dyntile.hip, panels.hpp: deterministic, no
+// atomics).  The others -- Bresenham, FP64, other shapes -- run here: one
+// thread walks one ray; N gathers x along the ray (x stays resident in L2 /
+// MALL) and writes y[ray] once; T scatters x[ray]*w into y with FP32
 // hardware atomics (no replicated sub-volumes).  The same walk, with FP
 // contraction disabled, fills the CSR in dynamic_materialize so a
 // precomputed matrix holds exactly the weights the dynamic product uses.
