@@ -51,7 +51,9 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 __global__ __launch_bounds__(512) void k_gemm_rhs(const uint16_t *__restrict__ M, int64_t ldm,
                                                   int64_t R, int64_t K, int S,
                                                   const float *__restrict__ X, int64_t ldx,
-                                                  int G, float *__restrict__ part) {
+                                                  int G, float *__restrict__ part,
+                                                  const int32_t *gate) {
+    if (gate && !*gate) return;
     __shared__ __attribute__((aligned(16))) uint16_t xh[kCols][kPadK];
     __shared__ __attribute__((aligned(16))) uint16_t xm[kCols][kPadK];
     __shared__ __attribute__((aligned(16))) uint16_t xl[kCols][kPadK];
@@ -138,7 +140,9 @@ __global__ __launch_bounds__(512) void k_gemm_rhs(const uint16_t *__restrict__ M
 // transpose for column-major stores of 32 consecutive rows
 __global__ __launch_bounds__(256) void k_gemm_reduce(const float *__restrict__ part, int64_t R,
                                                      int S, int G, float *__restrict__ Y,
-                                                     int64_t ldy, const float *__restrict__ sub) {
+                                                     int64_t ldy, const float *__restrict__ sub,
+                                                     const int32_t *gate) {
+    if (gate && !*gate) return;
     __shared__ float tile[kCols][33];
     const int t = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.x * 32;
@@ -191,7 +195,7 @@ bool gemm_rhs_shape_ok(int64_t R, int64_t K, int splits) {
 
 void gemm_rhs(const uint16_t *M, int64_t ldm, int64_t R, int64_t K, const float *X, int64_t ldx,
               int G, float *Y, int64_t ldy, const float *sub, int splits, float *part,
-              hipStream_t s) {
+              hipStream_t s, const int32_t *gate) {
     TV3_REQUIRE(G > 0 && G <= kCols, TV3_E_ARG, "1 to 64 right-hand sides per product");
     TV3_REQUIRE(gemm_rhs_shape_ok(R, K, splits), TV3_E_SHAPE,
                 "dense multi-RHS products need rows a multiple of 256 and K a multiple of "
@@ -200,9 +204,9 @@ void gemm_rhs(const uint16_t *M, int64_t ldm, int64_t R, int64_t K, const float 
                 "dense multi-RHS products need 16-byte aligned rows");
     const unsigned grid = (unsigned)((R / kRows) * splits);
     hipLaunchKernelGGL(k_gemm_rhs, dim3(grid), dim3(kThreads), 0, s, M, ldm, R, K, splits, X,
-                       ldx, G, part);
+                       ldx, G, part, gate);
     hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)(R / 32)), dim3(256), 0, s, part, R, splits,
-                       G, Y, ldy, sub);
+                       G, Y, ldy, sub, gate);
 }
 
 size_t gemm_rhs_scratch(int64_t R, int splits) { return (size_t)splits * R * kCols; }

@@ -163,8 +163,9 @@ struct EpiBatch {
     int P;
     const FCtl *C;
     int pred;
+    const int32_t *gate = nullptr;   // device count; the launch is a no-op while it is 0
     struct Pre { float s; };
-    __device__ bool skip() const { return false; }
+    __device__ bool skip() const { return gate && !*gate; }
     __device__ __forceinline__ bool skip_rows(int row) const {
         return !pred_on(C[find_prob(starts, P, row / GB)], pred);
     }
@@ -184,8 +185,9 @@ struct EpiSegBatch {
     int P;                       // sparse problems
     const FCtl *C;
     int pred;
+    const int32_t *gate = nullptr;   // device count; the launch is a no-op while it is 0
     struct Pre {};
-    __device__ bool skip() const { return false; }
+    __device__ bool skip() const { return gate && !*gate; }
     __device__ __forceinline__ bool skip_rows(int row) const {
         return !pred_on(C[find_prob(seg_starts, P, row)], pred);
     }
@@ -467,7 +469,9 @@ __global__ __launch_bounds__(512) void k_b_update_w_tile(const BDims *__restrict
 __global__ __launch_bounds__(256) void k_b_dd_norm(const BDims *__restrict__ D,
                                                    const int32_t *__restrict__ vst, int P,
                                                    const FCtl *__restrict__ C,
-                                                   const float *__restrict__ d, double *part) {
+                                                   const float *__restrict__ d, double *part,
+                                                   const int32_t *gate) {
+    if (gate && !*gate) return;
     const int k = find_prob(vst, P, blockIdx.x);
     if (!C[k].need_tau) return;
     const BDims dm = D[k];
@@ -494,7 +498,9 @@ __global__ __launch_bounds__(256) void k_b_mlp_sigma(const BDims *__restrict__ D
                                                      float *__restrict__ sx, float *__restrict__ sy,
                                                      float *__restrict__ sz, float *__restrict__ ox,
                                                      float *__restrict__ oy, float *__restrict__ oz,
-                                                     double *part) {
+                                                     double *part,
+                                                     const int32_t *gate) {
+    if (gate && !*gate) return;
     const int k = find_prob(vst, P, blockIdx.x);
     const FCtl &c = C[k];
     if (!c.outer_flag) return;
@@ -528,7 +534,9 @@ __global__ __launch_bounds__(256) void k_b_outer_d(const BDims *__restrict__ D,
                                                    float *__restrict__ d,
                                                    const float *__restrict__ g0,
                                                    const float *__restrict__ g1,
-                                                   float *__restrict__ dts, double *part) {
+                                                   float *__restrict__ dts, double *part,
+                                                   const int32_t *gate) {
+    if (gate && !*gate) return;
     const int k = find_prob(vst, P, blockIdx.x);
     const FCtl &c = C[k];
     if (!c.outer_flag) return;
@@ -555,7 +563,9 @@ __global__ __launch_bounds__(256) void k_b_relout(const BDims *__restrict__ D,
                                                   const FCtl *__restrict__ C,
                                                   const float *__restrict__ U0,
                                                   const float *__restrict__ U1,
-                                                  float *__restrict__ rcd, double *part) {
+                                                  float *__restrict__ rcd, double *part,
+                                                  const int32_t *gate) {
+    if (gate && !*gate) return;
     const int k = find_prob(vst, P, blockIdx.x);
     const FCtl &c = C[k];
     if (!c.outer_flag) return;
@@ -648,7 +658,9 @@ __global__ __launch_bounds__(256) void k_b_skip_after(const BDims *__restrict__ 
 __global__ __launch_bounds__(256) void k_b_sqnorm(const BDims *__restrict__ D,
                                                   const int32_t *__restrict__ rst, int P,
                                                   const FCtl *__restrict__ C,
-                                                  const float *__restrict__ y, double *part) {
+                                                  const float *__restrict__ y, double *part,
+                                                  const int32_t *gate) {
+    if (gate && !*gate) return;
     const int k = find_prob(rst, P, blockIdx.x);
     if (!C[k].need_tau) return;
     const BDims dm = D[k];
@@ -664,7 +676,9 @@ __global__ __launch_bounds__(256) void k_b_mlp_delta(const BDims *__restrict__ D
                                                      const float *__restrict__ Au0,
                                                      const float *__restrict__ Au1,
                                                      const float *__restrict__ b,
-                                                     float *__restrict__ delta, double *part) {
+                                                     float *__restrict__ delta, double *part,
+                                                     const int32_t *gate) {
+    if (gate && !*gate) return;
     const int k = find_prob(rst, P, blockIdx.x);
     const FCtl &c = C[k];
     if (!c.outer_flag) return;
@@ -746,7 +760,9 @@ __global__ __launch_bounds__(256) void k_b_skip_decide(FCtl *C, const int32_t *_
 
 __global__ __launch_bounds__(256) void k_b_tau(FCtl *C, const int32_t *__restrict__ vst,
                                                const int32_t *__restrict__ rst, int nbV, int nbR,
-                                               const double *pD, const double *pA) {
+                                               const double *pD, const double *pA,
+                                               const int32_t *gate) {
+    if (gate && !*gate) return;
     FCtl *c = C + blockIdx.x;
     if (!c->need_tau) return;
     double a[1], b[1];
@@ -765,19 +781,31 @@ __global__ __launch_bounds__(256) void k_b_init(FCtl *C, const int32_t *__restri
     if (threadIdx.x == 0) init_step(c, a, w);
 }
 
+// relative change of U over the inner loop; below tol the problem stops
+// (:1210-1218)
 __global__ __launch_bounds__(256) void k_b_relout_scalar(FCtl *C, const int32_t *__restrict__ vst,
-                                                         int nbV, const double *p) {
+                                                         int nbV, const double *p, double tol,
+                                                         const int32_t *gate) {
+    if (gate && !*gate) return;
     FCtl *c = C + blockIdx.x;
     if (!c->outer_flag) return;
     double a[1];
     seg_sums<1>(p, nbV, vst[blockIdx.x], vst[blockIdx.x + 1], a);
-    if (threadIdx.x == 0) c->relout = sqrt(a[0]) / sqrt(c->nrmup2);
+    if (threadIdx.x == 0) {
+        c->relout = sqrt(a[0]) / sqrt(c->nrmup2);
+        if (c->relout < tol) {
+            c->outer_flag = 0;
+            c->done = 1;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_b_outer_scalars(FCtl *C, const int32_t *__restrict__ vst,
                                                          const int32_t *__restrict__ rst, int nbV,
                                                          int nbR, const double *p4,
-                                                         const double *p5) {
+                                                         const double *p5,
+                                                         const int32_t *gate) {
+    if (gate && !*gate) return;
     FCtl *c = C + blockIdx.x;
     if (!c->outer_flag) return;
     double a[1], b[1];
@@ -786,8 +814,12 @@ __global__ __launch_bounds__(256) void k_b_outer_scalars(FCtl *C, const int32_t 
     if (threadIdx.x == 0) outer_step(c, a[0], b[0]);
 }
 
+// end of the outer update: d.d for the next inner loop, then the loop
+// condition of :1166 (outer iterations and total inner iterations)
 __global__ __launch_bounds__(256) void k_b_set_dd(FCtl *C, const int32_t *__restrict__ vst,
-                                                  int nbV, const double *p) {
+                                                  int nbV, const double *p, int maxcnt, int maxit,
+                                                  const int32_t *gate) {
+    if (gate && !*gate) return;
     FCtl *c = C + blockIdx.x;
     if (!c->outer_flag) return;
     double a[1];
@@ -795,7 +827,79 @@ __global__ __launch_bounds__(256) void k_b_set_dd(FCtl *C, const int32_t *__rest
     if (threadIdx.x == 0) {
         c->dd = a[0];
         c->outer_flag = 0;
+        if (c->outer >= maxcnt || c->total >= maxit) {
+            c->done = 1;
+            c->active = 0;
+            c->need_tau = 0;
+        }
     }
+}
+
+// ---------------------------------------------------------- rounds on the device
+// A round advances every unfinished problem by one step of its own loop:
+// the outer update when its inner loop has ended, the first step of an inner
+// loop (or of a fallback) when it needs one, then an inner iteration.  The
+// counts below gate the launches of the rarer steps (a kernel whose gate is
+// 0 returns at once); per problem, each kernel still acts only on the
+// problems whose own flags ask for it.  Gate layout: [0] problems in their
+// outer update, [1] problems needing a first step, [2] unfinished problems,
+// [3 + g] problems of dense group g needing a first step.
+constexpr int kGateOuter = 0, kGateTau = 1, kGateAlive = 2, kGateGroup = 3;
+
+// classification at the start of a round (the host loop of :1166-1220 per
+// problem): an inner loop that has ended starts the outer update, a
+// non-finite objective stops the problem
+__global__ __launch_bounds__(256) void k_b_classify(FCtl *C, int P, int32_t *gates) {
+    __shared__ int32_t cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int k = threadIdx.x; k < P; k += blockDim.x) {
+        FCtl &c = C[k];
+        if (c.done || c.need_tau || c.active) continue;
+        if (c.nonfinite) {
+            c.done = 1;
+            continue;
+        }
+        c.outer += 1;
+        c.outer_flag = 1;
+        ++mine;
+    }
+    if (mine) atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) gates[kGateOuter] = cnt;
+}
+
+// after the outer update: problems, and problems of each dense group, that
+// need a first step (grp: first problem of each group, and the end)
+__global__ __launch_bounds__(256) void k_b_gate_tau(const FCtl *C, int P,
+                                                    const int32_t *__restrict__ grp, int ngroups,
+                                                    int32_t *gates) {
+    __shared__ int32_t cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int k = threadIdx.x; k < P; k += blockDim.x) mine += C[k].need_tau != 0;
+    if (mine) atomicAdd(&cnt, mine);
+    for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
+        int any = 0;
+        for (int k = grp[g]; k < grp[g + 1]; ++k) any |= C[k].need_tau != 0;
+        gates[kGateGroup + g] = any;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) gates[kGateTau] = cnt;
+}
+
+// unfinished problems after the last round of a launch
+__global__ __launch_bounds__(256) void k_b_alive(const FCtl *C, int P, int32_t *gates) {
+    __shared__ int32_t cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int k = threadIdx.x; k < P; k += blockDim.x) mine += C[k].done == 0;
+    if (mine) atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) gates[kGateAlive] = cnt;
 }
 
 // ------------------------------------------------------------------ set-up kernels
@@ -883,6 +987,9 @@ struct Batch {
     };
     std::vector<DGroup> groups;
     DevBuf<float> gpart;           // split-K partials of the group products
+    DevBuf<int32_t> grp;           // first problem of each dense group, then the end
+    DevBuf<int32_t> gates;         // device counts gating a round's launches
+    PinnedBuf<int32_t> halive;     // unfinished problems, read after each launch
 
     ~Batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -897,16 +1004,17 @@ struct Batch {
     }
 
     void product(const SparseRows &S, const float *x, float *y, const float *sub,
-                 const int32_t *starts, int pred) {
-        launch_product<float>(S, x, EpiBatch{y, sub, starts, P, ctl.p, pred}, s);
+                 const int32_t *starts, int pred, const int32_t *gate = nullptr) {
+        launch_product<float>(S, x, EpiBatch{y, sub, starts, P, ctl.p, pred, gate}, s);
     }
     // dense groups: every column is computed (the outputs are scratch, read
     // only for the problems whose predicate holds)
-    void group_N(const DGroup &g, const float *x, float *y, int splits) {
+    void group_N(const DGroup &g, const float *x, float *y, int splits,
+                 const int32_t *gate = nullptr) {
         for (int c = 0; c < g.G; c += 64)
             gemm_rhs((const uint16_t *)g.op->dense_rm.p, g.op->n, g.op->m, g.op->n,
                      x + g.v0 + c * g.ldn, g.ldn, std::min(64, g.G - c), y + g.r0 + c * g.ldm,
-                     g.ldm, nullptr, splits, gpart.p, s);
+                     g.ldm, nullptr, splits, gpart.p, s, gate);
     }
     void group_T(const DGroup &g, const float *x, float *y, const float *sub, int splits) {
         for (int c = 0; c < g.G; c += 64)
@@ -914,16 +1022,20 @@ struct Batch {
                      x + g.r0 + c * g.ldm, g.ldm, std::min(64, g.G - c), y + g.v0 + c * g.ldn,
                      g.ldn, sub ? sub + g.v0 + c * g.ldn : nullptr, splits, gpart.p, s);
     }
-    void prod_N(const float *x, float *y, int pred) {
+    // gate (optional): device count gating the sparse launches; ggate: one
+    // per dense group
+    void prod_N(const float *x, float *y, int pred, const int32_t *gate = nullptr,
+                const int32_t *ggate = nullptr) {
         if (A.rows) {
             if (panels_A) {
-                launch_panel(PA, x, EpiSegBatch{RS.part.p, segst.p, ncsr, ctl.p, pred}, s);
-                launch_seg_sum(RS, EpiBatch{y, nullptr, rst.p, P, ctl.p, pred}, s);
+                launch_panel(PA, x, EpiSegBatch{RS.part.p, segst.p, ncsr, ctl.p, pred, gate}, s);
+                launch_seg_sum(RS, EpiBatch{y, nullptr, rst.p, P, ctl.p, pred, gate}, s);
             } else {
-                product(A, x, y, nullptr, rst.p, pred);
+                product(A, x, y, nullptr, rst.p, pred, gate);
             }
         }
-        for (const DGroup &g : groups) group_N(g, x, y, g.sN);
+        for (size_t i = 0; i < groups.size(); ++i)
+            group_N(groups[i], x, y, groups[i].sN, ggate ? ggate + i : nullptr);
     }
     void prod_T(const float *x, float *y, const float *sub, int pred) {
         if (AT.rows) {
@@ -993,18 +1105,54 @@ struct Batch {
     }
 
     // first-iteration step lengths of the problems that need one
+    // first step of an inner loop or of a fallback (get_tau, :620-660) for
+    // the problems that need one; gated by the device counts
     void tau_pass() {
         const dim3 bV(nbV), bR(nbR), t(256);
-        hipLaunchKernelGGL(k_b_dd_norm, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, d, pDd);
-        prod_N(d, AuT, P_TAU);
-        hipLaunchKernelGGL(k_b_sqnorm, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, AuT, pAd);
-        hipLaunchKernelGGL(k_b_tau, dim3(P), t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, pDd, pAd);
+        const int32_t *gt = gates.p + kGateTau;
+        hipLaunchKernelGGL(k_b_dd_norm, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, d, pDd, gt);
+        prod_N(d, AuT, P_TAU, gt, gates.p + kGateGroup);
+        hipLaunchKernelGGL(k_b_sqnorm, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, AuT, pAd, gt);
+        hipLaunchKernelGGL(k_b_tau, dim3(P), t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, pDd, pAd, gt);
     }
 
+    // outer update of the problems whose inner loop has ended (:1170-1220),
+    // gated by the device count of such problems
+    void outer_update() {
+        const dim3 bV(nbV), bR(nbR), bP(P), t(256);
+        const int32_t *go = gates.p + kGateOuter;
+        hipLaunchKernelGGL(k_b_relout, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, U[0], U[1], rcd, pRO,
+                           go);
+        hipLaunchKernelGGL(k_b_relout_scalar, bP, t, 0, s, ctl.p, vst.p, nbV, pRO, (double)o->tol,
+                           go);
+        hipLaunchKernelGGL(k_b_mlp_sigma, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, U[0], U[1], sx, sy,
+                           sz, ox, oy, oz, p4, go);
+        hipLaunchKernelGGL(k_b_mlp_delta, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, Au[0], Au[1], b,
+                           delta, p5, go);
+        hipLaunchKernelGGL(k_b_outer_scalars, bP, t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, p4, p5,
+                           go);
+        hipLaunchKernelGGL(k_b_outer_d, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, d, g[0], g[1], dts,
+                           pOD, go);
+        hipLaunchKernelGGL(k_b_set_dd, bP, t, 0, s, ctl.p, vst.p, nbV, pOD, (int)o->maxcnt,
+                           (int)o->maxit, go);
+    }
+
+    // one round: every unfinished problem takes one step of its own loop
+    void round() {
+        hipLaunchKernelGGL(k_b_classify, dim3(1), dim3(256), 0, s, ctl.p, P, gates.p);
+        outer_update();
+        hipLaunchKernelGGL(k_b_gate_tau, dim3(1), dim3(256), 0, s, ctl.p, P, grp.p,
+                           (int)groups.size(), gates.p);
+        tau_pass();
+        iteration();
+    }
+
+    // graph_iters rounds per launch, then the count of unfinished problems
     void capture() {
         hipGraph_t gr;
         TV3_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < graph_iters; ++k) iteration();
+        for (int k = 0; k < graph_iters; ++k) round();
+        hipLaunchKernelGGL(k_b_alive, dim3(1), dim3(256), 0, s, ctl.p, P, gates.p);
         TV3_HIP(hipStreamEndCapture(s, &gr));
         TV3_HIP(hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0));
         TV3_HIP(hipGraphDestroy(gr));
@@ -1346,6 +1494,21 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
     B.hist.alloc((size_t)P * B.hist_stride);
     B.ctl.alloc(P);
     B.hctl.alloc(P);
+    {   // round gates and the dense groups' problem ranges
+        std::vector<int32_t> gk;
+        for (size_t i = 0; i < B.groups.size(); ++i) {
+            TV3_REQUIRE(i == 0 || B.groups[i].k0 == B.groups[i - 1].k0 + B.groups[i - 1].G,
+                        TV3_E_INTERNAL, "dense groups are consecutive");
+            gk.push_back(B.groups[i].k0);
+        }
+        gk.push_back(B.groups.empty() ? P : B.groups.back().k0 + B.groups.back().G);
+        B.grp.alloc(gk.size());
+        TV3_HIP(hipMemcpyAsync(B.grp.p, gk.data(), sizeof(int32_t) * gk.size(),
+                               hipMemcpyHostToDevice, s));
+        B.gates.alloc(kGateGroup + B.groups.size());
+        TV3_HIP(hipMemsetAsync(B.gates.p, 0, B.gates.bytes(), s));
+        B.halive.alloc(1);
+    }
 
     hipEvent_t ev0, ev1, evl0, evl1;
     TV3_HIP(hipEventCreate(&ev0)); TV3_HIP(hipEventCreate(&ev1));
@@ -1448,73 +1611,25 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         B.write_ctl();
     }
 
-    // per-problem loop state lives in the control blocks; the host only
-    // classifies problems after each launch and runs the outer updates
+    // every round of a launch advances each unfinished problem by one step
+    // (outer update, first step, inner iteration: one problem's next step
+    // never waits for another's); after a launch the host reads one count.
+    // A problem takes at most maxit inner iterations, maxcnt outer updates
+    // and one first step per inner iteration, so the rounds are bounded.
     int graph_launches = 0;
-    B.read_ctl();
+    const int64_t max_rounds = 2 * ((int64_t)std::max(o.maxit, 0) + 1) *
+                                   ((int64_t)std::max(o.maxcnt, 0) + 2) + 16;
     for (;;) {
-        bool any_tau = false, any_active = false, any_outer = false;
-        for (int k = 0; k < P; ++k) {
-            FCtl &c = H[k];
-            if (c.done) continue;
-            if (c.need_tau) { any_tau = true; continue; }
-            if (c.active) { any_active = true; continue; }
-            if (c.nonfinite) { c.done = 1; continue; }
-            // the inner loop has ended (:1170-1200): outer update
-            c.outer += 1;
-            c.outer_flag = 1;
-            any_outer = true;
-        }
-        if (any_outer) {
-            B.write_ctl();
-            hipLaunchKernelGGL(k_b_relout, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P,
-                               B.ctl.p, B.U[0], B.U[1], B.rcd, B.pRO);
-            hipLaunchKernelGGL(k_b_relout_scalar, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p,
-                               B.nbV, B.pRO);
-            B.read_ctl();
-            for (int k = 0; k < P; ++k) {
-                FCtl &c = H[k];
-                if (!c.outer_flag) continue;
-                if (c.relout < o.tol) {          // :1218
-                    c.outer_flag = 0;
-                    c.done = 1;
-                }
-            }
-            B.write_ctl();
-            hipLaunchKernelGGL(k_b_mlp_sigma, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P,
-                               B.ctl.p, B.U[0], B.U[1], B.sx, B.sy, B.sz, B.ox, B.oy, B.oz, B.p4);
-            hipLaunchKernelGGL(k_b_mlp_delta, dim3(B.nbR), dim3(256), 0, s, B.ddims.p, B.rst.p, P,
-                               B.ctl.p, B.Au[0], B.Au[1], B.b, B.delta, B.p5);
-            hipLaunchKernelGGL(k_b_outer_scalars, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p,
-                               B.rst.p, B.nbV, B.nbR, B.p4, B.p5);
-            hipLaunchKernelGGL(k_b_outer_d, dim3(B.nbV), dim3(256), 0, s, B.ddims.p, B.vst.p, P,
-                               B.ctl.p, B.d, B.g[0], B.g[1], B.dts, B.pOD);
-            hipLaunchKernelGGL(k_b_set_dd, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p, B.nbV,
-                               B.pOD);
-            B.read_ctl();
-            for (int k = 0; k < P; ++k) {      // loop condition of :1166
-                FCtl &c = H[k];
-                if (c.done) continue;
-                if (c.outer >= o.maxcnt || c.total >= o.maxit) {
-                    c.done = 1;
-                    c.active = 0;
-                    c.need_tau = 0;
-                }
-            }
-            B.write_ctl();
-            continue;
-        }
-        if (any_tau) {                 // first iteration of an inner loop or a fallback
-            B.tau_pass();
-            B.iteration();
-        } else if (any_active) {
-            TV3_HIP(hipGraphLaunch(B.graph, s));
-            ++graph_launches;
-        } else {
-            break;
-        }
-        B.read_ctl();
+        TV3_REQUIRE((int64_t)graph_launches * B.graph_iters <= max_rounds, TV3_E_INTERNAL,
+                    "batched solve did not finish within its iteration limits");
+        TV3_HIP(hipGraphLaunch(B.graph, s));
+        ++graph_launches;
+        TV3_HIP(hipMemcpyAsync(B.halive.p, B.gates.p + kGateAlive, sizeof(int32_t),
+                               hipMemcpyDeviceToHost, s));
+        TV3_HIP(hipStreamSynchronize(s));
+        if (*B.halive.p == 0) break;
     }
+    B.read_ctl();
     TV3_HIP(hipEventRecord(evl1, s));
     for (int k = 0; k < P; ++k) {
         const BDims &bd = B.dims[k];

@@ -352,12 +352,14 @@ void dense_gemv_ws(tv3_op *op, const float *x, float *y, hipStream_t s, void *ws
 void noalloc_setup(int device);
 
 // dense multi-right-hand-side products on the matrix cores (dense_mfma.hip):
-// Y = M X (- sub), M row-major BF16 R x K, X/Y column-major, G <= 64
+// Y = M X (- sub), M row-major BF16 R x K, X/Y column-major, G <= 64; a
+// non-null gate in device memory turns the launch into a no-op while *gate
+// is 0 (graph-captured launches that only some rounds need)
 int gemm_rhs_rows();   // rows per workgroup (R must be a multiple)
 bool gemm_rhs_shape_ok(int64_t R, int64_t K, int splits);
 void gemm_rhs(const uint16_t *M, int64_t ldm, int64_t R, int64_t K, const float *X, int64_t ldx,
               int G, float *Y, int64_t ldy, const float *sub, int splits, float *part,
-              hipStream_t s);
+              hipStream_t s, const int32_t *gate = nullptr);
 size_t gemm_rhs_scratch(int64_t R, int splits);   // floats
 void dense_rowmajor(tv3_op *op, hipStream_t s);     // builds op->dense_rm once
 
