@@ -470,22 +470,27 @@ __global__ __launch_bounds__(256) void k_b_dd_norm(const BDims *__restrict__ D,
                                                    const int32_t *__restrict__ vst, int P,
                                                    const FCtl *__restrict__ C,
                                                    const float *__restrict__ d, double *part,
-                                                   const int32_t *gate) {
+                                                   int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    const int k = find_prob(vst, P, blockIdx.x);
-    if (!C[k].need_tau) return;
-    const BDims dm = D[k];
-    double acc[1] = {0};
-    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
-    Vox v;
-    if (voxel_at(dm, i, v)) {
-        const float d0 = d[i];
-        const float a = d[v.at(nxt(v.y, v.p), v.x, v.z)] - d0,
-                    b = d[v.at(v.y, nxt(v.x, v.q), v.z)] - d0,
-                    c = d[v.at(v.y, v.x, nxt(v.z, v.r))] - d0;
-        acc[0] += (double)a * a + (double)b * b + (double)c * c;
+    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
+    // off ends after that many workgroups, not ng
+    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
+        const int k = find_prob(vst, P, grp);
+        if (!C[k].need_tau) continue;
+        const BDims dm = D[k];
+        double acc[1] = {0};
+        const int64_t i = (int64_t)grp * GB + threadIdx.x;
+        Vox v;
+        if (voxel_at(dm, i, v)) {
+            const float d0 = d[i];
+            const float a = d[v.at(nxt(v.y, v.p), v.x, v.z)] - d0,
+                        b = d[v.at(v.y, nxt(v.x, v.q), v.z)] - d0,
+                        c = d[v.at(v.y, v.x, nxt(v.z, v.r))] - d0;
+            acc[0] += (double)a * a + (double)b * b + (double)c * c;
+        }
+        block_sum_store_at<1>(acc, part, ng, grp);
+        __syncthreads();   // the block sum's LDS is rewritten by the next group
     }
-    block_sum_store<1>(acc, part, gridDim.x);
 }
 
 // update_mlp, sigma part (:1036-1069): keeps the old sigma for the W of the
@@ -499,32 +504,37 @@ __global__ __launch_bounds__(256) void k_b_mlp_sigma(const BDims *__restrict__ D
                                                      float *__restrict__ sz, float *__restrict__ ox,
                                                      float *__restrict__ oy, float *__restrict__ oz,
                                                      double *part,
-                                                     const int32_t *gate) {
+                                                     int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    const int k = find_prob(vst, P, blockIdx.x);
-    const FCtl &c = C[k];
-    if (!c.outer_flag) return;
-    const BDims dm = D[k];
-    const float *U = c.cur ? U1 : U0;
-    double acc[1] = {0};
-    const float beta = (float)c.beta, ib = 1.f / beta;
-    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
-    Vox v;
-    if (voxel_at(dm, i, v)) {
-        const float u0 = U[i];
-        const float dy = U[v.at(nxt(v.y, v.p), v.x, v.z)] - u0,
-                    dx = U[v.at(v.y, nxt(v.x, v.q), v.z)] - u0,
-                    dz = U[v.at(v.y, v.x, nxt(v.z, v.r))] - u0;
-        const float a = sx[i], b = sy[i], cc = sz[i];
-        float wx, wy, wz;
-        shrink3(dx, dy, dz, a, b, cc, ib, c.iso, wx, wy, wz);
-        const float vx = dx - wx, vy = dy - wy, vz = dz - wz;
-        const float na = a - beta * vx, nb = b - beta * vy, nc = cc - beta * vz;
-        ox[i] = a; oy[i] = b; oz[i] = cc;
-        sx[i] = na; sy[i] = nb; sz[i] = nc;
-        acc[0] += (double)na * vx + (double)nb * vy + (double)nc * vz;
+    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
+    // off ends after that many workgroups, not ng
+    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
+        const int k = find_prob(vst, P, grp);
+        const FCtl &c = C[k];
+        if (!c.outer_flag) continue;
+        const BDims dm = D[k];
+        const float *U = c.cur ? U1 : U0;
+        double acc[1] = {0};
+        const float beta = (float)c.beta, ib = 1.f / beta;
+        const int64_t i = (int64_t)grp * GB + threadIdx.x;
+        Vox v;
+        if (voxel_at(dm, i, v)) {
+            const float u0 = U[i];
+            const float dy = U[v.at(nxt(v.y, v.p), v.x, v.z)] - u0,
+                        dx = U[v.at(v.y, nxt(v.x, v.q), v.z)] - u0,
+                        dz = U[v.at(v.y, v.x, nxt(v.z, v.r))] - u0;
+            const float a = sx[i], b = sy[i], cc = sz[i];
+            float wx, wy, wz;
+            shrink3(dx, dy, dz, a, b, cc, ib, c.iso, wx, wy, wz);
+            const float vx = dx - wx, vy = dy - wy, vz = dz - wz;
+            const float na = a - beta * vx, nb = b - beta * vy, nc = cc - beta * vz;
+            ox[i] = a; oy[i] = b; oz[i] = cc;
+            sx[i] = na; sy[i] = nb; sz[i] = nc;
+            acc[0] += (double)na * vx + (double)nb * vy + (double)nc * vz;
+        }
+        block_sum_store_at<1>(acc, part, ng, grp);
+        __syncthreads();   // the block sum's LDS is rewritten by the next group
     }
-    block_sum_store<1>(acc, part, gridDim.x);
 }
 
 // outer update of d and DtsAtd (:1239-1246) without a stored g2
@@ -535,26 +545,31 @@ __global__ __launch_bounds__(256) void k_b_outer_d(const BDims *__restrict__ D,
                                                    const float *__restrict__ g0,
                                                    const float *__restrict__ g1,
                                                    float *__restrict__ dts, double *part,
-                                                   const int32_t *gate) {
+                                                   int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    const int k = find_prob(vst, P, blockIdx.x);
-    const FCtl &c = C[k];
-    if (!c.outer_flag) return;
-    const BDims dm = D[k];
-    const float *g = c.cur ? g1 : g0;
-    double acc[1] = {0};
-    const float ratio = (float)c.ratio, mo = (float)c.mdb_old, mn = (float)c.muDbeta;
-    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
-    if (i - dm.v0 < dm.n) {
-        const float dv = d[i], gv = g[i];
-        const float g2 = dv - mo * gv - dts[i];
-        const float nd = ratio * dv;
-        const float dn = mn * gv + g2 + nd;
-        dts[i] = nd;
-        d[i] = dn;
-        acc[0] += (double)dn * dn;
+    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
+    // off ends after that many workgroups, not ng
+    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
+        const int k = find_prob(vst, P, grp);
+        const FCtl &c = C[k];
+        if (!c.outer_flag) continue;
+        const BDims dm = D[k];
+        const float *g = c.cur ? g1 : g0;
+        double acc[1] = {0};
+        const float ratio = (float)c.ratio, mo = (float)c.mdb_old, mn = (float)c.muDbeta;
+        const int64_t i = (int64_t)grp * GB + threadIdx.x;
+        if (i - dm.v0 < dm.n) {
+            const float dv = d[i], gv = g[i];
+            const float g2 = dv - mo * gv - dts[i];
+            const float nd = ratio * dv;
+            const float dn = mn * gv + g2 + nd;
+            dts[i] = nd;
+            d[i] = dn;
+            acc[0] += (double)dn * dn;
+        }
+        block_sum_store_at<1>(acc, part, ng, grp);
+        __syncthreads();   // the block sum's LDS is rewritten by the next group
     }
-    block_sum_store<1>(acc, part, gridDim.x);
 }
 
 // |U - rcdU|^2 then rcdU = U (:1208-1217)
@@ -564,21 +579,26 @@ __global__ __launch_bounds__(256) void k_b_relout(const BDims *__restrict__ D,
                                                   const float *__restrict__ U0,
                                                   const float *__restrict__ U1,
                                                   float *__restrict__ rcd, double *part,
-                                                  const int32_t *gate) {
+                                                  int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    const int k = find_prob(vst, P, blockIdx.x);
-    const FCtl &c = C[k];
-    if (!c.outer_flag) return;
-    const BDims dm = D[k];
-    const float *U = c.cur ? U1 : U0;
-    double acc[1] = {0};
-    const int64_t i = (int64_t)blockIdx.x * GB + threadIdx.x;
-    if (i - dm.v0 < dm.n) {
-        const float u = U[i], t = u - rcd[i];
-        acc[0] += (double)t * t;
-        rcd[i] = u;
+    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
+    // off ends after that many workgroups, not ng
+    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
+        const int k = find_prob(vst, P, grp);
+        const FCtl &c = C[k];
+        if (!c.outer_flag) continue;
+        const BDims dm = D[k];
+        const float *U = c.cur ? U1 : U0;
+        double acc[1] = {0};
+        const int64_t i = (int64_t)grp * GB + threadIdx.x;
+        if (i - dm.v0 < dm.n) {
+            const float u = U[i], t = u - rcd[i];
+            acc[0] += (double)t * t;
+            rcd[i] = u;
+        }
+        block_sum_store_at<1>(acc, part, ng, grp);
+        __syncthreads();   // the block sum's LDS is rewritten by the next group
     }
-    block_sum_store<1>(acc, part, gridDim.x);
 }
 
 // ------------------------------------------------------------------ rays
@@ -659,15 +679,20 @@ __global__ __launch_bounds__(256) void k_b_sqnorm(const BDims *__restrict__ D,
                                                   const int32_t *__restrict__ rst, int P,
                                                   const FCtl *__restrict__ C,
                                                   const float *__restrict__ y, double *part,
-                                                  const int32_t *gate) {
+                                                  int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    const int k = find_prob(rst, P, blockIdx.x);
-    if (!C[k].need_tau) return;
-    const BDims dm = D[k];
-    double acc[1] = {0};
-    const int64_t r = (int64_t)blockIdx.x * GB + threadIdx.x;
-    if (r - dm.r0 < dm.m) acc[0] = (double)y[r] * y[r];
-    block_sum_store<1>(acc, part, gridDim.x);
+    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
+    // off ends after that many workgroups, not ng
+    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
+        const int k = find_prob(rst, P, grp);
+        if (!C[k].need_tau) continue;
+        const BDims dm = D[k];
+        double acc[1] = {0};
+        const int64_t r = (int64_t)grp * GB + threadIdx.x;
+        if (r - dm.r0 < dm.m) acc[0] = (double)y[r] * y[r];
+        block_sum_store_at<1>(acc, part, ng, grp);
+        __syncthreads();   // the block sum's LDS is rewritten by the next group
+    }
 }
 
 __global__ __launch_bounds__(256) void k_b_mlp_delta(const BDims *__restrict__ D,
@@ -677,23 +702,28 @@ __global__ __launch_bounds__(256) void k_b_mlp_delta(const BDims *__restrict__ D
                                                      const float *__restrict__ Au1,
                                                      const float *__restrict__ b,
                                                      float *__restrict__ delta, double *part,
-                                                     const int32_t *gate) {
+                                                     int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    const int k = find_prob(rst, P, blockIdx.x);
-    const FCtl &c = C[k];
-    if (!c.outer_flag) return;
-    const BDims dm = D[k];
-    const float *Au = c.cur ? Au1 : Au0;
-    double acc[1] = {0};
-    const float mu = (float)c.mu;
-    const int64_t r = (int64_t)blockIdx.x * GB + threadIdx.x;
-    if (r - dm.r0 < dm.m) {
-        const float rv = Au[r] - b[r];
-        const float dn = delta[r] - mu * rv;
-        delta[r] = dn;
-        acc[0] += (double)dn * rv;
+    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
+    // off ends after that many workgroups, not ng
+    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
+        const int k = find_prob(rst, P, grp);
+        const FCtl &c = C[k];
+        if (!c.outer_flag) continue;
+        const BDims dm = D[k];
+        const float *Au = c.cur ? Au1 : Au0;
+        double acc[1] = {0};
+        const float mu = (float)c.mu;
+        const int64_t r = (int64_t)grp * GB + threadIdx.x;
+        if (r - dm.r0 < dm.m) {
+            const float rv = Au[r] - b[r];
+            const float dn = delta[r] - mu * rv;
+            delta[r] = dn;
+            acc[0] += (double)dn * rv;
+        }
+        block_sum_store_at<1>(acc, part, ng, grp);
+        __syncthreads();   // the block sum's LDS is rewritten by the next group
     }
-    block_sum_store<1>(acc, part, gridDim.x);
 }
 
 // accepted Au: Au_new = Au_c + alpha (Au_t - Au_c), Au_t itself when alpha = 1
@@ -1108,31 +1138,34 @@ struct Batch {
     // first step of an inner loop or of a fallback (get_tau, :620-660) for
     // the problems that need one; gated by the device counts
     void tau_pass() {
-        const dim3 bV(nbV), bR(nbR), t(256);
+        const dim3 bV(round_grid(nbV)), bR(round_grid(nbR)), t(256);
         const int32_t *gt = gates.p + kGateTau;
-        hipLaunchKernelGGL(k_b_dd_norm, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, d, pDd, gt);
+        hipLaunchKernelGGL(k_b_dd_norm, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, d, pDd, nbV, gt);
         prod_N(d, AuT, P_TAU, gt, gates.p + kGateGroup);
-        hipLaunchKernelGGL(k_b_sqnorm, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, AuT, pAd, gt);
+        hipLaunchKernelGGL(k_b_sqnorm, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, AuT, pAd, nbR, gt);
         hipLaunchKernelGGL(k_b_tau, dim3(P), t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, pDd, pAd, gt);
     }
+
+    // grid of the gated group kernels: they loop over their groups
+    static unsigned round_grid(int ng) { return (unsigned)std::max(1, std::min(ng, 2048)); }
 
     // outer update of the problems whose inner loop has ended (:1170-1220),
     // gated by the device count of such problems
     void outer_update() {
-        const dim3 bV(nbV), bR(nbR), bP(P), t(256);
+        const dim3 bV(round_grid(nbV)), bR(round_grid(nbR)), bP(P), t(256);
         const int32_t *go = gates.p + kGateOuter;
         hipLaunchKernelGGL(k_b_relout, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, U[0], U[1], rcd, pRO,
-                           go);
+                           nbV, go);
         hipLaunchKernelGGL(k_b_relout_scalar, bP, t, 0, s, ctl.p, vst.p, nbV, pRO, (double)o->tol,
                            go);
         hipLaunchKernelGGL(k_b_mlp_sigma, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, U[0], U[1], sx, sy,
-                           sz, ox, oy, oz, p4, go);
+                           sz, ox, oy, oz, p4, nbV, go);
         hipLaunchKernelGGL(k_b_mlp_delta, bR, t, 0, s, ddims.p, rst.p, P, ctl.p, Au[0], Au[1], b,
-                           delta, p5, go);
+                           delta, p5, nbR, go);
         hipLaunchKernelGGL(k_b_outer_scalars, bP, t, 0, s, ctl.p, vst.p, rst.p, nbV, nbR, p4, p5,
                            go);
         hipLaunchKernelGGL(k_b_outer_d, bV, t, 0, s, ddims.p, vst.p, P, ctl.p, d, g[0], g[1], dts,
-                           pOD, go);
+                           pOD, nbV, go);
         hipLaunchKernelGGL(k_b_set_dd, bP, t, 0, s, ctl.p, vst.p, nbV, pOD, (int)o->maxcnt,
                            (int)o->maxit, go);
     }
