@@ -13,6 +13,9 @@ of launches per iteration.
 - The A^T-skip heuristic decides per problem: each problem against the
   oracle running the same rule, batch against solo bitwise, and a rule that
   never skips against the run without it, bitwise.
+- The context keeps the last batch (layouts, launch shapes, graph): a call
+  with the same operators, shapes and captured options and new data reuses
+  it and gives the bits a fresh build gives.
 """
 import numpy as np
 import pytest
@@ -371,3 +374,26 @@ def test_batch_continuation_tracks_oracle(ctx, batch8):
         runs = _envelope_runs(p["oop"], p["dims"], p["b"], oo, p["u"])
         kk = check_continuation_tracks(hb[k], ho, runs, f"batch {p['dims']}")
         assert kk >= 15
+
+
+def test_batch_session_reuse_equals_fresh_build(ctx, batch):
+    """Three calls with the same operators and shapes: b, then 0.7 b (the
+    session is reused), then 0.7 b with a different graph size (a captured
+    option: the session is rebuilt; graph size does not change results).
+    The reused call equals the rebuilt one bit for bit, and the first call
+    equals a repeat of itself."""
+    import torch
+    o8, o4 = opts(graph_iters=8), opts(graph_iters=4)
+    U1, i1, _, h1 = solve(ctx, batch, o8, lanes=(-2048, -1024), panels=2048, panels_A=1024)
+    scaled = [dict(p, bt=(0.7 * p["bt"]).contiguous()) for p in batch]
+    U2, i2, b2, h2 = solve(ctx, scaled, o8, lanes=(-2048, -1024), panels=2048, panels_A=1024)
+    U3, i3, b3, h3 = solve(ctx, scaled, o4, lanes=(-2048, -1024), panels=2048, panels_A=1024)
+    U4, i4, _, h4 = solve(ctx, batch, o4, lanes=(-2048, -1024), panels=2048, panels_A=1024)
+    print("host seconds: reused", b2["host_secs"], "rebuilt", b3["host_secs"])
+    for k in range(len(batch)):
+        assert torch.equal(U2[k], U3[k])
+        assert i2[k]["total_iters"] == i3[k]["total_iters"] > 0
+        assert (h2[k]["f"] == h3[k]["f"]).all()
+        assert torch.equal(U1[k], U4[k])
+        assert (h1[k]["f"] == h4[k]["f"]).all()
+        assert not torch.equal(U1[k], U2[k])            # the data did change

@@ -947,18 +947,74 @@ __global__ void k_concat_ind(const int32_t *__restrict__ ind, int64_t nnz, int64
         out[j] = (int32_t)(ind[j] + off);
 }
 
-// out[i] = s in[map[i]] (natural -> internal order, scaled)
+// out[i] = s in[map[i]] (natural -> internal order, scaled; s in device memory)
 __global__ void k_gather_scale(const float *__restrict__ in, const int32_t *__restrict__ map,
-                               int64_t n, float s, float *__restrict__ out) {
+                               int64_t n, const float *__restrict__ sp, float *__restrict__ out) {
+    const float s = *sp;
     for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
         out[j] = s * in[map ? map[j] : j];
 }
 
 // out[map[i]] = in[i] / s (internal -> natural order, unscaled, :1275-1277)
 __global__ void k_scatter_unscale(const float *__restrict__ in, const int32_t *__restrict__ map,
-                                  int64_t n, float s, float *__restrict__ out) {
+                                  int64_t n, const float *__restrict__ sp,
+                                  float *__restrict__ out) {
+    const float s = *sp;
     for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256)
         out[map ? map[j] : j] = in[j] / s;
+}
+
+// scaleb (:225-266), one workgroup per problem: the entries of least and of
+// largest magnitude (the first such in index order, as a sequential scan
+// finds them), their distance dif, the factor 0.5 / dif below 0.5 and
+// 1.5 / dif above 1.5; *bad set when b holds NaN or Inf
+__global__ __launch_bounds__(1024) void k_b_scaleb(const float *const *__restrict__ bptr,
+                                                   const BDims *__restrict__ D,
+                                                   float *__restrict__ scl,
+                                                   int32_t *__restrict__ bad) {
+    __shared__ float sa[1024], sb[1024];
+    __shared__ int64_t ia[1024], ib[1024];
+    __shared__ int32_t snf;
+    const int k = blockIdx.x, t = threadIdx.x;
+    const float *b = bptr[k];
+    const int64_t m = D[k].m;
+    float amin = INFINITY, amax = -1.f;
+    int64_t imin = INT64_MAX, imax = INT64_MAX;
+    int nf = 0;
+    for (int64_t j = t; j < m; j += 1024) {
+        const float v = b[j], a = fabsf(v);
+        nf |= !isfinite(v);
+        if (a < amin) { amin = a; imin = j; }
+        if (a > amax) { amax = a; imax = j; }
+    }
+    if (t == 0) snf = 0;
+    sa[t] = amin; ia[t] = imin; sb[t] = amax; ib[t] = imax;
+    __syncthreads();
+    if (nf) atomicOr(&snf, 1);
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) {
+            if (sa[t + w] < sa[t] || (sa[t + w] == sa[t] && ia[t + w] < ia[t])) {
+                sa[t] = sa[t + w];
+                ia[t] = ia[t + w];
+            }
+            if (sb[t + w] > sb[t] || (sb[t + w] == sb[t] && ib[t + w] < ib[t])) {
+                sb[t] = sb[t + w];
+                ib[t] = ib[t + w];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        float s = 1.f;
+        if (snf) {
+            atomicOr(bad, 1);
+        } else if (m > 0) {
+            const float dif = fabsf(b[ib[0]] - b[ia[0]]);
+            if (dif < 0.5f) s = 0.5f / dif;
+            else if (dif > 1.5f) s = 1.5f / dif;
+        }
+        scl[k] = s;
+    }
 }
 
 inline int64_t pad256(int64_t v) { return (v + GB - 1) / GB * GB; }
@@ -1018,6 +1074,10 @@ struct Batch {
     std::vector<DGroup> groups;
     DevBuf<float> gpart;           // split-K partials of the group products
     DevBuf<int32_t> grp;           // first problem of each dense group, then the end
+    DevBuf<float> scl;             // per problem: the scaleb factor of the current solve
+    DevBuf<const float *> bptr;    // per problem: the caller's b of the current solve
+    DevBuf<int32_t> bad;           // the current b holds NaN or Inf
+    PinnedBuf<int32_t> hbad;
     DevBuf<int32_t> gates;         // device counts gating a round's launches
     PinnedBuf<int32_t> halive;     // unfinished problems, read after each launch
 
@@ -1370,18 +1430,67 @@ void concat_matrix(tv3_ctx *ctx, hipStream_t s, const std::vector<const SparseRo
 
 }  // namespace
 
-void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *dims_in,
-                   const tv3_opts &o, const float *const *b_in, float *const *U_in,
-                   const float *const *Ut_in, tv3_info *infos, tv3_history *hists,
-                   tv3_batch_info *binfo) {
-    auto t_host0 = std::chrono::steady_clock::now();
+// The last batched solve of a context, reused by the next call that has the
+// same operators (by serial number: a destroyed operator never matches, and
+// the session's buffers stay until the next batched call or the context's
+// end), the same volume shapes, the same options captured in the graph
+// (tol, maxcnt, maxit, whether the A^T-skip rule runs, graph size) and any
+// launch shapes the call fixes.
+struct BatchSession : CtxSession {
+    std::vector<uint64_t> uids;      // caller's order
+    std::vector<int32_t> dims;       // caller's order, three per problem
+    double tol = 0;
+    int32_t maxcnt = 0, maxit = 0;
+    tv3_opts oc;                     // the current call's options (Batch::o points here)
     Batch B;
-    B.ctx = ctx; B.s = ctx->stream; B.o = &o; B.P = P;
+    std::vector<std::unique_ptr<FusedLayout>> lay;   // internal order
+    std::vector<int> orig;           // orig[k]: caller's index of internal problem k
+};
+
+static bool session_matches(const BatchSession &S, int P, tv3_op *const *ops,
+                            const int32_t *dims, const tv3_opts &o,
+                            const tv3_batch_info *binfo) {
+    if ((int)S.uids.size() != P) return false;
+    for (int k = 0; k < P; ++k)
+        if (!ops[k] || ops[k]->uid != S.uids[k]) return false;
+    if (!std::equal(S.dims.begin(), S.dims.end(), dims)) return false;
+    const Batch &B = S.B;
+    if (S.tol != o.tol || S.maxcnt != o.maxcnt || S.maxit != o.maxit) return false;
+    if (B.skipping != (o.skipMulRatio != 0.0)) return false;
+    if ((o.graph_iters > 0 ? o.graph_iters : 8) != B.graph_iters) return false;
+    if (!binfo) return true;
+    if (binfo->lanes_N != 0 && binfo->lanes_T != 0 &&
+        (binfo->lanes_N != B.lanes_N || binfo->lanes_T != B.lanes_T))
+        return false;
+    auto panels_ok = [](int want, int have) { return want == 0 || (want < 0 ? 0 : want) == have; };
+    if (!panels_ok(binfo->panels_T, B.panels_T) || !panels_ok(binfo->panels_A, B.panels_A))
+        return false;
+    if (binfo->splits_N > 0 && binfo->splits_T > 0)
+        for (const auto &g : B.groups)
+            if (g.sN != binfo->splits_N || g.sT != binfo->splits_T) return false;
+    return true;
+}
+
+// Everything of a batched solve that depends only on its operators, volume
+// shapes and the options its graph captures: ranges, the fused numbering of
+// the sparse problems and their block-diagonal matrices, dense groups,
+// buffers, launch shapes (timed unless given), panel forms, the graph.
+static std::unique_ptr<BatchSession> build_session(tv3_ctx *ctx, int P, tv3_op *const *ops_in,
+                                                   const int32_t *dims_in, const tv3_opts &o,
+                                                   const tv3_batch_info *binfo) {
+    std::unique_ptr<BatchSession> SP(new BatchSession());
+    BatchSession &S = *SP;
+    S.oc = o;
+    for (int k = 0; k < P; ++k) S.uids.push_back(ops_in[k] ? ops_in[k]->uid : 0);
+    S.dims.assign(dims_in, dims_in + 3 * P);
+    S.tol = o.tol; S.maxcnt = o.maxcnt; S.maxit = o.maxit;
+    Batch &B = S.B;
+    B.ctx = ctx; B.s = ctx->stream; B.o = &S.oc; B.P = P;
     hipStream_t s = B.s;
     B.graph_iters = o.graph_iters > 0 ? o.graph_iters : 8;
     // internal order: sparse problems first, then the problems of each shared
     // dense operator consecutively; orig[k] = caller's index of problem k
-    std::vector<int> orig;
+    std::vector<int> &orig = S.orig;
     {
         std::vector<int> dense;
         for (int k = 0; k < P; ++k) {
@@ -1447,7 +1556,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
     // where the dims allow, rays in a locality order), then the block-diagonal
     // matrices over the sparse prefix; the renumbered copies are released
     // once concatenated.  Dense problems keep the natural order.
-    std::vector<std::unique_ptr<FusedLayout>> lay(P);
+    std::vector<std::unique_ptr<FusedLayout>> &lay = S.lay;
+    lay.resize(P);
     if (ncsr > 0) {
         std::vector<const SparseRows *> pa(ncsr), pt(ncsr);
         std::vector<int64_t> v0c(v0.begin(), v0.begin() + ncsr), r0c(r0.begin(), r0.begin() + ncsr);
@@ -1488,13 +1598,6 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         k = e;
     }
     if (gscratch) B.gpart.alloc(gscratch);
-    std::vector<const float *> b_dev(P), Ut_dev(P);
-    std::vector<float *> U_dev(P);
-    for (int k = 0; k < P; ++k) {
-        b_dev[k] = b_in[orig[k]];
-        U_dev[k] = U_in[orig[k]];
-        Ut_dev[k] = Ut_in ? Ut_in[orig[k]] : nullptr;
-    }
     // vectors
     const int nv = 2 + 1 + 2 + 1 + 1 + 3 + 3 + 1 + 1 + 1, mv = 2 + 1 + 1 + 1 + 1;
     B.pool.alloc((size_t)nv * N + (size_t)mv * M);
@@ -1543,10 +1646,6 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         B.halive.alloc(1);
     }
 
-    hipEvent_t ev0, ev1, evl0, evl1;
-    TV3_HIP(hipEventCreate(&ev0)); TV3_HIP(hipEventCreate(&ev1));
-    TV3_HIP(hipEventCreate(&evl0)); TV3_HIP(hipEventCreate(&evl1));
-
     // launch shapes of the two products (every problem active), unless given
     if (binfo && binfo->lanes_N != 0 && binfo->lanes_T != 0) {
         B.lanes_N = binfo->lanes_N;
@@ -1574,27 +1673,64 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
     } else {
         B.tune_groups();
     }
+    B.skipping = o.skipMulRatio != 0.0;
+    B.scl.alloc(P);
+    B.bptr.alloc(P);
+    B.bad.alloc(1);
+    B.hbad.alloc(1);
+    B.capture();
+    return SP;
+}
+
+void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *dims_in,
+                   const tv3_opts &o, const float *const *b_in, float *const *U_in,
+                   const float *const *Ut_in, tv3_info *infos, tv3_history *hists,
+                   tv3_batch_info *binfo) {
+    auto t_host0 = std::chrono::steady_clock::now();
+    // the context's last batch when this call has its operators, shapes and
+    // captured options; otherwise it is freed first and this one built
+    BatchSession *S = dynamic_cast<BatchSession *>(ctx->batch_session.get());
+    const bool reused = S && session_matches(*S, P, ops_in, dims_in, o, binfo);
+    if (!reused) {
+        ctx->batch_session.reset();
+        std::unique_ptr<BatchSession> ns = build_session(ctx, P, ops_in, dims_in, o, binfo);
+        S = ns.get();
+        ctx->batch_session = std::move(ns);
+    }
+    S->oc = o;
+    Batch &B = S->B;
+    hipStream_t s = B.s;
+    const std::vector<int> &orig = S->orig;
+    const std::vector<std::unique_ptr<FusedLayout>> &lay = S->lay;
+    const int64_t N = B.N, M = B.M;
+    std::vector<const float *> b_dev(P), Ut_dev(P);
+    std::vector<float *> U_dev(P);
+    for (int k = 0; k < P; ++k) {
+        b_dev[k] = b_in[orig[k]];
+        U_dev[k] = U_in[orig[k]];
+        Ut_dev[k] = Ut_in ? Ut_in[orig[k]] : nullptr;
+    }
+    hipEvent_t ev0, ev1, evl0, evl1;
+    TV3_HIP(hipEventCreate(&ev0)); TV3_HIP(hipEventCreate(&ev1));
+    TV3_HIP(hipEventCreate(&evl0)); TV3_HIP(hipEventCreate(&evl1));
+
+    TV3_HIP(hipMemsetAsync(B.pool.p, 0, B.pool.bytes(), s));
     TV3_HIP(hipEventRecord(ev0, s));
 
-    // scaleb (:225-266) per problem on a host copy of b
-    std::vector<float> scl(P, 1.f);
+    // scaleb (:225-266) per problem on the device; b gathered into the
+    // internal order and scaled by the factor found there
+    TV3_HIP(hipMemcpyAsync(B.bptr.p, b_dev.data(), sizeof(const float *) * P,
+                           hipMemcpyHostToDevice, s));
+    TV3_HIP(hipMemsetAsync(B.bad.p, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_b_scaleb, dim3(P), dim3(1024), 0, s, B.bptr.p, B.ddims.p, B.scl.p,
+                       B.bad.p);
+    TV3_HIP(hipMemcpyAsync(B.hbad.p, B.bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TV3_HIP(hipStreamSynchronize(s));
+    TV3_REQUIRE(*B.hbad.p == 0, TV3_E_NONFINITE, "b holds NaN or Inf");
     for (int k = 0; k < P; ++k) {
         const int64_t m = B.dims[k].m;
-        std::vector<float> bh(m);
-        TV3_HIP(hipMemcpyAsync(bh.data(), b_dev[k], sizeof(float) * m, hipMemcpyDeviceToHost, s));
-        TV3_HIP(hipStreamSynchronize(s));
-        float bmin = bh[0], bmax = bmin, amin = std::fabs(bmin), amax = amin;
-        for (float v : bh) {
-            TV3_REQUIRE(std::isfinite(v), TV3_E_NONFINITE, "b holds NaN or Inf");
-            const float a = std::fabs(v);
-            if (a < amin) { bmin = v; amin = a; }
-            if (a > amax) { bmax = v; amax = a; }
-        }
-        const float dif = std::fabs(bmax - bmin);
-        if (dif < 0.5f) scl[k] = 0.5f / dif;
-        else if (dif > 1.5f) scl[k] = 1.5f / dif;
         hipLaunchKernelGGL(k_gather_scale, dim3(grid_for(m)), dim3(256), 0, s, b_dev[k],
-                           lay[k] ? lay[k]->r_int2nat.p : nullptr, m, scl[k],
+                           lay[k] ? lay[k]->r_int2nat.p : nullptr, m, B.scl.p + k,
                            B.b + B.dims[k].r0);
     }
 
@@ -1611,7 +1747,6 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         c.skip_ratio = o.skipMulRatio; c.max_rel = o.maxRelativeChange;
         c.n_init = (int32_t)std::min<uint32_t>(o.numInitIterations, INT32_MAX);
     }
-    B.skipping = o.skipMulRatio != 0.0;
     B.write_ctl();
 
     // initialisation (:1128-1153), every problem at once
@@ -1620,7 +1755,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         const BDims &bd = B.dims[k];
         if (o.warm_start)
             hipLaunchKernelGGL(k_gather_scale, dim3(grid_for(bd.n)), dim3(256), 0, s, U_dev[k],
-                               lay[k] ? lay[k]->v_int2nat.p : nullptr, bd.n, scl[k],
+                               lay[k] ? lay[k]->v_int2nat.p : nullptr, bd.n, B.scl.p + k,
                                B.U[0] + bd.v0);
     }
     if (!o.warm_start)
@@ -1636,7 +1771,6 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
     hipLaunchKernelGGL(k_b_init, dim3(P), dim3(256), 0, s, B.ctl.p, B.vst.p, B.rst.p, B.nbV,
                        B.nbR, B.pN, B.pW);
     TV3_HIP(hipGetLastError());
-    B.capture();
     TV3_HIP(hipEventRecord(evl0, s));
     if (o.maxcnt <= 0 || o.maxit <= 0) {     // the outer loop of :1166 does not run
         B.read_ctl();
@@ -1668,7 +1802,8 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         const BDims &bd = B.dims[k];
         const float *Uk = H[k].cur ? B.U[1] : B.U[0];
         hipLaunchKernelGGL(k_scatter_unscale, dim3(grid_for(bd.n)), dim3(256), 0, s, Uk + bd.v0,
-                           lay[k] ? lay[k]->v_int2nat.p : nullptr, bd.n, scl[k], U_dev[k]);
+                           lay[k] ? lay[k]->v_int2nat.p : nullptr, bd.n, B.scl.p + k,
+                           U_dev[k]);
     }
     TV3_HIP(hipEventRecord(ev1, s));
     TV3_HIP(hipStreamSynchronize(s));

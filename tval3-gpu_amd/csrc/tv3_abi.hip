@@ -240,6 +240,7 @@ void tv3_ctx_destroy(tv3_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->batch_session.reset();
     ctx->red_partials.release();
     ctx->scratch.release();
     (void)hipStreamDestroy(ctx->stream);

@@ -5,6 +5,8 @@
 // library and become tv3_status codes at the C ABI (tv3_abi.cpp).
 #pragma once
 
+#include <atomic>
+
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -291,6 +293,20 @@ struct FusedLayout {
 
 }  // namespace tv3
 
+namespace tv3 {
+// State a context keeps between calls (the last batched solve's layouts,
+// tuned shapes and graphs).
+struct CtxSession {
+    virtual ~CtxSession() = default;
+};
+// serial number of an operator, never reused: a cached session recognises
+// its operators by it, not by their addresses
+inline uint64_t next_op_uid() {
+    static std::atomic<uint64_t> counter{0};
+    return ++counter;
+}
+}  // namespace tv3
+
 struct tv3_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -301,9 +317,11 @@ struct tv3_ctx {
     tv3::DevBuf<double> red_partials;
     tv3::PinnedBuf<double> red_host;
     tv3::DevBuf<uint8_t> scratch;   // general workspace (grown on demand)
+    std::unique_ptr<tv3::CtxSession> batch_session;   // the last batched solve, for reuse
 };
 
 struct tv3_op {
+    uint64_t uid = tv3::next_op_uid();
     tv3_ctx *ctx = nullptr;
     int device = 0;                  // device of ctx (destruction does not read ctx)
     tv3::OpKind kind = tv3::OpKind::CSR;
