@@ -372,14 +372,25 @@ __host__ __device__ __forceinline__ void dyn_local_xyz(int i, int &x, int &y, in
 //     operation on the same values.  A parallel axis has A = 0, B = +inf, so
 //     its next plane never comes; no plane beyond the box's exit is ever
 //     evaluated (the walk stops at its parameter first).
-template <typename F>
-__host__ __device__ __forceinline__ void dyn_seg_walk(const DynView &g, const DynPan &P, int ray,
-                                                      int c0, F emit) {
-#pragma clang fp contract(off)
+struct DynSeg {
+    float4 ra, rb;   // A[3], L; B[3], amin
+    float amax;
+    int c0;          // box-local first voxel x | y << 4 | z << 8
+};
+
+// a segment's traversal constants: its ray's record and its first voxel
+__host__ __device__ __forceinline__ DynSeg dyn_seg(const DynView &g, int ray, int c0) {
     const float *rec = g.ray + 12 * (int64_t)ray;
-    const float4 ra = reinterpret_cast<const float4 *>(rec)[0];
-    const float4 rb = reinterpret_cast<const float4 *>(rec)[1];
-    const float amax = rec[8];
+    return DynSeg{reinterpret_cast<const float4 *>(rec)[0],
+                  reinterpret_cast<const float4 *>(rec)[1], rec[8], c0};
+}
+
+template <typename F>
+__host__ __device__ __forceinline__ void dyn_seg_walk(const DynSeg &sr, const DynPan &P, F emit) {
+#pragma clang fp contract(off)
+    const float4 ra = sr.ra, rb = sr.rb;
+    const float amax = sr.amax;
+    const int c0 = sr.c0;
     const float A[3] = {ra.x, ra.y, ra.z}, B[3] = {rb.x, rb.y, rb.z};
     const int lc[3] = {c0 & 15, (c0 >> 4) & 15, (c0 >> 8) & 15};
     const int b0[3] = {P.bx, P.by, P.bz};
@@ -425,6 +436,12 @@ __host__ __device__ __forceinline__ void dyn_seg_walk(const DynView &g, const Dy
     }
 }
 
+template <typename F>
+__host__ __device__ __forceinline__ void dyn_seg_walk(const DynView &g, const DynPan &P, int ray,
+                                                      int c0, F emit) {
+    dyn_seg_walk(dyn_seg(g, ray, c0), P, emit);
+}
+
 // A x, first pass: per box its voxels staged in LDS, each thread walks
 // segments and stores their partial sums (walk order) in part[segment].
 // `active` (may be NULL) predicates the launch like an epilogue's skip().
@@ -460,28 +477,56 @@ __global__ __launch_bounds__(NT) void k_dyn_tile_T(DynView g, const T *__restric
     static_assert(ExtraBlocks<Epi>::value == 0, "no extra workgroups in the tile product");
     extern __shared__ __align__(16) unsigned char dyn_smem[];
     if (epi.skip()) return;
+    static_assert(NT >= 512, "one voxel row per thread (boxes of at most 8^3)");
     const DynPan P = g.pan[g.order[blockIdx.x]];
     T *pe = reinterpret_cast<T *>(dyn_smem);
-    const int t = threadIdx.x, ns = P.s1 - P.s0;
-    for (int j = t; j < ns; j += NT) {
+    const int t = threadIdx.x, ns = P.s1 - P.s0, nv = P.bs * P.bs * P.bs;
+    // phase 2's operands do not depend on phase 1: its row starts, first
+    // positions and epilogue operands are requested before the walks
+    const bool has_row = t < nv;
+    const int row = P.v0 + (has_row ? t : 0);
+    uint32_t q0 = 0, q1 = 0;
+    if (has_row) {
+        q0 = g.pptr[row];
+        q1 = g.pptr[row + 1];
+    }
+    uint2 u0 = make_uint2(0, 0);
+    if (q0 < q1) u0 = *reinterpret_cast<const uint2 *>(g.pos + q0);
+    const auto pre = epi.load(row);
+    // phase 1: the next segment's record, slots and residual are requested
+    // while the current one is walked
+    struct Cur {
+        DynSeg sr;
+        int e0, e1;
+        T xv;
+    };
+    auto fetch = [&](int j) {
         const int sg = P.s0 + j, ray = g.seg_ray[sg];
-        const int e1 = j + 1 < ns ? g.seg_eoff[sg + 1] : P.ne;
-        const T xv = x[ray];
-        int e = g.seg_eoff[sg];
-        dyn_seg_walk(g, P, ray, g.seg_c0[sg], [&](int, float w) {
-            if (e < e1) pe[e] = (T)w * xv;
-            ++e;
-        });
+        return Cur{dyn_seg(g, ray, g.seg_c0[sg]), g.seg_eoff[sg],
+                   j + 1 < ns ? g.seg_eoff[sg + 1] : P.ne, x[ray]};
+    };
+    if (t < ns) {
+        Cur cur = fetch(t);
+        for (int j = t; j < ns; j += NT) {
+            Cur nxt = cur;
+            if (j + NT < ns) nxt = fetch(j + NT);
+            int e = cur.e0;
+            const int e1 = cur.e1;
+            const T xv = cur.xv;
+            dyn_seg_walk(cur.sr, P, [&](int, float w) {
+                if (e < e1) pe[e] = (T)w * xv;
+                ++e;
+            });
+            cur = nxt;
+        }
     }
     if (t == 0) pe[P.ne] = 0;
     __syncthreads();
-    const int nv = P.bs * P.bs * P.bs;
-    for (int r = t; r < nv; r += NT) {
-        const int row = P.v0 + r;
-        const auto pre = epi.load(row);
-        const uint32_t q0 = g.pptr[row], q1 = g.pptr[row + 1];
+    if (has_row) {
         T acc = 0;
-        for (uint32_t q = q0; q < q1; q += 4) {
+        if (q0 < q1)
+            acc += (pe[u0.x & 0xffffu] + pe[u0.x >> 16]) + (pe[u0.y & 0xffffu] + pe[u0.y >> 16]);
+        for (uint32_t q = q0 + 4; q < q1; q += 4) {
             const uint2 u = *reinterpret_cast<const uint2 *>(g.pos + q);
             acc += (pe[u.x & 0xffffu] + pe[u.x >> 16]) + (pe[u.y & 0xffffu] + pe[u.y >> 16]);
         }
@@ -502,10 +547,6 @@ void launch_dyn_T(const DynTiles &D, const T *x, const Epi &epi, hipStream_t s) 
     if (D.bwd.npan == 0) return;
     const size_t lds = sizeof(T) * (size_t)(D.emax_cap + 1);
     switch (D.threads_T) {
-    case 256:
-        hipLaunchKernelGGL((k_dyn_tile_T<T, 256, Epi>), dim3((unsigned)D.bwd.npan), dim3(256), lds,
-                           s, dyn_view(D, D.bwd), x, epi);
-        break;
     case 512:
         hipLaunchKernelGGL((k_dyn_tile_T<T, 512, Epi>), dim3((unsigned)D.bwd.npan), dim3(512), lds,
                            s, dyn_view(D, D.bwd), x, epi);
@@ -514,7 +555,7 @@ void launch_dyn_T(const DynTiles &D, const T *x, const Epi &epi, hipStream_t s) 
         hipLaunchKernelGGL((k_dyn_tile_T<T, 1024, Epi>), dim3((unsigned)D.bwd.npan), dim3(1024),
                            lds, s, dyn_view(D, D.bwd), x, epi);
         break;
-    default: throw tv3_error(TV3_E_ARG, "the A^T tile product runs 256, 512 or 1024 threads");
+    default: throw tv3_error(TV3_E_ARG, "the A^T tile product runs 512 or 1024 threads");
     }
 }
 
