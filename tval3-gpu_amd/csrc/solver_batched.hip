@@ -68,6 +68,37 @@ __device__ __forceinline__ int find_prob(const int32_t *__restrict__ starts, int
     return lo;
 }
 
+// The groups of this workgroup (grp = blockIdx.x + i gridDim.x < ng) whose
+// problem passes want(k), found in one parallel pass over up to 256
+// candidates at a time (one thread each) instead of one dependent lookup
+// per group; then body(grp, k) for each of them, in whatever order they
+// were found (each group's partial sum has a slot of its own).  For
+// 256-thread workgroups.
+template <typename W, typename Bd>
+__device__ __forceinline__ void for_active_groups(int ng, const int32_t *__restrict__ starts, int P,
+                                                  W want, Bd body) {
+    __shared__ int32_t lg[256], lk[256];
+    __shared__ int32_t cnt;
+    const int G = (int)gridDim.x;
+    for (int base = 0; (int)blockIdx.x + base * G < ng; base += 256) {
+        if (threadIdx.x == 0) cnt = 0;
+        __syncthreads();
+        const int grp = (int)blockIdx.x + (base + (int)threadIdx.x) * G;
+        if (grp < ng) {
+            const int k = find_prob(starts, P, grp);
+            if (want(k)) {
+                const int slot = atomicAdd(&cnt, 1);
+                lg[slot] = grp;
+                lk[slot] = k;
+            }
+        }
+        __syncthreads();
+        const int n = cnt;
+        for (int i = 0; i < n; ++i) body(lg[i], lk[i]);
+        __syncthreads();                 // lg, lk and cnt are rewritten by the next pass
+    }
+}
+
 struct Vox {
     int y, x, z;
     int p, q, r, B, nby, nbx;
@@ -472,11 +503,10 @@ __global__ __launch_bounds__(256) void k_b_dd_norm(const BDims *__restrict__ D,
                                                    const float *__restrict__ d, double *part,
                                                    int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
-    // off ends after that many workgroups, not ng
-    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
-        const int k = find_prob(vst, P, grp);
-        if (!C[k].need_tau) continue;
+    // ng groups (256 voxels or rays) over a fixed grid (a launch whose gate
+    // is off ends after that many workgroups, not ng); the groups whose
+    // problem takes this step are found in one pass
+    for_active_groups(ng, vst, P, [&](int k) { return C[k].need_tau != 0; }, [&](int grp, int k) {
         const BDims dm = D[k];
         double acc[1] = {0};
         const int64_t i = (int64_t)grp * GB + threadIdx.x;
@@ -490,7 +520,7 @@ __global__ __launch_bounds__(256) void k_b_dd_norm(const BDims *__restrict__ D,
         }
         block_sum_store_at<1>(acc, part, ng, grp);
         __syncthreads();   // the block sum's LDS is rewritten by the next group
-    }
+    });
 }
 
 // update_mlp, sigma part (:1036-1069): keeps the old sigma for the W of the
@@ -506,12 +536,11 @@ __global__ __launch_bounds__(256) void k_b_mlp_sigma(const BDims *__restrict__ D
                                                      double *part,
                                                      int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
-    // off ends after that many workgroups, not ng
-    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
-        const int k = find_prob(vst, P, grp);
+    // ng groups (256 voxels or rays) over a fixed grid (a launch whose gate
+    // is off ends after that many workgroups, not ng); the groups whose
+    // problem takes this step are found in one pass
+    for_active_groups(ng, vst, P, [&](int k) { return C[k].outer_flag != 0; }, [&](int grp, int k) {
         const FCtl &c = C[k];
-        if (!c.outer_flag) continue;
         const BDims dm = D[k];
         const float *U = c.cur ? U1 : U0;
         double acc[1] = {0};
@@ -534,7 +563,7 @@ __global__ __launch_bounds__(256) void k_b_mlp_sigma(const BDims *__restrict__ D
         }
         block_sum_store_at<1>(acc, part, ng, grp);
         __syncthreads();   // the block sum's LDS is rewritten by the next group
-    }
+    });
 }
 
 // outer update of d and DtsAtd (:1239-1246) without a stored g2
@@ -547,12 +576,11 @@ __global__ __launch_bounds__(256) void k_b_outer_d(const BDims *__restrict__ D,
                                                    float *__restrict__ dts, double *part,
                                                    int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
-    // off ends after that many workgroups, not ng
-    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
-        const int k = find_prob(vst, P, grp);
+    // ng groups (256 voxels or rays) over a fixed grid (a launch whose gate
+    // is off ends after that many workgroups, not ng); the groups whose
+    // problem takes this step are found in one pass
+    for_active_groups(ng, vst, P, [&](int k) { return C[k].outer_flag != 0; }, [&](int grp, int k) {
         const FCtl &c = C[k];
-        if (!c.outer_flag) continue;
         const BDims dm = D[k];
         const float *g = c.cur ? g1 : g0;
         double acc[1] = {0};
@@ -569,7 +597,7 @@ __global__ __launch_bounds__(256) void k_b_outer_d(const BDims *__restrict__ D,
         }
         block_sum_store_at<1>(acc, part, ng, grp);
         __syncthreads();   // the block sum's LDS is rewritten by the next group
-    }
+    });
 }
 
 // |U - rcdU|^2 then rcdU = U (:1208-1217)
@@ -581,12 +609,11 @@ __global__ __launch_bounds__(256) void k_b_relout(const BDims *__restrict__ D,
                                                   float *__restrict__ rcd, double *part,
                                                   int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
-    // off ends after that many workgroups, not ng
-    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
-        const int k = find_prob(vst, P, grp);
+    // ng groups (256 voxels or rays) over a fixed grid (a launch whose gate
+    // is off ends after that many workgroups, not ng); the groups whose
+    // problem takes this step are found in one pass
+    for_active_groups(ng, vst, P, [&](int k) { return C[k].outer_flag != 0; }, [&](int grp, int k) {
         const FCtl &c = C[k];
-        if (!c.outer_flag) continue;
         const BDims dm = D[k];
         const float *U = c.cur ? U1 : U0;
         double acc[1] = {0};
@@ -598,7 +625,7 @@ __global__ __launch_bounds__(256) void k_b_relout(const BDims *__restrict__ D,
         }
         block_sum_store_at<1>(acc, part, ng, grp);
         __syncthreads();   // the block sum's LDS is rewritten by the next group
-    }
+    });
 }
 
 // ------------------------------------------------------------------ rays
@@ -681,18 +708,17 @@ __global__ __launch_bounds__(256) void k_b_sqnorm(const BDims *__restrict__ D,
                                                   const float *__restrict__ y, double *part,
                                                   int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
-    // off ends after that many workgroups, not ng
-    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
-        const int k = find_prob(rst, P, grp);
-        if (!C[k].need_tau) continue;
+    // ng groups (256 voxels or rays) over a fixed grid (a launch whose gate
+    // is off ends after that many workgroups, not ng); the groups whose
+    // problem takes this step are found in one pass
+    for_active_groups(ng, rst, P, [&](int k) { return C[k].need_tau != 0; }, [&](int grp, int k) {
         const BDims dm = D[k];
         double acc[1] = {0};
         const int64_t r = (int64_t)grp * GB + threadIdx.x;
         if (r - dm.r0 < dm.m) acc[0] = (double)y[r] * y[r];
         block_sum_store_at<1>(acc, part, ng, grp);
         __syncthreads();   // the block sum's LDS is rewritten by the next group
-    }
+    });
 }
 
 __global__ __launch_bounds__(256) void k_b_mlp_delta(const BDims *__restrict__ D,
@@ -704,12 +730,11 @@ __global__ __launch_bounds__(256) void k_b_mlp_delta(const BDims *__restrict__ D
                                                      float *__restrict__ delta, double *part,
                                                      int ng, const int32_t *gate) {
     if (gate && !*gate) return;
-    // ng groups (256 voxels or rays) over a fixed grid: a launch whose gate is
-    // off ends after that many workgroups, not ng
-    for (int grp = blockIdx.x; grp < ng; grp += gridDim.x) {
-        const int k = find_prob(rst, P, grp);
+    // ng groups (256 voxels or rays) over a fixed grid (a launch whose gate
+    // is off ends after that many workgroups, not ng); the groups whose
+    // problem takes this step are found in one pass
+    for_active_groups(ng, rst, P, [&](int k) { return C[k].outer_flag != 0; }, [&](int grp, int k) {
         const FCtl &c = C[k];
-        if (!c.outer_flag) continue;
         const BDims dm = D[k];
         const float *Au = c.cur ? Au1 : Au0;
         double acc[1] = {0};
@@ -723,7 +748,7 @@ __global__ __launch_bounds__(256) void k_b_mlp_delta(const BDims *__restrict__ D
         }
         block_sum_store_at<1>(acc, part, ng, grp);
         __syncthreads();   // the block sum's LDS is rewritten by the next group
-    }
+    });
 }
 
 // accepted Au: Au_new = Au_c + alpha (Au_t - Au_c), Au_t itself when alpha = 1
