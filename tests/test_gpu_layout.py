@@ -63,8 +63,10 @@ SHAPES = [
     (-1024, -2048, 600, 3), (-2048, -1024, 100, 3),
     (32, -1024, 16384, 2), (-1024, 64, 16384, 1),
     (8, -2048, 0, 0), (-4096, 16, 0, 0),
-    # A's panels over two tiles, whole and split into brick runs
+    # panels over two tiles (A: bit 2, A^T: bit 3), whole and split into
+    # brick runs
     (-2048, -2048, 16384, 7), (-1024, -1024, 300, 7), (-4096, 32, 16384, 5),
+    (-2048, -2048, 16384, 15), (16, -1024, 300, 10), (-1024, -4096, 16384, 11),
 ]
 
 

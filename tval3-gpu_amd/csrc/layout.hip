@@ -470,11 +470,12 @@ static float tune_shape(tv3_ctx *ctx, ColdTimer &time_launches, SparseRows &S, i
     return best;
 }
 
-// Panel forms of A^T (tiles of 512 internal voxels, panels of at most 16384
-// entries) and of A (panels of one tile, or of two -- 16 x 8 x 8 voxels, at
-// most 32768 entries -- whose rays leave fewer, longer segments and so fewer
-// partial sums for the second pass) timed over row blocks of 1024 and 2048
-// entries; each kept when faster than the tuned plain kernel (t_A, t_T in ms).
+// Panel forms of A^T and of A, each over panels of one tile (512 internal
+// voxels, at most 16384 entries) or of two consecutive tiles (16 x 8 x 8
+// voxels, at most 32768 entries: A's rays leave fewer, longer segments and
+// so fewer partial sums for the second pass; A^T stages the union of the two
+// tiles' rays once), timed over row blocks of 1024 and 2048 entries; each
+// kept when faster than the tuned plain kernel (t_A, t_T in ms).
 static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, float t_A,
                         float t_T) {
     hipStream_t s = ctx->stream;
@@ -490,12 +491,25 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
     DevBuf<float> xr(m > 0 ? m : 1), xv(n > 0 ? n : 1), y(n > m ? n : (m > 0 ? m : 1));
     TV3_HIP(hipMemsetAsync(xr.p, 0, xr.bytes(), s));
     TV3_HIP(hipMemsetAsync(xv.p, 0, xv.bytes(), s));
-    for (int nb : {1024, 2048}) {
-        Panels P;
-        build_panels_T(ctx, h, pl, nb, P);
-        const float t =
-            time_launches([&]() { launch_panel(P, xr.p, EpiStoreIf{y.p, nullptr}, s); });
-        if (t < t_T) { t_T = t; F.PT = std::move(P); F.panels_T = true; }
+    for (int unit : {512, 1024}) {
+        PanelPlan pl2;
+        if (unit != 512 && !plan_panels(h, unit, 16384 * unit / 512, pl2)) continue;
+        const PanelPlan &pt = unit == 512 ? pl : pl2;
+        for (int nb : {1024, 2048}) {
+            Panels P;
+            build_panels_T(ctx, h, pt, nb, P);
+            const float t =
+                time_launches([&]() { launch_panel(P, xr.p, EpiStoreIf{y.p, nullptr}, s); });
+            if (t < t_T) {
+                t_T = t;
+                F.PT = std::move(P);
+                F.panels_T = true;
+                F.unit_T = unit;
+                F.npan = (int64_t)pt.pan_row.size() - 1;
+                F.nseg = (int64_t)pt.seg_ray.size();
+                F.max_seg = (int)pt.max_seg;
+            }
+        }
     }
     for (int unit : {512, 1024}) {
         PanelPlan pl2;
@@ -658,7 +672,7 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
     F.session.reset();               // its graphs hold the old launch shapes
     tv3_ctx *ctx = op->ctx;
     const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0;
-    const int unit_A = (panels & 4) ? 1024 : 512;
+    const int unit_A = (panels & 4) ? 1024 : 512, unit_T = (panels & 8) ? 1024 : 512;
     if (pa || pt) {
         TV3_REQUIRE(pan_nnz > 0, TV3_E_ARG, "panels need pan_nnz > 0");
         for (int c : {pa ? code_N : -1024, pt ? code_T : -1024})
@@ -671,7 +685,17 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
         F.npan = (int64_t)pl.pan_row.size() - 1;
         F.nseg = (int64_t)pl.seg_ray.size();
         F.max_seg = (int)pl.max_seg;
-        if (pt) build_panels_T(ctx, h, pl, -code_T, F.PT);
+        if (pt && unit_T == 512) build_panels_T(ctx, h, pl, -code_T, F.PT);
+        if (pt && unit_T != 512) {
+            PanelPlan pl2;
+            TV3_REQUIRE(plan_panels(h, unit_T, 2 * pan_nnz, pl2), TV3_E_UNSUPPORTED,
+                        "a panel of this operator would need more rays than LDS holds");
+            build_panels_T(ctx, h, pl2, -code_T, F.PT);
+            F.npan = (int64_t)pl2.pan_row.size() - 1;
+            F.nseg = (int64_t)pl2.seg_ray.size();
+            F.max_seg = (int)pl2.max_seg;
+        }
+        F.unit_T = unit_T;
         if (pa && unit_A == 512) build_panels_A(ctx, h, pl, -code_N, F.PA, F.RS);
         if (pa && unit_A != 512) {
             PanelPlan pl2;

@@ -279,6 +279,7 @@ struct FusedLayout {
     SegSum RS;
     bool panels_T = false, panels_A = false;
     int unit_A = 512;                 // voxels per panel of A's panel form (1 or 2 tiles)
+    int unit_T = 512;                 // voxels per panel of A^T's panel form (1 or 2 tiles)
     int64_t npan = 0, nseg = 0;
     int max_seg = 0;
     DevBuf<int32_t> v_nat2int, v_int2nat;   // voxel maps (n)
