@@ -210,6 +210,13 @@ static bool plan_panels(const HostCSR &h, int unit, int pan_nnz, PanelPlan &pl) 
     return true;
 }
 
+// Row blocks per work item of a panel launch (one workgroup each): A's
+// panels in chunks of 6 (their sources, a contiguous voxel range, are cheap
+// to stage again); A^T's whole (each chunk would gather the panel's rays
+// again).  Measured at 128^3 (A, A^T in us): whole 66.0, 56.1; chunks of 3
+// 65.0, 64.3; of 6 64.7, 58.7; of 9 67.7, 56.2.
+constexpr int kPanelChunkA = 6, kPanelChunkT = 1 << 30;
+
 // Entry groups of the panel rows: A's rows (ray segments, ~8 entries at
 // 128^3) unpadded, A^T's (voxels, ~17) in groups of 4.  Measured at 128^3,
 // 35.8M FP16 entries, blocks of 2048: A 71.1 / 71.5 / 74.5 us with groups
@@ -231,7 +238,7 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
                         const std::vector<int32_t> &src_off, const std::vector<int32_t> *src,
                         const std::vector<int32_t> &rptr, const std::vector<uint16_t> &lidx,
                         const uint8_t *wb, size_t vb, tv3_dtype vt, int blk_nnz, int G,
-                        Panels &P) {
+                        int chunk_blocks, Panels &P) {
     hipStream_t s = ctx->stream;
     TV3_REQUIRE(G == 1 || G == 2 || G == 4, TV3_E_ARG, "panel groups of 1, 2 or 4 entries");
     const int64_t npan = (int64_t)pan_row.size() - 1, rows = pan_row.back();
@@ -269,14 +276,8 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
     }
     TV3_REQUIRE(acc < (int64_t(1) << 31), TV3_E_UNSUPPORTED, "panel entries exceed 2^31");
     pptr[rows] = (int32_t)acc;
-    std::vector<int64_t> work(npan);
     int max_src = 0;
-    for (int64_t p = 0; p < npan; ++p) {
-        const int64_t r0 = pan_row[p], r1 = pan_row[p + 1];
-        const int ns = src_off[p + 1] - src_off[p];
-        max_src = std::max(max_src, ns);
-        work[p] = ((int64_t)pptr[r1] - pptr[r0]) + 4 * (int64_t)ns;
-    }
+    for (int64_t p = 0; p < npan; ++p) max_src = std::max(max_src, src_off[p + 1] - src_off[p]);
     std::vector<uint32_t> pk;
     std::vector<uint16_t> ps;
     std::vector<float> pv;
@@ -295,10 +296,28 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
                 std::memcpy(&pv[q], wb + (size_t)k * 4, 4);
             }
         }
-    std::vector<int32_t> order(npan);
+    // work items: each panel's row blocks in chunks of at most chunk_blocks,
+    // one workgroup each (staging the panel's sources itself), so that the
+    // largest panels -- 29 blocks against 9 on average for the two-tile A
+    // panels at 128^3 -- do not run long chains after the others have ended;
+    // largest first
+    std::vector<int32_t> chunk, order;
+    std::vector<int64_t> cwork;
+    for (int64_t p = 0; p < npan; ++p) {
+        const int32_t b0 = pan_blk[p], nb = pan_blk[p + 1] - b0;
+        const int32_t nc = (int32_t)(((int64_t)nb + chunk_blocks - 1) / chunk_blocks);
+        const int ns = src_off[p + 1] - src_off[p];
+        for (int32_t i = 0; i < nc; ++i) {
+            const int32_t c0 = b0 + (int32_t)((int64_t)nb * i / nc);
+            const int32_t c1 = b0 + (int32_t)((int64_t)nb * (i + 1) / nc);
+            chunk.insert(chunk.end(), {(int32_t)p, c0, c1});
+            cwork.push_back((int64_t)pptr[blk[c1]] - pptr[blk[c0]] + 4 * (int64_t)ns);
+        }
+    }
+    order.resize(cwork.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return work[a] > work[b]; });
+                     [&](int32_t a, int32_t b) { return cwork[a] > cwork[b]; });
     P.rows = rows;
     P.npan = npan;
     P.nsrc = src ? (int64_t)src->size() : 0;
@@ -316,7 +335,8 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
     };
     up(P.order, order);
     up(P.src_off, src_off);
-    up(P.pan_blk, pan_blk);
+    up(P.chunk, chunk);
+    P.nitem = (int64_t)order.size();
     if (src) up(P.src, *src);
     else P.src.release();
     up(P.pptr, pptr);
@@ -355,7 +375,7 @@ static void build_panels_T(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
             lidx[k] = (uint16_t)local[h.ind[k]];
     }
     pack_panels(ctx, pl.pan_row, pl.pan_seg, &pl.seg_ray, h.ptr, lidx, h.val.data(), h.vb,
-                h.vtype, blk_nnz, kPanelGroupT, P);
+                h.vtype, blk_nnz, kPanelGroupT, kPanelChunkT, P);
 }
 
 // A in panel form: rows = ray segments (segment s = pan_seg[p] + position of
@@ -388,7 +408,7 @@ static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
     }
     // panels of the segment matrix: segment ranges pan_seg, voxel ranges pan_row
     pack_panels(ctx, pl.pan_seg, pl.pan_row, nullptr, sptr, lidx, wb.data(), h.vb, h.vtype,
-                blk_nnz, kPanelGroupA, P);
+                blk_nnz, kPanelGroupA, kPanelChunkA, P);
     // segments of each ray, in increasing segment number
     std::vector<int32_t> rptr(m + 1, 0), rseg(nseg > 0 ? nseg : 1);
     for (int64_t s = 0; s < nseg; ++s) ++rptr[pl.seg_ray[s] + 1];

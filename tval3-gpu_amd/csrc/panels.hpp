@@ -90,8 +90,9 @@ struct ExtraBlocks<E, std::void_t<decltype(E::kExtra)>> {
     static constexpr int value = E::kExtra;
 };
 
-// One workgroup per panel (largest panels first, so the launch ends with
-// small ones).  Block descriptors: p0 / len = start / length of the block's
+// One workgroup per work item: a chunk of a panel's row blocks, whose
+// workgroup stages the panel's sources (largest items first, so the launch
+// ends with small ones).  Block descriptors: p0 / len = start / length of the block's
 // stored entries (both multiples of 4); row r's entries start at pptr[r].
 // A panel's blocks are stored back to back, so block b + 1 starts where
 // block b ends: its entries are requested before block b's products are
@@ -104,7 +105,7 @@ struct ExtraBlocks<E, std::void_t<decltype(E::kExtra)>> {
 template <int NB, int G, typename V, bool CONTIG, typename Epi>
 __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order,
                                                const int32_t *__restrict__ src_off,
-                                               const int32_t *__restrict__ pan_blk,
+                                               const int32_t *__restrict__ chunk,
                                                const int32_t *__restrict__ src,
                                                const BlkDesc *__restrict__ desc,
                                                const int32_t *__restrict__ pptr,
@@ -125,9 +126,9 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
     }
     if (epi.skip()) return;
     const int t = threadIdx.x;
-    const int pan = order[blockIdx.x - NX];
+    const int item = order[blockIdx.x - NX];
+    const int pan = chunk[3 * item], bb = chunk[3 * item + 1], b1 = chunk[3 * item + 2];
     const int s0 = src_off[pan], ns = src_off[pan + 1] - s0;
-    const int bb = pan_blk[pan], b1 = pan_blk[pan + 1];
     BlkDesc d = desc[bb];
     if constexpr (HasSkipRows<Epi>::value) {
         // a panel's rows lie in one row range of the epilogue (tiles never
@@ -284,13 +285,13 @@ __global__ __launch_bounds__(256) void k_seg_sum(int64_t rows, const int32_t *__
 
 template <typename V, bool CONTIG, typename Epi>
 void launch_panel_v(const Panels &P, const float *x, const Epi &epi, hipStream_t s) {
-    const unsigned g = (unsigned)P.npan + ExtraBlocks<Epi>::value;
+    const unsigned g = (unsigned)P.nitem + ExtraBlocks<Epi>::value;
     const size_t lds = sizeof(float) * (size_t)(P.max_src > 0 ? P.max_src : 1);
     if (g == 0) return;
 #define TV3_PCASE(NBV, GV)                                                                    \
     case NBV * 8 + GV:                                                                        \
         hipLaunchKernelGGL((k_panel<NBV, GV, V, CONTIG, Epi>), dim3(g), dim3(256), lds, s,      \
-                           P.order.p, P.src_off.p, P.pan_blk.p, P.src.p, P.desc.p, P.pptr.p,  \
+                           P.order.p, P.src_off.p, P.chunk.p, P.src.p, P.desc.p, P.pptr.p,    \
                            P.pk.p, P.ps.p, P.pv.p, P.stored - 4, x, epi);                     \
         break;
     switch (P.blk_nnz * 8 + P.group) {

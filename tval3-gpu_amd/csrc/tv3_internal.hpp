@@ -193,9 +193,10 @@ struct Panels {
     int64_t stored = 4;        // stored entries (at least 4, a multiple of 4)
     bool contiguous = false;   // sources of panel p: x[src_off[p] .. src_off[p + 1])
     tv3_dtype vtype = TV3_F16;
-    DevBuf<int32_t> order;     // npan panels, decreasing work
+    int64_t nitem = 0;         // work items (a panel's row blocks, in chunks)
+    DevBuf<int32_t> order;     // work items, decreasing work
+    DevBuf<int32_t> chunk;     // per work item: panel, first block, end block
     DevBuf<int32_t> src_off;   // npan + 1 offsets into src (or index ranges)
-    DevBuf<int32_t> pan_blk;   // npan + 1 offsets into desc
     DevBuf<int32_t> src;       // nsrc source indices (empty when contiguous)
     DevBuf<int32_t> pptr;      // rows + 1 padded row starts
     DevBuf<BlkDesc> desc;      // row blocks (+1 sentinel); p0, len in padded entries
