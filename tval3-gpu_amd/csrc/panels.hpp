@@ -348,9 +348,10 @@ inline DynView dyn_view(const DynTiles &D, const DynBoxes &X) {
 
 // Box-local coordinates (x, y, z) of the voxel at offset i of a box in the
 // internal order of brick_index: bricks of 4^3 in (y, x, z) bit order, y
-// fastest in a brick (a 4^3 box holds one brick).
+// fastest in a brick (a 4^3 box holds one brick); in a box of two tiles
+// consecutive in y the second tile's offsets start at 512, y + 8.
 __host__ __device__ __forceinline__ void dyn_local_xyz(int i, int &x, int &y, int &z) {
-    y = (((i >> 6) & 1) << 2) | (i & 3);
+    y = (((i >> 9) & 1) << 3) | (((i >> 6) & 1) << 2) | (i & 3);
     x = (((i >> 7) & 1) << 2) | ((i >> 2) & 3);
     z = (((i >> 8) & 1) << 2) | ((i >> 4) & 3);
 }
@@ -358,7 +359,8 @@ __host__ __device__ __forceinline__ void dyn_local_xyz(int i, int &x, int &y, in
 // The traversal of one segment (the part of an internal ray inside box P,
 // starting in the voxel of box-local coordinates c0 = x | y << 4 | z << 8):
 // emit(offset of the voxel in the box, length) in traversal order, the
-// offset in linear box order y + bs x + bs^2 z.  These are the entries the
+// offset in linear box order y + ey x + ey bs z (edges bs, ey, bs along x,
+// y, z).  These are the entries the
 // walk of the whole ray (sid_walk, siddon.hpp) emits inside the box:
 //   * the ray record holds sid_setup's A, B, L, amin, amax (12 words:
 //     A[3], L, B[3], amin, amax, 3 unused);
@@ -395,8 +397,9 @@ __host__ __device__ __forceinline__ void dyn_seg_walk(const DynSeg &sr, const Dy
     const float A[3] = {ra.x, ra.y, ra.z}, B[3] = {rb.x, rb.y, rb.z};
     const int lc[3] = {c0 & 15, (c0 >> 4) & 15, (c0 >> 8) & 15};
     const int b0[3] = {P.bx, P.by, P.bz};
-    const int bs = P.bs;
-    const int stride[3] = {bs, 1, bs * bs};
+    const int bs = P.bs, ey = P.ey;
+    const int ext[3] = {bs, ey, bs};
+    const int stride[3] = {ey, 1, ey * bs};
     float kf[3], tk[3], st[3];
     int dstep[3], off = 0;
     float ts = rb.w, te = amax;
@@ -409,7 +412,7 @@ __host__ __device__ __forceinline__ void dyn_seg_walk(const DynSeg &sr, const Dy
         tk[a] = sid_fma(kf[a], A[a], B[a]);
         if (!par) {
             const float tin = sid_fma((float)(up ? c : c + 1), A[a], B[a]);
-            const float tout = sid_fma((float)(up ? b0[a] + bs : b0[a]), A[a], B[a]);
+            const float tout = sid_fma((float)(up ? b0[a] + ext[a] : b0[a]), A[a], B[a]);
             ts = tin > ts ? tin : ts;
             te = tout < te ? tout : te;
         }
@@ -417,9 +420,9 @@ __host__ __device__ __forceinline__ void dyn_seg_walk(const DynSeg &sr, const Dy
         dstep[a] = up ? stride[a] : (par ? 0 : -stride[a]);
     }
     // the offset is masked all the same, so that no index leaves the box
-    const int mask = bs * bs * bs - 1;
+    const int mask = ey * bs * bs - 1;
     float tcur = ts;
-    for (int step = 0; step < 3 * bs + 4; ++step) {
+    for (int step = 0; step < 2 * bs + ey + 4; ++step) {
         float tnext = te;
 #pragma unroll
         for (int a = 0; a < 3; ++a) tnext = tk[a] < tnext ? tk[a] : tnext;
@@ -451,13 +454,13 @@ __global__ __launch_bounds__(256) void k_dyn_tile_N(DynView g, const T *__restri
                                                     T *__restrict__ part,
                                                     const int32_t *active) {
     if (active && !*active) return;
-    __shared__ T xs[512];
+    __shared__ T xs[1024];
     const DynPan P = g.pan[g.order[blockIdx.x]];
-    const int t = threadIdx.x, bs = P.bs, nv = bs * bs * bs;
-    for (int i = t; i < nv; i += 256) {     // linear box order y + bs x + bs^2 z
+    const int t = threadIdx.x, bs = P.bs, ey = P.ey, nv = ey * bs * bs;
+    for (int i = t; i < nv; i += 256) {     // linear box order y + ey x + ey bs z
         int lx, ly, lz;
         dyn_local_xyz(i, lx, ly, lz);
-        xs[ly + bs * (lx + bs * lz)] = x[P.v0 + i];
+        xs[ly + ey * (lx + bs * lz)] = x[P.v0 + i];
     }
     __syncthreads();
     for (int sg = P.s0 + t; sg < P.s1; sg += 256) {
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(NT) void k_dyn_tile_T(DynView g, const T *__restric
     static_assert(NT >= 512, "one voxel row per thread (boxes of at most 8^3)");
     const DynPan P = g.pan[g.order[blockIdx.x]];
     T *pe = reinterpret_cast<T *>(dyn_smem);
-    const int t = threadIdx.x, ns = P.s1 - P.s0, nv = P.bs * P.bs * P.bs;
+    const int t = threadIdx.x, ns = P.s1 - P.s0, nv = P.ey * P.bs * P.bs;
     // phase 2's operands do not depend on phase 1: its row starts, first
     // positions and epilogue operands are requested before the walks
     const bool has_row = t < nv;

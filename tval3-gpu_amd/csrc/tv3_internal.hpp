@@ -226,17 +226,19 @@ struct SegSum {
 struct DynPan {
     int32_t v0;          // first internal voxel of the box (its voxels are contiguous)
     int32_t bx, by, bz;  // box corner in voxel coordinates (x, y, z)
-    int32_t bs;          // box edge: 8 (a tile) or 4 (a brick)
+    int32_t bs;          // box edge along x and z: 8 (tiles) or 4 (a brick)
+    int32_t ey;          // box edge along y: bs, or 16 for two tiles consecutive in y
     int32_t s0, s1;      // its segments
     int32_t ne;          // entries of its segments
 };
 
 // One direction's boxes and segments.
 struct DynBoxes {
-    int64_t npan = 0, nseg = 0;
+    int64_t npan = 0, nseg = 0;      // work items (pan), segments
     int32_t emax = 0, smax = 0;      // most entries / segments of one box
-    DevBuf<DynPan> pan;
-    DevBuf<int32_t> order;           // boxes, decreasing work
+    DevBuf<DynPan> pan;              // A^T r: boxes; A x: work items (a box, part of its
+                                     // segments)
+    DevBuf<int32_t> order;           // work items, decreasing work
     DevBuf<int32_t> seg_ray;         // internal ray of each segment
     DevBuf<int32_t> seg_eoff;        // first entry of each segment in its box (A^T)
     DevBuf<uint16_t> seg_c0;         // box-local first voxel of each segment: x | y << 4 | z << 8
