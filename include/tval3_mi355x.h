@@ -341,7 +341,8 @@ typedef struct tv3_layout_info {
     int32_t panels;         /* bit 0: A x in panel form (per-tile ray segments, then a
                                per-ray sum); bit 1: A^T r in panel form (per-tile rays
                                staged in LDS); bit 2 / bit 3: A's / A^T's panels
-                               span two tiles */
+                               span two tiles; bit 4 / bit 5: A's panels span
+                               2 x 2 x 1 / 2 x 2 x 2 tiles */
     int32_t code_N, code_T; /* launch shapes as tv3_op_autotune codes; for a panel form its
                                row block size as -NB */
     int64_t npan, nseg;     /* tiles (panels) and ray segments (tile, ray) of the plan */
@@ -358,10 +359,12 @@ tv3_status tv3_op_prepare(tv3_op *op, const int32_t dims[3], tv3_layout_info *in
 
 /* Override the layout's tuned launch shapes (tests, experiments).  panels:
  * bit 0 puts A x, bit 1 puts A^T r in panel form, rebuilt with panels of at
- * most pan_nnz entries (bit 2 / bit 3: A's / A^T's panels of two tiles, at
- * most 2 pan_nnz entries) and row blocks of -code_N / -code_T entries (-1024,
- * -2048 or -4096); a direction not in panel form takes code_N / code_T as a
- * tv3_op_set_lanes code (0 = keep). */
+ * most pan_nnz entries per tile (bit 2 / bit 3: A's / A^T's panels of two
+ * tiles; bit 4 / bit 5: A's panels of 2 x 2 x 1 / 2 x 2 x 2 tiles, which need
+ * 8^3 tiles and an even tile count along every axis; at most one of bits 2, 4
+ * and 5) and row blocks of -code_N / -code_T entries (-1024, -2048 or -4096);
+ * a direction not in panel form takes code_N / code_T as a tv3_op_set_lanes
+ * code (0 = keep). */
 tv3_status tv3_op_layout_force(tv3_op *op, const int32_t dims[3], int32_t code_N, int32_t code_T,
                                int32_t pan_nnz, int32_t panels, tv3_layout_info *info);
 

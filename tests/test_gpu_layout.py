@@ -13,6 +13,7 @@ import pytest
 
 import tv3_oracle as orc
 from tval3_amd import api, workload as wl
+from tval3_amd._lib import TV3Error
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +87,46 @@ def test_layout_products_exact(ctx, dims, brick):
         z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
         assert (y == y_ref).all(), (dims, code_N, pan, info)
         assert (z == z_ref).all(), (dims, code_T, pan, info)
+
+
+# A's panels over 2 x 2 x 1 tiles (bit 4) and 2 x 2 x 2 tiles (bit 5), whole
+# and split into brick runs, with A^T in either form
+SHAPES_SUPER = [
+    (-2048, -2048, 16384, 19), (-1024, -1024, 300, 19), (-4096, 16, 16384, 17),
+    (-2048, -2048, 16384, 43), (-1024, -2048, 100, 33), (-2048, -1024, 16384, 35),
+]
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 16, 32), (32, 32, 16)])
+def test_layout_supertile_panels_exact(ctx, dims):
+    """A in panel form over boxes of 4 and 8 tiles: the tile-ordered voxel
+    rows are regrouped by 2 x 2 x 2 blocks of tiles and the panels stage x
+    through a tile map; integer inputs give the oracle's bits."""
+    op, oop, x, r = integer_case(ctx, dims, 24, 64, "f16")
+    y_ref = orc.matvec(oop, 0, x.astype(np.float64)).astype(np.float32)
+    z_ref = orc.matvec(oop, 1, r.astype(np.float64)).astype(np.float32)
+    assert np.abs(z_ref).max() < 2 ** 24 and np.abs(y_ref).max() < 2 ** 24
+    op.prepare(dims)
+    for code_N, code_T, pan, panels in SHAPES_SUPER:
+        info = op.layout_force(dims, code_N, code_T, pan, panels)
+        assert info["panels"] == panels
+        y = H(op.layout_mat_vec_mul(dims, T(x)))
+        z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+        assert (y == y_ref).all(), (dims, code_N, pan, info)
+        assert (z == z_ref).all(), (dims, code_T, pan, info)
+
+
+def test_layout_supertile_panels_need_even_tiles(ctx):
+    """24 x 16 x 8 voxels hold 3 x 2 x 1 tiles: no 2 x 2 x 2 blocks, so the
+    larger panels are refused, and two bits for A's panel size at once are
+    an argument error."""
+    dims = (24, 16, 8)
+    op, _, _, _ = integer_case(ctx, dims, 8, 16, "f16")
+    op.prepare(dims)
+    with pytest.raises(TV3Error, match="even tile count"):
+        op.layout_force(dims, -2048, -2048, 16384, 19)
+    with pytest.raises(TV3Error, match="at most one"):
+        op.layout_force(dims, -2048, -2048, 16384, 1 | 4 | 16)
 
 
 def test_layout_panels_with_empty_tiles(ctx):

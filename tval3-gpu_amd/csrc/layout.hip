@@ -215,7 +215,10 @@ static bool plan_panels(const HostCSR &h, int unit, int pan_nnz, PanelPlan &pl) 
 // to stage again); A^T's whole (each chunk would gather the panel's rays
 // again).  Measured at 128^3 (A, A^T in us): whole 66.0, 56.1; chunks of 3
 // 65.0, 64.3; of 6 64.7, 58.7; of 9 67.7, 56.2.
-constexpr int kPanelChunkA = 6, kPanelChunkT = 1 << 30;
+#ifndef TV3_PANEL_CHUNK_A
+#define TV3_PANEL_CHUNK_A 6
+#endif
+constexpr int kPanelChunkA = TV3_PANEL_CHUNK_A, kPanelChunkT = 1 << 30;
 
 // Entry groups of the panel rows: A's rows (ray segments, ~8 entries at
 // 128^3) unpadded, A^T's (voxels, ~17) in groups of 4.  Measured at 128^3,
@@ -382,7 +385,8 @@ static void build_panels_T(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
 // its ray in panel p's list), sources = the panel's voxel rows; and the
 // per-ray segment lists of the second pass.
 static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, int blk_nnz,
-                           Panels &P, SegSum &R) {
+                           Panels &P, SegSum &R, int chunk_blocks = kPanelChunkA,
+                           const std::vector<int32_t> *tmap = nullptr) {
     const int64_t npan = (int64_t)pl.pan_row.size() - 1, nseg = (int64_t)pl.seg_ray.size();
     const int64_t m = h.cols;
     std::vector<int32_t> local(m > 0 ? m : 1, 0), sptr(nseg + 1, 0);
@@ -408,7 +412,13 @@ static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
     }
     // panels of the segment matrix: segment ranges pan_seg, voxel ranges pan_row
     pack_panels(ctx, pl.pan_seg, pl.pan_row, nullptr, sptr, lidx, wb.data(), h.vb, h.vtype,
-                blk_nnz, kPanelGroupA, kPanelChunkA, P);
+                blk_nnz, kPanelGroupA, chunk_blocks, P);
+    if (tmap) {
+        // h's rows in super-tile order: the panels stage x through the tile map
+        P.src.alloc(tmap->size());
+        TV3_HIP(hipMemcpyAsync(P.src.p, tmap->data(), sizeof(int32_t) * tmap->size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    }
     // segments of each ray, in increasing segment number
     std::vector<int32_t> rptr(m + 1, 0), rseg(nseg > 0 ? nseg : 1);
     for (int64_t s = 0; s < nseg; ++s) ++rptr[pl.seg_ray[s] + 1];
@@ -431,6 +441,48 @@ static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
         TV3_HIP(hipMemcpyAsync(R.seg.p, rseg.data(), sizeof(int32_t) * rseg.size(),
                                hipMemcpyHostToDevice, st));
     TV3_HIP(hipStreamSynchronize(st));
+}
+
+// The rows of A^T (internal voxels) in super-tile order: blocks of 2 x 2 x 2
+// tiles of the B = 8 numbering, the tiles of a block in (y, x, z) bit order,
+// so that 2, 4 and 8 consecutive tiles form boxes of 16 x 8 x 8, 16 x 16 x 8
+// and 16^3 voxels.  A ray crossing the volume leaves 14.9 segments in boxes of
+// 16 x 8 x 8 voxels, 12.0 in 16 x 16 x 8 and 9.1 in 16^3 at 128^3 (spherical
+// geometry, 256 x 1024 rays): fewer partial sums for the second pass.
+// tmap[t'] = internal tile of super-tile-order tile t'.  Needs an even tile
+// count along every axis.
+static bool supertile_rows(const FusedLayout &F, const HostCSR &h, HostCSR &hs,
+                           std::vector<int32_t> &tmap) {
+    if (F.B != 8) return false;
+    const int nby = F.nby, nbx = F.nbx, nbz = F.dims[2] / 8;
+    if (nby % 2 || nbx % 2 || nbz % 2) return false;
+    const int64_t ntile = (int64_t)nby * nbx * nbz;
+    if (h.rows != ntile * 512) return false;
+    tmap.assign((size_t)ntile, 0);
+    for (int tz = 0; tz < nbz; ++tz)
+        for (int tx = 0; tx < nbx; ++tx)
+            for (int ty = 0; ty < nby; ++ty) {
+                const int64_t t = ((int64_t)tz * nbx + tx) * nby + ty;
+                const int64_t S = ((int64_t)(tz >> 1) * (nbx / 2) + (tx >> 1)) * (nby / 2) + (ty >> 1);
+                const int w = ((tz & 1) << 2) | ((tx & 1) << 1) | (ty & 1);
+                tmap[(size_t)(S * 8 + w)] = (int32_t)t;
+            }
+    hs.rows = h.rows; hs.cols = h.cols; hs.nnz = h.nnz; hs.vb = h.vb; hs.vtype = h.vtype;
+    hs.ptr.assign((size_t)h.rows + 1, 0);
+    hs.ind.resize(h.ind.size());
+    hs.val.resize(h.val.size());
+    int64_t at = 0;
+    for (int64_t ts = 0; ts < ntile; ++ts) {
+        const int64_t r0 = (int64_t)tmap[(size_t)ts] * 512;
+        const int64_t k0 = h.ptr[r0], k1 = h.ptr[r0 + 512];
+        for (int i = 0; i < 512; ++i) hs.ptr[ts * 512 + i] = (int32_t)(at + h.ptr[r0 + i] - k0);
+        std::copy(h.ind.begin() + k0, h.ind.begin() + k1, hs.ind.begin() + at);
+        std::copy(h.val.begin() + k0 * (int64_t)h.vb, h.val.begin() + k1 * (int64_t)h.vb,
+                  hs.val.begin() + at * (int64_t)h.vb);
+        at += k1 - k0;
+    }
+    hs.ptr[(size_t)h.rows] = (int32_t)at;
+    return true;
 }
 
 // Median duration (ms) of `reps` launches of f on the context stream, each
@@ -494,8 +546,9 @@ static float tune_shape(tv3_ctx *ctx, ColdTimer &time_launches, SparseRows &S, i
 // voxels, at most 16384 entries) or of two consecutive tiles (16 x 8 x 8
 // voxels, at most 32768 entries: A's rays leave fewer, longer segments and
 // so fewer partial sums for the second pass; A^T stages the union of the two
-// tiles' rays once), timed over row blocks of 1024 and 2048 entries; each
-// kept when faster than the tuned plain kernel (t_A, t_T in ms).
+// tiles' rays once), timed over row blocks of 1024 and 2048 entries; A also
+// over panels of 4 and 8 tiles where the tile counts are even; each kept
+// when faster than the tuned plain kernel (t_A, t_T in ms).
 static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, float t_A,
                         float t_T) {
     hipStream_t s = ctx->stream;
@@ -539,6 +592,34 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
             Panels P;
             SegSum R;
             build_panels_A(ctx, h, pa, nb, P, R);
+            const float t = time_launches([&]() {
+                launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
+                launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
+            });
+            if (t < t_A) {
+                t_A = t;
+                F.PA = std::move(P);
+                F.RS = std::move(R);
+                F.panels_A = true;
+                F.unit_A = unit;
+            }
+        }
+    }
+    // A over panels of 4 and 8 tiles (16 x 16 x 8 and 16^3 voxels), row
+    // blocks of 2048 or 4096 entries.  Measured at 128^3 (A in us, work items
+    // of 6 blocks): two tiles 67.9 (blocks of 2048); four tiles 61.9 (2048),
+    // 63.7 (4096); eight tiles 64.3 (2048), 60.4 (4096).  Items of 3, 12 or
+    // 24 blocks were slower for every panel size.
+    HostCSR hs;
+    std::vector<int32_t> tmap;
+    if (!supertile_rows(F, h, hs, tmap)) return;
+    for (int unit : {2048, 4096}) {
+        PanelPlan pl2;
+        if (!plan_panels(hs, unit, 16384 * unit / 512, pl2)) continue;
+        for (int nb : {2048, 4096}) {
+            Panels P;
+            SegSum R;
+            build_panels_A(ctx, hs, pl2, nb, P, R, kPanelChunkA, &tmap);
             const float t = time_launches([&]() {
                 launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
                 launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
@@ -692,7 +773,8 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
     F.session.reset();               // its graphs hold the old launch shapes
     tv3_ctx *ctx = op->ctx;
     const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0;
-    const int unit_A = (panels & 4) ? 1024 : 512, unit_T = (panels & 8) ? 1024 : 512;
+    const int unit_A = (panels & 32) ? 4096 : (panels & 16) ? 2048 : (panels & 4) ? 1024 : 512;
+    const int unit_T = (panels & 8) ? 1024 : 512;
     if (pa || pt) {
         TV3_REQUIRE(pan_nnz > 0, TV3_E_ARG, "panels need pan_nnz > 0");
         for (int c : {pa ? code_N : -1024, pt ? code_T : -1024})
@@ -717,7 +799,18 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
         }
         F.unit_T = unit_T;
         if (pa && unit_A == 512) build_panels_A(ctx, h, pl, -code_N, F.PA, F.RS);
-        if (pa && unit_A != 512) {
+        if (pa && unit_A > 1024) {
+            HostCSR hs;
+            std::vector<int32_t> tmap;
+            TV3_REQUIRE(supertile_rows(F, h, hs, tmap), TV3_E_UNSUPPORTED,
+                        "panels of 4 or 8 tiles need 8^3 tiles and an even tile count along "
+                        "every axis");
+            PanelPlan pl2;
+            TV3_REQUIRE(plan_panels(hs, unit_A, pan_nnz * (unit_A / 512), pl2), TV3_E_UNSUPPORTED,
+                        "a panel of this operator would need more rays than LDS holds");
+            build_panels_A(ctx, hs, pl2, -code_N, F.PA, F.RS, kPanelChunkA, &tmap);
+        }
+        if (pa && unit_A == 1024) {
             PanelPlan pl2;
             TV3_REQUIRE(plan_panels(h, unit_A, 2 * pan_nnz, pl2), TV3_E_UNSUPPORTED,
                         "a panel of this operator would need more rays than LDS holds");

@@ -7,10 +7,13 @@
 // LDS (one coalesced or lightly scattered pass), after which every entry
 // reads x from LDS through a 16-bit index.
 //
-//   A^T r: rows = voxels of an 8x8x8 tile, sources = the rays crossing it.
-//   A u:   pass 1 rows = ray segments (the part of one ray inside one tile),
-//          sources = the tile's 512 voxels (a contiguous range); pass 2 sums
-//          each ray's segment partials (k_seg_sum) and runs the epilogue.
+//   A^T r: rows = voxels of one or two 8x8x8 tiles, sources = the rays
+//          crossing them.
+//   A u:   pass 1 rows = ray segments (the part of one ray inside a box of
+//          1, 2, 4 or 8 tiles), sources = the box's voxels (a contiguous
+//          range, or for 4 and 8 tiles one in a tile order of its own);
+//          pass 2 sums each ray's segment partials (k_seg_sum) and runs the
+//          epilogue.
 //
 // A row block's rows are stored back to back from a 4-aligned start, each
 // padded with zero entries to a multiple of the group size G (1, 2 or 4),
@@ -144,9 +147,17 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
     // stage the sources: batches of 8 per thread, all loads in flight
     for (int k0 = 0; k0 < ns; k0 += 8 * 256) {
         float xv[8];
-        if (CONTIG) {
+        if (CONTIG && src == nullptr) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) xv[i] = x[s0 + min(k0 + i * 256 + t, ns - 1)];
+        } else if (CONTIG) {
+            // index range in a tile order of its own: src maps its 512-row
+            // tiles to the tiles of x
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = s0 + min(k0 + i * 256 + t, ns - 1);
+                xv[i] = x[((int64_t)src[k >> 9] << 9) | (k & 511)];
+            }
         } else {
             int rv[8];
 #pragma unroll

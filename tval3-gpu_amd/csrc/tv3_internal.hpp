@@ -197,7 +197,9 @@ struct Panels {
     DevBuf<int32_t> order;     // work items, decreasing work
     DevBuf<int32_t> chunk;     // per work item: panel, first block, end block
     DevBuf<int32_t> src_off;   // npan + 1 offsets into src (or index ranges)
-    DevBuf<int32_t> src;       // nsrc source indices (empty when contiguous)
+    DevBuf<int32_t> src;       // nsrc source indices; when contiguous, empty, or the map
+                               // of index ranges in super-tile order: tile t' of the
+                               // range space is tile src[t'] of x (512 indices a tile)
     DevBuf<int32_t> pptr;      // rows + 1 padded row starts
     DevBuf<BlkDesc> desc;      // row blocks (+1 sentinel); p0, len in padded entries
     DevBuf<uint32_t> pk;       // padded packed entries (16-bit weights)
@@ -281,7 +283,7 @@ struct FusedLayout {
     Panels PT, PA;
     SegSum RS;
     bool panels_T = false, panels_A = false;
-    int unit_A = 512;                 // voxels per panel of A's panel form (1 or 2 tiles)
+    int unit_A = 512;                 // voxels per panel of A's panel form (1, 2, 4 or 8 tiles)
     int unit_T = 512;                 // voxels per panel of A^T's panel form (1 or 2 tiles)
     int64_t npan = 0, nseg = 0;
     int max_seg = 0;
