@@ -304,11 +304,13 @@ def test_fused_tiny_gradients_stay_finite(ctx):
 
 
 @pytest.mark.parametrize("code_N,code_T,panels", [(8, 8, 0), (-1024, -1024, 0), (-1024, -1024, 1),
-                                                  (-1024, -1024, 2), (-2048, -2048, 3)])
+                                                  (-1024, -1024, 2), (-2048, -2048, 3),
+                                                  (-2048, -2048, 19), (-4096, -2048, 43)])
 def test_fused_every_product_form_tracks_oracle(ctx, code_N, code_T, panels):
     """The fused solve with each form of its products forced (row groups, row
     blocks, the panel form of A x -- whose second pass stores the line-search
-    partial sums per workgroup -- and of A^T r): rounding-level agreement
+    partial sums per workgroup -- over boxes of 1, 4 and 8 tiles, and of A^T
+    r over one or two tiles): rounding-level agreement
     with the FP32 oracle at the start and the oracle's own perturbation
     envelope after (regression: a pass-2 grid that differed from the count
     its partial sums are read with gave wrong line-search sums)."""
