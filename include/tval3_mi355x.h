@@ -218,8 +218,10 @@ tv3_status tv3_matvec_noalloc(tv3_op *op, tv3_trans trans, const float *x_dev, f
  * multi-layer case (2d/mat_vec_mul.h:91-99).  A dense BF16 operator whose
  * sizes are multiples of 256 runs them on the matrix cores, 64 columns per
  * product (each FP32 value of X split into three BF16 terms, hi + mid + lo,
- * FP32 accumulation); any other
- * operator applies tv3_matvec per column.  Asynchronous on stream. */
+ * FP32 accumulation; the first such call builds copies of A and A^T in the
+ * matrix cores' operand order, 2 m n bytes each, kept with the operator);
+ * any other operator applies tv3_matvec per column.  Asynchronous on
+ * stream. */
 tv3_status tv3_matvec_multi(tv3_op *op, tv3_trans trans, int32_t nrhs, const float *X_dev,
                             int64_t ldx, float *Y_dev, int64_t ldy, void *stream);
 

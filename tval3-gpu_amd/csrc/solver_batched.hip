@@ -1127,15 +1127,15 @@ struct Batch {
     void group_N(const DGroup &g, const float *x, float *y, int splits,
                  const int32_t *gate = nullptr) {
         for (int c = 0; c < g.G; c += 64)
-            gemm_rhs((const uint16_t *)g.op->dense_rm.p, g.op->n, g.op->m, g.op->n,
-                     x + g.v0 + c * g.ldn, g.ldn, std::min(64, g.G - c), y + g.r0 + c * g.ldm,
-                     g.ldm, nullptr, splits, gpart.p, s, gate);
+            gemm_rhs((const uint16_t *)g.op->dense_oN.p, g.op->m, g.op->n, x + g.v0 + c * g.ldn,
+                     g.ldn, std::min(64, g.G - c), y + g.r0 + c * g.ldm, g.ldm, nullptr, splits,
+                     gpart.p, s, gate);
     }
     void group_T(const DGroup &g, const float *x, float *y, const float *sub, int splits) {
         for (int c = 0; c < g.G; c += 64)
-            gemm_rhs((const uint16_t *)g.op->dense.p, g.op->lda, g.op->n, g.op->m,
-                     x + g.r0 + c * g.ldm, g.ldm, std::min(64, g.G - c), y + g.v0 + c * g.ldn,
-                     g.ldn, sub ? sub + g.v0 + c * g.ldn : nullptr, splits, gpart.p, s);
+            gemm_rhs((const uint16_t *)g.op->dense_oT.p, g.op->n, g.op->m, x + g.r0 + c * g.ldm,
+                     g.ldm, std::min(64, g.G - c), y + g.v0 + c * g.ldn, g.ldn,
+                     sub ? sub + g.v0 + c * g.ldn : nullptr, splits, gpart.p, s);
     }
     // gate (optional): device count gating the sparse launches; ggate: one
     // per dense group
@@ -1613,7 +1613,7 @@ static std::unique_ptr<BatchSession> build_session(tv3_ctx *ctx, int P, tv3_op *
                     "dense operators of a batch must hold BF16 values (matrix-core products)");
         TV3_REQUIRE(op->m % 256 == 0 && op->n % 256 == 0 && op->lda % 8 == 0, TV3_E_SHAPE,
                     "dense operators of a batch need rows and columns multiples of 256");
-        dense_rowmajor(op, s);
+        dense_operand_order(op, s);
         Batch::DGroup g;
         g.op = op; g.k0 = k; g.G = e - k;
         g.v0 = v0[k]; g.r0 = r0[k]; g.ldn = pad512(op->n); g.ldm = pad256(op->m);

@@ -572,19 +572,11 @@ tv3_status tv3_matvec_multi(tv3_op *op, tv3_trans trans, int32_t nrhs, const flo
         const size_t need = gemm_rhs_scratch(R, splits) * sizeof(float);
         if (op->ctx->scratch.bytes() < need) op->ctx->scratch.alloc(need);
         float *part = reinterpret_cast<float *>(op->ctx->scratch.p);
-        const uint16_t *M;
-        int64_t ldm;
-        if (!T) {
-            dense_rowmajor(op, s);
-            M = (const uint16_t *)op->dense_rm.p;
-            ldm = op->n;
-        } else {
-            M = (const uint16_t *)op->dense.p;
-            ldm = op->lda;
-        }
+        dense_operand_order(op, s);
+        const uint16_t *M = (const uint16_t *)(T ? op->dense_oT.p : op->dense_oN.p);
         for (int32_t c = 0; c < nrhs; c += 64)
-            gemm_rhs(M, ldm, R, K, X + c * ldx, ldx, std::min(64, nrhs - c), Y + c * ldy, ldy,
-                     nullptr, splits, part, s);
+            gemm_rhs(M, R, K, X + c * ldx, ldx, std::min(64, nrhs - c), Y + c * ldy, ldy, nullptr,
+                     splits, part, s);
         TV3_HIP(hipGetLastError());
     });
 }

@@ -338,7 +338,8 @@ struct tv3_op {
     tv3_dtype dtype = TV3_F32;
     tv3::DevBuf<uint8_t> dense;
     int64_t lda = 0;
-    tv3::DevBuf<uint8_t> dense_rm;   // BF16 row-major copy (multi-RHS products), on demand
+    tv3::DevBuf<uint8_t> dense_oN, dense_oT;   // A and A^T in MFMA operand order (multi-RHS
+                                               // products), built on demand
     // DYNAMIC / HYBRID
     tv3::DynGeom geo;
     // CSR built from a geometry: the locality order of its rays (may be empty)
@@ -376,16 +377,17 @@ void dense_gemv_ws(tv3_op *op, const float *x, float *y, hipStream_t s, void *ws
 void noalloc_setup(int device);
 
 // dense multi-right-hand-side products on the matrix cores (dense_mfma.hip):
-// Y = M X (- sub), M row-major BF16 R x K, X/Y column-major, G <= 64; a
+// Y = M X (- sub), M BF16 R x K in MFMA operand order (dense_operand_order),
+// X/Y column-major, G <= 64; a
 // non-null gate in device memory turns the launch into a no-op while *gate
 // is 0 (graph-captured launches that only some rounds need)
 int gemm_rhs_rows();   // rows per workgroup (R must be a multiple)
 bool gemm_rhs_shape_ok(int64_t R, int64_t K, int splits);
-void gemm_rhs(const uint16_t *M, int64_t ldm, int64_t R, int64_t K, const float *X, int64_t ldx,
+void gemm_rhs(const uint16_t *M, int64_t R, int64_t K, const float *X, int64_t ldx,
               int G, float *Y, int64_t ldy, const float *sub, int splits, float *part,
               hipStream_t s, const int32_t *gate = nullptr);
 size_t gemm_rhs_scratch(int64_t R, int splits);   // floats
-void dense_rowmajor(tv3_op *op, hipStream_t s);     // builds op->dense_rm once
+void dense_operand_order(tv3_op *op, hipStream_t s);   // builds op->dense_oN / _oT once
 
 // dynamic (dynamic.hip)
 void dynamic_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_t s,
