@@ -284,9 +284,13 @@ typedef struct tv3_batch_info {
                                    out: the block size used, 0 when not used */
     int64_t dense_values;       /* entries of the shared dense operators                */
     int32_t panels_A;           /* A of the sparse problems in two-pass panel form
-                                   (segment sums per tile, then per ray); in and out
-                                   like panels_T                                        */
-    int32_t reserved;
+                                   (segment sums per panel, then per ray); in and out
+                                   like panels_T (4096 too with panel_box_A = 8)        */
+    int32_t panel_box_A;        /* tiles per panel of that form: 1 (8^3 voxels) or 8
+                                   (16^3: fewer, longer ray segments; every sparse
+                                   problem on 8^3 tiles with even tile counts); in: 0
+                                   lets the timing choose when panels_A is 0 and means
+                                   1 otherwise; out: the size used, 0 without panels */
 } tv3_batch_info;
 
 tv3_status tv3_solve_batched(tv3_ctx *ctx, int32_t nprob, tv3_op *const *ops,

@@ -71,17 +71,26 @@ def batch8(ctx):
     return [make(ctx, *pp) for pp in PROBS8]
 
 
+# even tile counts along every axis: A's panels may span 16^3 voxels
+PROBS16 = [((16, 16, 16), 24, 64, 41), ((32, 16, 16), 20, 56, 44), ((16, 32, 16), 16, 48, 47)]
+
+
+@pytest.fixture(scope="module")
+def batch16(ctx):
+    return [make(ctx, *pp) for pp in PROBS16]
+
+
 def opts(**kw):
     base = dict(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, tv_mode="iso", maxit=60)
     base.update(kw)
     return api.default_opts(**base)
 
 
-def solve(ctx, probs, o, lanes=(8, 8), history=True, panels=-1, panels_A=-1):
+def solve(ctx, probs, o, lanes=(8, 8), history=True, panels=-1, panels_A=-1, box=None):
     return api.tval3_gpu_3d_batched(ctx, [p["A"] for p in probs], [p["bt"] for p in probs],
                                     [p["dims"] for p in probs], o, Uts=[p["ut"] for p in probs],
                                     history=history, lanes=lanes, panels=panels,
-                                    panels_A=panels_A)
+                                    panels_A=panels_A, panel_box_A=box)
 
 
 @pytest.mark.parametrize("lanes,panels,panels_A", [((8, 8), -1, -1), ((-2048, -1024), -1, -1),
@@ -140,13 +149,16 @@ def first_divergence(h1, h2):
     return n
 
 
-@pytest.mark.parametrize("which,panels", [("batch", -1), ("batch8", 2048)])
-def test_batch_tracks_oracle(ctx, request, which, panels):
-    """(batch8 with A^T and A in panel form)"""
+@pytest.mark.parametrize("which,panels,box", [("batch", -1, None), ("batch8", 2048, None),
+                                              ("batch16", 2048, 8)])
+def test_batch_tracks_oracle(ctx, request, which, panels, box):
+    """(batch8 with A^T and A in panel form; batch16 with A's panels over
+    16^3 voxels)"""
     batch = request.getfixturevalue(which)
     o = opts()
-    Ub, ib, bi, hb = solve(ctx, batch, o, panels=panels, panels_A=panels)
+    Ub, ib, bi, hb = solve(ctx, batch, o, panels=panels, panels_A=panels, box=box)
     assert bi["panels_T"] == bi["panels_A"] == max(panels, 0)
+    assert bi["panel_box_A"] == (box or (1 if panels > 0 else 0))
     oo = orc.default_opts(mu=2.0 ** 9, mu0=2.0 ** 9, tol_inn=1e-3, maxit=60, tv_iso=1)
     for k, p in enumerate(batch):
         Uo, io, ho, _ = orc.solve(p["oop"], p["dims"], p["b"], oo, Ut=p["u"], dtype=np.float32)
@@ -324,6 +336,33 @@ def test_batch_panels_equal_solo_bitwise(ctx, batch8, blk):
     # timed choice (0): one of the two sizes or the row kernels, reported
     _, _, bt, _ = solve(ctx, batch8, o, panels=0, panels_A=0, history=False)
     assert bt["panels_T"] in (0, 1024, 2048) and bt["panels_A"] in (0, 1024, 2048)
+
+
+@pytest.mark.parametrize("blk", [2048, 4096])
+def test_batch_box_panels_equal_solo_bitwise(ctx, batch16, blk):
+    """A in two-pass panel form over 16^3 voxels (the tiles of each problem
+    regrouped by 2 x 2 x 2, x staged through a tile map): each problem of the
+    batch equals its solo solve bit for bit, and the batch agrees with the
+    one-tile panels to rounding level at the start; panels of 8 tiles are
+    refused for a batch with an odd tile count."""
+    o = opts(maxit=80, tol_inn=5e-3, mu0=2.0 ** 6, beta0=4.0)
+    Ub, ib, bi, hb = solve(ctx, batch16, o, panels=2048, panels_A=blk, box=8)
+    assert (bi["panels_A"], bi["panel_box_A"]) == (blk, 8)
+    for k, p in enumerate(batch16):
+        Us, is_, bs, hs = solve(ctx, [p], o, panels=2048, panels_A=blk, box=8)
+        assert (bs["panels_A"], bs["panel_box_A"]) == (blk, 8)
+        assert ib[k]["total_iters"] == is_[0]["total_iters"] > 0
+        assert np.array_equal(hb[k]["f"], hs[0]["f"])
+        assert np.array_equal(Ub[k].cpu().numpy(), Us[0].cpu().numpy()), k
+    U1, i1, b1, h1 = solve(ctx, batch16, o, panels=2048, panels_A=2048)
+    assert (b1["panels_A"], b1["panel_box_A"]) == (2048, 1)
+    for k in range(len(batch16)):
+        f, f1 = hb[k]["f"][:4], h1[k]["f"][:4]
+        assert (np.abs(f - f1) <= 1e-5 * np.abs(f1)).all()
+    _, _, bt, _ = solve(ctx, batch16, o, panels=0, panels_A=0, history=False)
+    assert bt["panel_box_A"] in (0, 1, 8)
+    with pytest.raises(TV3Error, match="even tile count"):
+        solve(ctx, [make(ctx, (24, 16, 16), 8, 16, 50)], o, panels_A=2048, box=8)
 
 
 def test_batch_backtracking_blends(ctx, batch):

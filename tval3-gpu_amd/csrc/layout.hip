@@ -449,24 +449,30 @@ static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
 // and 16^3 voxels.  A ray crossing the volume leaves 14.9 segments in boxes of
 // 16 x 8 x 8 voxels, 12.0 in 16 x 16 x 8 and 9.1 in 16^3 at 128^3 (spherical
 // geometry, 256 x 1024 rays): fewer partial sums for the second pass.
-// tmap[t'] = internal tile of super-tile-order tile t'.  Needs an even tile
-// count along every axis.
-static bool supertile_rows(const FusedLayout &F, const HostCSR &h, HostCSR &hs,
+// `vols` lists the volumes in the rows (one for a single operator, one per
+// sparse problem of a batch): their first row (a multiple of 4096) and tile
+// counts (even along every axis); rows outside them keep their place.
+// tmap[t'] = internal tile of super-tile-order tile t'.
+static bool supertile_rows(const std::vector<TileVolume> &vols, const HostCSR &h, HostCSR &hs,
                            std::vector<int32_t> &tmap) {
-    if (F.B != 8) return false;
-    const int nby = F.nby, nbx = F.nbx, nbz = F.dims[2] / 8;
-    if (nby % 2 || nbx % 2 || nbz % 2) return false;
-    const int64_t ntile = (int64_t)nby * nbx * nbz;
-    if (h.rows != ntile * 512) return false;
-    tmap.assign((size_t)ntile, 0);
-    for (int tz = 0; tz < nbz; ++tz)
-        for (int tx = 0; tx < nbx; ++tx)
-            for (int ty = 0; ty < nby; ++ty) {
-                const int64_t t = ((int64_t)tz * nbx + tx) * nby + ty;
-                const int64_t S = ((int64_t)(tz >> 1) * (nbx / 2) + (tx >> 1)) * (nby / 2) + (ty >> 1);
-                const int w = ((tz & 1) << 2) | ((tx & 1) << 1) | (ty & 1);
-                tmap[(size_t)(S * 8 + w)] = (int32_t)t;
-            }
+    if (h.rows % 512 != 0 || vols.empty()) return false;
+    const int64_t ntile = h.rows / 512;
+    tmap.resize((size_t)ntile);
+    std::iota(tmap.begin(), tmap.end(), 0);
+    for (const TileVolume &v : vols) {
+        if (v.row0 % 4096 != 0 || v.nby % 2 || v.nbx % 2 || v.nbz % 2) return false;
+        const int64_t t0 = v.row0 / 512;
+        if (t0 + (int64_t)v.nby * v.nbx * v.nbz > ntile) return false;
+        for (int tz = 0; tz < v.nbz; ++tz)
+            for (int tx = 0; tx < v.nbx; ++tx)
+                for (int ty = 0; ty < v.nby; ++ty) {
+                    const int64_t t = ((int64_t)tz * v.nbx + tx) * v.nby + ty;
+                    const int64_t S =
+                        ((int64_t)(tz >> 1) * (v.nbx / 2) + (tx >> 1)) * (v.nby / 2) + (ty >> 1);
+                    const int w = ((tz & 1) << 2) | ((tx & 1) << 1) | (ty & 1);
+                    tmap[(size_t)(t0 + S * 8 + w)] = (int32_t)(t0 + t);
+                }
+    }
     hs.rows = h.rows; hs.cols = h.cols; hs.nnz = h.nnz; hs.vb = h.vb; hs.vtype = h.vtype;
     hs.ptr.assign((size_t)h.rows + 1, 0);
     hs.ind.resize(h.ind.size());
@@ -482,6 +488,14 @@ static bool supertile_rows(const FusedLayout &F, const HostCSR &h, HostCSR &hs,
         at += k1 - k0;
     }
     hs.ptr[(size_t)h.rows] = (int32_t)at;
+    return true;
+}
+
+// the fused layout's volume as one TileVolume (B = 8 only)
+static bool layout_volume(const FusedLayout &F, std::vector<TileVolume> &vols) {
+    vols.clear();
+    if (F.B != 8) return false;
+    vols.push_back({0, F.nby, F.nbx, F.dims[2] / 8});
     return true;
 }
 
@@ -612,7 +626,8 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
     // 24 blocks were slower for every panel size.
     HostCSR hs;
     std::vector<int32_t> tmap;
-    if (!supertile_rows(F, h, hs, tmap)) return;
+    std::vector<TileVolume> vols;
+    if (!layout_volume(F, vols) || !supertile_rows(vols, h, hs, tmap)) return;
     for (int unit : {2048, 4096}) {
         PanelPlan pl2;
         if (!plan_panels(hs, unit, 16384 * unit / 512, pl2)) continue;
@@ -649,12 +664,19 @@ bool build_transposed_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vect
 
 bool build_forward_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
                           const std::vector<int64_t> &row_starts, std::vector<Panels> &P,
-                          std::vector<SegSum> &R, std::vector<int32_t> &seg_starts) {
+                          std::vector<SegSum> &R, std::vector<int32_t> &seg_starts,
+                          const std::vector<TileVolume> *boxes) {
     if (dtype_size(AT.vtype) != 2 && AT.vtype != TV3_F32) return false;
-    const HostCSR h = download_csr(ctx, AT);
+    const HostCSR h0 = download_csr(ctx, AT);
+    HostCSR hs;
+    std::vector<int32_t> tmap;
+    if (boxes && !supertile_rows(*boxes, h0, hs, tmap)) return false;
+    const HostCSR &h = boxes ? hs : h0;
+    const int unit = boxes ? 4096 : 512;
     PanelPlan pl;
-    if (!plan_panels(h, 512, 16384, pl)) return false;
-    // panels never straddle a 512-row unit, so each range's segments are
+    if (!plan_panels(h, unit, 16384 * unit / 512, pl)) return false;
+    // panels never straddle a unit (512 rows, or 4096 for 16^3 boxes, whose
+    // volumes start at multiples of 4096 rows), so each range's segments are
     // those of the panels from its first row on
     seg_starts.assign(row_starts.size() + 1, (int32_t)pl.seg_ray.size());
     size_t p = 0;
@@ -664,7 +686,8 @@ bool build_forward_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<
     }
     P.clear(); R.clear();
     P.resize(blks.size()); R.resize(blks.size());
-    for (size_t i = 0; i < blks.size(); ++i) build_panels_A(ctx, h, pl, blks[i], P[i], R[i]);
+    for (size_t i = 0; i < blks.size(); ++i)
+        build_panels_A(ctx, h, pl, blks[i], P[i], R[i], kPanelChunkA, boxes ? &tmap : nullptr);
     return true;
 }
 
@@ -802,7 +825,9 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
         if (pa && unit_A > 1024) {
             HostCSR hs;
             std::vector<int32_t> tmap;
-            TV3_REQUIRE(supertile_rows(F, h, hs, tmap), TV3_E_UNSUPPORTED,
+            std::vector<TileVolume> vols;
+            TV3_REQUIRE(layout_volume(F, vols) && supertile_rows(vols, h, hs, tmap),
+                        TV3_E_UNSUPPORTED,
                         "panels of 4 or 8 tiles need 8^3 tiles and an even tile count along "
                         "every axis");
             PanelPlan pl2;

@@ -1073,6 +1073,7 @@ struct Batch {
     SegSum RS;
     DevBuf<int32_t> segst;           // first segment of each sparse problem (+ total)
     int panels_A = 0;
+    int box_A = 0;                   // tiles per panel of that form: 1 or 8 (16^3 voxels)
     int ncsr = 0;                    // sparse problems (the first ncsr)
     DevBuf<BDims> ddims;
     DevBuf<int32_t> vst, rst;
@@ -1380,43 +1381,78 @@ struct Batch {
             if (t < best) { best = t; PT = std::move(cand[i]); panels_T = blks[i]; }
         }
     }
-    // A in two-pass panel form (segment sums per tile, then per ray): want as
-    // for choose_panels
-    void choose_panels_A(int want, const std::vector<int64_t> &v0) {
+    // A in two-pass panel form (segment sums per panel, then per ray): want as
+    // for choose_panels (4096 too for boxes of 8 tiles); box = tiles per panel,
+    // 1 (8^3) or 8 (16^3: fewer, longer segments), 0 = timed with want 0 and 1
+    // otherwise.  Boxes of 8 tiles need every sparse problem on 8^3 tiles with
+    // even tile counts.
+    void choose_panels_A(int want, int box, const std::vector<int64_t> &v0) {
         panels_A = 0;
+        box_A = 0;
         if (!A.rows || want < 0) return;
-        TV3_REQUIRE(want == 0 || want == 1024 || want == 2048, TV3_E_ARG,
-                    "panels_A must be 1024, 2048, -1 or 0");
-        const std::vector<int> blks = want ? std::vector<int>{want} : std::vector<int>{1024, 2048};
-        std::vector<Panels> cp;
-        std::vector<SegSum> cr;
-        std::vector<int32_t> st;
+        TV3_REQUIRE(box == 0 || box == 1 || box == 8, TV3_E_ARG, "panel_box_A must be 0, 1 or 8");
+        if (want != 0 && box == 0) box = 1;
+        TV3_REQUIRE(want == 0 || want == 1024 || want == 2048 || (want == 4096 && box == 8),
+                    TV3_E_ARG, "panels_A must be 1024, 2048 (4096 with boxes of 8 tiles), -1 or 0");
+        std::vector<TileVolume> vols;
+        bool boxes_ok = true;
+        for (int k = 0; k < ncsr; ++k) {
+            const BDims &d = dims[k];
+            if (d.B != 8 || d.v0 % 4096 != 0) { boxes_ok = false; break; }
+            vols.push_back({d.v0, d.p / 8, d.q / 8, d.r / 8});
+            if (vols.back().nby % 2 || vols.back().nbx % 2 || vols.back().nbz % 2) boxes_ok = false;
+        }
+        TV3_REQUIRE(box != 8 || boxes_ok, TV3_E_UNSUPPORTED,
+                    "panels of 8 tiles need every sparse problem on 8^3 tiles with an even tile "
+                    "count along every axis");
         const std::vector<int64_t> starts(v0.begin(), v0.begin() + ncsr);
-        if (!build_forward_panels(ctx, AT, blks, starts, cp, cr, st)) {
+        struct Cand { Panels P; SegSum R; DevBuf<int32_t> st; int blk, box; };
+        std::vector<Cand> cand;
+        auto add = [&](const std::vector<int> &blks, int bx) {
+            std::vector<Panels> cp;
+            std::vector<SegSum> cr;
+            std::vector<int32_t> st;
+            if (!build_forward_panels(ctx, AT, blks, starts, cp, cr, st,
+                                      bx == 8 ? &vols : nullptr))
+                return;
+            for (size_t i = 0; i < blks.size(); ++i) {
+                Cand c{std::move(cp[i]), std::move(cr[i]), DevBuf<int32_t>(st.size()), blks[i], bx};
+                TV3_HIP(hipMemcpyAsync(c.st.p, st.data(), sizeof(int32_t) * st.size(),
+                                       hipMemcpyHostToDevice, s));
+                cand.push_back(std::move(c));
+            }
+        };
+        if (want) {
+            add({want}, box);
+        } else {
+            if (box != 8) add({1024, 2048}, 1);
+            if (box != 1 && boxes_ok) add({2048, 4096}, 8);
+        }
+        if (cand.empty()) {
             TV3_REQUIRE(want == 0, TV3_E_UNSUPPORTED,
                         "a tile of this batch crosses too many rays for the panel form");
             return;
         }
-        segst.alloc(st.size());
-        TV3_HIP(hipMemcpyAsync(segst.p, st.data(), sizeof(int32_t) * st.size(),
-                               hipMemcpyHostToDevice, s));
-        for (SegSum &r : cr) r.rows = A.rows;      // the sparse problems' rays only
-        auto run = [&](const Panels &pp, const SegSum &rr) {
-            launch_panel(pp, U[0], EpiSegBatch{rr.part.p, segst.p, ncsr, ctl.p, P_ALL}, s);
-            launch_seg_sum(rr, EpiBatch{AuT, nullptr, rst.p, P, ctl.p, P_ALL}, s);
+        for (Cand &c : cand) c.R.rows = A.rows;      // the sparse problems' rays only
+        auto run = [&](Cand &c) {
+            launch_panel(c.P, U[0], EpiSegBatch{c.R.part.p, c.st.p, ncsr, ctl.p, P_ALL}, s);
+            launch_seg_sum(c.R, EpiBatch{AuT, nullptr, rst.p, P, ctl.p, P_ALL}, s);
         };
         int pick = want ? 0 : -1;
         if (!want) {
             float best = time_ms([&] { product(A, U[0], AuT, nullptr, rst.p, P_ALL); });
-            for (size_t i = 0; i < cp.size(); ++i) {
-                const float t = time_ms([&] { run(cp[i], cr[i]); });
+            for (size_t i = 0; i < cand.size(); ++i) {
+                const float t = time_ms([&] { run(cand[i]); });
                 if (t < best) { best = t; pick = (int)i; }
             }
         }
         if (pick >= 0) {
-            PA = std::move(cp[pick]);
-            RS = std::move(cr[pick]);
-            panels_A = blks[pick];
+            Cand &c = cand[pick];
+            segst = std::move(c.st);
+            PA = std::move(c.P);
+            RS = std::move(c.R);
+            panels_A = c.blk;
+            box_A = c.box;
         }
     }
     static constexpr int kMaxSplits = 32;
@@ -1490,6 +1526,10 @@ static bool session_matches(const BatchSession &S, int P, tv3_op *const *ops,
     auto panels_ok = [](int want, int have) { return want == 0 || (want < 0 ? 0 : want) == have; };
     if (!panels_ok(binfo->panels_T, B.panels_T) || !panels_ok(binfo->panels_A, B.panels_A))
         return false;
+    const int want_box = binfo->panel_box_A != 0 ? binfo->panel_box_A
+                         : binfo->panels_A > 0   ? 1
+                                                 : 0;
+    if (want_box != 0 && binfo->panels_A >= 0 && want_box != B.box_A) return false;
     if (binfo->splits_N > 0 && binfo->splits_T > 0)
         for (const auto &g : B.groups)
             if (g.sN != binfo->splits_N || g.sT != binfo->splits_T) return false;
@@ -1684,7 +1724,7 @@ static std::unique_ptr<BatchSession> build_session(tv3_ctx *ctx, int P, tv3_op *
     }
     B.choose_panels(binfo ? binfo->panels_T : 0, v0, ncsr);
     B.ncsr = ncsr;
-    B.choose_panels_A(binfo ? binfo->panels_A : 0, v0);
+    B.choose_panels_A(binfo ? binfo->panels_A : 0, binfo ? binfo->panel_box_A : 0, v0);
     if (binfo && binfo->splits_N > 0 && binfo->splits_T > 0) {
         for (auto &g : B.groups) {
             TV3_REQUIRE(gemm_rhs_shape_ok(g.op->m, g.op->n, binfo->splits_N) &&
@@ -1887,6 +1927,7 @@ void solve_batched(tv3_ctx *ctx, int P, tv3_op *const *ops_in, const int32_t *di
         }
         binfo->panels_T = B.panels_T;
         binfo->panels_A = B.panels_A;
+        binfo->panel_box_A = B.box_A;
         binfo->graph_iters = B.graph_iters;
         binfo->graph_launches = graph_launches;
         binfo->secs = ms / 1000.0;

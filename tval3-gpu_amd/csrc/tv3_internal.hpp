@@ -423,13 +423,22 @@ bool dyn_tile_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_
 // F16 or BF16
 bool build_transposed_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
                              std::vector<Panels> &out);
+// A volume of 8^3 tiles inside the rows of a prebuilt A^T: its first row and
+// tile counts along y, x, z
+struct TileVolume {
+    int64_t row0;
+    int nby, nbx, nbz;
+};
 // the two-pass panel form of A from a prebuilt A^T (rows of A^T = voxels in
 // units of 512), one per row-block size; seg_starts[k] = first segment of
 // the voxel range starting at row_starts[k] (row_starts ascending, multiples
-// of 512; seg_starts gets one more entry: the segment count)
+// of 512; seg_starts gets one more entry: the segment count).  With boxes,
+// panels over 16^3 voxels of those volumes (each starting at a multiple of
+// 4096 rows with even tile counts; false otherwise)
 bool build_forward_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
                           const std::vector<int64_t> &row_starts, std::vector<Panels> &P,
-                          std::vector<SegSum> &R, std::vector<int32_t> &seg_starts);
+                          std::vector<SegSum> &R, std::vector<int32_t> &seg_starts,
+                          const std::vector<TileVolume> *boxes = nullptr);
 // y = A x or A^T x with the layout's kernels, natural order in and out
 void layout_matvec(tv3_op *op, const int32_t dims[3], bool trans, const float *x, float *y,
                    hipStream_t s);

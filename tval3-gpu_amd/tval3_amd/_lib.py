@@ -85,7 +85,8 @@ class BatchInfo(C.Structure):
                 ("host_secs", C.c_double), ("total_iters", C.c_int64), ("nnz", C.c_int64),
                 ("voxels", C.c_int64), ("rays", C.c_int64), ("dense_groups", C.c_int32),
                 ("splits_N", C.c_int32), ("splits_T", C.c_int32), ("panels_T", C.c_int32),
-                ("dense_values", C.c_int64), ("panels_A", C.c_int32), ("reserved", C.c_int32)]
+                ("dense_values", C.c_int64), ("panels_A", C.c_int32),
+                ("panel_box_A", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -426,7 +426,8 @@ def tval3_gpu_3d(ctx, A, b, dims, opts=None, Ut=None, U0=None, history=True, his
 
 
 def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
-                         hist_cap=4096, lanes=None, splits=None, panels=None, panels_A=None):
+                         hist_cap=4096, lanes=None, splits=None, panels=None, panels_A=None,
+                         panel_box_A=None):
     """Solve many problems together (tv3_solve_batched): ops a list of sparse
     operators (one weight type), bs their b (float32 torch tensors on the
     device), dims a list of (dim_y, dim_x, dim_z).  Every kernel of an
@@ -437,7 +438,9 @@ def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
     products' K splits (default: timed); panels = 1024 or 2048 puts A^T of the
     sparse problems in panel form with row blocks of that size, -1 never
     (default: timed where possible); panels_A likewise for the two-pass
-    panel form of A.  Returns (list of U tensors, list of
+    panel form of A (4096 too with boxes of 8 tiles), panel_box_A = 1 or 8
+    its tiles per panel (8: 16^3 voxels; default: timed with panels_A unset,
+    else 1).  Returns (list of U tensors, list of
     info dicts, batch info dict, list of history dicts or None)."""
     import torch
     o = opts if opts is not None else default_opts()
@@ -478,6 +481,8 @@ def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
         bi.panels_T = panels
     if panels_A is not None:
         bi.panels_A = panels_A
+    if panel_box_A is not None:
+        bi.panel_box_A = panel_box_A
     torch.cuda.synchronize()
     ctx.check(L.load().tv3_solve_batched(ctx.h, P, opsa, d, C.byref(o), ba, ua, uta, infos,
                                          hp, C.byref(bi)))

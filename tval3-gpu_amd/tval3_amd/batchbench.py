@@ -119,20 +119,23 @@ def run(problems=40, maxit=100, seed=1000, fp64=True, sequential=True, dense=8, 
     splits = (bi0["splits_N"], bi0["splits_T"]) if dense > 0 else None
     panels, panels_A = bi0["panels_T"] or -1, bi0["panels_A"] or -1
     Ub, ib, bi, _ = api.tval3_gpu_3d_batched(ctx, ops, bs, dims, o, Uts=uts, lanes=lanes,
-                                             splits=splits, panels=panels, panels_A=panels_A)
+                                             splits=splits, panels=panels, panels_A=panels_A,
+                                             panel_box_A=bi0["panel_box_A"])
     out = dict(workload=dict(problems=problems, dense_problems=dense, rhs=rhs, voxels=nvox,
                              rays=rays, nnz=nnz, weights="bf16", maxit=maxit, seed=seed,
                              shapes=[list(p["dims"]) for p in probs if not p.get("dense")]),
                tuned=dict(lanes_N=bi0["lanes_N"], lanes_T=bi0["lanes_T"],
                           splits_N=bi0["splits_N"], splits_T=bi0["splits_T"],
-                          panels_T=bi0["panels_T"], panels_A=bi0["panels_A"]),
+                          panels_T=bi0["panels_T"], panels_A=bi0["panels_A"],
+                          panel_box_A=bi0["panel_box_A"]),
                batch=dict(loop_s=bi["loop_secs"], total_iters=bi["total_iters"],
                           iter_per_s=bi["total_iters"] / bi["loop_secs"],
                           graph_launches=bi["graph_launches"], secs=bi["secs"],
                           host_secs=bi["host_secs"]))
     log(f"batch: {bi['total_iters']} inner iterations in {bi['loop_secs']:.3f}s -> "
         f"{out['batch']['iter_per_s']:.0f} iter/s (lanes {lanes}, splits {splits}, "
-        f"A^T panels {bi['panels_T'] or 'off'}, A panels {bi['panels_A'] or 'off'})")
+        f"A^T panels {bi['panels_T'] or 'off'}, A panels {bi['panels_A'] or 'off'} over "
+        f"{bi['panel_box_A']} tiles)")
     sparse = [p for p in probs if not p.get("dense")]
     if dense > 0:
         dps = [p for p in probs if p.get("dense")]
