@@ -44,9 +44,14 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
+# A x boxes: 2 x 2 x 2 tiles where every tile count is even ("tiles", "bricks",
+# "box16"), tile pairs along y ("pairs"), single tiles ("singles"), bricks (B4)
 GEOMS = {"tiles": dict(E=24, R=40, dims=(16, 16, 16)),       # few rays: whole 8^3 tiles
          "bricks": dict(E=96, R=256, dims=(16, 16, 16)),     # dense: tiles split into bricks
-         "B4": dict(E=24, R=64, dims=(20, 12, 8))}           # bricks only (dims multiples of 4)
+         "B4": dict(E=24, R=64, dims=(20, 12, 8)),           # bricks only (dims multiples of 4)
+         "box16": dict(E=32, R=96, dims=(32, 16, 32)),
+         "pairs": dict(E=32, R=96, dims=(16, 24, 8)),
+         "singles": dict(E=32, R=96, dims=(24, 16, 8))}
 
 
 def geom(name, seed=2):
@@ -65,7 +70,7 @@ def test_device_csr_equals_oracle_fp32_walk(ctx, name):
     assert np.array_equal(v.astype(np.float32), ov)
 
 
-@pytest.mark.parametrize("name", ["tiles", "bricks", "B4"])
+@pytest.mark.parametrize("name", ["tiles", "bricks", "B4", "box16", "pairs", "singles"])
 def test_tile_products_same_matrix_and_repeatable(ctx, name):
     g = geom(name)
     dims = GEOMS[name]["dims"]
@@ -77,8 +82,10 @@ def test_tile_products_same_matrix_and_repeatable(ctx, name):
         assert lay["npan"] == ntiles                     # no tile split
     elif name == "bricks":
         assert lay["npan"] > ntiles                      # tiles split into bricks
-    else:
+    elif name == "B4":
         assert lay["npan"] == int(np.prod(dims)) // 64   # B = 4: bricks
+    else:
+        assert lay["npan"] >= ntiles                     # A^T: tiles or their bricks
     orp, oc, ov = orc.siddon_walk_csr(orc.siddon_op(g), np.float32)
     oop = orc.csr_op(dyn.m, dyn.n, orp, oc, ov.astype(np.float64))
     aop = orc.csr_op(dyn.m, dyn.n, orp, oc, np.abs(ov.astype(np.float64)))

@@ -137,9 +137,10 @@ def test_layout_panels_with_empty_tiles(ctx):
     y_ref = orc.matvec(oop, 0, x.astype(np.float64)).astype(np.float32)
     z_ref = orc.matvec(oop, 1, r.astype(np.float64)).astype(np.float32)
     assert (z_ref == 0).sum() > z_ref.size // 2
-    for code, pan in ((-1024, 16384), (-2048, 64)):
-        info = op.layout_force(dims, code, code, pan, 3)
-        assert info["panels"] == 3 and info["npan"] >= 16
+    for code, pan, panels in ((-1024, 16384, 3), (-2048, 64, 3), (-4096, 16384, 35),
+                              (-2048, 64, 19)):
+        info = op.layout_force(dims, code, code, pan, panels)
+        assert info["panels"] == panels and info["npan"] >= 16
         y = H(op.layout_mat_vec_mul(dims, T(x)))
         z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
         assert (y == y_ref).all()
@@ -154,12 +155,13 @@ def test_layout_panels_first_tile_empty(ctx):
     op, oop, x, r = integer_case(ctx, dims, 16, 32, "f16")
     y_ref = orc.matvec(oop, 0, x.astype(np.float64)).astype(np.float32)
     z_ref = orc.matvec(oop, 1, r.astype(np.float64)).astype(np.float32)
-    info = op.layout_force(dims, -1024, -1024, 16384, 3)
-    assert info["panels"] == 3 and info["brick"] == 8
-    y = H(op.layout_mat_vec_mul(dims, T(x)))
-    z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
-    assert (y == y_ref).all()
-    assert (z == z_ref).all()
+    for panels, code in ((3, -1024), (35, -4096)):   # A over tiles, over 16^3 boxes
+        info = op.layout_force(dims, code, -1024, 16384, panels)
+        assert info["panels"] == panels and info["brick"] == 8
+        y = H(op.layout_mat_vec_mul(dims, T(x)))
+        z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+        assert (y == y_ref).all()
+        assert (z == z_ref).all()
     assert (z_ref[:8] == 0).all()          # voxels (0..7, 0, 0) of the corner tile
 
 

@@ -3,8 +3,10 @@
 // sparsity pattern; the products (panels.hpp: k_dyn_tile_N, k_dyn_tile_T)
 // recompute every weight.
 //
-// Boxes: for A x, pairs of the fused solver's 8^3 tiles consecutive in y
-// (or single tiles, or 4^3 bricks when B = 4); for A^T r, the 8^3 tiles, or
+// Boxes: for A x, blocks of 2 x 2 x 2 of the fused solver's 8^3 tiles
+// (16^3 voxels) where every tile count is even, else pairs of tiles
+// consecutive in y (or single tiles, or 4^3 bricks when B = 4); for A^T r,
+// the 8^3 tiles, or
 // their 4^3 bricks where a tile holds more ray entries than the A^T product
 // keeps in LDS (and every brick when B = 4).  A box is convex, so a ray's
 // entries inside it are consecutive in traversal order: one segment.
@@ -25,6 +27,13 @@
 #endif
 #ifndef TV3_DYN_EMAX_BIG
 #define TV3_DYN_EMAX_BIG 19456
+#endif
+// work items of the A x boxes of 16^3 voxels: this many times the segments
+// and entries of a two-tile box's item (each item stages its box's 4,096
+// voxels).  Measured at 128^3: A 85-87 us with 1, 77 with 2, 73 with 4, 77
+// with 6, 83 with 8 (two-tile boxes: 81-83 us).
+#ifndef TV3_DYN_CHUNK16
+#define TV3_DYN_CHUNK16 4
 #endif
 
 #include "panels.hpp"
@@ -149,6 +158,34 @@ void add_pair_box(std::vector<DynPan> &pan, std::vector<int32_t> &box_of, int64_
     for (int k = 0; k < 16; ++k) box_of[(u << 3) + k] = (int32_t)pan.size() - 1;
 }
 
+// the A x box of the 2 x 2 x 2 tiles from tile u (tile coordinates all even):
+// 16^3 voxels in 8 tiles, at u, u + 1 (y), u + nby (x), u + nby nbx (z) and
+// their sums
+void add_box16(std::vector<DynPan> &pan, std::vector<int32_t> &box_of, int64_t u, int nby,
+               int nbx) {
+    const int ty = (int)(u % nby), tx = (int)((u / nby) % nbx),
+              tz = (int)(u / ((int64_t)nby * nbx));
+    pan.push_back(DynPan{(int32_t)(u << 9), 8 * tx, 8 * ty, 8 * tz, 16, 16, 0, 0, 0});
+    for (int sb = 0; sb < 8; ++sb) {
+        const int64_t t = u + (sb & 1) + ((sb >> 1) & 1) * (int64_t)nby +
+                          (sb >> 2) * (int64_t)nby * nbx;
+        for (int k = 0; k < 8; ++k) box_of[(t << 3) + k] = (int32_t)pan.size() - 1;
+    }
+}
+
+// box-local coordinates of internal voxel v in box P (numbering B = 8 or 4,
+// nby / nbx tiles or bricks along y / x)
+void box_local(int32_t v, const DynPan &P, int B, int nby, int nbx, int &x, int &y, int &z) {
+    const int sh = B == 8 ? 9 : 6, e = B == 8 ? 8 : 4;
+    const int64_t u = v >> sh;
+    int lx, ly, lz;
+    dyn_local_xyz(v & ((1 << sh) - 1), lx, ly, lz);
+    const int uy = (int)(u % nby), ux = (int)((u / nby) % nbx), uz = (int)(u / ((int64_t)nby * nbx));
+    x = e * ux + lx - P.bx;
+    y = e * uy + ly - P.by;
+    z = e * uz + lz - P.bz;
+}
+
 // Inside every box, segments by decreasing length in buckets of q entries,
 // ascending ray inside a bucket: the threads of a wavefront walk segments of
 // about the same length, and neighbouring threads read the records of
@@ -181,7 +218,8 @@ void sort_segments(SegPlan &S, int64_t npan, int q) {
 std::vector<uint16_t> segment_starts(const std::vector<int32_t> &ptr,
                                      const std::vector<int32_t> &ind, const SegPlan &S,
                                      const std::vector<int32_t> &box_of,
-                                     const std::vector<DynPan> &pan, int64_t m) {
+                                     const std::vector<DynPan> &pan, int64_t m, int B, int nby,
+                                     int nbx) {
     const int64_t nseg = S.seg_box_ptr.back();
     std::vector<uint16_t> c0(nseg > 0 ? nseg : 1, 0);
     for (int64_t r = 0; r < m; ++r) {
@@ -192,9 +230,9 @@ std::vector<uint16_t> segment_starts(const std::vector<int32_t> &ptr,
             last = sg;
             const DynPan &P = pan[box_of[ind[k] >> 6]];
             int x, y, z;
-            dyn_local_xyz(ind[k] - P.v0, x, y, z);
-            TV3_REQUIRE(x < P.bs && y < P.ey && z < P.bs, TV3_E_INTERNAL,
-                        "a segment starts outside its box");
+            box_local(ind[k], P, B, nby, nbx, x, y, z);
+            TV3_REQUIRE(x >= 0 && y >= 0 && z >= 0 && x < P.bs && y < P.ey && z < P.bs,
+                        TV3_E_INTERNAL, "a segment starts outside its box");
             c0[sg] = (uint16_t)(x | (y << 4) | (z << 8));
         }
     }
@@ -224,17 +262,20 @@ void finish_boxes(tv3_ctx *ctx, std::vector<DynPan> &pan, const SegPlan &S,
     }
     if (!offsets) {
         // A x: a box's segments in chunks of at most 512 segments and about
-        // 6,144 entries, one workgroup each (it stages the box's voxels, and
-        // each segment's partial sum is its own), so that the heaviest boxes
-        // do not run long chains after the others have ended
+        // 6,144 entries (TV3_DYN_CHUNK16 times that in boxes of 16^3 voxels),
+        // one workgroup each (it stages the box's voxels, and each segment's
+        // partial sum is its own), so that the heaviest boxes do not run long
+        // chains after the others have ended
         std::vector<DynPan> items;
         for (const DynPan &P : pan) {
+            const int scale = P.bs == 16 ? TV3_DYN_CHUNK16 : 1;
             int32_t sg = P.s0;
             while (sg < P.s1) {
                 DynPan c = P;
                 c.s0 = sg;
                 int32_t e = 0;
-                while (sg < P.s1 && sg - c.s0 < 512 && e < 6144) e += S.seg_len[sg++];
+                while (sg < P.s1 && sg - c.s0 < 512 * scale && e < 6144 * scale)
+                    e += S.seg_len[sg++];
                 c.s1 = sg;
                 c.ne = e;
                 items.push_back(c);
@@ -301,25 +342,40 @@ std::unique_ptr<FusedLayout> build_dyn_layout(tv3_op *op) {
     // forward boxes: whole units; backward: split where the LDS limits ask
     std::vector<DynPan> panF, panT;
     std::vector<int32_t> boxF((size_t)(n >> 6), -1), boxT((size_t)(n >> 6), -1);
-    // A x: tiles in pairs along y where the tile count along y is even (half
-    // the segments, so half the partial sums of the second pass), worked in
-    // chunks (finish_boxes).  Measured at 128^3: 92.6 us against 100-102 us
+    // A x: blocks of 2 x 2 x 2 tiles where every tile count is even (9.1
+    // segments per ray at 128^3 against 14.9 for pairs: 73 us against 81-83),
+    // else tiles in pairs
+    // along y where the tile count along y is even (half the segments, so
+    // half the partial sums of the second pass), worked in chunks
+    // (finish_boxes).  Measured at 128^3: pairs 92.6 us against 100-102 us
     // for single tiles, 104 us for single tiles in chunks.
+    const int nbz = B == 8 ? dims[2] / 8 : 0;
+    const bool box16 = B == 8 && nby % 2 == 0 && nbx % 2 == 0 && nbz % 2 == 0;
     const bool pairs = B == 8 && nby % 2 == 0;
     for (int64_t u = 0; u < nunit; ++u) {
-        if (!pairs) add_boxes(panF, boxF, B, u, nby, nbx, false);
-        else if (u % 2 == 0) add_pair_box(panF, boxF, u, nby, nbx);
+        const int64_t ty = u % nby, tx = (u / nby) % nbx, tz = u / ((int64_t)nby * nbx);
+        if (box16) {
+            if (ty % 2 == 0 && tx % 2 == 0 && tz % 2 == 0) add_box16(panF, boxF, u, nby, nbx);
+        } else if (!pairs) {
+            add_boxes(panF, boxF, B, u, nby, nbx, false);
+        } else if (u % 2 == 0) {
+            add_pair_box(panF, boxF, u, nby, nbx);
+        }
         add_boxes(panT, boxT, B, u, nby, nbx, unit_ent[u] > D->emax_cap);
+    }
+    if (B == 8) {
+        D->tile_dx = nby * 512;
+        D->tile_dz = nby * nbx * 512;
     }
     SegPlan SF = plan_segments(ptr, ind, boxF, (int64_t)panF.size(), m);
     SegPlan ST = plan_segments(ptr, ind, boxT, (int64_t)panT.size(), m);
     sort_segments(SF, (int64_t)panF.size(), 8);
     sort_segments(ST, (int64_t)panT.size(), 4);
-    finish_boxes(ctx, panF, SF, segment_starts(ptr, ind, SF, boxF, panF, m), false, D->fwd,
-                 nullptr);
+    finish_boxes(ctx, panF, SF, segment_starts(ptr, ind, SF, boxF, panF, m, B, nby, nbx), false,
+                 D->fwd, nullptr);
     std::vector<int32_t> eoffT;
-    finish_boxes(ctx, panT, ST, segment_starts(ptr, ind, ST, boxT, panT, m), true, D->bwd,
-                 &eoffT);
+    finish_boxes(ctx, panT, ST, segment_starts(ptr, ind, ST, boxT, panT, m, B, nby, nbx), true,
+                 D->bwd, &eoffT);
     TV3_REQUIRE(D->bwd.emax <= D->emax_cap, TV3_E_UNSUPPORTED,
                 "a 4^3 brick of the volume holds more ray entries than the tile products keep in "
                 "LDS");

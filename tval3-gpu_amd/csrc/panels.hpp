@@ -346,6 +346,7 @@ struct DynView {
     const uint32_t *pptr;
     const uint16_t *pos;
     int emax_cap;
+    int tile_dx, tile_dz;   // internal offset of the next tile along x, z
 };
 
 inline DynView dyn_view(const DynTiles &D, const DynBoxes &X) {
@@ -354,6 +355,8 @@ inline DynView dyn_view(const DynTiles &D, const DynBoxes &X) {
     v.seg_c0 = X.seg_c0.p;
     v.ray = D.ray.p; v.pptr = D.pptr.p; v.pos = D.pos.p;
     v.emax_cap = D.emax_cap;
+    v.tile_dx = D.tile_dx;
+    v.tile_dz = D.tile_dz;
     return v;
 }
 
@@ -465,13 +468,19 @@ __global__ __launch_bounds__(256) void k_dyn_tile_N(DynView g, const T *__restri
                                                     T *__restrict__ part,
                                                     const int32_t *active) {
     if (active && !*active) return;
-    __shared__ T xs[1024];
+    __shared__ T xs[4096];
     const DynPan P = g.pan[g.order[blockIdx.x]];
     const int t = threadIdx.x, bs = P.bs, ey = P.ey, nv = ey * bs * bs;
     for (int i = t; i < nv; i += 256) {     // linear box order y + ey x + ey bs z
+        // tile sb of the box (bits: y, x, z; boxes of one tile or brick: 0)
+        const int j = i & 511, sb = i >> 9;
         int lx, ly, lz;
-        dyn_local_xyz(i, lx, ly, lz);
-        xs[ly + ey * (lx + bs * lz)] = x[P.v0 + i];
+        dyn_local_xyz(j, lx, ly, lz);
+        lx += (sb & 2) << 2;
+        ly += (sb & 1) << 3;
+        lz += (sb & 4) << 1;
+        xs[ly + ey * (lx + bs * lz)] =
+            x[P.v0 + (sb & 1) * 512 + ((sb >> 1) & 1) * g.tile_dx + (sb >> 2) * g.tile_dz + j];
     }
     __syncthreads();
     for (int sg = P.s0 + t; sg < P.s1; sg += 256) {

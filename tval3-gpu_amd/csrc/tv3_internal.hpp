@@ -226,9 +226,10 @@ struct SegSum {
 // same bits on every run.  Only the pattern's positions are stored (16 bits
 // per entry); every weight is recomputed by the traversal of siddon.hpp.
 struct DynPan {
-    int32_t v0;          // first internal voxel of the box (its voxels are contiguous)
+    int32_t v0;          // first internal voxel of the box (of its first tile)
     int32_t bx, by, bz;  // box corner in voxel coordinates (x, y, z)
-    int32_t bs;          // box edge along x and z: 8 (tiles) or 4 (a brick)
+    int32_t bs;          // box edge along x and z: 16 (2 x 2 x 2 tiles), 8 (tiles) or 4
+                         // (a brick)
     int32_t ey;          // box edge along y: bs, or 16 for two tiles consecutive in y
     int32_t s0, s1;      // its segments
     int32_t ne;          // entries of its segments
@@ -252,7 +253,9 @@ struct DynTiles {
     int32_t threads_T = 512;              // workgroup of the A^T product
     float org[3] = {0, 0, 0}, h[3] = {1, 1, 1};
     int32_t N[3] = {0, 0, 0};        // voxels along x, y, z
-    DynBoxes fwd;                    // A x: whole tiles (or bricks when B = 4)
+    int32_t tile_dx = 0, tile_dz = 0;  // internal offset of the next 8^3 tile along x, z
+    DynBoxes fwd;                    // A x: 2 x 2 x 2 tiles, tile pairs along y, tiles or
+                                     // bricks (B = 4)
     DynBoxes bwd;                    // A^T r: tiles, split into bricks where a tile's
                                      // entries exceed the LDS limit
     DevBuf<float> ray;               // 12 per internal ray: sid_setup's A[3], L, B[3],
