@@ -370,7 +370,9 @@ tv3_status tv3_op_prepare(tv3_op *op, const int32_t dims[3], tv3_layout_info *in
  * 8^3 tiles and an even tile count along every axis; at most one of bits 2, 4
  * and 5) and row blocks of -code_N / -code_T entries (-1024, -2048 or -4096);
  * a direction not in panel form takes code_N / code_T as a tv3_op_set_lanes
- * code (0 = keep). */
+ * code (0 = keep).  A layout not yet prepared for dims is built here without
+ * timing any launch shape (a caller replaying shapes it stored, such as the
+ * Python binding's JSON tuning cache, skips the tuner's timings). */
 tv3_status tv3_op_layout_force(tv3_op *op, const int32_t dims[3], int32_t code_N, int32_t code_T,
                                int32_t pan_nnz, int32_t panels, tv3_layout_info *info);
 

@@ -742,7 +742,7 @@ void renumber_rows(tv3_ctx *ctx, const SparseRows &A, const std::vector<int32_t>
     build_transpose(ctx, N, L->AT);
 }
 
-FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
+FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3], bool tune) {
     if (op->fused && op->fused->dims[0] == dims[0] && op->fused->dims[1] == dims[1] &&
         op->fused->dims[2] == dims[2])
         return *op->fused;
@@ -755,10 +755,12 @@ FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]) {
     renumber_operator(op, dims, L.get());
     SparseRows &N = L->A;
     const int64_t n = N.cols, m = N.rows;
-    ColdTimer timer(ctx);
-    const float t_a = tune_shape(ctx, timer, N, n);
-    const float t_at = tune_shape(ctx, timer, L->AT, m);
-    tune_panels(ctx, timer, *L, t_a, t_at);
+    if (tune) {
+        ColdTimer timer(ctx);
+        const float t_a = tune_shape(ctx, timer, N, n);
+        const float t_at = tune_shape(ctx, timer, L->AT, m);
+        tune_panels(ctx, timer, *L, t_a, t_at);
+    }
     TV3_HIP(hipStreamSynchronize(s));
     op->fused = std::move(L);
     return *op->fused;
@@ -790,7 +792,9 @@ static void apply_code(tv3_ctx *ctx, SparseRows &S, int code) {
 
 void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int pan_nnz,
                   int panels) {
-    FusedLayout &F = fused_layout(op, dims);
+    // a layout not built yet is built without timing: the shapes given here
+    // replace whatever the timing would pick
+    FusedLayout &F = fused_layout(op, dims, false);
     TV3_REQUIRE(!F.dyn, TV3_E_UNSUPPORTED, "a matrix-free operator's tile products have no "
                 "launch shapes to force");
     F.session.reset();               // its graphs hold the old launch shapes

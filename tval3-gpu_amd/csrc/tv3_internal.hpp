@@ -406,7 +406,10 @@ void op_apply(tv3_op *op, bool trans, const double *x, double *y, hipStream_t s,
               bool acc64 = true);
 
 // internal layout (layout.hip)
-FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3]);
+// the fused layout of op for a volume shape, built on first use (launch
+// shapes timed on this GPU unless tune is false: then the row kernels'
+// defaults, no panel forms)
+FusedLayout &fused_layout(tv3_op *op, const int32_t dims[3], bool tune = true);
 // the layout's numbering only: voxel and ray maps, A and its CSC renumbered
 // (no panel forms, no tuned launch shapes)
 void renumber_operator(tv3_op *op, const int32_t dims[3], FusedLayout *L);
