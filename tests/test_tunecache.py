@@ -84,6 +84,14 @@ def test_cache_ignores_unreadable_or_other_versions(tmp_path, text):
     assert json.loads(path.read_text())["version"] == tunecache.VERSION
 
 
+def test_cache_leaves_matrix_free_layouts_alone(tmp_path):
+    path = tmp_path / "tune.json"
+    op = _Op()
+    op.kind = "dynamic"
+    tunecache.TuneCache(str(path), device="g").prepare(op, (4, 4, 4))
+    assert op.timed == 1 and not path.exists()
+
+
 def test_batch_key_names_shared_operators():
     c = tunecache.TuneCache("unused.json", device="g")
     a, b = _Op(), _Op()

@@ -79,6 +79,8 @@ class TuneCache:
         cache has them (no timing), else timed and stored.  Returns the
         layout description."""
         from ._lib import TV3Error
+        if op.kind == "dynamic":     # a matrix-free layout has no launch shapes to time
+            return op.prepare(dims)
         key = self.layout_key(op, dims)
         e = self.entries.get(key)
         if e is not None:
