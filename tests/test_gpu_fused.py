@@ -167,16 +167,21 @@ def test_fused_fallback_with_each_search_placement(ctx, panels, skip):
         assert (h["mulT"][h["cnt"] == 6] == 1).all()     # the fallback step multiplies
 
 
-def test_fused_fp16_weights_track_oracle(ctx):
+@pytest.mark.parametrize("shapes", [None, (-4096, -2048, 16384, 1 | 2 | 32 | 256),
+                                    (-2048, -2048, 16384, 1 | 2 | 16 | 256)])
+def test_fused_fp16_weights_track_oracle(ctx, shapes):
     """FP16 ray weights, FP32 accumulation: the fused solve with FP16 weights
     against the FP32 oracle solving the same FP16-rounded matrix, with the
-    envelope criterion of test_fused_tracks_oracle.  (Rounding the weights to
-    FP16 changes the problem by up to 2^-11 per weight, far more than the
-    rounding-level perturbations of that criterion, so the FP16 run is not
-    compared with the FP32-weight run here; equal error of the two is a
-    criterion of the full-size benchmark.)"""
+    envelope criterion of test_fused_tracks_oracle; the timed layout, and A
+    over 16^3 and 16 x 16 x 8 boxes with packed entries (12-bit indices).
+    (Rounding the weights to FP16 changes the problem by up to 2^-11 per
+    weight, far more than the rounding-level perturbations of that
+    criterion, so the FP16 run is not compared with the FP32-weight run here;
+    equal error of the two is a criterion of the full-size benchmark.)"""
     dims = (16, 16, 16)
     u, A16, A32, oop, b = setup(ctx, dims, 64, 128, seed=3, dtype="f16")
+    if shapes is not None:
+        assert A16.layout_force(dims, *shapes)["panels"] == shapes[3]
     rp, c, v = A16.download_csr()
     assert np.array_equal(v, v.astype(np.float16).astype(np.float64))
     oop16 = orc.csr_op(A16.m, A16.n, rp, c, v)

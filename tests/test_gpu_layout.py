@@ -90,10 +90,13 @@ def test_layout_products_exact(ctx, dims, brick):
 
 
 # A's panels over 2 x 2 x 1 tiles (bit 4) and 2 x 2 x 2 tiles (bit 5), whole
-# and split into brick runs, with A^T in either form
+# and split into brick runs, with A^T in either form; bit 8: their entries
+# packed (12-bit indices, FP16 weights)
 SHAPES_SUPER = [
     (-2048, -2048, 16384, 19), (-1024, -1024, 300, 19), (-4096, 16, 16384, 17),
     (-2048, -2048, 16384, 43), (-1024, -2048, 100, 33), (-2048, -1024, 16384, 35),
+    (-4096, -2048, 16384, 299), (-2048, -1024, 300, 275), (-4096, 16, 100, 289),
+    (-2048, -2048, 16384, 291),
 ]
 
 
@@ -127,6 +130,16 @@ def test_layout_supertile_panels_need_even_tiles(ctx):
         op.layout_force(dims, -2048, -2048, 16384, 19)
     with pytest.raises(TV3Error, match="at most one"):
         op.layout_force(dims, -2048, -2048, 16384, 1 | 4 | 16)
+    with pytest.raises(TV3Error, match="bit 4 or 5"):
+        op.layout_force(dims, -2048, -2048, 16384, 1 | 4 | 256)
+
+
+def test_layout_packed_entries_need_fp16(ctx):
+    """packed A entries hold FP16 weights: an FP32 operator is refused"""
+    dims = (16, 16, 16)
+    op, _, _, _ = integer_case(ctx, dims, 8, 16, "f32")
+    with pytest.raises(TV3Error, match="FP16"):
+        op.layout_force(dims, -2048, -2048, 16384, 1 | 32 | 256)
 
 
 def test_layout_panels_with_empty_tiles(ctx):

@@ -241,9 +241,13 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
                         const std::vector<int32_t> &src_off, const std::vector<int32_t> *src,
                         const std::vector<int32_t> &rptr, const std::vector<uint16_t> &lidx,
                         const uint8_t *wb, size_t vb, tv3_dtype vt, int blk_nnz, int G,
-                        int chunk_blocks, Panels &P) {
+                        int chunk_blocks, Panels &P, bool pack12 = false) {
     hipStream_t s = ctx->stream;
     TV3_REQUIRE(G == 1 || G == 2 || G == 4, TV3_E_ARG, "panel groups of 1, 2 or 4 entries");
+    TV3_REQUIRE(!pack12 || (vt == TV3_F16 && G == 1 && (blk_nnz == 2048 || blk_nnz == 4096)),
+                TV3_E_ARG, "packed entries take FP16 weights, unpadded rows and blocks of "
+                "2048 or 4096");
+    const int64_t align = pack12 ? 8 : 4;
     const int64_t npan = (int64_t)pan_row.size() - 1, rows = pan_row.back();
     auto plen = [&](int64_t r) { return ((int64_t)rptr[r + 1] - rptr[r] + G - 1) / G * G; };
     // row blocks: consecutive rows of one panel, at most blk_nnz padded
@@ -267,7 +271,7 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
         pan_blk.push_back((int32_t)blk.size() - 1);
     }
     // stored positions: a block's padded rows back to back from a 4-aligned
-    // start
+    // start (8-aligned for packed entries)
     std::vector<int32_t> pptr(rows + 1);
     int64_t acc = 0;
     for (size_t b = 0; b + 1 < blk.size(); ++b) {
@@ -275,22 +279,34 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
             pptr[r] = (int32_t)acc;
             acc += plen(r);
         }
-        acc = (acc + 3) & ~int64_t(3);
+        acc = (acc + align - 1) & ~(align - 1);
     }
     TV3_REQUIRE(acc < (int64_t(1) << 31), TV3_E_UNSUPPORTED, "panel entries exceed 2^31");
     pptr[rows] = (int32_t)acc;
     int max_src = 0;
     for (int64_t p = 0; p < npan; ++p) max_src = std::max(max_src, src_off[p + 1] - src_off[p]);
-    std::vector<uint32_t> pk;
+    std::vector<uint32_t> pk, pw;
     std::vector<uint16_t> ps;
     std::vector<float> pv;
-    const size_t np = acc > 4 ? (size_t)acc : 4;   // whole quads, at least one
-    if (vb == 2) pk.assign(np, 0u);
+    const size_t np = acc > align ? (size_t)acc : (size_t)align;   // whole quads (octets)
+    if (pack12) { pk.assign(np / 8 * 3, 0u); pw.assign(np / 8 * 4, 0u); }
+    else if (vb == 2) pk.assign(np, 0u);
     else { ps.assign(np, 0); pv.assign(np, 0.f); }
     for (int64_t r = 0; r < rows; ++r)
         for (int32_t k = rptr[r]; k < rptr[r + 1]; ++k) {
             const int64_t q = (int64_t)pptr[r] + (k - rptr[r]);
-            if (vb == 2) {
+            if (pack12) {
+                // entry e of octet o: index at bits 12 e of the octet's 96,
+                // weight in half e of its 16 bytes
+                uint16_t bits;
+                std::memcpy(&bits, wb + (size_t)k * 2, 2);
+                TV3_REQUIRE(lidx[k] < 4096, TV3_E_INTERNAL, "a packed local index needs 12 bits");
+                const int64_t o = q >> 3;
+                const int e = (int)(q & 7), bit = 12 * e, w = bit >> 5, off = bit & 31;
+                pk[3 * o + w] |= (uint32_t)lidx[k] << off;
+                if (off > 20) pk[3 * o + w + 1] |= (uint32_t)lidx[k] >> (32 - off);
+                pw[4 * o + (e >> 1)] |= (uint32_t)bits << (16 * (e & 1));
+            } else if (vb == 2) {
                 uint16_t bits;
                 std::memcpy(&bits, wb + (size_t)k * 2, 2);
                 pk[q] = (uint32_t)lidx[k] | ((uint32_t)bits << 16);
@@ -327,6 +343,7 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
     P.max_src = max_src;
     P.blk_nnz = blk_nnz;
     P.group = G;
+    P.fmt = pack12 ? 12 : 0;
     P.stored = (int64_t)np;
     P.contiguous = src == nullptr;
     P.vtype = vt;
@@ -347,7 +364,17 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
     P.desc.alloc(d.size());
     TV3_HIP(hipMemcpyAsync(P.desc.p, d.data(), sizeof(BlkDesc) * d.size(), hipMemcpyHostToDevice,
                            s));
-    if (vb == 2) {
+    P.pw.release();
+    if (pack12) {
+        P.pk.alloc(pk.size());
+        P.pw.alloc(pw.size());
+        P.ps.release();
+        P.pv.release();
+        TV3_HIP(hipMemcpyAsync(P.pk.p, pk.data(), sizeof(uint32_t) * pk.size(),
+                               hipMemcpyHostToDevice, s));
+        TV3_HIP(hipMemcpyAsync(P.pw.p, pw.data(), sizeof(uint32_t) * pw.size(),
+                               hipMemcpyHostToDevice, s));
+    } else if (vb == 2) {
         P.pk.alloc(pk.size());
         P.ps.release();
         P.pv.release();
@@ -386,7 +413,7 @@ static void build_panels_T(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
 // per-ray segment lists of the second pass.
 static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, int blk_nnz,
                            Panels &P, SegSum &R, int chunk_blocks = kPanelChunkA,
-                           const std::vector<int32_t> *tmap = nullptr) {
+                           const std::vector<int32_t> *tmap = nullptr, bool pack12 = false) {
     const int64_t npan = (int64_t)pl.pan_row.size() - 1, nseg = (int64_t)pl.seg_ray.size();
     const int64_t m = h.cols;
     std::vector<int32_t> local(m > 0 ? m : 1, 0), sptr(nseg + 1, 0);
@@ -412,7 +439,7 @@ static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
     }
     // panels of the segment matrix: segment ranges pan_seg, voxel ranges pan_row
     pack_panels(ctx, pl.pan_seg, pl.pan_row, nullptr, sptr, lidx, wb.data(), h.vb, h.vtype,
-                blk_nnz, kPanelGroupA, chunk_blocks, P);
+                blk_nnz, kPanelGroupA, chunk_blocks, P, pack12);
     if (tmap) {
         // h's rows in super-tile order: the panels stage x through the tile map
         P.src.alloc(tmap->size());
@@ -628,25 +655,29 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
     std::vector<int32_t> tmap;
     std::vector<TileVolume> vols;
     if (!layout_volume(F, vols) || !supertile_rows(vols, h, hs, tmap)) return;
+    // FP16 weights: also with packed entries (12-bit indices, 28 bytes per 8
+    // entries instead of 32)
+    const bool f16 = F.AT.vtype == TV3_F16;
     for (int unit : {2048, 4096}) {
         PanelPlan pl2;
         if (!plan_panels(hs, unit, 16384 * unit / 512, pl2)) continue;
-        for (int nb : {2048, 4096}) {
-            Panels P;
-            SegSum R;
-            build_panels_A(ctx, hs, pl2, nb, P, R, kPanelChunkA, &tmap);
-            const float t = time_launches([&]() {
-                launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
-                launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
-            });
-            if (t < t_A) {
-                t_A = t;
-                F.PA = std::move(P);
-                F.RS = std::move(R);
-                F.panels_A = true;
-                F.unit_A = unit;
+        for (int packed = 0; packed <= (f16 ? 1 : 0); ++packed)
+            for (int nb : {2048, 4096}) {
+                Panels P;
+                SegSum R;
+                build_panels_A(ctx, hs, pl2, nb, P, R, kPanelChunkA, &tmap, packed != 0);
+                const float t = time_launches([&]() {
+                    launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
+                    launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
+                });
+                if (t < t_A) {
+                    t_A = t;
+                    F.PA = std::move(P);
+                    F.RS = std::move(R);
+                    F.panels_A = true;
+                    F.unit_A = unit;
+                }
             }
-        }
     }
 }
 
@@ -837,7 +868,10 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
             PanelPlan pl2;
             TV3_REQUIRE(plan_panels(hs, unit_A, pan_nnz * (unit_A / 512), pl2), TV3_E_UNSUPPORTED,
                         "a panel of this operator would need more rays than LDS holds");
-            build_panels_A(ctx, hs, pl2, -code_N, F.PA, F.RS, kPanelChunkA, &tmap);
+            const bool packed = (panels & 256) != 0;
+            TV3_REQUIRE(!packed || (F.AT.vtype == TV3_F16 && code_N != -1024), TV3_E_UNSUPPORTED,
+                        "packed panel entries take FP16 weights and blocks of 2048 or 4096");
+            build_panels_A(ctx, hs, pl2, -code_N, F.PA, F.RS, kPanelChunkA, &tmap, packed);
         }
         if (pa && unit_A == 1024) {
             PanelPlan pl2;

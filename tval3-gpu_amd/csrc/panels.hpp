@@ -226,6 +226,140 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
     epi.finish();
 }
 
+// A's panel form with packed entries (Panels::fmt 12, FP16 weights, rows
+// unpadded): the stored entries in octets of 8, their 12-bit local indices
+// in 3 words (entry e at bits 12 e .. 12 e + 11 of the 96) and their 8 FP16
+// weights in 16 bytes -- 28 bytes for 8 entries instead of 32, for panels
+// of at most 4,096 sources (boxes of 16^3 voxels).  Row blocks start and end
+// 8-aligned; otherwise as k_panel with G = 1 and contiguous sources (staged
+// through the tile map src when one is given).
+struct Oct {
+    uint3 i;
+    uint4 w;
+};
+
+__device__ __forceinline__ Oct load_oct(const uint32_t *pi, const uint4 *pw, int64_t o) {
+    const uint32_t *q = pi + 3 * o;
+    return Oct{make_uint3(q[0], q[1], q[2]), pw[o]};
+}
+
+__device__ __forceinline__ float oct_w(uint32_t u, int hi) {
+    return __half2float(__ushort_as_half((unsigned short)(hi ? u >> 16 : u & 0xffffu)));
+}
+
+// the 8 products of an octet with the staged sources
+__device__ __forceinline__ void oct_prod(const Oct &c, const float *xs, float4 &p0, float4 &p1) {
+    const uint32_t a = c.i.x, b = c.i.y, d = c.i.z;
+    const int i0 = a & 0xfff, i1 = (a >> 12) & 0xfff, i2 = (a >> 24) | ((b & 0xf) << 8),
+              i3 = (b >> 4) & 0xfff, i4 = (b >> 16) & 0xfff, i5 = (b >> 28) | ((d & 0xff) << 4),
+              i6 = (d >> 8) & 0xfff, i7 = d >> 20;
+    p0 = make_float4(oct_w(c.w.x, 0) * xs[i0], oct_w(c.w.x, 1) * xs[i1],
+                     oct_w(c.w.y, 0) * xs[i2], oct_w(c.w.y, 1) * xs[i3]);
+    p1 = make_float4(oct_w(c.w.z, 0) * xs[i4], oct_w(c.w.z, 1) * xs[i5],
+                     oct_w(c.w.w, 0) * xs[i6], oct_w(c.w.w, 1) * xs[i7]);
+}
+
+template <int NB, typename Epi>
+__global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ order,
+                                                 const int32_t *__restrict__ src_off,
+                                                 const int32_t *__restrict__ chunk,
+                                                 const int32_t *__restrict__ tmap,
+                                                 const BlkDesc *__restrict__ desc,
+                                                 const int32_t *__restrict__ pptr,
+                                                 const uint32_t *__restrict__ pi,
+                                                 const uint4 *__restrict__ pw, int64_t olast,
+                                                 const float *__restrict__ x, Epi epi) {
+    constexpr int NO = NB / 8;                   // octets per block
+    constexpr int J = NO / 256;
+    static_assert(J >= 1, "blocks of at least 2048 entries");
+    extern __shared__ float xs[];                // the panel's source values
+    __shared__ __align__(16) float prod[NB];     // the block's products
+    constexpr int NX = ExtraBlocks<Epi>::value;
+    if constexpr (NX > 0) {
+        if ((int)blockIdx.x < NX) {
+            epi.extra();
+            return;
+        }
+    }
+    if (epi.skip()) return;
+    const int t = threadIdx.x;
+    const int item = order[blockIdx.x - NX];
+    const int pan = chunk[3 * item], bb = chunk[3 * item + 1], b1 = chunk[3 * item + 2];
+    const int s0 = src_off[pan], ns = src_off[pan + 1] - s0;
+    BlkDesc d = desc[bb];
+    if constexpr (HasSkipRows<Epi>::value) {
+        if (bb < b1 && epi.skip_rows(d.r0)) return;
+    }
+    const int64_t ohi = min(olast, max((int64_t)desc[b1].p0 / 8 - 1, (int64_t)d.p0 / 8));
+    Oct cur[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) cur[j] = load_oct(pi, pw, min((int64_t)d.p0 / 8 + j * 256 + t, ohi));
+    for (int k0 = 0; k0 < ns; k0 += 8 * 256) {
+        float xv[8];
+        if (tmap == nullptr) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xv[i] = x[s0 + min(k0 + i * 256 + t, ns - 1)];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = s0 + min(k0 + i * 256 + t, ns - 1);
+                xv[i] = x[((int64_t)tmap[k >> 9] << 9) | (k & 511)];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (k0 + i * 256 + t < ns) xs[k0 + i * 256 + t] = xv[i];
+    }
+    __syncthreads();
+    for (int b = bb; b < b1; ++b) {
+        const BlkDesc dn = desc[b + 1];      // in flight while this block runs
+        Oct nxt[J];
+        if (b + 1 < b1) {
+            const int64_t on = ((int64_t)d.p0 + d.len) / 8;
+#pragma unroll
+            for (int j = 0; j < J; ++j) nxt[j] = load_oct(pi, pw, min(on + j * 256 + t, ohi));
+        }
+        if (d.len <= NB) {
+            const int row = block_row(d);
+            const int a = pptr[row] - d.p0, e = pptr[row + 1] - d.p0;
+            const auto pre = epi.load(row);
+            if (d.len > 0) {
+#pragma unroll
+                for (int j = 0; j < J; ++j) {
+                    float4 v0, v1;
+                    oct_prod(cur[j], xs, v0, v1);
+                    const int q = 8 * (j * 256 + t);
+                    *reinterpret_cast<float4 *>(prod + q) = v0;
+                    *reinterpret_cast<float4 *>(prod + q + 4) = v1;
+                }
+            }
+            __syncthreads();
+            block_rows<NB>(prod, d, a, e, epi, row, pre);
+        } else {
+            // one row longer than NB
+            const auto pre = epi.load(d.r0);
+            float s = 0;
+            for (int g = t; g < (d.len >> 3); g += 256) {
+                float4 v0, v1;
+                oct_prod(load_oct(pi, pw, (int64_t)d.p0 / 8 + g), xs, v0, v1);
+                s += ((v0.x + v0.y) + (v0.z + v0.w)) + ((v1.x + v1.y) + (v1.z + v1.w));
+            }
+            s = lane_sum<64>(s);
+            if ((t & 63) == 0) prod[t >> 6] = s;
+            __syncthreads();
+            if (t == 0) s = (prod[0] + prod[1]) + (prod[2] + prod[3]);
+            epi.row(d.r0, s, t == 0, pre);
+        }
+        __syncthreads();             // prod is rewritten by the next row block
+        d = dn;
+        if (b + 1 < b1) {
+#pragma unroll
+            for (int j = 0; j < J; ++j) cur[j] = nxt[j];
+        }
+    }
+    epi.finish();
+}
+
 // Pass 2 of A x: L lanes per ray sum its segment partials (batches of 4 per
 // lane, loads clamped to the ray's last segment), then the epilogue.
 template <int L, int NLB, typename Epi>
@@ -319,6 +453,22 @@ void launch_panel_v(const Panels &P, const float *x, const Epi &epi, hipStream_t
 // Panel product y = M x with epilogue epi (FP32 accumulation).
 template <typename Epi>
 void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s) {
+    if (P.fmt == 12) {
+        const unsigned g = (unsigned)P.nitem + ExtraBlocks<Epi>::value;
+        const size_t lds = sizeof(float) * (size_t)(P.max_src > 0 ? P.max_src : 1);
+        if (g == 0) return;
+        if (P.blk_nnz == 2048)
+            hipLaunchKernelGGL((k_panel12<2048, Epi>), dim3(g), dim3(256), lds, s, P.order.p,
+                               P.src_off.p, P.chunk.p, P.src.p, P.desc.p, P.pptr.p, P.pk.p,
+                               reinterpret_cast<const uint4 *>(P.pw.p), P.stored / 8 - 1, x, epi);
+        else if (P.blk_nnz == 4096)
+            hipLaunchKernelGGL((k_panel12<4096, Epi>), dim3(g), dim3(256), lds, s, P.order.p,
+                               P.src_off.p, P.chunk.p, P.src.p, P.desc.p, P.pptr.p, P.pk.p,
+                               reinterpret_cast<const uint4 *>(P.pw.p), P.stored / 8 - 1, x, epi);
+        else
+            throw tv3_error(TV3_E_ARG, "packed panel entries take blocks of 2048 or 4096");
+        return;
+    }
     switch (P.vtype) {
     case TV3_F32:
         P.contiguous ? launch_panel_v<float, true, Epi>(P, x, epi, s)

@@ -190,7 +190,9 @@ struct Panels {
     int max_src = 0;           // largest source list of a panel (LDS floats)
     int blk_nnz = 0;
     int group = 4;             // rows padded to groups of 1, 2 or 4 entries
-    int64_t stored = 4;        // stored entries (at least 4, a multiple of 4)
+    int fmt = 0;               // 0, or 12: 16-bit weights with 12-bit indices packed,
+                               // 8 entries in pk (3 words) and pw (4 words)
+    int64_t stored = 4;        // stored entries (at least 4, a multiple of 4; of 8 for fmt 12)
     bool contiguous = false;   // sources of panel p: x[src_off[p] .. src_off[p + 1])
     tv3_dtype vtype = TV3_F16;
     int64_t nitem = 0;         // work items (a panel's row blocks, in chunks)
@@ -205,6 +207,7 @@ struct Panels {
     DevBuf<uint32_t> pk;       // padded packed entries (16-bit weights)
     DevBuf<uint16_t> ps;       // padded local indices (F32 weights)
     DevBuf<float> pv;          // padded weights (F32 weights)
+    DevBuf<uint32_t> pw;       // fmt 12: the FP16 weights, 8 per 4 words
 };
 
 // Second pass of the panel form of A x: y_i = sum of the partial sums of the
