@@ -100,7 +100,7 @@ SHAPES_SUPER = [
 ]
 
 
-@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 16, 32), (32, 32, 16)])
+@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 16, 32), (32, 32, 16), (48, 32, 64)])
 def test_layout_supertile_panels_exact(ctx, dims):
     """A in panel form over boxes of 4 and 8 tiles: the tile-ordered voxel
     rows are regrouped by 2 x 2 x 2 blocks of tiles and the panels stage x
