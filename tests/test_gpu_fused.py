@@ -168,12 +168,14 @@ def test_fused_fallback_with_each_search_placement(ctx, panels, skip):
 
 
 @pytest.mark.parametrize("shapes", [None, (-4096, -2048, 16384, 1 | 2 | 32 | 256),
-                                    (-2048, -2048, 16384, 1 | 2 | 16 | 256)])
+                                    (-2048, -2048, 16384, 1 | 2 | 16 | 256),
+                                    (-4096, -4096, 16384, 1 | 2 | 8 | 32 | 256 | 512)])
 def test_fused_fp16_weights_track_oracle(ctx, shapes):
     """FP16 ray weights, FP32 accumulation: the fused solve with FP16 weights
     against the FP32 oracle solving the same FP16-rounded matrix, with the
-    envelope criterion of test_fused_tracks_oracle; the timed layout, and A
-    over 16^3 and 16 x 16 x 8 boxes with packed entries (12-bit indices).
+    envelope criterion of test_fused_tracks_oracle; the timed layout, A
+    over 16^3 and 16 x 16 x 8 boxes with packed entries (12-bit indices), and
+    both products with packed entries.
     (Rounding the weights to FP16 changes the problem by up to 2^-11 per
     weight, far more than the rounding-level perturbations of that
     criterion, so the FP16 run is not compared with the FP32-weight run here;

@@ -68,6 +68,10 @@ SHAPES = [
     # brick runs
     (-2048, -2048, 16384, 7), (-1024, -1024, 300, 7), (-4096, 32, 16384, 5),
     (-2048, -2048, 16384, 15), (16, -1024, 300, 10), (-1024, -4096, 16384, 11),
+    # A^T's entries packed (bit 9: 12-bit local ray indices, FP16 weights,
+    # rows in groups of 4), over one and two tiles, whole and in brick runs
+    (-2048, -2048, 16384, 3 | 512), (32, -4096, 300, 2 | 512), (-1024, -2048, 16384, 11 | 512),
+    (-2048, -4096, 300, 15 | 512),
 ]
 
 
@@ -91,12 +95,13 @@ def test_layout_products_exact(ctx, dims, brick):
 
 # A's panels over 2 x 2 x 1 tiles (bit 4) and 2 x 2 x 2 tiles (bit 5), whole
 # and split into brick runs, with A^T in either form; bit 8: their entries
-# packed (12-bit indices, FP16 weights)
+# packed (12-bit indices, FP16 weights); bit 9: A^T's entries packed
 SHAPES_SUPER = [
     (-2048, -2048, 16384, 19), (-1024, -1024, 300, 19), (-4096, 16, 16384, 17),
     (-2048, -2048, 16384, 43), (-1024, -2048, 100, 33), (-2048, -1024, 16384, 35),
     (-4096, -2048, 16384, 299), (-2048, -1024, 300, 275), (-4096, 16, 100, 289),
-    (-2048, -2048, 16384, 291),
+    (-2048, -2048, 16384, 291), (-4096, -2048, 16384, 299 | 512),
+    (-2048, -4096, 300, 291 | 512),
 ]
 
 
@@ -135,11 +140,45 @@ def test_layout_supertile_panels_need_even_tiles(ctx):
 
 
 def test_layout_packed_entries_need_fp16(ctx):
-    """packed A entries hold FP16 weights: an FP32 operator is refused"""
+    """packed entries hold FP16 weights: an FP32 operator is refused in
+    either direction; packed A^T entries need A^T's panel form and blocks
+    of 2048 or 4096"""
     dims = (16, 16, 16)
     op, _, _, _ = integer_case(ctx, dims, 8, 16, "f32")
     with pytest.raises(TV3Error, match="FP16"):
         op.layout_force(dims, -2048, -2048, 16384, 1 | 32 | 256)
+    with pytest.raises(TV3Error, match="FP16"):
+        op.layout_force(dims, -2048, -2048, 16384, 2 | 512)
+    op16, _, _, _ = integer_case(ctx, dims, 8, 16, "f16")
+    with pytest.raises(TV3Error, match="bit 1"):
+        op16.layout_force(dims, -2048, -2048, 16384, 1 | 512)
+    with pytest.raises(TV3Error, match="2048 or 4096"):
+        op16.layout_force(dims, -2048, -1024, 16384, 2 | 512)
+
+
+@pytest.mark.parametrize("unit_bits", [0, 8])
+def test_layout_packed_transposed_equals_unpacked(ctx, unit_bits):
+    """A^T with packed entries forms the same group sums and row sums as the
+    4-byte form with the same block size, so real-valued products agree bit
+    for bit (panels over one tile and over two; the packed plan caps a
+    panel's rays at 4096)"""
+    dims = (32, 32, 32)
+    g = wl.transducer_geometry(48, 96, dims=dims)
+    op = api.DynamicOperator(ctx, g, "siddon").materialize("f16")
+    rng = np.random.default_rng(11)
+    r = rng.uniform(-1, 1, op.m).astype(np.float32)
+    op.prepare(dims)
+    for code_T in (-2048, -4096):
+        base = 2 | unit_bits
+        i0 = op.layout_force(dims, -2048, code_T, 16384, base)
+        assert i0["panels"] == base
+        # no panel exceeds 4096 rays, so both forms share one plan and blocks
+        assert i0["max_seg"] <= 4096
+        z0 = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+        i1 = op.layout_force(dims, -2048, code_T, 16384, base | 512)
+        assert i1["panels"] == base | 512 and i1["npan"] == i0["npan"]
+        z1 = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+        assert np.array_equal(z0, z1), (code_T, unit_bits)
 
 
 def test_layout_panels_with_empty_tiles(ctx):

@@ -159,14 +159,16 @@ static HostCSR download_csr(tv3_ctx *ctx, const SparseRows &S) {
 
 // Tiles of the voxel rows of A^T: panel p = rows [pan_row[p], pan_row[p+1])
 // (a unit of `unit` rows, split at brick boundaries where it holds more than
-// pan_nnz entries or more than kPanelSegCap distinct rays) and its sorted
-// distinct rays seg_ray[pan_seg[p] .. pan_seg[p+1]).
+// pan_nnz entries or more than seg_cap distinct rays: kPanelSegCap, what LDS
+// holds, or 4096 for 12-bit local indices) and its sorted distinct rays
+// seg_ray[pan_seg[p] .. pan_seg[p+1]).
 struct PanelPlan {
     std::vector<int32_t> pan_row{0}, pan_seg{0}, seg_ray;
     int64_t max_seg = 0;
 };
 
-static bool plan_panels(const HostCSR &h, int unit, int pan_nnz, PanelPlan &pl) {
+static bool plan_panels(const HostCSR &h, int unit, int pan_nnz, PanelPlan &pl,
+                        int64_t seg_cap = kPanelSegCap) {
     const int64_t n = h.rows, m = h.cols;
     const std::vector<int32_t> &ptr = h.ptr, &ind = h.ind;
     const int64_t sub = unit >= 64 ? 64 : unit;   // split granularity: one brick
@@ -196,12 +198,12 @@ static bool plan_panels(const HostCSR &h, int unit, int pan_nnz, PanelPlan &pl) 
                 }
                 const bool empty = (int64_t)pl.pan_row.back() == c0;
                 if (empty || (cur_nnz + cn <= pan_nnz &&
-                              (int64_t)(rays.size() + fresh.size()) <= kPanelSegCap))
+                              (int64_t)(rays.size() + fresh.size()) <= seg_cap))
                     break;
                 for (int32_t r : fresh) stamp[r] = -1;   // the chunk opens a new panel
                 close(c0);
             }
-            if ((int64_t)(rays.size() + fresh.size()) > kPanelSegCap) return false;
+            if ((int64_t)(rays.size() + fresh.size()) > seg_cap) return false;
             rays.insert(rays.end(), fresh.begin(), fresh.end());
             cur_nnz += cn;
         }
@@ -244,9 +246,10 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
                         int chunk_blocks, Panels &P, bool pack12 = false) {
     hipStream_t s = ctx->stream;
     TV3_REQUIRE(G == 1 || G == 2 || G == 4, TV3_E_ARG, "panel groups of 1, 2 or 4 entries");
-    TV3_REQUIRE(!pack12 || (vt == TV3_F16 && G == 1 && (blk_nnz == 2048 || blk_nnz == 4096)),
-                TV3_E_ARG, "packed entries take FP16 weights, unpadded rows and blocks of "
-                "2048 or 4096");
+    TV3_REQUIRE(!pack12 || (vt == TV3_F16 && (G == 1 || G == 4) &&
+                            (blk_nnz == 2048 || blk_nnz == 4096)),
+                TV3_E_ARG, "packed entries take FP16 weights, rows unpadded or in groups of 4 "
+                "and blocks of 2048 or 4096");
     const int64_t align = pack12 ? 8 : 4;
     const int64_t npan = (int64_t)pan_row.size() - 1, rows = pan_row.back();
     auto plen = [&](int64_t r) { return ((int64_t)rptr[r + 1] - rptr[r] + G - 1) / G * G; };
@@ -393,8 +396,9 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
 }
 
 // A^T in panel form: rows = voxels, sources = the tile's rays.
+// pack12: packed entries (the plan's panels hold at most 4096 rays)
 static void build_panels_T(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, int blk_nnz,
-                           Panels &P) {
+                           Panels &P, bool pack12 = false) {
     std::vector<int32_t> local(h.cols > 0 ? h.cols : 1, 0);
     std::vector<uint16_t> lidx(h.nnz > 0 ? h.nnz : 1);
     const int64_t npan = (int64_t)pl.pan_row.size() - 1;
@@ -405,7 +409,7 @@ static void build_panels_T(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
             lidx[k] = (uint16_t)local[h.ind[k]];
     }
     pack_panels(ctx, pl.pan_row, pl.pan_seg, &pl.seg_ray, h.ptr, lidx, h.val.data(), h.vb,
-                h.vtype, blk_nnz, kPanelGroupT, kPanelChunkT, P);
+                h.vtype, blk_nnz, kPanelGroupT, kPanelChunkT, P, pack12);
 }
 
 // A in panel form: rows = ray segments (segment s = pan_seg[p] + position of
@@ -526,23 +530,46 @@ static bool layout_volume(const FusedLayout &F, std::vector<TileVolume> &vols) {
     return true;
 }
 
+// Reads a buffer larger than the MALL (the 256 MiB memory-side cache) so
+// that the next kernel starts with cold caches, as a product does inside the
+// solver, where the other kernels of an iteration stream ~0.5 GB in between.
+__global__ void k_scrub(const float4 *__restrict__ p, int64_t n, float *sink) {
+    float a = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = p[i];
+        a += v.x + v.y + v.z + v.w;
+    }
+    if (a == -1.f) *sink = a;   // the buffer is zero: never taken, keeps the loads
+}
+
 // Median duration (ms) of `reps` launches of f on the context stream, each
-// timed alone after a 512 MiB write that evicts the operator from the
-// 256 MiB memory-side cache, as the solver's other kernels do between two
+// timed alone after k_scrub has read 512 MiB, which evicts the operator from
+// the 256 MiB memory-side cache as the solver's other kernels do between two
 // products (back-to-back launches would time a cache-resident operator and
-// can rank the shapes differently); one untimed launch first.
+// can rank the shapes differently); one untimed launch first.  A 512 MiB
+// write instead leaves the caches full of dirty lines whose write-back
+// lands in the timed launch: at 128^3 it ranked A^T's packed row blocks of
+// 2048 entries first, whose product takes 5% longer in the solver than
+// with the blocks of 4096 the read ranks first.  15 launches: candidates
+// 2% apart (A's packed and 4-byte entries) are ranked as the solver ranks
+// them, which the median of 5 did not always do.
 struct ColdTimer {
+    static constexpr int64_t kScrub = (int64_t)(512 << 20) / 16;
     tv3_ctx *ctx;
-    DevBuf<uint8_t> flush;
-    explicit ColdTimer(tv3_ctx *c) : ctx(c), flush(size_t(512) << 20) {}
+    DevBuf<float4> scrub;
+    DevBuf<float> sink;
+    explicit ColdTimer(tv3_ctx *c) : ctx(c), scrub(kScrub), sink(1) {
+        TV3_HIP(hipMemsetAsync(scrub.p, 0, scrub.bytes(), c->stream));
+    }
     template <typename F>
-    float operator()(F &&f, int reps = 5) {
+    float operator()(F &&f, int reps = 15) {
         hipStream_t s = ctx->stream;
         std::vector<hipEvent_t> ev(2 * reps);
         for (auto &e : ev) TV3_HIP(hipEventCreate(&e));
         f();
         for (int r = 0; r < reps; ++r) {
-            TV3_HIP(hipMemsetAsync(flush.p, r & 0xFF, flush.bytes(), s));
+            hipLaunchKernelGGL(k_scrub, dim3(4096), dim3(256), 0, s, scrub.p, kScrub, sink.p);
             TV3_HIP(hipEventRecord(ev[2 * r], s));
             f();
             TV3_HIP(hipEventRecord(ev[2 * r + 1], s));
@@ -605,26 +632,33 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
     DevBuf<float> xr(m > 0 ? m : 1), xv(n > 0 ? n : 1), y(n > m ? n : (m > 0 ? m : 1));
     TV3_HIP(hipMemsetAsync(xr.p, 0, xr.bytes(), s));
     TV3_HIP(hipMemsetAsync(xv.p, 0, xv.bytes(), s));
-    for (int unit : {512, 1024}) {
-        PanelPlan pl2;
-        if (unit != 512 && !plan_panels(h, unit, 16384 * unit / 512, pl2)) continue;
-        const PanelPlan &pt = unit == 512 ? pl : pl2;
-        for (int nb : {1024, 2048}) {
-            Panels P;
-            build_panels_T(ctx, h, pt, nb, P);
-            const float t =
-                time_launches([&]() { launch_panel(P, xr.p, EpiStoreIf{y.p, nullptr}, s); });
-            if (t < t_T) {
-                t_T = t;
-                F.PT = std::move(P);
-                F.panels_T = true;
-                F.unit_T = unit;
-                F.npan = (int64_t)pt.pan_row.size() - 1;
-                F.nseg = (int64_t)pt.seg_ray.size();
-                F.max_seg = (int)pt.max_seg;
+    // FP16 weights: also with packed entries (panels of at most 4096 rays)
+    const bool f16 = F.AT.vtype == TV3_F16;
+    for (int unit : {512, 1024})
+        for (int packed = 0; packed <= (f16 ? 1 : 0); ++packed) {
+            PanelPlan pl2;
+            const bool own = unit != 512 || packed;
+            if (own && !plan_panels(h, unit, 16384 * unit / 512, pl2,
+                                    packed ? 4096 : kPanelSegCap))
+                continue;
+            const PanelPlan &pt = own ? pl2 : pl;
+            const int nbs[2][2] = {{1024, 2048}, {2048, 4096}};
+            for (int nb : nbs[packed]) {
+                Panels P;
+                build_panels_T(ctx, h, pt, nb, P, packed != 0);
+                const float t =
+                    time_launches([&]() { launch_panel(P, xr.p, EpiStoreIf{y.p, nullptr}, s); });
+                if (t < t_T) {
+                    t_T = t;
+                    F.PT = std::move(P);
+                    F.panels_T = true;
+                    F.unit_T = unit;
+                    F.npan = (int64_t)pt.pan_row.size() - 1;
+                    F.nseg = (int64_t)pt.seg_ray.size();
+                    F.max_seg = (int)pt.max_seg;
+                }
             }
         }
-    }
     for (int unit : {512, 1024}) {
         PanelPlan pl2;
         if (unit != 512 && !plan_panels(h, unit, 16384 * unit / 512, pl2)) continue;
@@ -657,7 +691,6 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
     if (!layout_volume(F, vols) || !supertile_rows(vols, h, hs, tmap)) return;
     // FP16 weights: also with packed entries (12-bit indices, 28 bytes per 8
     // entries instead of 32)
-    const bool f16 = F.AT.vtype == TV3_F16;
     for (int unit : {2048, 4096}) {
         PanelPlan pl2;
         if (!plan_panels(hs, unit, 16384 * unit / 512, pl2)) continue;
@@ -830,9 +863,11 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
                 "launch shapes to force");
     F.session.reset();               // its graphs hold the old launch shapes
     tv3_ctx *ctx = op->ctx;
-    const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0;
+    const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0, packed_T = (panels & 512) != 0;
     const int unit_A = (panels & 32) ? 4096 : (panels & 16) ? 2048 : (panels & 4) ? 1024 : 512;
     const int unit_T = (panels & 8) ? 1024 : 512;
+    TV3_REQUIRE(!packed_T || (pt && F.AT.vtype == TV3_F16 && code_T != -1024), TV3_E_UNSUPPORTED,
+                "packed panel entries take FP16 weights and blocks of 2048 or 4096");
     if (pa || pt) {
         TV3_REQUIRE(pan_nnz > 0, TV3_E_ARG, "panels need pan_nnz > 0");
         for (int c : {pa ? code_N : -1024, pt ? code_T : -1024})
@@ -845,12 +880,14 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
         F.npan = (int64_t)pl.pan_row.size() - 1;
         F.nseg = (int64_t)pl.seg_ray.size();
         F.max_seg = (int)pl.max_seg;
-        if (pt && unit_T == 512) build_panels_T(ctx, h, pl, -code_T, F.PT);
-        if (pt && unit_T != 512) {
+        if (pt && unit_T == 512 && !packed_T) build_panels_T(ctx, h, pl, -code_T, F.PT);
+        if (pt && (unit_T != 512 || packed_T)) {
             PanelPlan pl2;
-            TV3_REQUIRE(plan_panels(h, unit_T, 2 * pan_nnz, pl2), TV3_E_UNSUPPORTED,
+            TV3_REQUIRE(plan_panels(h, unit_T, pan_nnz * (unit_T / 512), pl2,
+                                    packed_T ? 4096 : kPanelSegCap),
+                        TV3_E_UNSUPPORTED,
                         "a panel of this operator would need more rays than LDS holds");
-            build_panels_T(ctx, h, pl2, -code_T, F.PT);
+            build_panels_T(ctx, h, pl2, -code_T, F.PT, packed_T);
             F.npan = (int64_t)pl2.pan_row.size() - 1;
             F.nseg = (int64_t)pl2.seg_ray.size();
             F.max_seg = (int)pl2.max_seg;
@@ -890,19 +927,6 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
 __global__ void k_fill(float *p, float v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
-}
-
-// Reads a buffer larger than the MALL (the 256 MiB memory-side cache) so
-// that the next kernel starts with cold caches, as a product does inside the
-// solver, where the other kernels of an iteration stream ~0.5 GB in between.
-__global__ void k_scrub(const float4 *__restrict__ p, int64_t n, float *sink) {
-    float a = 0.f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const float4 v = p[i];
-        a += v.x + v.y + v.z + v.w;
-    }
-    if (a == -1.f) *sink = a;   // the buffer is zero: never taken, keeps the loads
 }
 
 // Bandwidth probes: a streaming copy (read n, write n float4) and a streaming

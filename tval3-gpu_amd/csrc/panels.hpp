@@ -226,13 +226,16 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
     epi.finish();
 }
 
-// A's panel form with packed entries (Panels::fmt 12, FP16 weights, rows
-// unpadded): the stored entries in octets of 8, their 12-bit local indices
-// in 3 words (entry e at bits 12 e .. 12 e + 11 of the 96) and their 8 FP16
-// weights in 16 bytes -- 28 bytes for 8 entries instead of 32, for panels
-// of at most 4,096 sources (boxes of 16^3 voxels).  Row blocks start and end
-// 8-aligned; otherwise as k_panel with G = 1 and contiguous sources (staged
-// through the tile map src when one is given).
+// The panel form with packed entries (Panels::fmt 12, FP16 weights): the
+// stored entries in octets of 8, their 12-bit local indices in 3 words
+// (entry e at bits 12 e .. 12 e + 11 of the 96) and their 8 FP16 weights in
+// 16 bytes -- 28 bytes for 8 entries instead of 32, for panels of at most
+// 4,096 sources (A's boxes of 16^3 voxels, A^T panels of at most 4,096
+// rays).  Row blocks start and end 8-aligned; rows unpadded (G = 1) or
+// padded to groups of 4 (G = 4: an octet holds two groups, whose sums are
+// formed as k_panel forms them, so both kernels give the same bits for the
+// same rows and blocks).  Sources are contiguous (src null), staged through
+// a tile map (src, gather = 0) or gathered from a ray list (src, gather = 1).
 struct Oct {
     uint3 i;
     uint4 w;
@@ -259,11 +262,11 @@ __device__ __forceinline__ void oct_prod(const Oct &c, const float *xs, float4 &
                      oct_w(c.w.w, 0) * xs[i6], oct_w(c.w.w, 1) * xs[i7]);
 }
 
-template <int NB, typename Epi>
+template <int NB, int G, typename Epi>
 __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ order,
                                                  const int32_t *__restrict__ src_off,
                                                  const int32_t *__restrict__ chunk,
-                                                 const int32_t *__restrict__ tmap,
+                                                 const int32_t *__restrict__ src, int gather,
                                                  const BlkDesc *__restrict__ desc,
                                                  const int32_t *__restrict__ pptr,
                                                  const uint32_t *__restrict__ pi,
@@ -272,8 +275,9 @@ __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ ord
     constexpr int NO = NB / 8;                   // octets per block
     constexpr int J = NO / 256;
     static_assert(J >= 1, "blocks of at least 2048 entries");
+    static_assert(G == 1 || G == 4, "rows unpadded or in groups of 4");
     extern __shared__ float xs[];                // the panel's source values
-    __shared__ __align__(16) float prod[NB];     // the block's products
+    __shared__ __align__(16) float prod[NB / G]; // the block's products (group sums)
     constexpr int NX = ExtraBlocks<Epi>::value;
     if constexpr (NX > 0) {
         if ((int)blockIdx.x < NX) {
@@ -296,15 +300,21 @@ __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ ord
     for (int j = 0; j < J; ++j) cur[j] = load_oct(pi, pw, min((int64_t)d.p0 / 8 + j * 256 + t, ohi));
     for (int k0 = 0; k0 < ns; k0 += 8 * 256) {
         float xv[8];
-        if (tmap == nullptr) {
+        if (src == nullptr) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) xv[i] = x[s0 + min(k0 + i * 256 + t, ns - 1)];
-        } else {
+        } else if (!gather) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int k = s0 + min(k0 + i * 256 + t, ns - 1);
-                xv[i] = x[((int64_t)tmap[k >> 9] << 9) | (k & 511)];
+                xv[i] = x[((int64_t)src[k >> 9] << 9) | (k & 511)];
             }
+        } else {
+            int rv[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) rv[i] = src[s0 + min(k0 + i * 256 + t, ns - 1)];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xv[i] = x[rv[i]];
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i)
@@ -321,20 +331,26 @@ __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ ord
         }
         if (d.len <= NB) {
             const int row = block_row(d);
-            const int a = pptr[row] - d.p0, e = pptr[row + 1] - d.p0;
+            const int a = (pptr[row] - d.p0) / G, e = (pptr[row + 1] - d.p0) / G;
             const auto pre = epi.load(row);
             if (d.len > 0) {
 #pragma unroll
                 for (int j = 0; j < J; ++j) {
                     float4 v0, v1;
                     oct_prod(cur[j], xs, v0, v1);
-                    const int q = 8 * (j * 256 + t);
-                    *reinterpret_cast<float4 *>(prod + q) = v0;
-                    *reinterpret_cast<float4 *>(prod + q + 4) = v1;
+                    const int o = j * 256 + t;
+                    if constexpr (G == 4) {
+                        *reinterpret_cast<float2 *>(prod + 2 * o) =
+                            make_float2((v0.x + v0.y) + (v0.z + v0.w),
+                                        (v1.x + v1.y) + (v1.z + v1.w));
+                    } else {
+                        *reinterpret_cast<float4 *>(prod + 8 * o) = v0;
+                        *reinterpret_cast<float4 *>(prod + 8 * o + 4) = v1;
+                    }
                 }
             }
             __syncthreads();
-            block_rows<NB>(prod, d, a, e, epi, row, pre);
+            block_rows<NB / G>(prod, d, a, e, epi, row, pre);
         } else {
             // one row longer than NB
             const auto pre = epi.load(d.r0);
@@ -457,16 +473,22 @@ void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s
         const unsigned g = (unsigned)P.nitem + ExtraBlocks<Epi>::value;
         const size_t lds = sizeof(float) * (size_t)(P.max_src > 0 ? P.max_src : 1);
         if (g == 0) return;
-        if (P.blk_nnz == 2048)
-            hipLaunchKernelGGL((k_panel12<2048, Epi>), dim3(g), dim3(256), lds, s, P.order.p,
-                               P.src_off.p, P.chunk.p, P.src.p, P.desc.p, P.pptr.p, P.pk.p,
-                               reinterpret_cast<const uint4 *>(P.pw.p), P.stored / 8 - 1, x, epi);
-        else if (P.blk_nnz == 4096)
-            hipLaunchKernelGGL((k_panel12<4096, Epi>), dim3(g), dim3(256), lds, s, P.order.p,
-                               P.src_off.p, P.chunk.p, P.src.p, P.desc.p, P.pptr.p, P.pk.p,
-                               reinterpret_cast<const uint4 *>(P.pw.p), P.stored / 8 - 1, x, epi);
-        else
-            throw tv3_error(TV3_E_ARG, "packed panel entries take blocks of 2048 or 4096");
+        // sources: A's contiguous range or tile map, A^T's ray list
+        const int gather = P.contiguous ? 0 : 1;
+#define TV3_P12CASE(NBV, GV)                                                                  \
+    case NBV * 8 + GV:                                                                        \
+        hipLaunchKernelGGL((k_panel12<NBV, GV, Epi>), dim3(g), dim3(256), lds, s, P.order.p,    \
+                           P.src_off.p, P.chunk.p, P.src.p, gather, P.desc.p, P.pptr.p,       \
+                           P.pk.p, reinterpret_cast<const uint4 *>(P.pw.p), P.stored / 8 - 1,  \
+                           x, epi);                                                           \
+        break;
+        switch (P.blk_nnz * 8 + P.group) {
+            TV3_P12CASE(2048, 1) TV3_P12CASE(4096, 1) TV3_P12CASE(2048, 4) TV3_P12CASE(4096, 4)
+        default:
+            throw tv3_error(TV3_E_ARG, "packed panel entries take blocks of 2048 or 4096 and "
+                                       "groups of 1 or 4");
+        }
+#undef TV3_P12CASE
         return;
     }
     switch (P.vtype) {

@@ -830,6 +830,7 @@ static void fill_layout_info(const FusedLayout &F, tv3_layout_info *info) {
                    (F.panels_A && F.unit_A == 2048 ? 16 : 0) |
                    (F.panels_A && F.unit_A == 4096 ? 32 : 0) |
                    (F.panels_A && F.PA.fmt == 12 ? 256 : 0) |
+                   (F.panels_T && F.PT.fmt == 12 ? 512 : 0) |
                    (F.panels_T && F.unit_T != 512 ? 8 : 0);
     info->code_N = F.panels_A ? -F.PA.blk_nnz : code(F.A);
     info->code_T = F.panels_T ? -F.PT.blk_nnz : code(F.AT);
@@ -863,10 +864,12 @@ tv3_status tv3_op_layout_force(tv3_op *op, const int32_t dims[3], int32_t code_N
     return guard(op->ctx, [&] {
         TV3_REQUIRE(ok(code_N) && ok(code_T), TV3_E_ARG,
                     "launch codes are 0, a power of two in [2, 64], -1024, -2048 or -4096");
-        TV3_REQUIRE(pan_nnz >= 0 && panels >= 0 && panels <= 511 && (panels & 192) == 0,
-                    TV3_E_ARG, "pan_nnz must be >= 0 and panels bits 0-5 and 8");
+        TV3_REQUIRE(pan_nnz >= 0 && panels >= 0 && panels <= 1023 && (panels & 192) == 0,
+                    TV3_E_ARG, "pan_nnz must be >= 0 and panels bits 0-5, 8 and 9");
         TV3_REQUIRE(!(panels & 256) || (panels & (16 | 32)), TV3_E_ARG,
                     "packed entries (bit 8) go with A's panels of 4 or 8 tiles (bit 4 or 5)");
+        TV3_REQUIRE(!(panels & 512) || (panels & 2), TV3_E_ARG,
+                    "packed A^T entries (bit 9) go with A^T's panel form (bit 1)");
         const int ua = panels & (4 | 16 | 32);
         TV3_REQUIRE((ua & (ua - 1)) == 0, TV3_E_ARG,
                     "at most one of the panel bits 2, 4 and 5 (A's panels of 2, 4 or 8 tiles)");
