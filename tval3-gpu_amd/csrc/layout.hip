@@ -614,9 +614,10 @@ static float tune_shape(tv3_ctx *ctx, ColdTimer &time_launches, SparseRows &S, i
 // voxels, at most 16384 entries) or of two consecutive tiles (16 x 8 x 8
 // voxels, at most 32768 entries: A's rays leave fewer, longer segments and
 // so fewer partial sums for the second pass; A^T stages the union of the two
-// tiles' rays once), timed over row blocks of 1024 and 2048 entries; A also
-// over panels of 4 and 8 tiles where the tile counts are even; each kept
-// when faster than the tuned plain kernel (t_A, t_T in ms).
+// tiles' rays once), timed over row blocks of 1024 and 2048 entries (A^T
+// also 4096, and with FP16 weights packed entries in blocks of 2048 or
+// 4096); A also over panels of 4 and 8 tiles where the tile counts are
+// even; each kept when faster than the tuned plain kernel (t_A, t_T in ms).
 static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, float t_A,
                         float t_T) {
     hipStream_t s = ctx->stream;
@@ -642,8 +643,8 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
                                     packed ? 4096 : kPanelSegCap))
                 continue;
             const PanelPlan &pt = own ? pl2 : pl;
-            const int nbs[2][2] = {{1024, 2048}, {2048, 4096}};
-            for (int nb : nbs[packed]) {
+            for (int nb : {1024, 2048, 4096}) {
+                if (packed && nb == 1024) continue;      // packed blocks: 2048 or 4096
                 Panels P;
                 build_panels_T(ctx, h, pt, nb, P, packed != 0);
                 const float t =
