@@ -279,8 +279,9 @@ typedef struct tv3_batch_info {
     int32_t panels_T;           /* A^T of the sparse problems in panel form (tiles of
                                    512 voxels, each tile's rays staged in LDS; every
                                    problem's voxel range is padded to a multiple of 512);
-                                   in: 1024 or 2048 uses it with row blocks of that many
-                                   entries, -1 never, 0 timed against the row kernels;
+                                   in: 1024, 2048 or 4096 uses it with row blocks of that
+                                   many entries, -1 never, 0 timed against the row
+                                   kernels;
                                    out: the block size used, 0 when not used */
     int64_t dense_values;       /* entries of the shared dense operators                */
     int32_t panels_A;           /* A of the sparse problems in two-pass panel form

@@ -333,24 +333,25 @@ def test_batch_panels_equal_solo_bitwise(ctx, batch8, blk):
     for k in range(len(batch8)):
         f, fr = hb[k]["f"][:4], hr[k]["f"][:4]
         assert (np.abs(f - fr) <= 1e-5 * np.abs(fr)).all()
-    # timed choice (0): one of the two sizes or the row kernels, reported
+    # timed choice (0): one of the block sizes or the row kernels, reported
     _, _, bt, _ = solve(ctx, batch8, o, panels=0, panels_A=0, history=False)
-    assert bt["panels_T"] in (0, 1024, 2048) and bt["panels_A"] in (0, 1024, 2048)
+    assert bt["panels_T"] in (0, 1024, 2048, 4096) and bt["panels_A"] in (0, 1024, 2048)
 
 
 @pytest.mark.parametrize("blk", [2048, 4096])
 def test_batch_box_panels_equal_solo_bitwise(ctx, batch16, blk):
     """A in two-pass panel form over 16^3 voxels (the tiles of each problem
     regrouped by 2 x 2 x 2, x staged through a tile map): each problem of the
-    batch equals its solo solve bit for bit, and the batch agrees with the
-    one-tile panels to rounding level at the start; panels of 8 tiles are
-    refused for a batch with an odd tile count."""
+    batch equals its solo solve bit for bit (A^T's panels in row blocks of
+    the same size), and the batch agrees with the one-tile panels to
+    rounding level at the start; panels of 8 tiles are refused for a batch
+    with an odd tile count."""
     o = opts(maxit=80, tol_inn=5e-3, mu0=2.0 ** 6, beta0=4.0)
-    Ub, ib, bi, hb = solve(ctx, batch16, o, panels=2048, panels_A=blk, box=8)
-    assert (bi["panels_A"], bi["panel_box_A"]) == (blk, 8)
+    Ub, ib, bi, hb = solve(ctx, batch16, o, panels=blk, panels_A=blk, box=8)
+    assert (bi["panels_T"], bi["panels_A"], bi["panel_box_A"]) == (blk, blk, 8)
     for k, p in enumerate(batch16):
-        Us, is_, bs, hs = solve(ctx, [p], o, panels=2048, panels_A=blk, box=8)
-        assert (bs["panels_A"], bs["panel_box_A"]) == (blk, 8)
+        Us, is_, bs, hs = solve(ctx, [p], o, panels=blk, panels_A=blk, box=8)
+        assert (bs["panels_T"], bs["panels_A"], bs["panel_box_A"]) == (blk, blk, 8)
         assert ib[k]["total_iters"] == is_[0]["total_iters"] > 0
         assert np.array_equal(hb[k]["f"], hs[0]["f"])
         assert np.array_equal(Ub[k].cpu().numpy(), Us[0].cpu().numpy()), k

@@ -1353,15 +1353,16 @@ struct Batch {
         if (!AT.rows || want < 0) return;
         bool aligned = true;
         for (int k = 0; k < ncsr; ++k) aligned = aligned && v0[k] % 512 == 0;
-        TV3_REQUIRE(want == 0 || want == 1024 || want == 2048, TV3_E_ARG,
-                    "panels_T must be 1024, 2048, -1 or 0");
+        TV3_REQUIRE(want == 0 || want == 1024 || want == 2048 || want == 4096, TV3_E_ARG,
+                    "panels_T must be 1024, 2048, 4096, -1 or 0");
         if (!aligned) {
             TV3_REQUIRE(want == 0, TV3_E_UNSUPPORTED,
                         "panel form of A^T needs every sparse problem's voxel count to be a "
                         "multiple of 512");
             return;
         }
-        const std::vector<int> blks = want ? std::vector<int>{want} : std::vector<int>{1024, 2048};
+        const std::vector<int> blks =
+            want ? std::vector<int>{want} : std::vector<int>{1024, 2048, 4096};
         std::vector<Panels> cand;
         if (!build_transposed_panels(ctx, AT, blks, cand)) {
             TV3_REQUIRE(want == 0, TV3_E_UNSUPPORTED,
