@@ -372,9 +372,10 @@ tv3_status tv3_op_prepare(tv3_op *op, const int32_t dims[3], tv3_layout_info *in
  * tiles; bit 4 / bit 5: A's panels of 2 x 2 x 1 / 2 x 2 x 2 tiles, which need
  * 8^3 tiles and an even tile count along every axis; at most one of bits 2, 4
  * and 5; bit 8 with bit 4 or 5: their entries packed, 12-bit local indices
- * and FP16 weights in 28 bytes per 8 entries, FP16 operators and blocks of
- * 2048 or 4096; bit 9 with bit 1: A^T's entries packed the same way, its
- * panels split to at most 4096 rays) and row blocks of -code_N / -code_T
+ * and FP16 weights in 28 bytes per 8 entries, rows padded to pairs, FP16
+ * operators and blocks of 2048 or 4096; bit 9 with bit 1: A^T's entries
+ * packed the same way, rows in groups of 4, its panels split to at most
+ * 4096 rays) and row blocks of -code_N / -code_T
  * entries (-1024, -2048 or -4096);
  * a direction not in panel form takes code_N / code_T as a tv3_op_set_lanes
  * code (0 = keep).  A layout not yet prepared for dims is built here without

@@ -233,6 +233,12 @@ constexpr int kPanelChunkA = TV3_PANEL_CHUNK_A, kPanelChunkT = 1 << 30;
 #define TV3_PANEL_GROUP_T 4
 #endif
 constexpr int kPanelGroupA = TV3_PANEL_GROUP_A, kPanelGroupT = TV3_PANEL_GROUP_T;
+// A's packed entries: rows padded to pairs.  The block's products then take
+// half the LDS (16^3 boxes stage 16 KB of x besides), so more workgroups are
+// resident: at 128^3 A takes 57.0 / 56.1 us in row blocks of 4096 / 2048
+// entries against 59.0 / 64.0 us with unpadded rows, for about 1.2M padding
+// entries (4 MB).
+constexpr int kPanelGroupA12 = 2;
 
 // Pad, block and upload a panelled matrix.  Panel p holds rows [pan_row[p],
 // pan_row[p+1]) and sources src_off[p] .. src_off[p+1] (indices into *src,
@@ -246,10 +252,8 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
                         int chunk_blocks, Panels &P, bool pack12 = false) {
     hipStream_t s = ctx->stream;
     TV3_REQUIRE(G == 1 || G == 2 || G == 4, TV3_E_ARG, "panel groups of 1, 2 or 4 entries");
-    TV3_REQUIRE(!pack12 || (vt == TV3_F16 && (G == 1 || G == 4) &&
-                            (blk_nnz == 2048 || blk_nnz == 4096)),
-                TV3_E_ARG, "packed entries take FP16 weights, rows unpadded or in groups of 4 "
-                "and blocks of 2048 or 4096");
+    TV3_REQUIRE(!pack12 || (vt == TV3_F16 && (blk_nnz == 2048 || blk_nnz == 4096)), TV3_E_ARG,
+                "packed entries take FP16 weights and blocks of 2048 or 4096");
     const int64_t align = pack12 ? 8 : 4;
     const int64_t npan = (int64_t)pan_row.size() - 1, rows = pan_row.back();
     auto plen = [&](int64_t r) { return ((int64_t)rptr[r + 1] - rptr[r] + G - 1) / G * G; };
@@ -443,7 +447,7 @@ static void build_panels_A(tv3_ctx *ctx, const HostCSR &h, const PanelPlan &pl, 
     }
     // panels of the segment matrix: segment ranges pan_seg, voxel ranges pan_row
     pack_panels(ctx, pl.pan_seg, pl.pan_row, nullptr, sptr, lidx, wb.data(), h.vb, h.vtype,
-                blk_nnz, kPanelGroupA, chunk_blocks, P, pack12);
+                blk_nnz, pack12 ? kPanelGroupA12 : kPanelGroupA, chunk_blocks, P, pack12);
     if (tmap) {
         // h's rows in super-tile order: the panels stage x through the tile map
         P.src.alloc(tmap->size());

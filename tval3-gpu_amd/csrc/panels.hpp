@@ -232,10 +232,11 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
 // 16 bytes -- 28 bytes for 8 entries instead of 32, for panels of at most
 // 4,096 sources (A's boxes of 16^3 voxels, A^T panels of at most 4,096
 // rays).  Row blocks start and end 8-aligned; rows unpadded (G = 1) or
-// padded to groups of 4 (G = 4: an octet holds two groups, whose sums are
-// formed as k_panel forms them, so both kernels give the same bits for the
-// same rows and blocks).  Sources are contiguous (src null), staged through
-// a tile map (src, gather = 0) or gathered from a ray list (src, gather = 1).
+// padded to groups of 2 or 4 (an octet holds four or two groups, whose sums
+// are formed as k_panel forms them, so both kernels give the same bits for
+// the same rows and blocks).  Sources are contiguous (src null), staged
+// through a tile map (src, gather = 0) or gathered from a ray list (src,
+// gather = 1).
 struct Oct {
     uint3 i;
     uint4 w;
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ ord
     constexpr int NO = NB / 8;                   // octets per block
     constexpr int J = NO / 256;
     static_assert(J >= 1, "blocks of at least 2048 entries");
-    static_assert(G == 1 || G == 4, "rows unpadded or in groups of 4");
+    static_assert(G == 1 || G == 2 || G == 4, "rows in groups of 1, 2 or 4");
     extern __shared__ float xs[];                // the panel's source values
     __shared__ __align__(16) float prod[NB / G]; // the block's products (group sums)
     constexpr int NX = ExtraBlocks<Epi>::value;
@@ -343,6 +344,9 @@ __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ ord
                         *reinterpret_cast<float2 *>(prod + 2 * o) =
                             make_float2((v0.x + v0.y) + (v0.z + v0.w),
                                         (v1.x + v1.y) + (v1.z + v1.w));
+                    } else if constexpr (G == 2) {
+                        *reinterpret_cast<float4 *>(prod + 4 * o) =
+                            make_float4(v0.x + v0.y, v0.z + v0.w, v1.x + v1.y, v1.z + v1.w);
                     } else {
                         *reinterpret_cast<float4 *>(prod + 8 * o) = v0;
                         *reinterpret_cast<float4 *>(prod + 8 * o + 4) = v1;
@@ -483,10 +487,11 @@ void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s
                            x, epi);                                                           \
         break;
         switch (P.blk_nnz * 8 + P.group) {
-            TV3_P12CASE(2048, 1) TV3_P12CASE(4096, 1) TV3_P12CASE(2048, 4) TV3_P12CASE(4096, 4)
+            TV3_P12CASE(2048, 1) TV3_P12CASE(4096, 1) TV3_P12CASE(2048, 2) TV3_P12CASE(4096, 2)
+            TV3_P12CASE(2048, 4) TV3_P12CASE(4096, 4)
         default:
             throw tv3_error(TV3_E_ARG, "packed panel entries take blocks of 2048 or 4096 and "
-                                       "groups of 1 or 4");
+                                       "groups of 1, 2 or 4");
         }
 #undef TV3_P12CASE
         return;
