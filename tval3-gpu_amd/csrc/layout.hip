@@ -221,6 +221,10 @@ static bool plan_panels(const HostCSR &h, int unit, int pan_nnz, PanelPlan &pl,
 #define TV3_PANEL_CHUNK_A 6
 #endif
 constexpr int kPanelChunkA = TV3_PANEL_CHUNK_A, kPanelChunkT = 1 << 30;
+// A's packed 16^3 panels (rows in pairs, more workgroups resident) in chunks
+// of 12: measured at 128^3 with row blocks of 2048 / 4096 entries, chunks of
+// 3 60.0 / 53.7 us, of 6 56.0 / 57.2, of 12 51.7 / 52.8.
+constexpr int kPanelChunkA12 = 12;
 
 // Entry groups of the panel rows: A's rows (ray segments, ~8 entries at
 // 128^3) unpadded, A^T's (voxels, ~17) in groups of 4.  Measured at 128^3,
@@ -703,7 +707,8 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
             for (int nb : {2048, 4096}) {
                 Panels P;
                 SegSum R;
-                build_panels_A(ctx, hs, pl2, nb, P, R, kPanelChunkA, &tmap, packed != 0);
+                build_panels_A(ctx, hs, pl2, nb, P, R, packed ? kPanelChunkA12 : kPanelChunkA,
+                               &tmap, packed != 0);
                 const float t = time_launches([&]() {
                     launch_panel(P, xv.p, EpiStoreIf{R.part.p, nullptr}, s);
                     launch_seg_sum(R, EpiStoreIf{y.p, nullptr}, s);
@@ -913,7 +918,8 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
             const bool packed = (panels & 256) != 0;
             TV3_REQUIRE(!packed || (F.AT.vtype == TV3_F16 && code_N != -1024), TV3_E_UNSUPPORTED,
                         "packed panel entries take FP16 weights and blocks of 2048 or 4096");
-            build_panels_A(ctx, hs, pl2, -code_N, F.PA, F.RS, kPanelChunkA, &tmap, packed);
+            build_panels_A(ctx, hs, pl2, -code_N, F.PA, F.RS,
+                           packed ? kPanelChunkA12 : kPanelChunkA, &tmap, packed);
         }
         if (pa && unit_A == 1024) {
             PanelPlan pl2;
