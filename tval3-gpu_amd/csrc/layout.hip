@@ -221,10 +221,25 @@ static bool plan_panels(const HostCSR &h, int unit, int pan_nnz, PanelPlan &pl,
 #define TV3_PANEL_CHUNK_A 6
 #endif
 constexpr int kPanelChunkA = TV3_PANEL_CHUNK_A, kPanelChunkT = 1 << 30;
-// A's packed 16^3 panels (rows in pairs, more workgroups resident) in chunks
-// of 12: measured at 128^3 with row blocks of 2048 / 4096 entries, chunks of
-// 3 60.0 / 53.7 us, of 6 56.0 / 57.2, of 12 51.7 / 52.8.
-constexpr int kPanelChunkA12 = 12;
+// A's packed 16^3 panels (rows in pairs): chunk_blocks 0 = the fewest
+// blocks per work item whose items all fit one round of resident
+// workgroups.  A second, nearly empty round or an under-filled first one
+// costs more than the x staging of smaller items: measured at 128^3 with row
+// blocks of 2048 / 4096 entries (8 / 6 resident workgroups per CU), chunks
+// of 3 60.0 / 53.7 us, 6 56.0 / 57.2, 9 61.7 / 50.1, 12 51.7 / 52.6,
+// 16 54.7 / 56.4; the rule picks 12 / 9.
+constexpr int kPanelChunkA12 = 0;
+
+// Workgroups of the packed A kernel (rows in pairs) resident at once on the
+// GPU, with its dynamic LDS for max_src staged sources.
+static int64_t packed_A_slots(tv3_ctx *ctx, int blk_nnz, int max_src) {
+    const size_t lds = sizeof(float) * (size_t)std::max(max_src, 1);
+    const void *k = blk_nnz == 2048 ? reinterpret_cast<const void *>(&k_panel12<2048, 2, EpiStoreIf>)
+                                    : reinterpret_cast<const void *>(&k_panel12<4096, 2, EpiStoreIf>);
+    int per_cu = 0;
+    TV3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds));
+    return (int64_t)std::max(1, per_cu) * std::max(1, ctx->num_cus);
+}
 
 // Entry groups of the panel rows: A's rows (ray segments, ~8 entries at
 // 128^3) unpadded, A^T's (voxels, ~17) in groups of 4.  Measured at 128^3,
@@ -326,6 +341,16 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
                 std::memcpy(&pv[q], wb + (size_t)k * 4, 4);
             }
         }
+    if (chunk_blocks <= 0) {
+        TV3_REQUIRE(pack12 && G == 2, TV3_E_INTERNAL, "chunks sized to one round: packed A only");
+        const int64_t slots = packed_A_slots(ctx, blk_nnz, max_src);
+        chunk_blocks = 64;
+        for (int c = 3; c < 64; ++c) {
+            int64_t items = 0;
+            for (int64_t p = 0; p < npan; ++p) items += (pan_blk[p + 1] - pan_blk[p] + c - 1) / c;
+            if (items <= slots) { chunk_blocks = c; break; }
+        }
+    }
     // work items: each panel's row blocks in chunks of at most chunk_blocks,
     // one workgroup each (staging the panel's sources itself), so that the
     // largest panels -- 29 blocks against 9 on average for the two-tile A
