@@ -255,13 +255,16 @@ def main():
     bA, bAT = bytes_A(nnz, m, n, sv), bytes_AT(nnz, m, n, sv)
     gA, gAT = bA / tA, bAT / tAT
     log(f"SpMV A: {tA * 1e6:.1f} us {gA / 1e9:.0f} GB/s; A^T: {tAT * 1e6:.1f} us {gAT / 1e9:.0f} GB/s")
+    # the panel kernels: k_panel12 with packed entries (layout bits 8 and 9)
+    kA = "k_panel12" if panels & 256 else "k_panel"
+    kT = "k_panel12" if panels & 512 else "k_panel"
     if tAT >= tA:
         dom_t, dom_b, dom_key = tAT, bAT, "AT"
-        dom_name = ("A^T product, panel form (k_panel, rays staged in LDS)" if panels & 2
+        dom_name = (f"A^T product, panel form ({kT}, rays staged in LDS)" if panels & 2
                     else "A^T product (CSC row blocks)")
     else:
         dom_t, dom_b, dom_key = tA, bA, "A"
-        dom_name = ("A product, panel form (k_panel segment sums + k_seg_sum)" if panels & 1
+        dom_name = (f"A product, panel form ({kA} segment sums + k_seg_sum)" if panels & 1
                     else "A product (CSR row blocks)")
     traffic = measured_traffic(dom_key, N, args.path)
 
