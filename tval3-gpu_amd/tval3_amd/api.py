@@ -192,7 +192,9 @@ class Operator:
     def layout_force(self, dims, code_N=0, code_T=0, pan_nnz=16384, panels=0):
         """Override the layout's launch shapes.  panels: bit 0 = A x, bit 1 =
         A^T r in panel form (block sizes -code_N / -code_T, at most pan_nnz
-        entries per panel); other directions take set_lanes codes."""
+        entries per panel; the other bits as tv3_op_layout_force documents
+        them: panel sizes, and bits 8 / 9 packed entries of A / A^T); other
+        directions take set_lanes codes."""
         d = np.ascontiguousarray(dims, np.int32)
         info = L.LayoutInfo()
         self.ctx.check(L.load().tv3_op_layout_force(self.h, _vp(d), code_N, code_T, pan_nnz,
@@ -435,10 +437,10 @@ def tval3_gpu_3d_batched(ctx, ops, bs, dims, opts=None, Uts=None, history=False,
     stopping.  Problems sharing one dense BF16 operator run its products as
     one matrix-core product.  lanes = (lanes_N, lanes_T) fixes the sparse
     products' launch shapes, splits = (splits_N, splits_T) the dense
-    products' K splits (default: timed); panels = 1024 or 2048 puts A^T of the
-    sparse problems in panel form with row blocks of that size, -1 never
+    products' K splits (default: timed); panels = 1024, 2048 or 4096 puts A^T
+    of the sparse problems in panel form with row blocks of that size, -1 never
     (default: timed where possible); panels_A likewise for the two-pass
-    panel form of A (4096 too with boxes of 8 tiles), panel_box_A = 1 or 8
+    panel form of A (1024 or 2048; 4096 too with boxes of 8 tiles), panel_box_A = 1 or 8
     its tiles per panel (8: 16^3 voxels; default: timed with panels_A unset,
     else 1).  Returns (list of U tensors, list of
     info dicts, batch info dict, list of history dicts or None)."""
