@@ -230,12 +230,15 @@ constexpr int kPanelChunkA = TV3_PANEL_CHUNK_A, kPanelChunkT = 1 << 30;
 // 16 54.7 / 56.4; the rule picks 12 / 9.
 constexpr int kPanelChunkA12 = 0;
 
-// Workgroups of the packed A kernel (rows in pairs) resident at once on the
-// GPU, with its dynamic LDS for max_src staged sources.
-static int64_t packed_A_slots(tv3_ctx *ctx, int blk_nnz, int max_src) {
+// Workgroups of the packed A kernel (rows in groups of G = 2 or 4) resident
+// at once on the GPU, with its dynamic LDS for max_src staged sources.
+static int64_t packed_A_slots(tv3_ctx *ctx, int blk_nnz, int G, int max_src) {
     const size_t lds = sizeof(float) * (size_t)std::max(max_src, 1);
-    const void *k = blk_nnz == 2048 ? reinterpret_cast<const void *>(&k_panel12<2048, 2, EpiStoreIf>)
-                                    : reinterpret_cast<const void *>(&k_panel12<4096, 2, EpiStoreIf>);
+    const void *k =
+        G == 4 ? (blk_nnz == 2048 ? reinterpret_cast<const void *>(&k_panel12<2048, 4, EpiStoreIf>)
+                                  : reinterpret_cast<const void *>(&k_panel12<4096, 4, EpiStoreIf>))
+               : (blk_nnz == 2048 ? reinterpret_cast<const void *>(&k_panel12<2048, 2, EpiStoreIf>)
+                                  : reinterpret_cast<const void *>(&k_panel12<4096, 2, EpiStoreIf>));
     int per_cu = 0;
     TV3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds));
     return (int64_t)std::max(1, per_cu) * std::max(1, ctx->num_cus);
@@ -256,8 +259,13 @@ constexpr int kPanelGroupA = TV3_PANEL_GROUP_A, kPanelGroupT = TV3_PANEL_GROUP_T
 // half the LDS (16^3 boxes stage 16 KB of x besides), so more workgroups are
 // resident: at 128^3 A takes 57.0 / 56.1 us in row blocks of 4096 / 2048
 // entries against 59.0 / 64.0 us with unpadded rows, for about 1.2M padding
-// entries (4 MB).
-constexpr int kPanelGroupA12 = 2;
+// entries (4 MB).  Groups of 4 (8 workgroups per CU also in blocks of 4096,
+// 3.6M padding entries): 51.0 against 50.3 us with work items sized to one
+// round, equal in blocks of 2048.
+#ifndef TV3_PANEL_GROUP_A12
+#define TV3_PANEL_GROUP_A12 2
+#endif
+constexpr int kPanelGroupA12 = TV3_PANEL_GROUP_A12;
 
 // Pad, block and upload a panelled matrix.  Panel p holds rows [pan_row[p],
 // pan_row[p+1]) and sources src_off[p] .. src_off[p+1] (indices into *src,
@@ -342,8 +350,9 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
             }
         }
     if (chunk_blocks <= 0) {
-        TV3_REQUIRE(pack12 && G == 2, TV3_E_INTERNAL, "chunks sized to one round: packed A only");
-        const int64_t slots = packed_A_slots(ctx, blk_nnz, max_src);
+        TV3_REQUIRE(pack12 && (G == 2 || G == 4), TV3_E_INTERNAL,
+                    "chunks sized to one round: packed A only");
+        const int64_t slots = packed_A_slots(ctx, blk_nnz, G, max_src);
         chunk_blocks = 64;
         for (int c = 3; c < 64; ++c) {
             int64_t items = 0;
