@@ -350,7 +350,7 @@ typedef struct tv3_layout_info {
                                staged in LDS); bit 2 / bit 3: A's / A^T's panels
                                span two tiles; bit 4 / bit 5: A's panels span
                                2 x 2 x 1 / 2 x 2 x 2 tiles; bit 8: their entries
-                               packed (12-bit indices, FP16 weights); bit 9: A^T's
+                               packed (12-bit indices, FP16/BF16 weights); bit 9: A^T's
                                entries packed (panels of at most 4096 rays) */
     int32_t code_N, code_T; /* launch shapes as tv3_op_autotune codes; for a panel form its
                                row block size as -NB */
@@ -372,8 +372,8 @@ tv3_status tv3_op_prepare(tv3_op *op, const int32_t dims[3], tv3_layout_info *in
  * tiles; bit 4 / bit 5: A's panels of 2 x 2 x 1 / 2 x 2 x 2 tiles, which need
  * 8^3 tiles and an even tile count along every axis; at most one of bits 2, 4
  * and 5; bit 8 with bit 4 or 5: their entries packed, 12-bit local indices
- * and FP16 weights in 28 bytes per 8 entries, rows padded to pairs, FP16
- * operators and blocks of 2048 or 4096; bit 9 with bit 1: A^T's entries
+ * and 16-bit weights in 28 bytes per 8 entries, rows padded to pairs, FP16
+ * or BF16 operators and blocks of 2048 or 4096; bit 9 with bit 1: A^T's entries
  * packed the same way, rows in groups of 4, its panels split to at most
  * 4096 rays) and row blocks of -code_N / -code_T
  * entries (-1024, -2048 or -4096);

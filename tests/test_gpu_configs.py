@@ -124,9 +124,14 @@ def test_cfg1_fused_tracks_oracle(ctx, cfg1):
     check_envelope(h, ho, runs, io, info, U, Uo, u, "cfg1 fused")
 
 
-def test_fused_bf16_weights_track_oracle(ctx):
-    dims = (24, 24, 24)
+@pytest.mark.parametrize("dims,shapes", [
+    ((24, 24, 24), None),
+    # both products with packed entries (A over 16^3 boxes, A^T over one tile)
+    ((32, 32, 32), (-4096, -4096, 16384, 1 | 2 | 32 | 256 | 512))])
+def test_fused_bf16_weights_track_oracle(ctx, dims, shapes):
     u, A16, oop16, b = problem(ctx, dims, 64, 128, seed=3, dtype="bf16")
+    if shapes is not None:
+        assert A16.layout_force(dims, *shapes)["panels"] == shapes[3]
     rp, c, v = A16.download_csr()
     bf = (np.asarray(v, np.float32).view(np.uint32) & 0xFFFF0000).view(np.float32)
     assert np.array_equal(v, bf.astype(np.float64))          # values are BF16 numbers

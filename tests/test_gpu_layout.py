@@ -140,8 +140,8 @@ def test_layout_supertile_panels_need_even_tiles(ctx):
 
 
 def test_layout_packed_entries_need_fp16(ctx):
-    """packed entries hold FP16 weights: an FP32 operator is refused in
-    either direction; packed A^T entries need A^T's panel form and blocks
+    """packed entries hold FP16 or BF16 weights: an FP32 operator is refused
+    in either direction; packed A^T entries need A^T's panel form and blocks
     of 2048 or 4096"""
     dims = (16, 16, 16)
     op, _, _, _ = integer_case(ctx, dims, 8, 16, "f32")
@@ -154,6 +154,26 @@ def test_layout_packed_entries_need_fp16(ctx):
         op16.layout_force(dims, -2048, -2048, 16384, 1 | 512)
     with pytest.raises(TV3Error, match="2048 or 4096"):
         op16.layout_force(dims, -2048, -1024, 16384, 2 | 512)
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 16, 32)])
+def test_layout_packed_bf16_exact(ctx, dims):
+    """packed entries with BF16 weights (the upper half of an FP32): A over
+    16^3 and 16 x 16 x 8 boxes and A^T over one and two tiles, integer
+    inputs give the oracle's bits"""
+    op, oop, x, r = integer_case(ctx, dims, 24, 64, "bf16")
+    y_ref = orc.matvec(oop, 0, x.astype(np.float64)).astype(np.float32)
+    z_ref = orc.matvec(oop, 1, r.astype(np.float64)).astype(np.float32)
+    op.prepare(dims)
+    for code_N, code_T, pan, panels in [(-4096, -4096, 16384, 1 | 2 | 32 | 256 | 512),
+                                        (-2048, -2048, 300, 1 | 2 | 8 | 16 | 256 | 512),
+                                        (-2048, -4096, 16384, 1 | 2 | 8 | 32 | 256 | 512)]:
+        info = op.layout_force(dims, code_N, code_T, pan, panels)
+        assert info["panels"] == panels
+        y = H(op.layout_mat_vec_mul(dims, T(x)))
+        z = H(op.layout_mat_vec_mul(dims, T(r), trans=True))
+        assert (y == y_ref).all(), (dims, code_N, pan, info)
+        assert (z == z_ref).all(), (dims, code_T, pan, info)
 
 
 @pytest.mark.parametrize("unit_bits", [0, 8])

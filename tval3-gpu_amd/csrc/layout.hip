@@ -231,14 +231,17 @@ constexpr int kPanelChunkA = TV3_PANEL_CHUNK_A, kPanelChunkT = 1 << 30;
 constexpr int kPanelChunkA12 = 0;
 
 // Workgroups of the packed A kernel (rows in groups of G = 2 or 4) resident
-// at once on the GPU, with its dynamic LDS for max_src staged sources.
+// at once on the GPU, with its dynamic LDS for max_src staged sources (the
+// FP16 and BF16 instances differ only in the weight decode).
 static int64_t packed_A_slots(tv3_ctx *ctx, int blk_nnz, int G, int max_src) {
     const size_t lds = sizeof(float) * (size_t)std::max(max_src, 1);
     const void *k =
-        G == 4 ? (blk_nnz == 2048 ? reinterpret_cast<const void *>(&k_panel12<2048, 4, EpiStoreIf>)
-                                  : reinterpret_cast<const void *>(&k_panel12<4096, 4, EpiStoreIf>))
-               : (blk_nnz == 2048 ? reinterpret_cast<const void *>(&k_panel12<2048, 2, EpiStoreIf>)
-                                  : reinterpret_cast<const void *>(&k_panel12<4096, 2, EpiStoreIf>));
+        G == 4 ? (blk_nnz == 2048
+                      ? reinterpret_cast<const void *>(&k_panel12<2048, 4, __half, EpiStoreIf>)
+                      : reinterpret_cast<const void *>(&k_panel12<4096, 4, __half, EpiStoreIf>))
+               : (blk_nnz == 2048
+                      ? reinterpret_cast<const void *>(&k_panel12<2048, 2, __half, EpiStoreIf>)
+                      : reinterpret_cast<const void *>(&k_panel12<4096, 2, __half, EpiStoreIf>));
     int per_cu = 0;
     TV3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds));
     return (int64_t)std::max(1, per_cu) * std::max(1, ctx->num_cus);
@@ -279,8 +282,9 @@ static void pack_panels(tv3_ctx *ctx, const std::vector<int32_t> &pan_row,
                         int chunk_blocks, Panels &P, bool pack12 = false) {
     hipStream_t s = ctx->stream;
     TV3_REQUIRE(G == 1 || G == 2 || G == 4, TV3_E_ARG, "panel groups of 1, 2 or 4 entries");
-    TV3_REQUIRE(!pack12 || (vt == TV3_F16 && (blk_nnz == 2048 || blk_nnz == 4096)), TV3_E_ARG,
-                "packed entries take FP16 weights and blocks of 2048 or 4096");
+    TV3_REQUIRE(!pack12 || ((vt == TV3_F16 || vt == TV3_BF16) &&
+                            (blk_nnz == 2048 || blk_nnz == 4096)),
+                TV3_E_ARG, "packed entries take FP16 or BF16 weights and blocks of 2048 or 4096");
     const int64_t align = pack12 ? 8 : 4;
     const int64_t npan = (int64_t)pan_row.size() - 1, rows = pan_row.back();
     auto plen = [&](int64_t r) { return ((int64_t)rptr[r + 1] - rptr[r] + G - 1) / G * G; };
@@ -657,7 +661,7 @@ static float tune_shape(tv3_ctx *ctx, ColdTimer &time_launches, SparseRows &S, i
 // voxels, at most 32768 entries: A's rays leave fewer, longer segments and
 // so fewer partial sums for the second pass; A^T stages the union of the two
 // tiles' rays once), timed over row blocks of 1024 and 2048 entries (A^T
-// also 4096, and with FP16 weights packed entries in blocks of 2048 or
+// also 4096, and with FP16 or BF16 weights packed entries in blocks of 2048 or
 // 4096); A also over panels of 4 and 8 tiles where the tile counts are
 // even; each kept when faster than the tuned plain kernel (t_A, t_T in ms).
 static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, float t_A,
@@ -675,8 +679,8 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
     DevBuf<float> xr(m > 0 ? m : 1), xv(n > 0 ? n : 1), y(n > m ? n : (m > 0 ? m : 1));
     TV3_HIP(hipMemsetAsync(xr.p, 0, xr.bytes(), s));
     TV3_HIP(hipMemsetAsync(xv.p, 0, xv.bytes(), s));
-    // FP16 weights: also with packed entries (panels of at most 4096 rays)
-    const bool f16 = F.AT.vtype == TV3_F16;
+    // 16-bit weights: also with packed entries (panels of at most 4096 rays)
+    const bool f16 = F.AT.vtype == TV3_F16 || F.AT.vtype == TV3_BF16;   // 16-bit weights
     for (int unit : {512, 1024})
         for (int packed = 0; packed <= (f16 ? 1 : 0); ++packed) {
             PanelPlan pl2;
@@ -732,7 +736,7 @@ static void tune_panels(tv3_ctx *ctx, ColdTimer &time_launches, FusedLayout &F, 
     std::vector<int32_t> tmap;
     std::vector<TileVolume> vols;
     if (!layout_volume(F, vols) || !supertile_rows(vols, h, hs, tmap)) return;
-    // FP16 weights: also with packed entries (12-bit indices, 28 bytes per 8
+    // 16-bit weights: also with packed entries (12-bit indices, 28 bytes per 8
     // entries instead of 32)
     for (int unit : {2048, 4096}) {
         PanelPlan pl2;
@@ -910,8 +914,9 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
     const bool pa = (panels & 1) != 0, pt = (panels & 2) != 0, packed_T = (panels & 512) != 0;
     const int unit_A = (panels & 32) ? 4096 : (panels & 16) ? 2048 : (panels & 4) ? 1024 : 512;
     const int unit_T = (panels & 8) ? 1024 : 512;
-    TV3_REQUIRE(!packed_T || (pt && F.AT.vtype == TV3_F16 && code_T != -1024), TV3_E_UNSUPPORTED,
-                "packed panel entries take FP16 weights and blocks of 2048 or 4096");
+    const bool half_w = F.AT.vtype == TV3_F16 || F.AT.vtype == TV3_BF16;
+    TV3_REQUIRE(!packed_T || (pt && half_w && code_T != -1024), TV3_E_UNSUPPORTED,
+                "packed panel entries take FP16 or BF16 weights and blocks of 2048 or 4096");
     if (pa || pt) {
         TV3_REQUIRE(pan_nnz > 0, TV3_E_ARG, "panels need pan_nnz > 0");
         for (int c : {pa ? code_N : -1024, pt ? code_T : -1024})
@@ -950,8 +955,9 @@ void layout_force(tv3_op *op, const int32_t dims[3], int code_N, int code_T, int
             TV3_REQUIRE(plan_panels(hs, unit_A, pan_nnz * (unit_A / 512), pl2), TV3_E_UNSUPPORTED,
                         "a panel of this operator would need more rays than LDS holds");
             const bool packed = (panels & 256) != 0;
-            TV3_REQUIRE(!packed || (F.AT.vtype == TV3_F16 && code_N != -1024), TV3_E_UNSUPPORTED,
-                        "packed panel entries take FP16 weights and blocks of 2048 or 4096");
+            TV3_REQUIRE(!packed || (half_w && code_N != -1024), TV3_E_UNSUPPORTED,
+                        "packed panel entries take FP16 or BF16 weights and blocks of 2048 or "
+                        "4096");
             build_panels_A(ctx, hs, pl2, -code_N, F.PA, F.RS,
                            packed ? kPanelChunkA12 : kPanelChunkA, &tmap, packed);
         }

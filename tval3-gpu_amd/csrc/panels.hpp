@@ -226,9 +226,9 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ order
     epi.finish();
 }
 
-// The panel form with packed entries (Panels::fmt 12, FP16 weights): the
-// stored entries in octets of 8, their 12-bit local indices in 3 words
-// (entry e at bits 12 e .. 12 e + 11 of the 96) and their 8 FP16 weights in
+// The panel form with packed entries (Panels::fmt 12, FP16 or BF16 weights,
+// W): the stored entries in octets of 8, their 12-bit local indices in 3
+// words (entry e at bits 12 e .. 12 e + 11 of the 96) and their 8 weights in
 // 16 bytes -- 28 bytes for 8 entries instead of 32, for panels of at most
 // 4,096 sources (A's boxes of 16^3 voxels, A^T panels of at most 4,096
 // rays).  Row blocks start and end 8-aligned; rows unpadded (G = 1) or
@@ -247,23 +247,29 @@ __device__ __forceinline__ Oct load_oct(const uint32_t *pi, const uint4 *pw, int
     return Oct{make_uint3(q[0], q[1], q[2]), pw[o]};
 }
 
-__device__ __forceinline__ float oct_w(uint32_t u, int hi) {
+// weight e of a word (low or high half): FP16, or BF16 (the top half of a float)
+template <typename W> __device__ __forceinline__ float oct_w(uint32_t u, int hi);
+template <> __device__ __forceinline__ float oct_w<__half>(uint32_t u, int hi) {
     return __half2float(__ushort_as_half((unsigned short)(hi ? u >> 16 : u & 0xffffu)));
+}
+template <> __device__ __forceinline__ float oct_w<__hip_bfloat16>(uint32_t u, int hi) {
+    return __uint_as_float(hi ? (u & 0xffff0000u) : (u << 16));
 }
 
 // the 8 products of an octet with the staged sources
+template <typename W>
 __device__ __forceinline__ void oct_prod(const Oct &c, const float *xs, float4 &p0, float4 &p1) {
     const uint32_t a = c.i.x, b = c.i.y, d = c.i.z;
     const int i0 = a & 0xfff, i1 = (a >> 12) & 0xfff, i2 = (a >> 24) | ((b & 0xf) << 8),
               i3 = (b >> 4) & 0xfff, i4 = (b >> 16) & 0xfff, i5 = (b >> 28) | ((d & 0xff) << 4),
               i6 = (d >> 8) & 0xfff, i7 = d >> 20;
-    p0 = make_float4(oct_w(c.w.x, 0) * xs[i0], oct_w(c.w.x, 1) * xs[i1],
-                     oct_w(c.w.y, 0) * xs[i2], oct_w(c.w.y, 1) * xs[i3]);
-    p1 = make_float4(oct_w(c.w.z, 0) * xs[i4], oct_w(c.w.z, 1) * xs[i5],
-                     oct_w(c.w.w, 0) * xs[i6], oct_w(c.w.w, 1) * xs[i7]);
+    p0 = make_float4(oct_w<W>(c.w.x, 0) * xs[i0], oct_w<W>(c.w.x, 1) * xs[i1],
+                     oct_w<W>(c.w.y, 0) * xs[i2], oct_w<W>(c.w.y, 1) * xs[i3]);
+    p1 = make_float4(oct_w<W>(c.w.z, 0) * xs[i4], oct_w<W>(c.w.z, 1) * xs[i5],
+                     oct_w<W>(c.w.w, 0) * xs[i6], oct_w<W>(c.w.w, 1) * xs[i7]);
 }
 
-template <int NB, int G, typename Epi>
+template <int NB, int G, typename W, typename Epi>
 __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ order,
                                                  const int32_t *__restrict__ src_off,
                                                  const int32_t *__restrict__ chunk,
@@ -338,7 +344,7 @@ __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ ord
 #pragma unroll
                 for (int j = 0; j < J; ++j) {
                     float4 v0, v1;
-                    oct_prod(cur[j], xs, v0, v1);
+                    oct_prod<W>(cur[j], xs, v0, v1);
                     const int o = j * 256 + t;
                     if constexpr (G == 4) {
                         *reinterpret_cast<float2 *>(prod + 2 * o) =
@@ -361,7 +367,7 @@ __global__ __launch_bounds__(256) void k_panel12(const int32_t *__restrict__ ord
             float s = 0;
             for (int g = t; g < (d.len >> 3); g += 256) {
                 float4 v0, v1;
-                oct_prod(load_oct(pi, pw, (int64_t)d.p0 / 8 + g), xs, v0, v1);
+                oct_prod<W>(load_oct(pi, pw, (int64_t)d.p0 / 8 + g), xs, v0, v1);
                 s += ((v0.x + v0.y) + (v0.z + v0.w)) + ((v1.x + v1.y) + (v1.z + v1.w));
             }
             s = lane_sum<64>(s);
@@ -479,12 +485,21 @@ void launch_panel(const Panels &P, const float *x, const Epi &epi, hipStream_t s
         if (g == 0) return;
         // sources: A's contiguous range or tile map, A^T's ray list
         const int gather = P.contiguous ? 0 : 1;
+        const bool bf = P.vtype == TV3_BF16;
+        TV3_REQUIRE(bf || P.vtype == TV3_F16, TV3_E_FORMAT, "packed entries hold FP16 or BF16");
 #define TV3_P12CASE(NBV, GV)                                                                  \
     case NBV * 8 + GV:                                                                        \
-        hipLaunchKernelGGL((k_panel12<NBV, GV, Epi>), dim3(g), dim3(256), lds, s, P.order.p,    \
-                           P.src_off.p, P.chunk.p, P.src.p, gather, P.desc.p, P.pptr.p,       \
-                           P.pk.p, reinterpret_cast<const uint4 *>(P.pw.p), P.stored / 8 - 1,  \
-                           x, epi);                                                           \
+        if (bf)                                                                               \
+            hipLaunchKernelGGL((k_panel12<NBV, GV, __hip_bfloat16, Epi>), dim3(g), dim3(256),   \
+                               lds, s, P.order.p, P.src_off.p, P.chunk.p, P.src.p, gather,    \
+                               P.desc.p, P.pptr.p, P.pk.p,                                    \
+                               reinterpret_cast<const uint4 *>(P.pw.p), P.stored / 8 - 1, x,  \
+                               epi);                                                          \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_panel12<NBV, GV, __half, Epi>), dim3(g), dim3(256), lds, s,   \
+                               P.order.p, P.src_off.p, P.chunk.p, P.src.p, gather, P.desc.p,  \
+                               P.pptr.p, P.pk.p, reinterpret_cast<const uint4 *>(P.pw.p),     \
+                               P.stored / 8 - 1, x, epi);                                     \
         break;
         switch (P.blk_nnz * 8 + P.group) {
             TV3_P12CASE(2048, 1) TV3_P12CASE(4096, 1) TV3_P12CASE(2048, 2) TV3_P12CASE(4096, 2)
