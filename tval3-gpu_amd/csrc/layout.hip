@@ -766,11 +766,18 @@ bool build_transposed_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vect
                              std::vector<Panels> &out) {
     if (dtype_size(AT.vtype) != 2 && AT.vtype != TV3_F32) return false;
     const HostCSR h = download_csr(ctx, AT);
-    PanelPlan pl;
+    PanelPlan pl, pl12;
     if (!plan_panels(h, 512, 16384, pl)) return false;
+    // 16-bit weights in row blocks of 2048 or 4096: packed entries, over a
+    // plan whose panels hold at most 4096 rays
+    const bool half_w = AT.vtype == TV3_F16 || AT.vtype == TV3_BF16;
+    const bool have12 = half_w && plan_panels(h, 512, 16384, pl12, 4096);
     out.clear();
     out.resize(blks.size());
-    for (size_t i = 0; i < blks.size(); ++i) build_panels_T(ctx, h, pl, blks[i], out[i]);
+    for (size_t i = 0; i < blks.size(); ++i) {
+        const bool pk = have12 && blks[i] >= 2048;
+        build_panels_T(ctx, h, pk ? pl12 : pl, blks[i], out[i], pk);
+    }
     return true;
 }
 
@@ -798,8 +805,14 @@ bool build_forward_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<
     }
     P.clear(); R.clear();
     P.resize(blks.size()); R.resize(blks.size());
-    for (size_t i = 0; i < blks.size(); ++i)
-        build_panels_A(ctx, h, pl, blks[i], P[i], R[i], kPanelChunkA, boxes ? &tmap : nullptr);
+    // 16^3 boxes with 16-bit weights in row blocks of 2048 or 4096: packed
+    // entries, rows in pairs, work items sized to one round
+    const bool half_w = AT.vtype == TV3_F16 || AT.vtype == TV3_BF16;
+    for (size_t i = 0; i < blks.size(); ++i) {
+        const bool pk = boxes && half_w && blks[i] >= 2048;
+        build_panels_A(ctx, h, pl, blks[i], P[i], R[i], pk ? kPanelChunkA12 : kPanelChunkA,
+                       boxes ? &tmap : nullptr, pk);
+    }
     return true;
 }
 

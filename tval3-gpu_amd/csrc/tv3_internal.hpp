@@ -429,7 +429,8 @@ bool dyn_tile_apply(tv3_op *op, bool trans, const float *x, float *y, hipStream_
 // panel forms of a prebuilt A^T (rows = voxels in tiles of 512 rows, sources =
 // rays), one per row-block size in blks; false when a panel would need more
 // distinct rays than its local indices address or the weights are not F32,
-// F16 or BF16
+// F16 or BF16.  16-bit weights in blocks of 2048 or 4096 are packed (12-bit
+// indices over panels of at most 4096 rays)
 bool build_transposed_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
                              std::vector<Panels> &out);
 // A volume of 8^3 tiles inside the rows of a prebuilt A^T: its first row and
@@ -443,7 +444,8 @@ struct TileVolume {
 // the voxel range starting at row_starts[k] (row_starts ascending, multiples
 // of 512; seg_starts gets one more entry: the segment count).  With boxes,
 // panels over 16^3 voxels of those volumes (each starting at a multiple of
-// 4096 rows with even tile counts; false otherwise)
+// 4096 rows with even tile counts; false otherwise), packed for 16-bit
+// weights in blocks of 2048 or 4096
 bool build_forward_panels(tv3_ctx *ctx, const SparseRows &AT, const std::vector<int> &blks,
                           const std::vector<int64_t> &row_starts, std::vector<Panels> &P,
                           std::vector<SegSum> &R, std::vector<int32_t> &seg_starts,
